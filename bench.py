@@ -1,0 +1,158 @@
+"""Benchmark of the reaction-diffusion hot path on one MI355X.
+
+Workload: uniform 128^3 grid (4,096 blocks of 8^3), FP32 tumour state, seeded
+brain phantom, Dw ~ logU[0.002, 0.015] cm^2/day and rho ~ logU[0.002, 0.2]/day
+(seeded draw), T = 400 days, explicit Euler with dt = 0.9 h^2/(6 Dw).
+
+One bench "step" is one full forward solve (all Euler steps to T) starting from
+the same initial condition, which is restored from a device-side checkpoint.
+Inputs are resident in HBM before the timed region starts.
+
+value      = leaf cells x Euler steps x solves / wall seconds (cell-updates/s)
+roofline   = 16 B per cell-update (read u, write u, read p_w, read p_g) x cells
+             per launch / average rd_step_kernel duration (HIP events on the
+             context stream), against 8 TB/s
+cpu_baseline = the reference's ReactionDiffusionOperator + UpdateTumor
+             (oracle/_ref/rd_ref_f32, built from the reference sources) on the
+             host cores, on a bounded sample of the same workload; the oracle
+             restatement (single thread) when that build is absent.
+
+Prints exactly one JSON line on stdout; progress goes to stderr.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import subprocess
+import sys
+import tempfile
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+BYTES_PER_CELL_UPDATE_F32 = 16
+METRIC = "RD cell-updates/s (% HBM roofline); batched fwd+adjoint evals/s vs plain"
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "rd_step_uniform128_f32_traffic.json")
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(case, pw, pg, phi0, target_s=12.0):
+    """Bounded sample of the same workload on the host CPU."""
+    ref = os.path.join(ROOT, "oracle", "_ref", "rd_ref_f32")
+    threads = min(os.cpu_count() or 1, 16)
+    if os.path.exists(ref):
+        with tempfile.TemporaryDirectory() as tmp:
+            inp, out = os.path.join(tmp, "in.bin"), os.path.join(tmp, "out.bin")
+
+            def run(k):
+                with open(inp, "wb") as f:
+                    f.write(np.array([case.n], np.int32).tobytes())
+                    f.write(np.array([k], np.int64).tobytes())
+                    f.write(np.array([case.Dw, case.Dg, case.rho, 1.0 / case.n, case.dt], np.float64).tobytes())
+                    for a in (pw, pg, phi0):
+                        f.write(np.ascontiguousarray(a, np.float32).tobytes())
+                r = subprocess.run([ref, inp, out, str(threads)], capture_output=True, text=True, check=True,
+                                   timeout=300)
+                return float(r.stdout.split("elapsed_s=")[1].split()[0])
+
+            t2 = run(2)
+            k = int(max(2, min(400, target_s / max(t2 / 2, 1e-6))))
+            el = run(k)
+            return {"value": case.ncells * k / el, "unit": "cell-updates/s", "cores": threads,
+                    "kind": "reference",
+                    "sample": "%d Euler steps of the 128^3 FP32 workload (%.1f s), reference operator over 8^3 "
+                              "blocks split across %d threads" % (k, el, threads)}
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from conftest import Oracle  # checker library, used here only as the CPU baseline
+    o = Oracle()
+    t0 = time.perf_counter()
+    o.rd_steps(phi0, pw, pg, case.Dw, case.Dg, case.rho, 1.0 / case.n, case.dt, 2)
+    t2 = time.perf_counter() - t0
+    k = int(max(2, min(400, target_s / max(t2 / 2, 1e-6))))
+    t0 = time.perf_counter()
+    o.rd_steps(phi0, pw, pg, case.Dw, case.Dg, case.rho, 1.0 / case.n, case.dt, k)
+    el = time.perf_counter() - t0
+    return {"value": case.ncells * k / el, "unit": "cell-updates/s", "cores": 1, "kind": "port",
+            "sample": "%d Euler steps of the 128^3 FP32 workload (%.1f s), oracle restatement, 1 thread" % (k, el)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10, help="timed full solves")
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--seed", type=int, default=2)
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print("bench.py runs on one GPU only: this build has no multi-GPU path (samples shard with no "
+              "intra-stage traffic; see the design notes). Use --gpus 1.", file=sys.stderr)
+        return 2
+
+    import gliomasolver_amd as gh
+    from gliomasolver_amd import FIELD_PG, FIELD_PHI, FIELD_PW
+    from gliomasolver_amd import workloads as wl
+
+    case = wl.config_uniform128(a.seed)
+    log("workload", json.dumps(case.describe()))
+    ctx = wl.load_context(case)
+    pw, pg, phi0 = ctx.get_field(FIELD_PW), ctx.get_field(FIELD_PG), ctx.get_field(FIELD_PHI)
+    ctx.checkpoint_save(0)
+    for i in range(a.warmup):
+        ctx.checkpoint_load(0)
+        ctx.rd_steps(case.nsteps, case.dt)
+    ctx.sync()
+    log("warmup done")
+    kernel_ms = 0.0
+    ctx.sync()
+    t0 = time.perf_counter()
+    for i in range(a.steps):
+        ctx.checkpoint_load(0)
+        kernel_ms += ctx.rd_steps(case.nsteps, case.dt, timed=True)
+    ctx.sync()
+    wall = time.perf_counter() - t0
+    log("timed %d solves x %d Euler steps in %.3f s" % (a.steps, case.nsteps, wall))
+    final = ctx.get_field(FIELD_PHI)
+    ctx.close()
+
+    updates = case.ncells * case.nsteps * a.steps
+    value = updates / wall
+    launches = case.nsteps * a.steps
+    avg_launch_s = kernel_ms * 1e-3 / launches
+    achieved = BYTES_PER_CELL_UPDATE_F32 * case.ncells / avg_launch_s / 1e9
+    traffic = None
+    if os.path.exists(TRAFFIC_FILE):
+        traffic = json.load(open(TRAFFIC_FILE)).get("bytes_per_launch")
+    roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+            "kernel": "rd_step_kernel<float,true>", "avg_launch_us": round(avg_launch_s * 1e6, 3),
+            "algorithmic_bytes_per_launch": BYTES_PER_CELL_UPDATE_F32 * case.ncells}
+    cpu = None
+    if not a.no_cpu:
+        log("cpu baseline ...")
+        cpu = cpu_baseline(case, pw, pg, phi0)
+        log("cpu baseline", json.dumps(cpu))
+    line = {"metric": METRIC, "value": value, "unit": "cell-updates/s", "n_gpus": 1, "steps": a.steps,
+            "warmup": a.warmup, "ms_per_step": wall * 1e3 / a.steps, "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+            "config": {"workload": "uniform 128^3 RD to T=400 d, FP32 state, bit-exact to the reference operator",
+                       "grid": "128^3 (4096 blocks of 8^3)", "euler_steps_per_solve": case.nsteps,
+                       "Dw_phys": case.Dw_phys, "rho": case.rho, "dt_days": case.dt, "seed": case.seed,
+                       "final_phi_sum": float(final.astype(np.float64).sum())},
+            "roofline": roof, "cpu_baseline": cpu}
+    if cpu:
+        line["gpu_over_cpu"] = round(value / cpu["value"], 1)
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,154 @@
+"""gliomasolver_amd -- MI355X-native reaction-diffusion hot path of GliomaSolver.
+
+Python mirror of the C ABI in include/gliomahip.h.  All compute runs in
+libgliomahip.so (HIP, gfx950); this module only moves numpy arrays across the
+ABI.  Names follow the reference (Glioma/Glioma_ReactionDiffusion.cpp,
+Glioma/Operators/ReactionDiffusionOperator.h).
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _capi
+from ._capi import (FIELD_DPHIDT, FIELD_PCSF, FIELD_PG, FIELD_PHI, FIELD_PW, GH_REAL_F32, GH_REAL_F64,
+                    GHError, check, load)
+
+__all__ = ["Context", "GHError", "phantom", "schedule", "gaussian_ic", "real_kind", "np_real", "load",
+           "FIELD_PHI", "FIELD_PW", "FIELD_PG", "FIELD_PCSF", "FIELD_DPHIDT", "GH_REAL_F32", "GH_REAL_F64"]
+
+
+def real_kind(real) -> int:
+    if real in ("f32", "float", "float32", np.float32, GH_REAL_F32):
+        return GH_REAL_F32
+    if real in ("f64", "double", "float64", np.float64, GH_REAL_F64):
+        return GH_REAL_F64
+    raise ValueError("real must be f32 or f64, got %r" % (real,))
+
+
+def np_real(kind: int):
+    return np.float32 if kind == GH_REAL_F32 else np.float64
+
+
+def _ptr(a: np.ndarray):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def phantom(n: int, seed: int):
+    """Seeded synthetic brain: returns (gm, wm, csf, seed_voxel) as n^3 float64 arrays (x fastest)."""
+    lib = load()
+    gm = np.zeros((n, n, n), np.float64)
+    wm = np.zeros_like(gm)
+    csf = np.zeros_like(gm)
+    sv = (C.c_int * 3)()
+    check(lib.gh_phantom(n, C.c_uint64(seed), _ptr(gm), _ptr(wm), _ptr(csf), sv))
+    return gm, wm, csf, (sv[0], sv[1], sv[2])
+
+
+def schedule(real, Dw_phys: float, L: float, tend: float, gpd: int, cfl: float = 0.99):
+    """Rescaled Dw, Dg, dt and step count of Glioma_ReactionDiffusion::run (all in Real)."""
+    lib = load()
+    Dw, Dg, dt = C.c_double(), C.c_double(), C.c_double()
+    ns = C.c_int64()
+    check(lib.gh_schedule(real_kind(real), Dw_phys, L, tend, gpd, cfl, C.byref(Dw), C.byref(Dg), C.byref(dt),
+                          C.byref(ns)))
+    return Dw.value, Dg.value, dt.value, ns.value
+
+
+def gaussian_ic(real, p_w: np.ndarray, p_g: np.ndarray, center, sigma: float = 1.0, peak: float = 0.1):
+    k = real_kind(real)
+    t = np_real(k)
+    n = p_w.shape[0]
+    pw = np.ascontiguousarray(p_w, t)
+    pg = np.ascontiguousarray(p_g, t)
+    out = np.zeros_like(pw)
+    c = (C.c_double * 3)(*[float(v) for v in center])
+    check(load().gh_gaussian_ic(k, n, _ptr(pw), _ptr(pg), c, sigma, peak, _ptr(out)))
+    return out
+
+
+class Context:
+    """One device context (gh_ctx): grid topology plus block-major fields in HBM."""
+
+    def __init__(self, device: int = 0, real="f32"):
+        self.lib = load()
+        self.kind = real_kind(real)
+        self.dtype = np_real(self.kind)
+        h = C.c_void_p()
+        check(self.lib.gh_ctx_create(device, self.kind, C.byref(h)))
+        self.h = h
+        self.n = 0
+
+    def close(self):
+        if getattr(self, "h", None) is not None and self.h.value:
+            self.lib.gh_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # ---- grid ----
+    def grid_init_uniform(self, bpd: int):
+        check(self.lib.gh_grid_init_uniform(self.h, bpd))
+        self.n = 8 * bpd
+
+    def nblocks(self) -> int:
+        v = C.c_int()
+        check(self.lib.gh_grid_nblocks(self.h, C.byref(v)))
+        return v.value
+
+    def blocks(self):
+        nb = self.nblocks()
+        lev = np.zeros(nb, np.int32)
+        idx = np.zeros((nb, 3), np.int32)
+        check(self.lib.gh_grid_blocks(self.h, _ptr(lev), _ptr(idx)))
+        return lev, idx
+
+    # ---- fields ----
+    def set_anatomy(self, gm, wm, csf):
+        n = gm.shape[0]
+        arrs = [np.ascontiguousarray(a, np.float64) for a in (gm, wm, csf)]
+        check(self.lib.gh_set_anatomy(self.h, n, *[_ptr(a) for a in arrs]))
+
+    def set_field(self, field: int, values: np.ndarray):
+        v = np.ascontiguousarray(values, self.dtype)
+        check(self.lib.gh_set_field_dense(self.h, field, v.shape[0], _ptr(v)))
+
+    def get_field(self, field: int) -> np.ndarray:
+        out = np.zeros((self.n, self.n, self.n), self.dtype)
+        check(self.lib.gh_get_field_dense(self.h, field, self.n, _ptr(out)))
+        return out
+
+    # ---- reaction-diffusion ----
+    def rd_prepare(self, Dw: float, Dg: float, rho: float):
+        check(self.lib.gh_rd_prepare(self.h, Dw, Dg, rho))
+
+    def rd_rhs(self):
+        check(self.lib.gh_rd_rhs(self.h))
+
+    def rd_update(self, dt: float):
+        check(self.lib.gh_rd_update(self.h, dt))
+
+    def rd_steps(self, nsteps: int, dt: float, timed: bool = False):
+        ms = C.c_double()
+        check(self.lib.gh_rd_steps(self.h, nsteps, dt, C.byref(ms) if timed else None))
+        return ms.value if timed else None
+
+    def checkpoint_save(self, slot: int):
+        check(self.lib.gh_checkpoint_save(self.h, slot))
+
+    def checkpoint_load(self, slot: int):
+        check(self.lib.gh_checkpoint_load(self.h, slot))
+
+    def sync(self):
+        check(self.lib.gh_sync(self.h))

@@ -1,0 +1,61 @@
+// gh_ctx.h -- device context shared by the HIP translation units.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "gh_internal.h"
+
+namespace gh {
+
+#define GH_HIP_CHECK(expr)                                                                     \
+    do {                                                                                       \
+        hipError_t _e = (expr);                                                                \
+        if (_e != hipSuccess)                                                                  \
+            return ::gh::set_error(GH_EHIP, std::string(#expr) + ": " + hipGetErrorString(_e)); \
+    } while (0)
+
+// Device buffer owned by the context (plain hipMalloc; freed in the dtor).
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    int ensure(size_t nbytes);
+    void release();
+    template <typename T> T* as() const { return static_cast<T*>(p); }
+};
+
+// Per-context device state.  Field arrays are block-major [slot][z][y][x],
+// 512 cells per slot, element type Real (float or double per real_kind).
+struct Ctx {
+    int device = 0;
+    int real_kind = GH_REAL_F32;
+    size_t real_size = 4;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    Topology topo;
+
+    DevBuf phi[2];      // ping-pong tumor density
+    int cur = 0;        // phi[cur] holds the current state
+    DevBuf dphidt;      // two-pass right-hand side
+    DevBuf pw, pg, pcsf;
+    DevBuf statD;       // 3 * ncells: harmonic diffusivity of the +x,+y,+z face of each cell
+    DevBuf statTau;     // ncells: neighbour is outside (chf < 0.1) iff phi < tau
+    DevBuf statSig;     // ncells: cell is in the domain iff phi > sigma
+    DevBuf nbr;         // 6 * nblocks int32
+    std::vector<DevBuf> ckpt;  // device-side copies of phi (gh_checkpoint_*)
+    bool prepared = false;
+    double Dw = 0, Dg = 0, rho = 0;
+    double dt_last = 0;  // dt of the running gh_rd_steps call
+
+    size_t ncells() const { return (size_t)topo.nblocks() * BCELLS; }
+    int alloc_fields();
+    int upload_topology();
+};
+
+Ctx* as_ctx(gh_ctx* c);
+
+// RD launchers (gh_rd.hip)
+int rd_prepare(Ctx& c, double Dw, double Dg, double rho);
+int rd_rhs(Ctx& c);
+int rd_update(Ctx& c, double dt);
+int rd_steps(Ctx& c, int64_t nsteps, double dt);
+
+}  // namespace gh

@@ -1,0 +1,115 @@
+/*
+ * gliomahip.h -- C ABI of libgliomahip.so, the MI355X (gfx950) implementation
+ * of GliomaSolver's reaction-diffusion hot path.
+ *
+ * Conventions
+ *   - every entry point returns int status: GH_OK (0) or a negative GH_E* code;
+ *     gh_last_error() returns the message of the calling thread's last failure.
+ *     Nothing aborts across the ABI.
+ *   - host pointers are borrowed for the duration of the call only.
+ *   - "real" buffers hold float when the context was created with GH_REAL_F32
+ *     and double with GH_REAL_F64 (the reference's Real, MRAG/MRAGcore/MRAGCommon.h:43-48).
+ *   - dense arrays are x-fastest, index i + n*(j + n*k) (Matrix::D3D,
+ *     Glioma/Matrix.h:593-601).  Block-major arrays are [block][z][y][x] with
+ *     8^3 cells per block, blocks in the order returned by gh_grid_blocks().
+ *   - one context per host thread; all device work is ordered on the context's
+ *     HIP stream.
+ */
+#ifndef GLIOMAHIP_H
+#define GLIOMAHIP_H
+#include <stdint.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GH_OK 0
+#define GH_EINVAL -1   /* bad argument or size */
+#define GH_EHIP -2     /* HIP runtime failure */
+#define GH_ESTATE -3   /* call out of order (e.g. no grid yet) */
+#define GH_ENOMEM -4   /* device or host allocation failed */
+#define GH_EIO -5      /* file could not be read or written */
+
+#define GH_REAL_F32 0
+#define GH_REAL_F64 1
+
+typedef struct gh_ctx gh_ctx;
+
+/* ---- context --------------------------------------------------------------
+ * Replaces Environment::setup + the BlockCollection owned by Grid
+ * (MRAG/MRAGcore/MRAGEnvironment.h:59-72, MRAG/MRAGcore/MRAGrid.inl:20-54). */
+int gh_ctx_create(int device, int real_kind, gh_ctx** out);
+int gh_ctx_destroy(gh_ctx* ctx);
+const char* gh_last_error(void);
+int gh_sync(gh_ctx* ctx);
+
+/* ---- grid topology ---------------------------------------------------------
+ * gh_grid_init_uniform: Grid<W,B>(bpd,bpd,bpd) -- all blocks at the finest
+ * level log2(bpd) (MRAG/MRAGcore/MRAGrid.inl:92-111, MRAGridHelpers.h:9-50).
+ * gh_grid_blocks: BlockInfo list (Grid::getBlocksInfo, MRAGrid.inl:190-219):
+ * level[b], index[3b..3b+2]; h = 2^-level / 8, origin = index * 2^-level. */
+int gh_grid_init_uniform(gh_ctx* ctx, int bpd);
+int gh_grid_nblocks(gh_ctx* ctx, int* nblocks);
+int gh_grid_blocks(gh_ctx* ctx, int* level, int* index);
+
+/* ---- fields ----------------------------------------------------------------
+ * gh_set_anatomy: Glioma_ReactionDiffusion::_ic anatomy part
+ * (Glioma/Glioma_ReactionDiffusion.cpp:79-152): n^3 GM/WM/CSF fractions as
+ * read from GM.dat/WM.dat/CSF.dat, majority vote, normalised p_w, p_g, p_csf.
+ * Requires a uniform grid with n = 8*bpd (voxel (i,j,k) -> cell (i,j,k)). */
+int gh_set_anatomy(gh_ctx* ctx, int n, const double* gm, const double* wm, const double* csf);
+/* dense n^3 real arrays <-> the uniform grid's cells */
+#define GH_FIELD_PHI 0
+#define GH_FIELD_PW 1
+#define GH_FIELD_PG 2
+#define GH_FIELD_PCSF 3
+#define GH_FIELD_DPHIDT 4
+int gh_set_field_dense(gh_ctx* ctx, int field, int n, const void* values);
+int gh_get_field_dense(gh_ctx* ctx, int field, int n, void* values);
+
+/* ---- schedule --------------------------------------------------------------
+ * Glioma_ReactionDiffusion::run (Glioma/Glioma_ReactionDiffusion.cpp:296-320):
+ * Dw/(L*L), Dg = 0.1*Dw, dt = cfl*h^2/(6*max(Dw,Dg)) with h = 1/gpd, and the
+ * number of iterations of while(t<=tend){t+=dt}, all in Real. */
+int gh_schedule(int real_kind, double Dw_phys, double L, double tend, int gpd, double cfl,
+                double* Dw, double* Dg, double* dt, int64_t* nsteps);
+
+/* ---- reaction-diffusion ----------------------------------------------------
+ * gh_rd_prepare: per-solve constants of ReactionDiffusionOperator(Dw,Dg,rho)
+ * (Glioma/Operators/ReactionDiffusionOperator.h:29-35): harmonic face
+ * diffusivities (:148-163) and the per-cell no-flux / in-domain thresholds
+ * (:103, :166-181).  Must be called after the anatomy changes.
+ * gh_rd_rhs: pipeline_process(vInfo, coll, bInfo, rhs) -> dphidt
+ *   (Glioma_ReactionDiffusion.cpp:185, ReactionDiffusionOperator.h:98-139).
+ * gh_rd_update: BlockProcessing::process(vInfo, coll, UpdateTumor(dt))
+ *   (Glioma_ReactionDiffusion.cpp:186, ReactionDiffusionOperator.h:215-222).
+ * gh_rd_steps: nsteps of _reactionDiffusionStep (Glioma_ReactionDiffusion.cpp:
+ *   176-187, 317-322) fused into one kernel per step; bit-identical to
+ *   alternating gh_rd_rhs and gh_rd_update.  If elapsed_ms is not NULL the
+ *   call synchronises and returns the device time of the steps. */
+int gh_rd_prepare(gh_ctx* ctx, double Dw, double Dg, double rho);
+int gh_rd_rhs(gh_ctx* ctx);
+int gh_rd_update(gh_ctx* ctx, double dt);
+int gh_rd_steps(gh_ctx* ctx, int64_t nsteps, double dt, double* elapsed_ms);
+
+/* ---- checkpoints (no reference counterpart: the reference keeps one copy of
+ * the state) -- device-side copies of phi, used to restart solves from one
+ * initial condition and for the adjoint's checkpoint/recompute.  Slots are
+ * allocated on first save (0 <= slot <= 4096). */
+int gh_checkpoint_save(gh_ctx* ctx, int slot);
+int gh_checkpoint_load(gh_ctx* ctx, int slot);
+
+/* ---- synthetic inputs (no reference counterpart: the reference reads
+ * patient .dat files) -------------------------------------------------------
+ * gh_phantom: seeded brain phantom on an n^3 vertex grid -- outer CSF shell,
+ * folded GM cortex, WM interior, two CSF ventricles, tanh phase fields of
+ * width 1.5 cells.  seed_voxel receives a seeded WM voxel (i,j,k).
+ * gh_gaussian_ic: phi = peak*exp(-|x-x0|^2/(2 sigma^2)) (cells), 0 where
+ * p_w+p_g == 0; written as real_kind values. */
+int gh_phantom(int n, uint64_t seed, double* gm, double* wm, double* csf, int* seed_voxel);
+int gh_gaussian_ic(int real_kind, int n, const void* p_w, const void* p_g, const double* center,
+                   double sigma, double peak, void* phi);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,61 @@
+/*
+ * gh_oracle.h -- CPU oracle for the GliomaSolver reaction-diffusion hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker: only tests/,
+ * __graft_entry__.smoke() and bench.py's cpu_baseline leg may load it.  The
+ * product (gliomasolver_amd/libgliomahip.so) never links or calls it.
+ *
+ * Every function restates the reference algorithm in plain C and cites the
+ * reference file:line it follows.  Arrays are x-fastest, index
+ * i + n*(j + n*k), the layout of Matrix::D3D (Glioma/Matrix.h:593-601).
+ */
+#ifndef GH_ORACLE_H
+#define GH_ORACLE_H
+#include <stdint.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- anatomy: Glioma/Glioma_ReactionDiffusion.cpp:139-152 (majority vote) */
+void gho_anatomy_f32(int64_t ncells, const double* gm, const double* wm, const double* csf,
+                     float* p_w, float* p_g, float* p_csf);
+void gho_anatomy_f64(int64_t ncells, const double* gm, const double* wm, const double* csf,
+                     double* p_w, double* p_g, double* p_csf);
+
+/* ---- time step and step count: Glioma/Glioma_ReactionDiffusion.cpp:302-320 */
+/* outputs: Dw, Dg (rescaled by L^2), dt, nsteps (iterations of while(t<=tend)) */
+void gho_schedule_f32(double Dw_phys, double L, double tend, int gpd, double cfl,
+                      float* Dw, float* Dg, float* dt, int64_t* nsteps);
+void gho_schedule_f64(double Dw_phys, double L, double tend, int gpd, double cfl,
+                      double* Dw, double* Dg, double* dt, int64_t* nsteps);
+
+/* ---- uniform periodic grid, n^3 cells, spacing h (= info.h[0]).
+ * nsteps two-pass Euler steps: rhs (ReactionDiffusionOperator.h:98-139)
+ * then UpdateTumor (:215-222).  dphidt is an n^3 scratch array. */
+void gho_rd_steps_f32(int n, float* phi, float* dphidt, const float* p_w, const float* p_g,
+                      float Dw, float Dg, float rho, float h, double dt, int64_t nsteps);
+void gho_rd_steps_f64(int n, double* phi, double* dphidt, const double* p_w, const double* p_g,
+                      double Dw, double Dg, double rho, double h, double dt, int64_t nsteps);
+
+/* ---- likelihood: tools/pi4u_lite/Inference/Likelihood/source/HGG_Likelihood.cpp:40-192.
+ * roi may be NULL (bROI=0).  out[0]=L_PET, out[1]=L_T1, out[2]=L_T2 (log-likelihoods),
+ * returns -(L_PET+L_T1+L_T2) as long double rounded to double; *npet gets N. */
+double gho_likelihood(int nx, int ny, int nz, const double* model, const double* pet,
+                      const double* t1c, const double* flair, const double* roi, int nroi,
+                      double PETsigma2, double PETscale, double T1uc, double T2uc, double slope,
+                      int stepPET, int stepTi, double out[3], int64_t* npet);
+long double gho_log_bernoulli(double u, double y, double uc, double slope);
+
+/* ---- Matrix::D3D .dat files: Glioma/Matrix.h:625-665 */
+int gho_dat_write(const char* path, int nx, int ny, int nz, const double* data);
+int gho_dat_read_header(const char* path, int dims[3]);
+int gho_dat_read(const char* path, int nx, int ny, int nz, double* data);
+
+/* ---- wavelets: MRAG/MRAGcore/MRAGWavelets_StaticData.cpp:13-39 */
+/* order 2 or 4; filter in {0:Ha,1:Ga,2:Hs,3:Gs}; writes support[2] and taps; returns ntaps */
+int gho_wavelet_taps(int order, int filter, int support[2], double* taps);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,129 @@
+"""Shared fixtures.  The oracle (oracle/liboracle.so) is the checker; tests are
+the only product-side code allowed to load it."""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+ORACLE_SO = os.path.join(ROOT, "oracle", "liboracle.so")
+REF_DIR = os.path.join(ROOT, "oracle", "_ref")
+GOLDEN = os.path.join(ROOT, "tests", "golden")
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs an MI355X (HIP device)")
+
+
+_vp, _i, _d, _i64 = C.c_void_p, C.c_int, C.c_double, C.c_int64
+
+
+def _load_oracle():
+    if not os.path.exists(ORACLE_SO):
+        subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle"), "oracle"], check=True)
+    lib = C.CDLL(ORACLE_SO)
+    sig = {
+        "gho_anatomy_f32": (None, [_i64, _vp, _vp, _vp, _vp, _vp, _vp]),
+        "gho_anatomy_f64": (None, [_i64, _vp, _vp, _vp, _vp, _vp, _vp]),
+        "gho_schedule_f32": (None, [_d, _d, _d, _i, _d, _vp, _vp, _vp, _vp]),
+        "gho_schedule_f64": (None, [_d, _d, _d, _i, _d, _vp, _vp, _vp, _vp]),
+        "gho_rd_steps_f32": (None, [_i, _vp, _vp, _vp, _vp, C.c_float, C.c_float, C.c_float, C.c_float, _d, _i64]),
+        "gho_rd_steps_f64": (None, [_i, _vp, _vp, _vp, _vp, _d, _d, _d, _d, _d, _i64]),
+        "gho_likelihood": (_d, [_i, _i, _i, _vp, _vp, _vp, _vp, _vp, _i, _d, _d, _d, _d, _d, _i, _i, _vp, _vp]),
+        "gho_dat_write": (_i, [C.c_char_p, _i, _i, _i, _vp]),
+        "gho_dat_read": (_i, [C.c_char_p, _i, _i, _i, _vp]),
+        "gho_wavelet_taps": (_i, [_i, _i, _vp, _vp]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    return lib
+
+
+class Oracle:
+    """Thin numpy façade over liboracle.so."""
+
+    def __init__(self):
+        self.lib = _load_oracle()
+
+    @staticmethod
+    def p(a):
+        return a.ctypes.data_as(C.c_void_p)
+
+    def anatomy(self, gm, wm, csf, real):
+        t = np.float32 if real == "f32" else np.float64
+        pw, pg, pc = (np.zeros(gm.shape, t) for _ in range(3))
+        f = self.lib.gho_anatomy_f32 if real == "f32" else self.lib.gho_anatomy_f64
+        f(gm.size, self.p(gm), self.p(wm), self.p(csf), self.p(pw), self.p(pg), self.p(pc))
+        return pw, pg, pc
+
+    def schedule(self, real, Dw_phys, L, tend, gpd, cfl):
+        t = C.c_float if real == "f32" else C.c_double
+        Dw, Dg, dt, ns = t(), t(), t(), C.c_int64()
+        f = self.lib.gho_schedule_f32 if real == "f32" else self.lib.gho_schedule_f64
+        f(Dw_phys, L, tend, gpd, cfl, C.byref(Dw), C.byref(Dg), C.byref(dt), C.byref(ns))
+        return Dw.value, Dg.value, dt.value, ns.value
+
+    def rd_steps(self, phi, pw, pg, Dw, Dg, rho, h, dt, nsteps):
+        """Returns phi after nsteps (input not modified)."""
+        real = "f32" if phi.dtype == np.float32 else "f64"
+        phi = phi.copy()
+        dphi = np.zeros_like(phi)
+        f = self.lib.gho_rd_steps_f32 if real == "f32" else self.lib.gho_rd_steps_f64
+        f(phi.shape[0], self.p(phi), self.p(dphi), self.p(pw), self.p(pg), Dw, Dg, rho, h, dt, nsteps)
+        return phi
+
+    def likelihood(self, model, pet, t1c, flair, roi=None, PETsigma2=0.000361, PETscale=0.85, T1uc=0.7, T2uc=0.25,
+                   slope=2.0, stepPET=1, stepTi=1):
+        nx, ny, nz = model.shape[2], model.shape[1], model.shape[0]
+        out = np.zeros(3)
+        npet = C.c_int64()
+        r = None if roi is None else np.ascontiguousarray(roi.T, np.float64)  # column-major (i + n*col)
+        total = self.lib.gho_likelihood(nx, ny, nz, self.p(model), self.p(pet), self.p(t1c), self.p(flair),
+                                        None if r is None else self.p(r), 0 if roi is None else roi.shape[0],
+                                        PETsigma2, PETscale, T1uc, T2uc, slope, stepPET, stepTi, self.p(out),
+                                        C.byref(npet))
+        return total, out, npet.value
+
+
+@pytest.fixture(scope="session")
+def oracle():
+    return Oracle()
+
+
+def ref_available(name: str) -> bool:
+    return os.path.exists(os.path.join(REF_DIR, name))
+
+
+def run_rd_ref(real: str, phi, pw, pg, Dw, Dg, rho, h, dt, nsteps, workdir):
+    """Run oracle/_ref/rd_ref_{f32,f64} (the reference operator) on dense arrays."""
+    exe = os.path.join(REF_DIR, "rd_ref_" + real)
+    t = np.float32 if real == "f32" else np.float64
+    n = phi.shape[0]
+    inp = os.path.join(workdir, "rd_in.bin")
+    out = os.path.join(workdir, "rd_out.bin")
+    with open(inp, "wb") as f:
+        f.write(np.array([n], np.int32).tobytes())
+        f.write(np.array([nsteps], np.int64).tobytes())
+        f.write(np.array([Dw, Dg, rho, h, dt], np.float64).tobytes())
+        for a in (pw, pg, phi):
+            f.write(np.ascontiguousarray(a, t).tobytes())
+    subprocess.run([exe, inp, out], check=True, capture_output=True)
+    return np.fromfile(out, t).reshape(phi.shape)
+
+
+def gpu_available() -> bool:
+    try:
+        import torch
+        return torch.cuda.is_available()
+    except Exception:
+        return False
