@@ -15,7 +15,7 @@ from . import _capi
 from ._capi import (FIELD_DPHIDT, FIELD_PCSF, FIELD_PG, FIELD_PHI, FIELD_PW, GH_REAL_F32, GH_REAL_F64,
                     GHError, check, load)
 
-__all__ = ["Context", "GHError", "phantom", "schedule", "gaussian_ic", "real_kind", "np_real", "load",
+__all__ = ["Context", "GHError", "phantom", "schedule", "gaussian_ic", "rd_thresholds", "real_kind", "np_real", "load",
            "FIELD_PHI", "FIELD_PW", "FIELD_PG", "FIELD_PCSF", "FIELD_DPHIDT", "GH_REAL_F32", "GH_REAL_F64"]
 
 
@@ -66,6 +66,17 @@ def gaussian_ic(real, p_w: np.ndarray, p_g: np.ndarray, center, sigma: float = 1
     c = (C.c_double * 3)(*[float(v) for v in center])
     check(load().gh_gaussian_ic(k, n, _ptr(pw), _ptr(pg), c, sigma, peak, _ptr(out)))
     return out
+
+
+def rd_thresholds(real, p_w: np.ndarray, p_g: np.ndarray):
+    """(tau, sigma) per cell: neighbour-outside iff phi < tau, in-domain iff phi > sigma."""
+    t = np_real(real_kind(real))
+    pw = np.ascontiguousarray(p_w, t)
+    pg = np.ascontiguousarray(p_g, t)
+    tau = np.zeros_like(pw)
+    sig = np.zeros_like(pw)
+    check(load().gh_rd_thresholds(real_kind(real), pw.size, _ptr(pw), _ptr(pg), _ptr(tau), _ptr(sig)))
+    return tau, sig
 
 
 class Context:

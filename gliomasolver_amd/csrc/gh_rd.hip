@@ -27,71 +27,12 @@
 #include <cmath>
 
 #include "gh_ctx.h"
+#include "gh_rd_common.h"
 
 #pragma clang fp contract(off)
 
 namespace gh {
 namespace {
-
-// ---- ordered integer keys of float/double (monotone in the value) ----------
-__device__ inline int64_t okey(float x) {
-    int32_t b = __float_as_int(x);
-    return b >= 0 ? b : (b ^ 0x7fffffff);
-}
-__device__ inline float okey_inv(int64_t k, float) {
-    int32_t kk = (int32_t)k;
-    return __int_as_float(kk >= 0 ? kk : (kk ^ 0x7fffffff));
-}
-__device__ inline int64_t okey(double x) {
-    int64_t b = __double_as_longlong(x);
-    return b >= 0 ? b : (b ^ 0x7fffffffffffffffLL);
-}
-__device__ inline double okey_inv(int64_t k, double) {
-    return __longlong_as_double(k >= 0 ? k : (k ^ 0x7fffffffffffffffLL));
-}
-template <typename Real> __device__ inline Real real_max();
-template <> __device__ inline float real_max<float>() { return FLT_MAX; }
-template <> __device__ inline double real_max<double>() { return DBL_MAX; }
-
-// df = p_w*Dw + p_g*Dg in Real (ReactionDiffusionOperator.h:105-111)
-template <typename Real>
-__device__ inline Real diffusivity(Real pw, Real pg, Real Dw, Real Dg) {
-    Real a = pw * Dw;
-    Real b = pg * Dg;
-    return a + b;
-}
-
-// _harmonic_mean (ReactionDiffusionOperator.h:148-163): from the cell's view
-// with df_loc = dc and neighbour dn.  (2*dn)*dc and (2*dc)*dn are the same
-// double and dn+dc commutes, so the value is the same from both sides.
-template <typename Real>
-__device__ inline Real harmonic(Real dn, Real dc) {
-    const Real s = dn + dc;
-    if (s < (Real)1.0e-08) return (Real)0.;
-    return (Real)(((2. * (double)dn) * (double)dc) / (double)s);
-}
-
-// smallest phi with !(phi + p_w + p_g < 0.1)  (ReactionDiffusionOperator.h:
-// 115-120, 166-171); the neighbour is "outside" iff phi < tau.
-template <typename Real>
-__device__ inline Real chf_threshold(Real pw, Real pg) {
-    const Real c = (Real)0.1;
-    auto outside = [&](Real p) {
-        Real t = p + pw;
-        t = t + pg;
-        return t < c;
-    };
-    const Real M = real_max<Real>();
-    if (!outside(-M)) return -(Real)INFINITY;
-    if (outside(M)) return (Real)INFINITY;
-    int64_t lo = okey(-M), hi = okey(M);  // outside(lo), !outside(hi)
-    while (hi - lo > 1) {
-        const int64_t mid = lo + (int64_t)(((uint64_t)hi - (uint64_t)lo) >> 1);
-        if (outside(okey_inv(mid, (Real)0))) lo = mid;
-        else hi = mid;
-    }
-    return okey_inv(hi, (Real)0);
-}
 
 template <typename Real>
 __global__ __launch_bounds__(256) void rd_statics_kernel(int nslots, const int32_t* __restrict__ nbr,
@@ -114,8 +55,7 @@ __global__ __launch_bounds__(256) void rd_statics_kernel(int nslots, const int32
     D[nc + gid] = harmonic(diffusivity(pw[ny], pg[ny], Dw, Dg), dc);
     D[2 * nc + gid] = harmonic(diffusivity(pw[nz], pg[nz], Dw, Dg), dc);
     tau[gid] = chf_threshold(pwc, pgc);
-    const Real ssum = pwc + pgc;
-    sig[gid] = -ssum;
+    sig[gid] = domain_threshold(pwc, pgc);
 }
 
 __device__ inline int P(int x, int y, int z) { return (z + 1) * 100 + (y + 1) * 10 + (x + 1); }
@@ -298,4 +238,24 @@ int rd_steps(Ctx& c, int64_t nsteps, double dt) {
     return GH_OK;
 }
 
+template <typename Real>
+static void thresholds_host(int64_t n, const Real* pw, const Real* pg, Real* tau, Real* sig) {
+    for (int64_t i = 0; i < n; ++i) {
+        tau[i] = chf_threshold(pw[i], pg[i]);
+        sig[i] = domain_threshold(pw[i], pg[i]);
+    }
+}
+
 }  // namespace gh
+
+extern "C" int gh_rd_thresholds(int real_kind, int64_t n, const void* p_w, const void* p_g, void* tau, void* sigma) {
+    if (n < 0 || (n > 0 && (!p_w || !p_g || !tau || !sigma)))
+        return gh::set_error(GH_EINVAL, "gh_rd_thresholds: bad argument");
+    if (real_kind == GH_REAL_F32)
+        gh::thresholds_host(n, (const float*)p_w, (const float*)p_g, (float*)tau, (float*)sigma);
+    else if (real_kind == GH_REAL_F64)
+        gh::thresholds_host(n, (const double*)p_w, (const double*)p_g, (double*)tau, (double*)sigma);
+    else
+        return gh::set_error(GH_EINVAL, "gh_rd_thresholds: bad real_kind");
+    return GH_OK;
+}

@@ -90,6 +90,12 @@ int gh_rd_prepare(gh_ctx* ctx, double Dw, double Dg, double rho);
 int gh_rd_rhs(gh_ctx* ctx);
 int gh_rd_update(gh_ctx* ctx, double dt);
 int gh_rd_steps(gh_ctx* ctx, int64_t nsteps, double dt, double* elapsed_ms);
+/* gh_rd_thresholds (host-side, no device): the per-cell predicates that
+ * gh_rd_prepare stores, for inspection and testing.  tau: the neighbour test
+ * "phi + p_w + p_g < 0.1" of ReactionDiffusionOperator.h:115-120,166-181 holds
+ * iff phi < tau; sigma: the domain test "p_w + p_g + phi > 0" (:103) holds iff
+ * phi > sigma.  n cells of real_kind values. */
+int gh_rd_thresholds(int real_kind, int64_t n, const void* p_w, const void* p_g, void* tau, void* sigma);
 
 /* ---- checkpoints (no reference counterpart: the reference keeps one copy of
  * the state) -- device-side copies of phi, used to restart solves from one

@@ -82,3 +82,37 @@ def test_gaussian_ic_gated_by_tissue():
     assert phi[8, 8, 8] == np.float32(0.1)
     assert phi[8, 8, 9] == np.float32(0.1 * np.exp(-0.5))
     assert np.all(phi[pw + pg == 0] == 0)
+
+
+@pytest.mark.parametrize("real", ["f32", "f64"])
+def test_rd_thresholds_equal_brute_force_predicates(real):
+    """tau/sigma reproduce the reference predicates exactly: checked at tau,
+    one ulp below it, and on random phi, with numpy's IEEE arithmetic."""
+    t = np.float32 if real == "f32" else np.float64
+    rng = np.random.default_rng(4)
+    m = 4000
+    pw = rng.uniform(-0.05, 1, m).astype(t)
+    pg = rng.uniform(-0.05, 1, m).astype(t)
+    pw[:500] = rng.uniform(0, 0.1, 500).astype(t)
+    pg[:500] = rng.uniform(0, 0.01, 500).astype(t)
+    pw[500:600] = 0
+    pg[500:600] = 0
+    tau, sig = gh.rd_thresholds(real, pw, pg)
+
+    def outside(phi):
+        return ((phi + pw).astype(t) + pg).astype(t) < t(0.1)
+
+    def domain(phi):
+        return ((pw + pg).astype(t) + phi).astype(t) > 0
+
+    fin = np.isfinite(tau)
+    assert fin[:600].all()
+    at = tau.copy()
+    below = np.nextafter(tau, t(-np.inf))
+    assert not outside(at)[fin].any()
+    assert outside(below)[fin].all()
+    for _ in range(20):
+        phi = rng.uniform(-0.2, 1.2, m).astype(t)
+        assert np.array_equal(outside(phi), phi < tau)
+        assert np.array_equal(domain(phi), phi > sig)
+    assert np.array_equal(domain(np.zeros(m, t)), np.zeros(m, t) > sig)
