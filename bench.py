@@ -64,7 +64,7 @@ def cpu_baseline(case, pw, pg, phi0, target_s=12.0):
                 return float(r.stdout.split("elapsed_s=")[1].split()[0])
 
             t2 = run(2)
-            k = int(max(2, min(400, target_s / max(t2 / 2, 1e-6))))
+            k = int(max(2, min(20000, target_s / max(t2 / 2, 1e-6))))
             el = run(k)
             return {"value": case.ncells * k / el, "unit": "cell-updates/s", "cores": threads,
                     "kind": "reference",
@@ -76,7 +76,7 @@ def cpu_baseline(case, pw, pg, phi0, target_s=12.0):
     t0 = time.perf_counter()
     o.rd_steps(phi0, pw, pg, case.Dw, case.Dg, case.rho, 1.0 / case.n, case.dt, 2)
     t2 = time.perf_counter() - t0
-    k = int(max(2, min(400, target_s / max(t2 / 2, 1e-6))))
+    k = int(max(2, min(20000, target_s / max(t2 / 2, 1e-6))))
     t0 = time.perf_counter()
     o.rd_steps(phi0, pw, pg, case.Dw, case.Dg, case.rho, 1.0 / case.n, case.dt, k)
     el = time.perf_counter() - t0
@@ -106,6 +106,8 @@ def main():
     ctx = wl.load_context(case)
     pw, pg, phi0 = ctx.get_field(FIELD_PW), ctx.get_field(FIELD_PG), ctx.get_field(FIELD_PHI)
     ctx.checkpoint_save(0)
+    tile = ctx.rd_autotune(case.dt, 100)
+    log("autotuned tile: %d blocks of 8^3 per workgroup" % tile)
     for i in range(a.warmup):
         ctx.checkpoint_load(0)
         ctx.rd_steps(case.nsteps, case.dt)
@@ -133,7 +135,7 @@ def main():
         traffic = json.load(open(TRAFFIC_FILE)).get("bytes_per_launch")
     roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-            "kernel": "rd_step_kernel<float,true>", "avg_launch_us": round(avg_launch_s * 1e6, 3),
+            "kernel": "rd_step_kernel<float,true,%d>" % tile, "avg_launch_us": round(avg_launch_s * 1e6, 3),
             "algorithmic_bytes_per_launch": BYTES_PER_CELL_UPDATE_F32 * case.ncells}
     cpu = None
     if not a.no_cpu:

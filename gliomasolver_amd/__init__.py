@@ -155,6 +155,14 @@ class Context:
         check(self.lib.gh_rd_steps(self.h, nsteps, dt, C.byref(ms) if timed else None))
         return ms.value if timed else None
 
+    def rd_set_tile(self, blocks_per_workgroup: int):
+        check(self.lib.gh_rd_set_tile(self.h, blocks_per_workgroup))
+
+    def rd_autotune(self, dt: float, trial_steps: int = 50) -> int:
+        best = C.c_int()
+        check(self.lib.gh_rd_autotune(self.h, dt, trial_steps, C.byref(best)))
+        return best.value
+
     def checkpoint_save(self, slot: int):
         check(self.lib.gh_checkpoint_save(self.h, slot))
 

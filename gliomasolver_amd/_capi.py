@@ -44,6 +44,8 @@ SIGNATURES = {
     "gh_rd_rhs": (_i, [_vp]),
     "gh_rd_update": (_i, [_vp, _d]),
     "gh_rd_steps": (_i, [_vp, _i64, _d, _pd]),
+    "gh_rd_set_tile": (_i, [_vp, _i]),
+    "gh_rd_autotune": (_i, [_vp, _d, _i, _pi]),
     "gh_rd_thresholds": (_i, [_i, _i64, _vp, _vp, _vp, _vp]),
     "gh_checkpoint_save": (_i, [_vp, _i]),
     "gh_checkpoint_load": (_i, [_vp, _i]),

@@ -36,14 +36,14 @@ struct Ctx {
     int cur = 0;        // phi[cur] holds the current state
     DevBuf dphidt;      // two-pass right-hand side
     DevBuf pw, pg, pcsf;
-    DevBuf statD;       // 3 * ncells: harmonic diffusivity of the +x,+y,+z face of each cell
-    DevBuf statTau;     // ncells: neighbour is outside (chf < 0.1) iff phi < tau
+    DevBuf statD;       // 4 * ncells: per cell {D of the +x,+y,+z face, tau}: outside iff phi < tau
     DevBuf statSig;     // ncells: cell is in the domain iff phi > sigma
     DevBuf nbr;         // 6 * nblocks int32
     std::vector<DevBuf> ckpt;  // device-side copies of phi (gh_checkpoint_*)
     bool prepared = false;
     double Dw = 0, Dg = 0, rho = 0;
     double dt_last = 0;  // dt of the running gh_rd_steps call
+    int rd_bpw = 2;      // blocks per workgroup of the step kernel (1, 2 or 4)
 
     size_t ncells() const { return (size_t)topo.nblocks() * BCELLS; }
     int alloc_fields();
@@ -57,5 +57,6 @@ int rd_prepare(Ctx& c, double Dw, double Dg, double rho);
 int rd_rhs(Ctx& c);
 int rd_update(Ctx& c, double dt);
 int rd_steps(Ctx& c, int64_t nsteps, double dt);
+int rd_autotune(Ctx& c, double dt, int trial_steps, int* best);
 
 }  // namespace gh

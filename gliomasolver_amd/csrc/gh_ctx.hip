@@ -40,7 +40,7 @@ int Ctx::alloc_fields() {
     const size_t n = ncells() * real_size;
     int rc;
     if ((rc = phi[0].ensure(n)) || (rc = phi[1].ensure(n)) || (rc = dphidt.ensure(n)) || (rc = pw.ensure(n)) ||
-        (rc = pg.ensure(n)) || (rc = pcsf.ensure(n)) || (rc = statD.ensure(3 * n)) || (rc = statTau.ensure(n)) ||
+        (rc = pg.ensure(n)) || (rc = pcsf.ensure(n)) || (rc = statD.ensure(4 * n)) ||
         (rc = statSig.ensure(n)))
         return rc;
     GH_HIP_CHECK(hipMemsetAsync(phi[0].p, 0, n, stream));
@@ -240,7 +240,7 @@ int gh_ctx_destroy(gh_ctx* ctx) {
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     DevBuf* bufs[] = {&c->phi[0], &c->phi[1], &c->dphidt, &c->pw, &c->pg, &c->pcsf,
-                      &c->statD, &c->statTau, &c->statSig, &c->nbr};
+                      &c->statD, &c->statSig, &c->nbr};
     for (DevBuf* b : bufs) b->release();
     for (DevBuf& b : c->ckpt) b.release();
     if (c->ev0) (void)hipEventDestroy(c->ev0);
@@ -384,6 +384,24 @@ int gh_rd_steps(gh_ctx* ctx, int64_t nsteps, double dt, double* elapsed_ms) {
         *elapsed_ms = ms;
     }
     return GH_OK;
+}
+
+int gh_rd_set_tile(gh_ctx* ctx, int blocks_per_workgroup) {
+    Ctx* c = as_ctx(ctx);
+    if (!c) return set_error(GH_EINVAL, "null context");
+    if (blocks_per_workgroup != 1 && blocks_per_workgroup != 2 && blocks_per_workgroup != 4)
+        return set_error(GH_EINVAL, "gh_rd_set_tile: blocks_per_workgroup must be 1, 2 or 4");
+    c->rd_bpw = blocks_per_workgroup;
+    return GH_OK;
+}
+
+int gh_rd_autotune(gh_ctx* ctx, double dt, int trial_steps, int* best) {
+    Ctx* c = as_ctx(ctx);
+    if (!c) return set_error(GH_EINVAL, "null context");
+    if (!c->prepared) return set_error(GH_ESTATE, "gh_rd_autotune: call gh_rd_prepare first");
+    if (trial_steps < 1 || trial_steps > 100000) return set_error(GH_EINVAL, "gh_rd_autotune: bad trial_steps");
+    GH_HIP_CHECK(hipSetDevice(c->device));
+    return rd_autotune(*c, dt, trial_steps, best);
 }
 
 int gh_checkpoint_save(gh_ctx* ctx, int slot) {

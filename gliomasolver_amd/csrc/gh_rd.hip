@@ -34,11 +34,19 @@
 namespace gh {
 namespace {
 
+// Per-cell constants of one solve, one 4-Real record per cell so a lane
+// fetches them with one vector load: harmonic diffusivity of the cell's +x,
+// +y, +z faces and the outside threshold tau.
+template <typename Real>
+struct alignas(4 * sizeof(Real)) Stat4 {
+    Real dx, dy, dz, tau;
+};
+
 template <typename Real>
 __global__ __launch_bounds__(256) void rd_statics_kernel(int nslots, const int32_t* __restrict__ nbr,
                                                          const Real* __restrict__ pw, const Real* __restrict__ pg,
-                                                         Real Dw, Real Dg, Real* __restrict__ D,
-                                                         Real* __restrict__ tau, Real* __restrict__ sig) {
+                                                         Real Dw, Real Dg, Stat4<Real>* __restrict__ st,
+                                                         Real* __restrict__ sig) {
     const size_t nc = (size_t)nslots * BCELLS;
     const size_t gid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (gid >= nc) return;
@@ -48,83 +56,35 @@ __global__ __launch_bounds__(256) void rd_statics_kernel(int nslots, const int32
     const Real pwc = pw[gid], pgc = pg[gid];
     const Real dc = diffusivity(pwc, pgc, Dw, Dg);
     // +x, +y, +z neighbour cells (same slot or the same-level neighbour slot)
-    size_t nx = (x < 7) ? gid + 1 : (size_t)nbr[6 * s + FXP] * BCELLS + (l - 7);
-    size_t ny = (y < 7) ? gid + 8 : (size_t)nbr[6 * s + FYP] * BCELLS + (l - 56);
-    size_t nz = (z < 7) ? gid + 64 : (size_t)nbr[6 * s + FZP] * BCELLS + (l - 448);
-    D[gid] = harmonic(diffusivity(pw[nx], pg[nx], Dw, Dg), dc);
-    D[nc + gid] = harmonic(diffusivity(pw[ny], pg[ny], Dw, Dg), dc);
-    D[2 * nc + gid] = harmonic(diffusivity(pw[nz], pg[nz], Dw, Dg), dc);
-    tau[gid] = chf_threshold(pwc, pgc);
+    const size_t nx = (x < 7) ? gid + 1 : (size_t)nbr[6 * s + FXP] * BCELLS + (l - 7);
+    const size_t ny = (y < 7) ? gid + 8 : (size_t)nbr[6 * s + FYP] * BCELLS + (l - 56);
+    const size_t nz = (z < 7) ? gid + 64 : (size_t)nbr[6 * s + FZP] * BCELLS + (l - 448);
+    Stat4<Real> r;
+    r.dx = harmonic(diffusivity(pw[nx], pg[nx], Dw, Dg), dc);
+    r.dy = harmonic(diffusivity(pw[ny], pg[ny], Dw, Dg), dc);
+    r.dz = harmonic(diffusivity(pw[nz], pg[nz], Dw, Dg), dc);
+    r.tau = chf_threshold(pwc, pgc);
+    st[gid] = r;
     sig[gid] = domain_threshold(pwc, pgc);
 }
 
 __device__ inline int P(int x, int y, int z) { return (z + 1) * 100 + (y + 1) * 10 + (x + 1); }
 
-// One workgroup of 512 lanes per block; lane = cell (x fastest).
-// Slot mapping is XCD-aware: consecutive workgroups land on different XCDs,
-// so workgroup w serves slot (w % 8) * per_xcd + w / 8 and each XCD sweeps a
-// contiguous range of blocks whose halos stay in its own L2.
-template <typename Real, bool FUSED>
-__global__ __launch_bounds__(512) void rd_step_kernel(int nslots, int per_xcd, const int32_t* __restrict__ nbr,
-                                                      const Real* __restrict__ phi, Real* __restrict__ out,
-                                                      const Real* __restrict__ D, const Real* __restrict__ tau,
-                                                      const Real* __restrict__ sig, Real rho, double ih2,
-                                                      double dt) {
-    const int s = (int)(blockIdx.x % 8) * per_xcd + (int)(blockIdx.x / 8);
-    if (s >= nslots) return;
-    __shared__ Real s_phi[1000];
-    __shared__ unsigned char s_out[1000];
-    __shared__ Real s_Dx[576], s_Dy[576], s_Dz[576];
-
-    const size_t nc = (size_t)nslots * BCELLS;
-    const int t = threadIdx.x;
-    const int x = t & 7, y = (t >> 3) & 7, z = t >> 6;
-    const size_t g = (size_t)s * BCELLS + t;
-
-    const Real pc = phi[g];
-    const Real tc = tau[g];
-    const Real sc = sig[g];
-    const Real dxp = D[g], dyp = D[nc + g], dzp = D[2 * nc + g];
-    s_phi[P(x, y, z)] = pc;
-    s_out[P(x, y, z)] = pc < tc;
-    s_Dx[z * 72 + y * 9 + x + 1] = dxp;
-    s_Dy[z * 72 + (y + 1) * 8 + x] = dyp;
-    s_Dz[(z + 1) * 64 + y * 8 + x] = dzp;
-
-    if (t < 384) {  // six face halos of 64 cells, one lane each
-        const int f = t >> 6, u = t & 7, v = (t >> 3) & 7;
-        const size_t nb = (size_t)nbr[6 * s + f] * BCELLS;
-        int src, dst;
-        switch (f) {
-            case FXM: src = 7 + 8 * u + 64 * v; dst = P(-1, u, v); break;
-            case FXP: src = 0 + 8 * u + 64 * v; dst = P(8, u, v); break;
-            case FYM: src = u + 8 * 7 + 64 * v; dst = P(u, -1, v); break;
-            case FYP: src = u + 8 * 0 + 64 * v; dst = P(u, 8, v); break;
-            case FZM: src = u + 8 * v + 64 * 7; dst = P(u, v, -1); break;
-            default:  src = u + 8 * v + 64 * 0; dst = P(u, v, 8); break;
-        }
-        const Real ph = phi[nb + src];
-        s_phi[dst] = ph;
-        s_out[dst] = ph < tau[nb + src];
-        if (f == FXM) s_Dx[v * 72 + u * 9] = D[nb + src];
-        else if (f == FYM) s_Dy[v * 72 + u] = D[nc + nb + src];
-        else if (f == FZM) s_Dz[v * 8 + u] = D[2 * nc + nb + src];
-    }
-    __syncthreads();
-
-    Real d0 = s_Dx[z * 72 + y * 9 + x], d1 = dxp;
-    Real d2 = s_Dy[z * 72 + y * 8 + x], d3 = dyp;
-    Real d4 = s_Dz[z * 64 + y * 8 + x], d5 = dzp;
-    const Real p0 = s_phi[P(x - 1, y, z)], p1 = s_phi[P(x + 1, y, z)];
-    const Real p2 = s_phi[P(x, y - 1, z)], p3 = s_phi[P(x, y + 1, z)];
-    const Real p4 = s_phi[P(x, y, z - 1)], p5 = s_phi[P(x, y, z + 1)];
+// dphidt of one cell (ReactionDiffusionOperator.h:103-138) from its face
+// coefficients d0..d5 (-x,+x,-y,+y,-z,+z, before the no-flux rule), the
+// neighbour values p0..p5 with their outside flags o0..o5, its own value pc
+// and domain threshold sc.
+template <typename Real>
+__device__ inline Real rd_cell_rhs(Real d0, Real d1, Real d2, Real d3, Real d4, Real d5, Real p0, Real p1, Real p2,
+                                   Real p3, Real p4, Real p5, bool o0, bool o1, bool o2, bool o3, bool o4, bool o5,
+                                   Real pc, Real sc, Real rho, double ih2) {
     // _applyNoFluxBC: an outside neighbour doubles the opposite face (exact)
-    if (s_out[P(x - 1, y, z)]) d1 = (Real)((double)d1 * 2.0);
-    if (s_out[P(x + 1, y, z)]) d0 = (Real)((double)d0 * 2.0);
-    if (s_out[P(x, y - 1, z)]) d3 = (Real)((double)d3 * 2.0);
-    if (s_out[P(x, y + 1, z)]) d2 = (Real)((double)d2 * 2.0);
-    if (s_out[P(x, y, z - 1)]) d5 = (Real)((double)d5 * 2.0);
-    if (s_out[P(x, y, z + 1)]) d4 = (Real)((double)d4 * 2.0);
+    if (o0) d1 = (Real)((double)d1 * 2.0);
+    if (o1) d0 = (Real)((double)d0 * 2.0);
+    if (o2) d3 = (Real)((double)d3 * 2.0);
+    if (o3) d2 = (Real)((double)d2 * 2.0);
+    if (o4) d5 = (Real)((double)d5 * 2.0);
+    if (o5) d4 = (Real)((double)d4 * 2.0);
     // diffusionFluxIn = ih2 * (Real sum of df_i*phi_i, left to right)
     Real sin_ = d0 * p0;
     sin_ = sin_ + d1 * p1;
@@ -144,16 +104,96 @@ __global__ __launch_bounds__(512) void rd_step_kernel(int nslots, int per_xcd, c
     // reactionFlux = (Real)(rho*phi) * (1. - phi)
     const Real rp = rho * pc;
     const double react = (double)rp * (1. - (double)pc);
-    Real dphidt = (Real)((fout + fin) + react);
-    if (!(pc > sc)) dphidt = (Real)0.;  // outside the brain domain (:103, :137-138)
-    if (FUSED) {
-        // UpdateTumor: phi += dt*dphidt (double), then max(0,.), min(1,.)
-        Real p = (Real)((double)pc + dt * (double)dphidt);
-        p = ((Real)0. < p) ? p : (Real)0.;
-        p = (p < (Real)1.) ? p : (Real)1.;
-        out[g] = p;
-    } else {
-        out[g] = dphidt;
+    const Real dphidt = (Real)((fout + fin) + react);
+    return (pc > sc) ? dphidt : (Real)0.;  // outside the brain domain (:103, :137-138)
+}
+
+// UpdateTumor (ReactionDiffusionOperator.h:219-221): phi += dt*dphidt in
+// double, then std::max((Real)0, .), std::min((Real)1, .)
+template <typename Real>
+__device__ inline Real rd_update(Real pc, Real dphidt, double dt) {
+    Real p = (Real)((double)pc + dt * (double)dphidt);
+    p = ((Real)0. < p) ? p : (Real)0.;
+    return (p < (Real)1.) ? p : (Real)1.;
+}
+
+// BPW blocks per workgroup of 512 lanes; lane = cell (x fastest) of each
+// block.  All loads of a lane (BPW cells, one vector load of constants each)
+// are issued before the first wait.  Group mapping is XCD-aware: consecutive
+// workgroups land on different XCDs, so workgroup w serves group
+// (w % 8) * per_xcd + w / 8 and each XCD sweeps a contiguous range of blocks
+// whose halos stay in its own L2.
+template <typename Real, bool FUSED, int BPW>
+__global__ __launch_bounds__(512) void rd_step_kernel(int nslots, int per_xcd, const int32_t* __restrict__ nbr,
+                                                      const Real* __restrict__ phi, Real* __restrict__ out,
+                                                      const Stat4<Real>* __restrict__ st,
+                                                      const Real* __restrict__ sig, Real rho, double ih2,
+                                                      double dt) {
+    const int grp = (int)(blockIdx.x % 8) * per_xcd + (int)(blockIdx.x / 8);
+    if (grp * BPW >= nslots) return;
+    __shared__ Real s_phi[BPW][1000];
+    __shared__ unsigned char s_out[BPW][1000];
+    __shared__ Real s_Dx[BPW][576], s_Dy[BPW][576], s_Dz[BPW][576];
+
+    const int t = threadIdx.x;
+    const int x = t & 7, y = (t >> 3) & 7, z = t >> 6;
+    Real pc[BPW], sc[BPW];
+    Stat4<Real> sv[BPW];
+#pragma unroll
+    for (int j = 0; j < BPW; ++j) {
+        const int s = grp * BPW + j;
+        if (s >= nslots) break;
+        const size_t g = (size_t)s * BCELLS + t;
+        pc[j] = phi[g];
+        sv[j] = st[g];
+        sc[j] = sig[g];
+    }
+    if (t < 384) {  // six face halos of 64 cells, one lane each
+        const int f = t >> 6, u = t & 7, v = (t >> 3) & 7;
+        int src, dst;
+        switch (f) {
+            case FXM: src = 7 + 8 * u + 64 * v; dst = P(-1, u, v); break;
+            case FXP: src = 0 + 8 * u + 64 * v; dst = P(8, u, v); break;
+            case FYM: src = u + 8 * 7 + 64 * v; dst = P(u, -1, v); break;
+            case FYP: src = u + 8 * 0 + 64 * v; dst = P(u, 8, v); break;
+            case FZM: src = u + 8 * v + 64 * 7; dst = P(u, v, -1); break;
+            default:  src = u + 8 * v + 64 * 0; dst = P(u, v, 8); break;
+        }
+#pragma unroll
+        for (int j = 0; j < BPW; ++j) {
+            const int s = grp * BPW + j;
+            if (s >= nslots) break;
+            const size_t nb = (size_t)nbr[6 * s + f] * BCELLS + src;
+            const Real ph = phi[nb];
+            const Stat4<Real> hs = st[nb];
+            s_phi[j][dst] = ph;
+            s_out[j][dst] = ph < hs.tau;
+            if (f == FXM) s_Dx[j][v * 72 + u * 9] = hs.dx;
+            else if (f == FYM) s_Dy[j][v * 72 + u] = hs.dy;
+            else if (f == FZM) s_Dz[j][v * 8 + u] = hs.dz;
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < BPW; ++j) {
+        if (grp * BPW + j >= nslots) break;
+        s_phi[j][P(x, y, z)] = pc[j];
+        s_out[j][P(x, y, z)] = pc[j] < sv[j].tau;
+        s_Dx[j][z * 72 + y * 9 + x + 1] = sv[j].dx;
+        s_Dy[j][z * 72 + (y + 1) * 8 + x] = sv[j].dy;
+        s_Dz[j][(z + 1) * 64 + y * 8 + x] = sv[j].dz;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < BPW; ++j) {
+        const int s = grp * BPW + j;
+        if (s >= nslots) break;
+        const Real d = rd_cell_rhs<Real>(
+            s_Dx[j][z * 72 + y * 9 + x], sv[j].dx, s_Dy[j][z * 72 + y * 8 + x], sv[j].dy,
+            s_Dz[j][z * 64 + y * 8 + x], sv[j].dz, s_phi[j][P(x - 1, y, z)], s_phi[j][P(x + 1, y, z)],
+            s_phi[j][P(x, y - 1, z)], s_phi[j][P(x, y + 1, z)], s_phi[j][P(x, y, z - 1)], s_phi[j][P(x, y, z + 1)],
+            s_out[j][P(x - 1, y, z)], s_out[j][P(x + 1, y, z)], s_out[j][P(x, y - 1, z)],
+            s_out[j][P(x, y + 1, z)], s_out[j][P(x, y, z - 1)], s_out[j][P(x, y, z + 1)], pc[j], sc[j], rho, ih2);
+        out[(size_t)s * BCELLS + t] = FUSED ? rd_update(pc[j], d, dt) : d;
     }
 }
 
@@ -162,10 +202,7 @@ __global__ __launch_bounds__(256) void rd_update_kernel(size_t nc, Real* __restr
                                                         const Real* __restrict__ dphidt, double dt) {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nc) return;
-    Real p = (Real)((double)phi[i] + dt * (double)dphidt[i]);
-    p = ((Real)0. < p) ? p : (Real)0.;
-    p = (p < (Real)1.) ? p : (Real)1.;
-    phi[i] = p;
+    phi[i] = rd_update(phi[i], dphidt[i], dt);
 }
 
 inline double block_h(const Ctx& c) {
@@ -179,8 +216,8 @@ int prepare_t(Ctx& c, double Dw, double Dg, double rho) {
     const size_t nc = c.ncells();
     const unsigned grid = (unsigned)((nc + 255) / 256);
     hipLaunchKernelGGL(rd_statics_kernel<Real>, dim3(grid), dim3(256), 0, c.stream, ns, c.nbr.as<int32_t>(),
-                       c.pw.as<Real>(), c.pg.as<Real>(), (Real)Dw, (Real)Dg, c.statD.as<Real>(),
-                       c.statTau.as<Real>(), c.statSig.as<Real>());
+                       c.pw.as<Real>(), c.pg.as<Real>(), (Real)Dw, (Real)Dg, c.statD.as<Stat4<Real>>(),
+                       c.statSig.as<Real>());
     GH_HIP_CHECK(hipGetLastError());
     c.Dw = Dw;
     c.Dg = Dg;
@@ -189,15 +226,25 @@ int prepare_t(Ctx& c, double Dw, double Dg, double rho) {
     return GH_OK;
 }
 
-template <typename Real, bool FUSED>
-int step_t(Ctx& c, Real* out) {
+template <typename Real, bool FUSED, int BPW>
+void launch_step(Ctx& c, Real* out) {
     const int ns = c.topo.nblocks();
-    const int per_xcd = (ns + 7) / 8;
+    const int groups = (ns + BPW - 1) / BPW;
+    const int per_xcd = (groups + 7) / 8;
     const double h = block_h(c);
     const double ih2 = 1. / (h * h);
-    hipLaunchKernelGGL((rd_step_kernel<Real, FUSED>), dim3(8 * per_xcd), dim3(512), 0, c.stream, ns, per_xcd,
-                       c.nbr.as<int32_t>(), c.phi[c.cur].as<Real>(), out, c.statD.as<Real>(), c.statTau.as<Real>(),
+    hipLaunchKernelGGL((rd_step_kernel<Real, FUSED, BPW>), dim3(8 * per_xcd), dim3(512), 0, c.stream, ns,
+                       per_xcd, c.nbr.as<int32_t>(), c.phi[c.cur].as<Real>(), out, c.statD.as<Stat4<Real>>(),
                        c.statSig.as<Real>(), (Real)c.rho, ih2, c.dt_last);
+}
+
+template <typename Real, bool FUSED>
+int step_t(Ctx& c, Real* out) {
+    switch (c.rd_bpw) {
+        case 1: launch_step<Real, FUSED, 1>(c, out); break;
+        case 4: launch_step<Real, FUSED, 4>(c, out); break;
+        default: launch_step<Real, FUSED, 2>(c, out); break;
+    }
     GH_HIP_CHECK(hipGetLastError());
     return GH_OK;
 }
@@ -224,6 +271,36 @@ int rd_update(Ctx& c, double dt) {
         hipLaunchKernelGGL(rd_update_kernel<double>, dim3(grid), dim3(256), 0, c.stream, nc,
                            c.phi[c.cur].as<double>(), c.dphidt.as<double>(), dt);
     GH_HIP_CHECK(hipGetLastError());
+    return GH_OK;
+}
+
+// Times trial steps of each blocks-per-workgroup tile on the current state
+// (saved in the dphidt scratch and restored after every trial) and keeps the
+// fastest.
+int rd_autotune(Ctx& c, double dt, int trial_steps, int* best_out) {
+    static const int cand[3] = {1, 2, 4};
+    const size_t nb = c.ncells() * c.real_size;
+    GH_HIP_CHECK(hipMemcpyAsync(c.dphidt.p, c.phi[c.cur].p, nb, hipMemcpyDeviceToDevice, c.stream));
+    const int cur0 = c.cur;
+    float best_ms = 1e30f;
+    int best = c.rd_bpw;
+    for (int k = 0; k < 3; ++k) {
+        c.rd_bpw = cand[k];
+        int rc = rd_steps(c, 2, dt);  // warm
+        if (rc) return rc;
+        GH_HIP_CHECK(hipEventRecord(c.ev0, c.stream));
+        if ((rc = rd_steps(c, trial_steps, dt))) return rc;
+        GH_HIP_CHECK(hipEventRecord(c.ev1, c.stream));
+        GH_HIP_CHECK(hipEventSynchronize(c.ev1));
+        float ms = 0;
+        GH_HIP_CHECK(hipEventElapsedTime(&ms, c.ev0, c.ev1));
+        if (ms < best_ms) { best_ms = ms; best = cand[k]; }
+        c.cur = cur0;
+        GH_HIP_CHECK(hipMemcpyAsync(c.phi[c.cur].p, c.dphidt.p, nb, hipMemcpyDeviceToDevice, c.stream));
+    }
+    c.rd_bpw = best;
+    GH_HIP_CHECK(hipStreamSynchronize(c.stream));
+    if (best_out) *best_out = best;
     return GH_OK;
 }
 

@@ -90,6 +90,12 @@ int gh_rd_prepare(gh_ctx* ctx, double Dw, double Dg, double rho);
 int gh_rd_rhs(gh_ctx* ctx);
 int gh_rd_update(gh_ctx* ctx, double dt);
 int gh_rd_steps(gh_ctx* ctx, int64_t nsteps, double dt, double* elapsed_ms);
+/* gh_rd_set_tile / gh_rd_autotune (no reference counterpart): the step
+ * kernel's tile is 1, 2 or 4 blocks of 8^3 per workgroup; the autotuner times
+ * trial_steps of each on the current state (restored afterwards) and keeps the
+ * fastest, returned in *best.  Results do not depend on the tile. */
+int gh_rd_set_tile(gh_ctx* ctx, int blocks_per_workgroup);
+int gh_rd_autotune(gh_ctx* ctx, double dt, int trial_steps, int* best);
 /* gh_rd_thresholds (host-side, no device): the per-cell predicates that
  * gh_rd_prepare stores, for inspection and testing.  tau: the neighbour test
  * "phi + p_w + p_g < 0.1" of ReactionDiffusionOperator.h:115-120,166-181 holds

@@ -140,3 +140,40 @@ def test_gpu_checkpoint_restart_is_exact():
     b = ctx.get_field(FIELD_PHI)
     ctx.close()
     assert bits_equal(a, b) and bits_equal(a, c["phi_ref"])
+
+
+@pytest.mark.parametrize("bpw", [1, 2, 4])
+def test_gpu_tiles_are_bit_identical(bpw):
+    c = load_case("rd_f32_random16")
+    say("tile", bpw)
+    n = c["phi0"].shape[0]
+    ctx = gh.Context(0, "f32")
+    ctx.grid_init_uniform(n // 8)
+    ctx.set_field(FIELD_PW, c["p_w"])
+    ctx.set_field(FIELD_PG, c["p_g"])
+    ctx.set_field(FIELD_PHI, c["phi0"])
+    ctx.rd_prepare(float(c["Dw"]), float(c["Dg"]), float(c["rho"]))
+    ctx.rd_set_tile(bpw)
+    ctx.rd_steps(int(c["nsteps"]), float(c["dt"]))
+    out = ctx.get_field(FIELD_PHI)
+    ctx.close()
+    assert bits_equal(out, c["phi_ref"])
+
+
+def test_gpu_autotune_restores_state():
+    c = load_case("rd_f64_phantom32")
+    say("autotune f64 phantom32")
+    n = c["phi0"].shape[0]
+    ctx = gh.Context(0, "f64")
+    ctx.grid_init_uniform(n // 8)
+    ctx.set_field(FIELD_PW, c["p_w"])
+    ctx.set_field(FIELD_PG, c["p_g"])
+    ctx.set_field(FIELD_PHI, c["phi0"])
+    ctx.rd_prepare(float(c["Dw"]), float(c["Dg"]), float(c["rho"]))
+    best = ctx.rd_autotune(float(c["dt"]), 5)
+    assert best in (1, 2, 4)
+    assert bits_equal(ctx.get_field(FIELD_PHI), c["phi0"])
+    ctx.rd_steps(int(c["nsteps"]), float(c["dt"]))
+    out = ctx.get_field(FIELD_PHI)
+    ctx.close()
+    assert bits_equal(out, c["phi_ref"])
