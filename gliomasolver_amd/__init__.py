@@ -163,6 +163,35 @@ class Context:
         check(self.lib.gh_rd_autotune(self.h, dt, trial_steps, C.byref(best)))
         return best.value
 
+    # ---- likelihood (HGG_Likelihood) ----
+    LIK_DEFAULT = (0.000361, 0.85, 0.7, 0.25, 2.0)  # LikelihoodInput.txt
+
+    def lik_set_data(self, t1c, flair, pet, roi=None):
+        arrs = [np.ascontiguousarray(a, np.float64) for a in (t1c, flair, pet)]
+        r = None if roi is None else np.ascontiguousarray(np.asarray(roi, np.float64).T)
+        check(self.lib.gh_lik_set_data(self.h, arrs[0].shape[0], *[_ptr(a) for a in arrs],
+                                       None if r is None else _ptr(r), 0 if roi is None else len(roi)))
+
+    def likelihood(self, params=LIK_DEFAULT, stepPET=1, stepTi=1, use_roi=False):
+        """Returns (-logL, (L_PET, L_T1, L_T2), N_PET) for the current phi."""
+        p = (C.c_double * 5)(*params)
+        out = (C.c_double * 4)()
+        npet = C.c_int64()
+        check(self.lib.gh_likelihood(self.h, p, stepPET, stepTi, int(use_roi), out, C.byref(npet)))
+        return out[0], (out[1], out[2], out[3]), npet.value
+
+    def likelihood_dense(self, model, t1c, flair, pet, roi=None, params=LIK_DEFAULT, stepPET=1, stepTi=1):
+        arrs = [np.ascontiguousarray(a, np.float64) for a in (model, t1c, flair, pet)]
+        nz, ny, nx = arrs[0].shape
+        r = None if roi is None else np.ascontiguousarray(np.asarray(roi, np.float64).T)
+        p = (C.c_double * 5)(*params)
+        out = (C.c_double * 4)()
+        npet = C.c_int64()
+        check(self.lib.gh_likelihood_dense(self.h, nx, ny, nz, *[_ptr(a) for a in arrs],
+                                           None if r is None else _ptr(r), 0 if roi is None else len(roi), p,
+                                           stepPET, stepTi, out, C.byref(npet)))
+        return out[0], (out[1], out[2], out[3]), npet.value
+
     def checkpoint_save(self, slot: int):
         check(self.lib.gh_checkpoint_save(self.h, slot))
 

@@ -47,6 +47,9 @@ SIGNATURES = {
     "gh_rd_set_tile": (_i, [_vp, _i]),
     "gh_rd_autotune": (_i, [_vp, _d, _i, _pi]),
     "gh_rd_thresholds": (_i, [_i, _i64, _vp, _vp, _vp, _vp]),
+    "gh_lik_set_data": (_i, [_vp, _i, _vp, _vp, _vp, _vp, _i]),
+    "gh_likelihood": (_i, [_vp, _pd, _i, _i, _i, _pd, _pi64]),
+    "gh_likelihood_dense": (_i, [_vp, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _i, _pd, _i, _i, _pd, _pi64]),
     "gh_checkpoint_save": (_i, [_vp, _i]),
     "gh_checkpoint_load": (_i, [_vp, _i]),
     "gh_phantom": (_i, [_i, _u64, _vp, _vp, _vp, _pi]),

@@ -40,6 +40,10 @@ struct Ctx {
     DevBuf statSig;     // ncells: cell is in the domain iff phi > sigma
     DevBuf nbr;         // 6 * nblocks int32
     std::vector<DevBuf> ckpt;  // device-side copies of phi (gh_checkpoint_*)
+    // likelihood data (block-major): y == 1 flags of T1c/FLAIR, PET values, ROI {cell, ix, iy, iz}
+    DevBuf obsT1, obsFl, obsPet, obsRoi, likPart;
+    int nroi = 0;
+    bool lik_ready = false;
     bool prepared = false;
     double Dw = 0, Dg = 0, rho = 0;
     double dt_last = 0;  // dt of the running gh_rd_steps call

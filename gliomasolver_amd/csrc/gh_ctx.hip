@@ -240,7 +240,8 @@ int gh_ctx_destroy(gh_ctx* ctx) {
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     DevBuf* bufs[] = {&c->phi[0], &c->phi[1], &c->dphidt, &c->pw, &c->pg, &c->pcsf,
-                      &c->statD, &c->statSig, &c->nbr};
+                      &c->statD, &c->statSig, &c->nbr,
+                      &c->obsT1, &c->obsFl, &c->obsPet, &c->obsRoi, &c->likPart};
     for (DevBuf* b : bufs) b->release();
     for (DevBuf& b : c->ckpt) b.release();
     if (c->ev0) (void)hipEventDestroy(c->ev0);

@@ -1,0 +1,306 @@
+// gh_lik.hip -- TMCMC likelihood of a tumour field against imaging data.
+//
+// Reference: tools/pi4u_lite/Inference/Likelihood/source/HGG_Likelihood.cpp
+//   PET term        :40-75   N = #{PET > 0 on the stepPET lattice},
+//                            L = -N/2 log(2 pi s2) - 1/(2 s2) sum (u - b PET)^2
+//   Bernoulli terms :77-156  alpha = 1/2 + 1/2 sgn(u-uc) (1 - exp(-w2/slope)),
+//                            w2 = 1 if u > uc else (u-uc)^2; log alpha if y == 1
+//                            else log(1-alpha); all voxels, or ROI points on
+//                            the stepMRI lattice
+//   run             :176-192 -logL = -(L_PET + L_T1 + L_T2)
+// The reference accumulates in long double; here every per-voxel term is
+// double and the sums are FP64 with a fixed-order two-stage reduction, so a
+// result is reproducible run to run.
+#include "gh_ctx.h"
+
+namespace gh {
+namespace {
+
+constexpr int LIK_THREADS = 256;
+constexpr int LIK_GRID = 1024;  // fixed: the reduction order never changes
+
+struct LikParams {
+    double s2, b, uc1, uc2, is2;
+    int stepPET, stepTi;
+};
+
+__device__ inline double log_bernoulli(double u, bool y, double uc, double is2) {
+    const double diff = u - uc;
+    const double omega2 = (diff > 0.) ? 1. : diff * diff;
+    const int sg = (diff < -0.0) ? -1 : (diff > 0.0);
+    const double alpha = 0.5 + 0.5 * (double)sg * (1. - exp(-omega2 * is2));
+    return y ? log(alpha) : log(1. - alpha);
+}
+
+// cell index -> lattice coordinates for dense (x-fastest) or block-major
+// uniform layouts
+struct Layout {
+    int nx, ny, nz;  // dense when bpd == 0
+    int bpd;
+    __device__ inline void coords(size_t c, int& ix, int& iy, int& iz) const {
+        if (bpd == 0) {
+            ix = (int)(c % nx);
+            iy = (int)((c / nx) % ny);
+            iz = (int)(c / ((size_t)nx * ny));
+        } else {
+            const int s = (int)(c / BCELLS), l = (int)(c % BCELLS);
+            ix = (s % bpd) * BS + (l & 7);
+            iy = ((s / bpd) % bpd) * BS + ((l >> 3) & 7);
+            iz = (s / (bpd * bpd)) * BS + (l >> 6);
+        }
+    }
+};
+
+__device__ inline double wave_sum(double v) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// stage 1: per-workgroup partials {sse, npet, L_T1, L_T2}
+template <typename T>
+__global__ __launch_bounds__(LIK_THREADS) void lik_partials_kernel(size_t ncells, Layout lay, const T* __restrict__ u,
+                                                                   const unsigned char* __restrict__ t1,
+                                                                   const unsigned char* __restrict__ fl,
+                                                                   const double* __restrict__ pet,
+                                                                   const int4* __restrict__ roi, int nroi,
+                                                                   LikParams p, double* __restrict__ partials) {
+    double sse = 0, npet = 0, l1 = 0, l2 = 0;
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x; c < ncells; c += stride) {
+        int ix, iy, iz;
+        lay.coords(c, ix, iy, iz);
+        const double uc = (double)u[c];
+        if ((ix % p.stepPET == 0) && (iy % p.stepPET == 0) && (iz % p.stepPET == 0)) {
+            const double pv = pet[c];
+            if (pv > 0.) {
+                const double r = uc - p.b * pv;
+                sse += r * r;
+                npet += 1;
+            }
+        }
+        if (roi == nullptr) {
+            l1 += log_bernoulli(uc, t1[c] != 0, p.uc1, p.is2);
+            l2 += log_bernoulli(uc, fl[c] != 0, p.uc2, p.is2);
+        }
+    }
+    if (roi != nullptr) {
+        for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < (size_t)nroi; i += stride) {
+            const int4 q = roi[i];  // {cell, ix, iy, iz}
+            if ((q.y % p.stepTi == 0) && (q.z % p.stepTi == 0) && (q.w % p.stepTi == 0)) {
+                const double uc = (double)u[q.x];
+                l1 += log_bernoulli(uc, t1[q.x] != 0, p.uc1, p.is2);
+                l2 += log_bernoulli(uc, fl[q.x] != 0, p.uc2, p.is2);
+            }
+        }
+    }
+    __shared__ double red[4][LIK_THREADS / 64];
+    const int w = threadIdx.x / 64, lane = threadIdx.x % 64;
+    sse = wave_sum(sse);
+    npet = wave_sum(npet);
+    l1 = wave_sum(l1);
+    l2 = wave_sum(l2);
+    if (lane == 0) {
+        red[0][w] = sse;
+        red[1][w] = npet;
+        red[2][w] = l1;
+        red[3][w] = l2;
+    }
+    __syncthreads();
+    if (threadIdx.x < 4) {
+        double s = 0;
+        for (int k = 0; k < LIK_THREADS / 64; ++k) s += red[threadIdx.x][k];
+        partials[4 * blockIdx.x + threadIdx.x] = s;
+    }
+}
+
+// stage 2: one workgroup sums the partials in a fixed order
+__global__ __launch_bounds__(256) void lik_final_kernel(int nparts, const double* __restrict__ partials,
+                                                        double* __restrict__ out) {
+    __shared__ double red[4][256];
+    for (int q = 0; q < 4; ++q) {
+        double s = 0;
+        for (int i = threadIdx.x; i < nparts; i += 256) s += partials[4 * i + q];
+        red[q][threadIdx.x] = s;
+    }
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w)
+            for (int q = 0; q < 4; ++q) red[q][threadIdx.x] += red[q][threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x < 4) out[threadIdx.x] = red[threadIdx.x][0];
+}
+
+int check_params(const double* params, int stepPET, int stepTi, LikParams& p) {
+    if (!params) return set_error(GH_EINVAL, "likelihood: params is NULL");
+    if (stepPET < 1 || stepTi < 1) return set_error(GH_EINVAL, "likelihood: steps must be >= 1");
+    if (!(params[0] > 0) || !(params[4] != 0)) return set_error(GH_EINVAL, "likelihood: PETsigma2 > 0 and slope != 0");
+    p.s2 = params[0];
+    p.b = params[1];
+    p.uc1 = params[2];
+    p.uc2 = params[3];
+    p.is2 = 1. / params[4];
+    p.stepPET = stepPET;
+    p.stepTi = stepTi;
+    return GH_OK;
+}
+
+// combine on the host in long double, as HGG_Likelihood::run does
+void finish(const double sums[4], const LikParams& p, double out[4], int64_t* npet) {
+    const long double p1 = (long double)(-0.5 * (double)sums[1] * std::log(2. * M_PI * p.s2));
+    const long double p2 = (long double)(-0.5 * (1. / p.s2)) * (long double)sums[0];
+    const long double lpet = p1 + p2;
+    const long double cost = lpet + (long double)sums[2] + (long double)sums[3];
+    out[0] = (double)(-cost);
+    out[1] = (double)lpet;
+    out[2] = sums[2];
+    out[3] = sums[3];
+    if (npet) *npet = (int64_t)llround(sums[1]);
+}
+
+template <typename T>
+int run_lik(Ctx& c, size_t ncells, Layout lay, const T* u, const unsigned char* t1, const unsigned char* fl,
+            const double* pet, const int4* roi, int nroi, const LikParams& p, double out[4], int64_t* npet) {
+    int rc = c.likPart.ensure(sizeof(double) * 4 * (LIK_GRID + 1));
+    if (rc) return rc;
+    double* parts = c.likPart.as<double>();
+    hipLaunchKernelGGL(lik_partials_kernel<T>, dim3(LIK_GRID), dim3(LIK_THREADS), 0, c.stream, ncells, lay, u, t1,
+                       fl, pet, roi, nroi, p, parts);
+    GH_HIP_CHECK(hipGetLastError());
+    hipLaunchKernelGGL(lik_final_kernel, dim3(1), dim3(256), 0, c.stream, LIK_GRID, parts, parts + 4 * LIK_GRID);
+    GH_HIP_CHECK(hipGetLastError());
+    double sums[4];
+    GH_HIP_CHECK(hipMemcpyAsync(sums, parts + 4 * LIK_GRID, sizeof(sums), hipMemcpyDeviceToHost, c.stream));
+    GH_HIP_CHECK(hipStreamSynchronize(c.stream));
+    finish(sums, p, out, npet);
+    return GH_OK;
+}
+
+// dense x-fastest -> block-major for a uniform grid (slot order bx + bpd*(by + bpd*bz))
+template <typename T>
+void to_blocks(int n, const T* dense, T* bm) {
+    const int bpd = n / BS;
+    for (size_t c = 0; c < (size_t)n * n * n; ++c) {
+        const int ix = (int)(c % n), iy = (int)((c / n) % n), iz = (int)(c / ((size_t)n * n));
+        const size_t s = (size_t)(ix / BS) + (size_t)bpd * ((iy / BS) + (size_t)bpd * (iz / BS));
+        bm[s * BCELLS + (ix % BS) + BS * ((iy % BS) + BS * (iz % BS))] = dense[c];
+    }
+}
+
+}  // namespace
+}  // namespace gh
+
+using namespace gh;
+
+extern "C" {
+
+int gh_lik_set_data(gh_ctx* ctx, int n, const double* t1c, const double* flair, const double* pet, const double* roi,
+                    int nroi) {
+    Ctx* c = as_ctx(ctx);
+    if (!c) return set_error(GH_EINVAL, "null context");
+    if (c->topo.uniform_bpd <= 0 || n != BS * c->topo.uniform_bpd)
+        return set_error(GH_ESTATE, "gh_lik_set_data: needs a uniform grid with n = 8*bpd");
+    if (!t1c || !flair || !pet || nroi < 0 || (nroi > 0 && !roi))
+        return set_error(GH_EINVAL, "gh_lik_set_data: null array");
+    GH_HIP_CHECK(hipSetDevice(c->device));
+    const size_t nc = (size_t)n * n * n;
+    std::vector<unsigned char> y1(nc), y2(nc), b1(nc), b2(nc);
+    for (size_t i = 0; i < nc; ++i) {
+        y1[i] = t1c[i] == 1;
+        y2[i] = flair[i] == 1;
+    }
+    std::vector<double> pb(nc);
+    to_blocks(n, y1.data(), b1.data());
+    to_blocks(n, y2.data(), b2.data());
+    to_blocks(n, pet, pb.data());
+    int rc;
+    if ((rc = c->obsT1.ensure(nc)) || (rc = c->obsFl.ensure(nc)) || (rc = c->obsPet.ensure(nc * sizeof(double))))
+        return rc;
+    GH_HIP_CHECK(hipMemcpyAsync(c->obsT1.p, b1.data(), nc, hipMemcpyHostToDevice, c->stream));
+    GH_HIP_CHECK(hipMemcpyAsync(c->obsFl.p, b2.data(), nc, hipMemcpyHostToDevice, c->stream));
+    GH_HIP_CHECK(hipMemcpyAsync(c->obsPet.p, pb.data(), nc * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    c->nroi = 0;
+    if (nroi > 0) {
+        // Matrix::D2D ROI(i, 0..2) as stored x-fastest: roi[i + nroi*col]
+        std::vector<int4> q(nroi);
+        const int bpd = c->topo.uniform_bpd;
+        for (int i = 0; i < nroi; ++i) {
+            const int ix = (int)roi[i], iy = (int)roi[i + nroi], iz = (int)roi[i + 2 * (size_t)nroi];
+            if (ix < 0 || iy < 0 || iz < 0 || ix >= n || iy >= n || iz >= n)
+                return set_error(GH_EINVAL, "gh_lik_set_data: ROI point outside the grid");
+            const size_t s = (size_t)(ix / BS) + (size_t)bpd * ((iy / BS) + (size_t)bpd * (iz / BS));
+            q[i] = make_int4((int)(s * BCELLS + (ix % BS) + BS * ((iy % BS) + BS * (iz % BS))), ix, iy, iz);
+        }
+        if ((rc = c->obsRoi.ensure(nroi * sizeof(int4)))) return rc;
+        GH_HIP_CHECK(hipMemcpyAsync(c->obsRoi.p, q.data(), nroi * sizeof(int4), hipMemcpyHostToDevice, c->stream));
+        c->nroi = nroi;
+    }
+    GH_HIP_CHECK(hipStreamSynchronize(c->stream));
+    c->lik_ready = true;
+    return GH_OK;
+}
+
+int gh_likelihood(gh_ctx* ctx, const double* params, int stepPET, int stepTi, int use_roi, double* out,
+                  int64_t* npet) {
+    Ctx* c = as_ctx(ctx);
+    if (!c || !out) return set_error(GH_EINVAL, "null argument");
+    if (!c->lik_ready) return set_error(GH_ESTATE, "gh_likelihood: call gh_lik_set_data first");
+    if (use_roi && c->nroi == 0) return set_error(GH_ESTATE, "gh_likelihood: no ROI was given");
+    LikParams p;
+    int rc = check_params(params, stepPET, stepTi, p);
+    if (rc) return rc;
+    GH_HIP_CHECK(hipSetDevice(c->device));
+    Layout lay{0, 0, 0, c->topo.uniform_bpd};
+    const int4* roi = use_roi ? c->obsRoi.as<int4>() : nullptr;
+    if (c->real_kind == GH_REAL_F32)
+        return run_lik<float>(*c, c->ncells(), lay, c->phi[c->cur].as<float>(), c->obsT1.as<unsigned char>(),
+                              c->obsFl.as<unsigned char>(), c->obsPet.as<double>(), roi, c->nroi, p, out, npet);
+    return run_lik<double>(*c, c->ncells(), lay, c->phi[c->cur].as<double>(), c->obsT1.as<unsigned char>(),
+                           c->obsFl.as<unsigned char>(), c->obsPet.as<double>(), roi, c->nroi, p, out, npet);
+}
+
+int gh_likelihood_dense(gh_ctx* ctx, int nx, int ny, int nz, const double* model, const double* t1c,
+                        const double* flair, const double* pet, const double* roi, int nroi, const double* params,
+                        int stepPET, int stepTi, double* out, int64_t* npet) {
+    Ctx* c = as_ctx(ctx);
+    if (!c || !out || !model || !t1c || !flair || !pet) return set_error(GH_EINVAL, "null argument");
+    if (nx <= 0 || ny <= 0 || nz <= 0 || nroi < 0 || (nroi > 0 && !roi))
+        return set_error(GH_EINVAL, "gh_likelihood_dense: bad size");
+    LikParams p;
+    int rc = check_params(params, stepPET, stepTi, p);
+    if (rc) return rc;
+    GH_HIP_CHECK(hipSetDevice(c->device));
+    const size_t nc = (size_t)nx * ny * nz;
+    std::vector<unsigned char> y1(nc), y2(nc);
+    for (size_t i = 0; i < nc; ++i) {
+        y1[i] = t1c[i] == 1;
+        y2[i] = flair[i] == 1;
+    }
+    std::vector<int4> q(nroi);
+    for (int i = 0; i < nroi; ++i) {
+        const int ix = (int)roi[i], iy = (int)roi[i + nroi], iz = (int)roi[i + 2 * (size_t)nroi];
+        if (ix < 0 || iy < 0 || iz < 0 || ix >= nx || iy >= ny || iz >= nz)
+            return set_error(GH_EINVAL, "gh_likelihood_dense: ROI point outside the grid");
+        q[i] = make_int4((int)(ix + (size_t)nx * (iy + (size_t)ny * iz)), ix, iy, iz);
+    }
+    DevBuf dm, dy1, dy2, dp, dr;
+    if ((rc = dm.ensure(nc * 8)) || (rc = dy1.ensure(nc)) || (rc = dy2.ensure(nc)) || (rc = dp.ensure(nc * 8)) ||
+        (rc = dr.ensure((size_t)nroi * sizeof(int4)))) {
+        dm.release(); dy1.release(); dy2.release(); dp.release(); dr.release();
+        return rc;
+    }
+    GH_HIP_CHECK(hipMemcpyAsync(dm.p, model, nc * 8, hipMemcpyHostToDevice, c->stream));
+    GH_HIP_CHECK(hipMemcpyAsync(dy1.p, y1.data(), nc, hipMemcpyHostToDevice, c->stream));
+    GH_HIP_CHECK(hipMemcpyAsync(dy2.p, y2.data(), nc, hipMemcpyHostToDevice, c->stream));
+    GH_HIP_CHECK(hipMemcpyAsync(dp.p, pet, nc * 8, hipMemcpyHostToDevice, c->stream));
+    if (nroi > 0)
+        GH_HIP_CHECK(hipMemcpyAsync(dr.p, q.data(), nroi * sizeof(int4), hipMemcpyHostToDevice, c->stream));
+    Layout lay{nx, ny, nz, 0};
+    rc = run_lik<double>(*c, nc, lay, dm.as<double>(), dy1.as<unsigned char>(), dy2.as<unsigned char>(),
+                         dp.as<double>(), nroi > 0 ? dr.as<int4>() : nullptr, nroi, p, out, npet);
+    (void)hipStreamSynchronize(c->stream);
+    dm.release(); dy1.release(); dy2.release(); dp.release(); dr.release();
+    return rc;
+}
+
+}  // extern "C"

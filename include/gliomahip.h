@@ -103,6 +103,27 @@ int gh_rd_autotune(gh_ctx* ctx, double dt, int trial_steps, int* best);
  * phi > sigma.  n cells of real_kind values. */
 int gh_rd_thresholds(int real_kind, int64_t n, const void* p_w, const void* p_g, void* tau, void* sigma);
 
+/* ---- likelihood ------------------------------------------------------------
+ * HGG_Likelihood (tools/pi4u_lite/Inference/Likelihood/source/HGG_Likelihood.cpp:
+ * 40-192) evaluated on the device in FP64.
+ * gh_lik_set_data: the observations tumT1c.dat, tumFLAIR.dat, tumPET.dat (n^3,
+ *   x-fastest, y == 1 marks a positive voxel) and optionally ROI.dat points
+ *   (nroi rows, stored column-major roi[i + nroi*col], col = x,y,z), for a
+ *   uniform grid with n = 8*bpd.
+ * gh_likelihood: model u = the context's current phi (HGG_data.dat of a
+ *   uniform grid).  params = {PETsigma2, PETscale, T1uc, T2uc, slope}
+ *   (LikelihoodInput.txt order), stepPET / stepTi = -stepPET / -stepMRI,
+ *   use_roi = -bROI.  out = {-logL, L_PET, L_T1, L_T2}; *npet = PET points.
+ * gh_likelihood_dense: the same on host arrays of any nx*ny*nz (e.g. an
+ *   HGG_data.dat read from disk). */
+int gh_lik_set_data(gh_ctx* ctx, int n, const double* t1c, const double* flair, const double* pet,
+                    const double* roi, int nroi);
+int gh_likelihood(gh_ctx* ctx, const double* params, int stepPET, int stepTi, int use_roi, double* out,
+                  int64_t* npet);
+int gh_likelihood_dense(gh_ctx* ctx, int nx, int ny, int nz, const double* model, const double* t1c,
+                        const double* flair, const double* pet, const double* roi, int nroi,
+                        const double* params, int stepPET, int stepTi, double* out, int64_t* npet);
+
 /* ---- checkpoints (no reference counterpart: the reference keeps one copy of
  * the state) -- device-side copies of phi, used to restart solves from one
  * initial condition and for the adjoint's checkpoint/recompute.  Slots are

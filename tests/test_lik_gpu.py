@@ -1,0 +1,79 @@
+"""GPU likelihood (FP64 device sums) against the reference executable's
+golden values and the long-double oracle."""
+from __future__ import annotations
+
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import gliomasolver_amd as gh
+from gliomasolver_amd import FIELD_PHI
+from gliomasolver_amd import workloads as wl
+from conftest import GOLDEN
+
+pytestmark = pytest.mark.gpu
+REL = 1e-12
+
+
+def say(*a):
+    print("[gpu]", *a, flush=True)
+    sys.stdout.flush()
+
+
+def golden():
+    z = np.load(os.path.join(GOLDEN, "likelihood24.npz"))
+    return {k: z[k] for k in z.files}
+
+
+@pytest.mark.parametrize("tag", ["full", "stride", "roiA", "roiB"])
+def test_gpu_likelihood_dense_matches_reference_and_oracle(oracle, tag):
+    g = golden()
+    total_ref, lpet, lt1, lt2, sp, sm = g[tag]
+    roi = g["roi"] if tag.startswith("roi") else None
+    say("likelihood dense 24^3", tag)
+    with gh.Context(0, "f64") as ctx:
+        total, comps, npet = ctx.likelihood_dense(g["model"], g["t1c"], g["flair"], g["pet"], roi=roi,
+                                                  stepPET=int(sp), stepTi=int(sm))
+    o_total, o_comps, o_npet = oracle.likelihood(g["model"], g["pet"], g["t1c"], g["flair"], roi=roi,
+                                                 stepPET=int(sp), stepTi=int(sm))
+    assert npet == o_npet
+    assert total == pytest.approx(o_total, rel=REL)
+    for a, b in zip(comps, o_comps):
+        assert a == pytest.approx(b, rel=REL, abs=1e-9)
+    assert abs(total - total_ref) < 2e-6  # reference prints %Lf
+
+
+def synthetic_observations(u, seed):
+    rng = np.random.default_rng(seed)
+    t1 = (u >= 0.7).astype(np.float64)
+    fl = (u >= 0.25).astype(np.float64)
+    for m in (t1, fl):
+        flip = rng.uniform(0, 1, u.shape) < 0.05
+        m[flip] = 1 - m[flip]
+    pet = np.where(t1 > 0, u + rng.normal(0, 0.05, u.shape), 0.0)
+    return t1, fl, pet
+
+
+@pytest.mark.parametrize("real", ["f32", "f64"])
+def test_gpu_likelihood_of_solved_field(oracle, real):
+    """In-pipeline: likelihood of the context's phi after an RD solve on a
+    64^3 phantom, full grid and ROI, vs the oracle on the same field."""
+    case = wl.make_case(64, real, 5, Dw_phys=0.013, rho=0.1, tend=150.0)
+    say("likelihood in-pipeline", real, case.describe())
+    ctx = wl.load_context(case)
+    ctx.rd_steps(case.nsteps, case.dt)
+    u = ctx.get_field(FIELD_PHI).astype(np.float64)
+    t1, fl, pet = synthetic_observations(u, 9)
+    roi = np.argwhere(fl > 0)[:, ::-1].astype(np.float64)
+    ctx.lik_set_data(t1, fl, pet, roi=roi)
+    for use_roi, sp, sm in ((False, 1, 1), (False, 2, 1), (True, 3, 2)):
+        total, comps, npet = ctx.likelihood(stepPET=sp, stepTi=sm, use_roi=use_roi)
+        o_total, o_comps, o_npet = oracle.likelihood(u, pet, t1, fl, roi=roi if use_roi else None, stepPET=sp,
+                                                     stepTi=sm)
+        assert npet == o_npet
+        assert total == pytest.approx(o_total, rel=REL)
+        for a, b in zip(comps, o_comps):
+            assert a == pytest.approx(b, rel=REL, abs=1e-9)
+    ctx.close()
