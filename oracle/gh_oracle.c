@@ -289,3 +289,103 @@ int gho_wavelet_taps(int order, int filter, int support[2], double* taps) {
     for (int i = 0; i < nt; ++i) taps[i] = t[i];
     return nt;
 }
+
+/* ------------------------------------------------------------------------ */
+/* tangent-linear reference (FP64)                                           */
+/* ------------------------------------------------------------------------ */
+static double dlog_bern(double u, double y, double uc, double slope) {
+    /* d/du of log(alpha) or log(1-alpha), alpha of HGG_Likelihood.cpp:149-152 */
+    double diff = u - uc;
+    if (!(diff < 0.)) return 0.; /* alpha is constant for u > uc; slope 0 at u == uc */
+    double a = 0.5 * exp(-diff * diff / slope);
+    double da = a * (-2. * diff / slope);
+    return (y == 1) ? da / a : -da / (1. - a);
+}
+
+void gho_dloss_du(int n, const double* u, const double* t1c, const double* flair, const double* pet,
+                  const double* lik, int stepPET, int stepTi, double* g) {
+    (void)stepTi; /* full grid: every voxel counts for the Bernoulli terms */
+    const double s2 = lik[0], b = lik[1], uc1 = lik[2], uc2 = lik[3], slope = lik[4];
+    for (int k = 0; k < n; ++k)
+        for (int j = 0; j < n; ++j)
+            for (int i = 0; i < n; ++i) {
+                int64_t c = i + (int64_t)n * (j + (int64_t)n * k);
+                double dl = dlog_bern(u[c], t1c[c], uc1, slope) + dlog_bern(u[c], flair[c], uc2, slope);
+                if (i % stepPET == 0 && j % stepPET == 0 && k % stepPET == 0 && pet[c] > 0.)
+                    dl += -(u[c] - b * pet[c]) / s2;
+                g[c] = -dl; /* J = -logL */
+            }
+}
+
+double gho_tlm_f64(int n, const double* phi0, const double* p_w, const double* p_g, double Dw, double Dg,
+                   double rho, double h, double T, int64_t N, double sigma, const double* x0, const double* dir,
+                   const double* t1c, const double* flair, const double* pet, const double* lik, int stepPET,
+                   int stepTi, double* loss) {
+    const int64_t nc = (int64_t)n * n * n;
+    double* u = (double*)malloc(nc * sizeof(double));
+    double* un = (double*)malloc(nc * sizeof(double));
+    double* du = (double*)malloc(nc * sizeof(double));
+    double* dun = (double*)malloc(nc * sizeof(double));
+    double* dphidt = (double*)malloc(nc * sizeof(double));
+    const double dt = T / (double)N, ddt = dir[2] / (double)N;
+    const double ih2 = 1. / (h * h);
+    memcpy(u, phi0, nc * sizeof(double));
+    for (int k = 0; k < n; ++k)
+        for (int j = 0; j < n; ++j)
+            for (int i = 0; i < n; ++i) {
+                int64_t c = i + (int64_t)n * (j + (int64_t)n * k);
+                du[c] = phi0[c] * ((i - x0[0]) * dir[3] + (j - x0[1]) * dir[4] + (k - x0[2]) * dir[5]) /
+                        (sigma * sigma);
+            }
+    for (int64_t s = 0; s < N; ++s) {
+        memcpy(un, u, nc * sizeof(double));
+        gho_rd_steps_f64(n, un, dphidt, p_w, p_g, Dw, Dg, rho, h, dt, 1);
+        for (int k = 0; k < n; ++k)
+            for (int j = 0; j < n; ++j)
+                for (int i = 0; i < n; ++i) {
+                    const int64_t c = i + (int64_t)n * (j + (int64_t)n * k);
+                    const int64_t nb[6] = {
+                        ((i + n - 1) % n) + (int64_t)n * (j + (int64_t)n * k),
+                        ((i + 1) % n) + (int64_t)n * (j + (int64_t)n * k),
+                        i + (int64_t)n * (((j + n - 1) % n) + (int64_t)n * k),
+                        i + (int64_t)n * (((j + 1) % n) + (int64_t)n * k),
+                        i + (int64_t)n * (j + (int64_t)n * ((k + n - 1) % n)),
+                        i + (int64_t)n * (j + (int64_t)n * ((k + 1) % n))};
+                    const int m = un[c] < 1.; /* upper clamp frozen; lower clamp inactive */
+                    const int dom = (p_w[c] + p_g[c]) + u[c] > 0.;
+                    if (!dom) { dun[c] = m ? du[c] : 0.; continue; }
+                    double df_loc = p_w[c] * Dw + p_g[c] * Dg, df[6];
+                    for (int f = 0; f < 6; ++f) {
+                        double d = p_w[nb[f]] * Dw + p_g[nb[f]] * Dg, s_ = d + df_loc;
+                        df[f] = (s_ < 1e-8) ? 0. : 2. * d * df_loc / s_;
+                    }
+                    int out[6];
+                    for (int f = 0; f < 6; ++f) out[f] = (u[nb[f]] + p_w[nb[f]]) + p_g[nb[f]] < 0.1;
+                    if (out[0]) df[1] *= 2; if (out[1]) df[0] *= 2;
+                    if (out[2]) df[3] *= 2; if (out[3]) df[2] *= 2;
+                    if (out[4]) df[5] *= 2; if (out[5]) df[4] *= 2;
+                    double rdiff = 0., drdiff = 0.;
+                    for (int f = 0; f < 6; ++f) {
+                        rdiff += df[f] * (u[nb[f]] - u[c]);
+                        drdiff += df[f] * (du[nb[f]] - du[c]);
+                    }
+                    rdiff *= ih2;
+                    drdiff *= ih2;
+                    const double R = rdiff + rho * u[c] * (1. - u[c]);
+                    const double dR = drdiff + (dir[0] / Dw) * rdiff + rho * (1. - 2. * u[c]) * du[c] +
+                                      dir[1] * u[c] * (1. - u[c]);
+                    dun[c] = m ? du[c] + dt * dR + ddt * R : 0.;
+                }
+        double* t = u; u = un; un = t;
+        t = du; du = dun; dun = t;
+    }
+    double comps[3];
+    double J = gho_likelihood(n, n, n, u, pet, t1c, flair, NULL, 0, lik[0], lik[1], lik[2], lik[3], lik[4],
+                              stepPET, stepTi, comps, NULL);
+    if (loss) *loss = J;
+    gho_dloss_du(n, u, t1c, flair, pet, lik, stepPET, stepTi, dun);
+    double dJ = 0.;
+    for (int64_t c = 0; c < nc; ++c) dJ += dun[c] * du[c];
+    free(u); free(un); free(du); free(dun); free(dphidt);
+    return dJ;
+}

@@ -55,6 +55,23 @@ int gho_dat_read(const char* path, int nx, int ny, int nz, double* data);
 /* order 2 or 4; filter in {0:Ha,1:Ga,2:Hs,3:Gs}; writes support[2] and taps; returns ntaps */
 int gho_wavelet_taps(int order, int filter, int support[2], double* taps);
 
+/* ---- tangent-linear reference (no reference counterpart: the reference has
+ * no derivatives).  Differentiable model: N steps with dt = T/N of the FP64
+ * operator above; the no-flux flags, the in-domain test and the upper clamp are
+ * frozen at their values along the trajectory (a.e. derivative); the lower
+ * clamp is taken as inactive.  IC u0 = the given phi0 (a Gaussian of width
+ * sigma around x0 in cells, gated by tissue), so du0/dx0 = u0 (i - x0)/sigma^2.
+ * J = -logL(u_N) with the likelihood above (full grid, no ROI).
+ * dir = {dDw, drho, dT, dx0, dy0, dz0} (Dw = the operator's rescaled Dw with
+ * Dg = 0.1 Dw).  Returns dJ along dir; *loss = J. */
+double gho_tlm_f64(int n, const double* phi0, const double* p_w, const double* p_g, double Dw, double Dg,
+                   double rho, double h, double T, int64_t N, double sigma, const double* x0, const double* dir,
+                   const double* t1c, const double* flair, const double* pet, const double* lik, int stepPET,
+                   int stepTi, double* loss);
+/* dJ/du of J = -logL (HGG_Likelihood.cpp:40-156 differentiated), full grid */
+void gho_dloss_du(int n, const double* u, const double* t1c, const double* flair, const double* pet,
+                  const double* lik, int stepPET, int stepTi, double* g);
+
 #ifdef __cplusplus
 }
 #endif

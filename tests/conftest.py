@@ -41,6 +41,9 @@ def _load_oracle():
         "gho_dat_write": (_i, [C.c_char_p, _i, _i, _i, _vp]),
         "gho_dat_read": (_i, [C.c_char_p, _i, _i, _i, _vp]),
         "gho_wavelet_taps": (_i, [_i, _i, _vp, _vp]),
+        "gho_tlm_f64": (_d, [_i, _vp, _vp, _vp, _d, _d, _d, _d, _d, _i64, _d, _vp, _vp, _vp, _vp, _vp, _vp, _i, _i,
+                             _vp]),
+        "gho_dloss_du": (None, [_i, _vp, _vp, _vp, _vp, _vp, _i, _i, _vp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -93,6 +96,28 @@ class Oracle:
                                         PETsigma2, PETscale, T1uc, T2uc, slope, stepPET, stepTi, self.p(out),
                                         C.byref(npet))
         return total, out, npet.value
+
+    LIK = (0.000361, 0.85, 0.7, 0.25, 2.0)
+
+    def tlm(self, phi0, pw, pg, Dw, Dg, rho, h, T, N, sigma, x0, direction, t1c, flair, pet, lik=LIK, stepPET=1,
+            stepTi=1):
+        """(J, dJ along direction) of the FP64 tangent-linear model."""
+        n = phi0.shape[0]
+        x0a = np.asarray(x0, np.float64)
+        da = np.asarray(direction, np.float64)
+        la = np.asarray(lik, np.float64)
+        loss = C.c_double()
+        dj = self.lib.gho_tlm_f64(n, self.p(phi0), self.p(pw), self.p(pg), Dw, Dg, rho, h, T, N, sigma, self.p(x0a),
+                                  self.p(da), self.p(t1c), self.p(flair), self.p(pet), self.p(la), stepPET, stepTi,
+                                  C.byref(loss))
+        return loss.value, dj
+
+    def dloss_du(self, u, t1c, flair, pet, lik=LIK, stepPET=1, stepTi=1):
+        g = np.zeros_like(u)
+        la = np.asarray(lik, np.float64)
+        self.lib.gho_dloss_du(u.shape[0], self.p(u), self.p(t1c), self.p(flair), self.p(pet), self.p(la), stepPET,
+                              stepTi, self.p(g))
+        return g
 
 
 @pytest.fixture(scope="session")
