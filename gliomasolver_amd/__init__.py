@@ -192,6 +192,18 @@ class Context:
                                            stepPET, stepTi, out, C.byref(npet)))
         return out[0], (out[1], out[2], out[3]), npet.value
 
+    def rd_adjoint(self, nsteps: int, T: float, ic_center, ic_sigma: float = 1.0, ckpt_every: int = 64,
+                   params=LIK_DEFAULT, stepPET=1, stepTi=1, use_roi=False):
+        """Forward + adjoint from the current phi: returns (loss[4], grad[6])
+        with grad = dJ/d(Dw, rho, T, x0, y0, z0), Dw as passed to rd_prepare."""
+        p = (C.c_double * 5)(*params)
+        ic = (C.c_double * 3)(*[float(v) for v in ic_center])
+        loss = (C.c_double * 4)()
+        grad = (C.c_double * 6)()
+        check(self.lib.gh_rd_adjoint(self.h, nsteps, T, ckpt_every, p, stepPET, stepTi, int(use_roi), ic, ic_sigma,
+                                     loss, grad))
+        return np.array(loss[:]), np.array(grad[:])
+
     def checkpoint_save(self, slot: int):
         check(self.lib.gh_checkpoint_save(self.h, slot))
 

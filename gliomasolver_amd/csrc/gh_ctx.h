@@ -42,6 +42,8 @@ struct Ctx {
     std::vector<DevBuf> ckpt;  // device-side copies of phi (gh_checkpoint_*)
     // likelihood data (block-major): y == 1 flags of T1c/FLAIR, PET values, ROI {cell, ix, iy, iz}
     DevBuf obsT1, obsFl, obsPet, obsRoi, likPart;
+    // adjoint: forward checkpoints, segment tape, adjoint ping-pong (FP64), parameter partials
+    DevBuf adjCk, adjSeg, adjLam[2], adjPart;
     int nroi = 0;
     bool lik_ready = false;
     bool prepared = false;
@@ -61,6 +63,15 @@ int rd_prepare(Ctx& c, double Dw, double Dg, double rho);
 int rd_rhs(Ctx& c);
 int rd_update(Ctx& c, double dt);
 int rd_steps(Ctx& c, int64_t nsteps, double dt);
+// one fused step from an explicit state buffer into another (same layout as phi)
+int rd_step_buffers(Ctx& c, const void* in, void* out, double dt);
+double block_h_of(const Ctx& c);
 int rd_autotune(Ctx& c, double dt, int trial_steps, int* best);
+
+// likelihood helpers (gh_lik.hip): -logL of the current phi; dJ/du of a state buffer
+int lik_eval_current(Ctx& c, const double* params, int stepPET, int stepTi, int use_roi, double* out,
+                     int64_t* npet);
+int lik_seed_gradient(Ctx& c, const void* u, double* g, const double* params, int stepPET, int stepTi,
+                      int use_roi);
 
 }  // namespace gh

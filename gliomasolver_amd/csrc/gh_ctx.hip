@@ -241,7 +241,8 @@ int gh_ctx_destroy(gh_ctx* ctx) {
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     DevBuf* bufs[] = {&c->phi[0], &c->phi[1], &c->dphidt, &c->pw, &c->pg, &c->pcsf,
                       &c->statD, &c->statSig, &c->nbr,
-                      &c->obsT1, &c->obsFl, &c->obsPet, &c->obsRoi, &c->likPart};
+                      &c->obsT1, &c->obsFl, &c->obsPet, &c->obsRoi, &c->likPart,
+                      &c->adjCk, &c->adjSeg, &c->adjLam[0], &c->adjLam[1], &c->adjPart};
     for (DevBuf* b : bufs) b->release();
     for (DevBuf& b : c->ckpt) b.release();
     if (c->ev0) (void)hipEventDestroy(c->ev0);

@@ -176,6 +176,51 @@ int run_lik(Ctx& c, size_t ncells, Layout lay, const T* u, const unsigned char* 
     return GH_OK;
 }
 
+// d/du of log(alpha) (y == 1) or log(1 - alpha): 0 for u >= uc (alpha is
+// constant above uc and flat at uc), alpha = e/2 with e = exp(-(u-uc)^2/slope) below
+__device__ inline double dlog_bernoulli(double u, bool y, double uc, double is2) {
+    const double diff = u - uc;
+    if (!(diff < 0.)) return 0.;
+    const double a = 0.5 * exp(-diff * diff * is2);
+    const double da = a * (-2. * diff * is2);
+    return y ? da / a : -da / (1. - a);
+}
+
+// g = dJ/du of J = -logL: PET lattice term and (without ROI) the Bernoulli terms
+template <typename T>
+__global__ __launch_bounds__(256) void lik_seed_kernel(size_t ncells, Layout lay, const T* __restrict__ u,
+                                                       const unsigned char* __restrict__ t1,
+                                                       const unsigned char* __restrict__ fl,
+                                                       const double* __restrict__ pet, LikParams p, int full_bern,
+                                                       double* __restrict__ g) {
+    const size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ncells) return;
+    int ix, iy, iz;
+    lay.coords(c, ix, iy, iz);
+    const double uc = (double)u[c];
+    double v = 0;
+    if ((ix % p.stepPET == 0) && (iy % p.stepPET == 0) && (iz % p.stepPET == 0) && pet[c] > 0.)
+        v += (uc - p.b * pet[c]) / p.s2;
+    if (full_bern) v -= dlog_bernoulli(uc, t1[c] != 0, p.uc1, p.is2) + dlog_bernoulli(uc, fl[c] != 0, p.uc2, p.is2);
+    g[c] = v;
+}
+
+// Bernoulli terms at ROI points on the stepMRI lattice (a point listed twice counts twice)
+template <typename T>
+__global__ __launch_bounds__(256) void lik_seed_roi_kernel(int nroi, const int4* __restrict__ roi,
+                                                           const T* __restrict__ u,
+                                                           const unsigned char* __restrict__ t1,
+                                                           const unsigned char* __restrict__ fl, LikParams p,
+                                                           double* __restrict__ g) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nroi) return;
+    const int4 q = roi[i];
+    if ((q.y % p.stepTi != 0) || (q.z % p.stepTi != 0) || (q.w % p.stepTi != 0)) return;
+    const double uc = (double)u[q.x];
+    atomicAdd(&g[q.x], -(dlog_bernoulli(uc, t1[q.x] != 0, p.uc1, p.is2) +
+                         dlog_bernoulli(uc, fl[q.x] != 0, p.uc2, p.is2)));
+}
+
 // dense x-fastest -> block-major for a uniform grid (slot order bx + bpd*(by + bpd*bz))
 template <typename T>
 void to_blocks(int n, const T* dense, T* bm) {
@@ -188,6 +233,56 @@ void to_blocks(int n, const T* dense, T* bm) {
 }
 
 }  // namespace
+
+int lik_eval_current(Ctx& c, const double* params, int stepPET, int stepTi, int use_roi, double* out,
+                     int64_t* npet) {
+    if (!c.lik_ready) return set_error(GH_ESTATE, "likelihood: call gh_lik_set_data first");
+    if (use_roi && c.nroi == 0) return set_error(GH_ESTATE, "likelihood: no ROI was given");
+    LikParams p;
+    int rc = check_params(params, stepPET, stepTi, p);
+    if (rc) return rc;
+    Layout lay{0, 0, 0, c.topo.uniform_bpd};
+    const int4* roi = use_roi ? c.obsRoi.as<int4>() : nullptr;
+    if (c.real_kind == GH_REAL_F32)
+        return run_lik<float>(c, c.ncells(), lay, c.phi[c.cur].as<float>(), c.obsT1.as<unsigned char>(),
+                              c.obsFl.as<unsigned char>(), c.obsPet.as<double>(), roi, c.nroi, p, out, npet);
+    return run_lik<double>(c, c.ncells(), lay, c.phi[c.cur].as<double>(), c.obsT1.as<unsigned char>(),
+                           c.obsFl.as<unsigned char>(), c.obsPet.as<double>(), roi, c.nroi, p, out, npet);
+}
+
+int lik_seed_gradient(Ctx& c, const void* u, double* g, const double* params, int stepPET, int stepTi,
+                      int use_roi) {
+    LikParams p;
+    int rc = check_params(params, stepPET, stepTi, p);
+    if (rc) return rc;
+    if (use_roi && c.nroi == 0) return set_error(GH_ESTATE, "likelihood: no ROI was given");
+    Layout lay{0, 0, 0, c.topo.uniform_bpd};
+    const size_t nc = c.ncells();
+    const unsigned grid = (unsigned)((nc + 255) / 256);
+    if (c.real_kind == GH_REAL_F32)
+        hipLaunchKernelGGL(lik_seed_kernel<float>, dim3(grid), dim3(256), 0, c.stream, nc, lay, (const float*)u,
+                           c.obsT1.as<unsigned char>(), c.obsFl.as<unsigned char>(), c.obsPet.as<double>(), p,
+                           use_roi ? 0 : 1, g);
+    else
+        hipLaunchKernelGGL(lik_seed_kernel<double>, dim3(grid), dim3(256), 0, c.stream, nc, lay, (const double*)u,
+                           c.obsT1.as<unsigned char>(), c.obsFl.as<unsigned char>(), c.obsPet.as<double>(), p,
+                           use_roi ? 0 : 1, g);
+    GH_HIP_CHECK(hipGetLastError());
+    if (use_roi) {
+        const unsigned rg = (unsigned)((c.nroi + 255) / 256);
+        if (c.real_kind == GH_REAL_F32)
+            hipLaunchKernelGGL(lik_seed_roi_kernel<float>, dim3(rg), dim3(256), 0, c.stream, c.nroi,
+                               c.obsRoi.as<int4>(), (const float*)u, c.obsT1.as<unsigned char>(),
+                               c.obsFl.as<unsigned char>(), p, g);
+        else
+            hipLaunchKernelGGL(lik_seed_roi_kernel<double>, dim3(rg), dim3(256), 0, c.stream, c.nroi,
+                               c.obsRoi.as<int4>(), (const double*)u, c.obsT1.as<unsigned char>(),
+                               c.obsFl.as<unsigned char>(), p, g);
+        GH_HIP_CHECK(hipGetLastError());
+    }
+    return GH_OK;
+}
+
 }  // namespace gh
 
 using namespace gh;
@@ -244,19 +339,8 @@ int gh_likelihood(gh_ctx* ctx, const double* params, int stepPET, int stepTi, in
                   int64_t* npet) {
     Ctx* c = as_ctx(ctx);
     if (!c || !out) return set_error(GH_EINVAL, "null argument");
-    if (!c->lik_ready) return set_error(GH_ESTATE, "gh_likelihood: call gh_lik_set_data first");
-    if (use_roi && c->nroi == 0) return set_error(GH_ESTATE, "gh_likelihood: no ROI was given");
-    LikParams p;
-    int rc = check_params(params, stepPET, stepTi, p);
-    if (rc) return rc;
     GH_HIP_CHECK(hipSetDevice(c->device));
-    Layout lay{0, 0, 0, c->topo.uniform_bpd};
-    const int4* roi = use_roi ? c->obsRoi.as<int4>() : nullptr;
-    if (c->real_kind == GH_REAL_F32)
-        return run_lik<float>(*c, c->ncells(), lay, c->phi[c->cur].as<float>(), c->obsT1.as<unsigned char>(),
-                              c->obsFl.as<unsigned char>(), c->obsPet.as<double>(), roi, c->nroi, p, out, npet);
-    return run_lik<double>(*c, c->ncells(), lay, c->phi[c->cur].as<double>(), c->obsT1.as<unsigned char>(),
-                           c->obsFl.as<unsigned char>(), c->obsPet.as<double>(), roi, c->nroi, p, out, npet);
+    return lik_eval_current(*c, params, stepPET, stepTi, use_roi, out, npet);
 }
 
 int gh_likelihood_dense(gh_ctx* ctx, int nx, int ny, int nz, const double* model, const double* t1c,

@@ -34,14 +34,6 @@
 namespace gh {
 namespace {
 
-// Per-cell constants of one solve, one 4-Real record per cell so a lane
-// fetches them with one vector load: harmonic diffusivity of the cell's +x,
-// +y, +z faces and the outside threshold tau.
-template <typename Real>
-struct alignas(4 * sizeof(Real)) Stat4 {
-    Real dx, dy, dz, tau;
-};
-
 template <typename Real>
 __global__ __launch_bounds__(256) void rd_statics_kernel(int nslots, const int32_t* __restrict__ nbr,
                                                          const Real* __restrict__ pw, const Real* __restrict__ pg,
@@ -227,26 +219,31 @@ int prepare_t(Ctx& c, double Dw, double Dg, double rho) {
 }
 
 template <typename Real, bool FUSED, int BPW>
-void launch_step(Ctx& c, Real* out) {
+void launch_step(Ctx& c, const Real* in, Real* out, double dt) {
     const int ns = c.topo.nblocks();
     const int groups = (ns + BPW - 1) / BPW;
     const int per_xcd = (groups + 7) / 8;
     const double h = block_h(c);
     const double ih2 = 1. / (h * h);
     hipLaunchKernelGGL((rd_step_kernel<Real, FUSED, BPW>), dim3(8 * per_xcd), dim3(512), 0, c.stream, ns,
-                       per_xcd, c.nbr.as<int32_t>(), c.phi[c.cur].as<Real>(), out, c.statD.as<Stat4<Real>>(),
-                       c.statSig.as<Real>(), (Real)c.rho, ih2, c.dt_last);
+                       per_xcd, c.nbr.as<int32_t>(), in, out, c.statD.as<Stat4<Real>>(), c.statSig.as<Real>(),
+                       (Real)c.rho, ih2, dt);
+}
+
+template <typename Real, bool FUSED>
+int step_t(Ctx& c, const Real* in, Real* out, double dt) {
+    switch (c.rd_bpw) {
+        case 1: launch_step<Real, FUSED, 1>(c, in, out, dt); break;
+        case 4: launch_step<Real, FUSED, 4>(c, in, out, dt); break;
+        default: launch_step<Real, FUSED, 2>(c, in, out, dt); break;
+    }
+    GH_HIP_CHECK(hipGetLastError());
+    return GH_OK;
 }
 
 template <typename Real, bool FUSED>
 int step_t(Ctx& c, Real* out) {
-    switch (c.rd_bpw) {
-        case 1: launch_step<Real, FUSED, 1>(c, out); break;
-        case 4: launch_step<Real, FUSED, 4>(c, out); break;
-        default: launch_step<Real, FUSED, 2>(c, out); break;
-    }
-    GH_HIP_CHECK(hipGetLastError());
-    return GH_OK;
+    return step_t<Real, FUSED>(c, c.phi[c.cur].as<Real>(), out, c.dt_last);
 }
 
 }  // namespace
@@ -302,6 +299,13 @@ int rd_autotune(Ctx& c, double dt, int trial_steps, int* best_out) {
     GH_HIP_CHECK(hipStreamSynchronize(c.stream));
     if (best_out) *best_out = best;
     return GH_OK;
+}
+
+double block_h_of(const Ctx& c) { return block_h(c); }
+
+int rd_step_buffers(Ctx& c, const void* in, void* out, double dt) {
+    return c.real_kind == GH_REAL_F32 ? step_t<float, true>(c, (const float*)in, (float*)out, dt)
+                                      : step_t<double, true>(c, (const double*)in, (double*)out, dt);
 }
 
 int rd_steps(Ctx& c, int64_t nsteps, double dt) {

@@ -16,6 +16,14 @@
 
 namespace gh {
 
+// Per-cell constants of one solve, one 4-Real record per cell so a lane
+// fetches them with one vector load: harmonic diffusivity of the cell's +x,
+// +y, +z faces and the outside threshold tau.
+template <typename Real>
+struct alignas(4 * sizeof(Real)) Stat4 {
+    Real dx, dy, dz, tau;
+};
+
 // ordered integer keys: okey(a) < okey(b) iff a < b for non-NaN a, b
 // (except -0 < +0, harmless: the predicates below agree on both zeros)
 __host__ __device__ inline int64_t okey(float x) {

@@ -124,6 +124,22 @@ int gh_likelihood_dense(gh_ctx* ctx, int nx, int ny, int nz, const double* model
                         const double* flair, const double* pet, const double* roi, int nroi,
                         const double* params, int stepPET, int stepTi, double* out, int64_t* npet);
 
+/* ---- adjoint (no reference counterpart: the reference has no derivatives) --
+ * gh_rd_adjoint: forward solve of nsteps fused steps with dt = T/nsteps from
+ * the current phi (the IC), J = -logL(u_N) as gh_likelihood, then the
+ * hand-written discrete adjoint backwards with checkpoint/recompute every
+ * ckpt_every steps.  The no-flux flags, the in-domain test and the upper clamp
+ * are frozen along the trajectory; the lower clamp is treated as inactive.
+ * Requires gh_rd_prepare(Dw, Dg = Dw/10, rho) and gh_lik_set_data.  The IC is
+ * taken to be a Gaussian of width ic_sigma (cells) centred at ic_center
+ * (cells), so dJ/dx0 = sum lambda_0 u_0 (i - x0)/sigma^2.
+ * loss[4] = {J, L_PET, L_T1, L_T2}; grad[6] = dJ/d{Dw (as passed to
+ * gh_rd_prepare), rho, T, x0, y0, z0}, accumulated in FP64.  phi holds u_N on
+ * return. */
+int gh_rd_adjoint(gh_ctx* ctx, int64_t nsteps, double T, int ckpt_every, const double* lik_params,
+                  int stepPET, int stepTi, int use_roi, const double* ic_center, double ic_sigma,
+                  double* loss, double* grad);
+
 /* ---- checkpoints (no reference counterpart: the reference keeps one copy of
  * the state) -- device-side copies of phi, used to restart solves from one
  * initial condition and for the adjoint's checkpoint/recompute.  Slots are

@@ -1,0 +1,152 @@
+"""GPU adjoint (through the C ABI) against the FP64 tangent-linear oracle,
+central finite differences of GPU forward solves, and torch gradcheck."""
+from __future__ import annotations
+
+import sys
+
+import numpy as np
+import pytest
+
+import gliomasolver_amd as gh
+from gliomasolver_amd import FIELD_PG, FIELD_PHI, FIELD_PW
+from test_oracle_tlm import central_fd, gaussian, observations, small_problem
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print("[gpu]", *a, flush=True)
+    sys.stdout.flush()
+
+
+def gpu_setup(real, n, pw, pg, obs):
+    ctx = gh.Context(0, real)
+    ctx.grid_init_uniform(n // 8)
+    ctx.set_field(FIELD_PW, pw)
+    ctx.set_field(FIELD_PG, pg)
+    ctx.lik_set_data(obs[0], obs[1], obs[2])
+    return ctx
+
+
+def gpu_adjoint(ctx, u0, Dw, rho, T, N, x0, sigma, K):
+    ctx.set_field(FIELD_PHI, u0)
+    ctx.rd_prepare(Dw, 0.1 * Dw, rho)
+    return ctx.rd_adjoint(N, T, x0, sigma, K)
+
+
+@pytest.mark.parametrize("real,rel", [("f64", 1e-9), ("f32", 2e-3)])
+def test_gpu_adjoint_matches_tlm(oracle, real, rel):
+    n, pw, pg, ball, x0, th, N, sigma, peak = small_problem()
+    obs = observations(oracle, n, pw, pg, ball, N)
+    u0 = gaussian(n, x0, sigma, peak, ball)
+    say("adjoint vs tlm", real)
+    ctx = gpu_setup(real, n, pw, pg, obs)
+    loss, grad = gpu_adjoint(ctx, u0, th["Dw"], th["rho"], th["T"], N, x0, sigma, 3)
+    ctx.close()
+    for k in range(6):
+        e = np.zeros(6)
+        e[k] = 1
+        J, dJ = oracle.tlm(u0, pw, pg, th["Dw"], 0.1 * th["Dw"], th["rho"], 1.0 / n, th["T"], N, sigma, x0, e,
+                           obs[0], obs[1], obs[2])
+        if k == 0:
+            assert loss[0] == pytest.approx(J, rel=1e-12 if real == "f64" else 1e-5)
+        say("  d%d adjoint %.10e tlm %.10e" % (k, grad[k], dJ))
+        assert grad[k] == pytest.approx(dJ, rel=rel, abs=1e-9 * abs(J))
+
+
+def test_gpu_adjoint_checkpoint_interval_invariant(oracle):
+    n, pw, pg, ball, x0, th, N, sigma, peak = small_problem()
+    obs = observations(oracle, n, pw, pg, ball, N)
+    u0 = gaussian(n, x0, sigma, peak, ball)
+    ctx = gpu_setup("f64", n, pw, pg, obs)
+    ref = None
+    for K in (1, 3, 4, 64):
+        say("adjoint ckpt_every", K)
+        loss, grad = gpu_adjoint(ctx, u0, th["Dw"], th["rho"], th["T"], N, x0, sigma, K)
+        if ref is None:
+            ref = (loss, grad)
+        else:
+            assert np.array_equal(loss, ref[0]) and np.array_equal(grad, ref[1])
+    ctx.close()
+
+
+def test_gpu_adjoint_matches_central_fd_of_gpu_forward(oracle):
+    """FD step 1e-4 relative (shrunk if a voxel would cross T1uc/T2uc)."""
+    n, pw, pg, ball, x0, th, N, sigma, peak = small_problem()
+    obs = observations(oracle, n, pw, pg, ball, N)
+    ctx = gpu_setup("f64", n, pw, pg, obs)
+    theta = np.array([th["Dw"], th["rho"], th["T"], *x0])
+
+    def forward(t):
+        ctx.set_field(FIELD_PHI, gaussian(n, t[3:], sigma, peak, ball))
+        ctx.rd_prepare(t[0], 0.1 * t[0], t[1])
+        ctx.rd_steps(N, t[2] / N)
+        return ctx.likelihood()[0], ctx.get_field(FIELD_PHI)
+
+    _, grad = gpu_adjoint(ctx, gaussian(n, x0, sigma, peak, ball), th["Dw"], th["rho"], th["T"], N, x0, sigma, 4)
+    for k in range(6):
+        eps = 1e-4 * (abs(theta[k]) if k < 3 else max(abs(theta[k]), 1.0))
+        fd = central_fd(forward, theta, k, eps)
+        say("  d%d adjoint %.8e fd %.8e" % (k, grad[k], fd))
+        assert grad[k] == pytest.approx(fd, rel=3e-5)
+    ctx.close()
+
+
+def test_gpu_adjoint_phantom64_fp32_vs_tlm(oracle):
+    """64^3 phantom, FP32 state, FP64 adjoint accumulation, 64-step checkpoints;
+    error against the FP64 tangent-linear oracle stated."""
+    from gliomasolver_amd import workloads as wl
+    case = wl.make_case(64, "f32", 4, Dw_phys=0.013, rho=0.05, tend=120.0)
+    N = case.nsteps
+    say("phantom64 f32 adjoint N=%d" % N)
+    ctx = wl.load_context(case)
+    pw, pg, u0 = ctx.get_field(FIELD_PW), ctx.get_field(FIELD_PG), ctx.get_field(FIELD_PHI)
+    rng = np.random.default_rng(1)
+    u_true = oracle.rd_steps(u0.astype(np.float64), pw.astype(np.float64), pg.astype(np.float64), case.Dw * 1.3,
+                             case.Dg * 1.3, 0.08, 1.0 / 64, case.tend / N, N)
+    t1 = (u_true >= 0.7).astype(float)
+    fl = (u_true >= 0.25).astype(float)
+    pet = np.where(t1 > 0, u_true + rng.normal(0, 0.05, u_true.shape), 0.0)
+    ctx.lik_set_data(t1, fl, pet)
+    loss, grad = ctx.rd_adjoint(N, case.tend, case.seed_voxel, 1.0, 64)
+    ctx.close()
+    pw64, pg64, _ = oracle.anatomy(case.gm, case.wm, case.csf, "f64")
+    u064 = gh.gaussian_ic("f64", pw64, pg64, case.seed_voxel, 1.0, 0.1)
+    Dw64 = 0.013 / 17.6 ** 2
+    for k in (0, 1, 2, 3):
+        e = np.zeros(6)
+        e[k] = 1
+        J, dJ = oracle.tlm(u064, pw64, pg64, Dw64, 0.1 * Dw64, case.rho, 1.0 / 64, case.tend, N, 1.0,
+                           np.array(case.seed_voxel, float), e, t1, fl, pet)
+        # the FP32 context's Dw is the float-rounded rescaled Dw
+        scale = 1.0
+        relerr = abs(grad[k] * scale - dJ) / abs(dJ)
+        say("  d%d fp32-state adjoint %.6e fp64 tlm %.6e rel err %.2e" % (k, grad[k], dJ, relerr))
+        assert relerr < 5e-3
+
+
+def test_torch_gradcheck_over_c_abi():
+    torch = pytest.importorskip("torch")
+    from gliomasolver_amd.autograd import AdjointProblem, neg_log_likelihood
+    n = 16
+    z, y, x = np.meshgrid(*(np.arange(n),) * 3, indexing="ij")
+    ball = ((x - 7.5) ** 2 + (y - 7.5) ** 2 + (z - 7.5) ** 2) < 6.5 ** 2
+    pw = np.where(ball, 0.8, 0.0)
+    pg = np.where(ball, 0.2, 0.0)
+    rng = np.random.default_rng(0)
+    t1 = (rng.uniform(0, 1, pw.shape) < 0.1).astype(float)
+    fl = (rng.uniform(0, 1, pw.shape) < 0.3).astype(float)
+    pet = np.where(t1 > 0, rng.uniform(0, 0.5, pw.shape), 0.0)
+    ctx = gh.Context(0, "f64")
+    ctx.grid_init_uniform(2)
+    ctx.set_field(FIELD_PW, pw)
+    ctx.set_field(FIELD_PG, pg)
+    ctx.lik_set_data(t1, fl, pet)
+    # thresholds above every simulated value keep J smooth for torch's FD
+    prob = AdjointProblem(ctx, L=17.6, nsteps=8, sigma=1.5, peak=0.4, ckpt_every=3,
+                          lik=(0.000361, 0.85, 0.95, 0.9, 2.0))
+    theta = torch.tensor([0.05, 0.1, 16.0, 7.3, 8.1, 7.6], dtype=torch.float64, requires_grad=True)
+    say("gradcheck")
+    assert torch.autograd.gradcheck(lambda t: neg_log_likelihood(t, prob), (theta,), eps=1e-6, atol=1e-4,
+                                    rtol=1e-4)
+    ctx.close()
