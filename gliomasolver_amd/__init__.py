@@ -125,6 +125,45 @@ class Context:
         check(self.lib.gh_grid_blocks(self.h, _ptr(lev), _ptr(idx)))
         return lev, idx
 
+    # ---- wavelet-adaptive grid ----
+    def grid_configure(self, wavelet_order: int = 4, max_level: int = 4, min_level: int = 0):
+        check(self.lib.gh_grid_configure(self.h, wavelet_order, max_level, min_level))
+
+    def grid_init_leaves(self, levels, indices):
+        lev = np.ascontiguousarray(levels, np.int32)
+        idx = np.ascontiguousarray(indices, np.int32).reshape(-1, 3)
+        check(self.lib.gh_grid_init_leaves(self.h, len(lev), _ptr(lev), _ptr(idx)))
+        self.n = 0
+
+    def fwt_details(self) -> np.ndarray:
+        out = np.zeros(self.nblocks(), np.float64)
+        check(self.lib.gh_fwt_details(self.h, _ptr(out)))
+        return out
+
+    def refine(self, tol: float, max_loops: int = 1) -> int:
+        n = C.c_int()
+        check(self.lib.gh_refine(self.h, tol, max_loops, C.byref(n)))
+        return n.value
+
+    def compress(self, tol: float, max_loops: int = -1) -> int:
+        n = C.c_int()
+        check(self.lib.gh_compress(self.h, tol, max_loops, C.byref(n)))
+        return n.value
+
+    def set_field_blocks(self, field: int, values: np.ndarray):
+        v = np.ascontiguousarray(values, self.dtype).reshape(self.nblocks(), 8, 8, 8)
+        check(self.lib.gh_set_field_blocks(self.h, field, _ptr(v)))
+
+    def get_field_blocks(self, field: int) -> np.ndarray:
+        out = np.zeros((self.nblocks(), 8, 8, 8), self.dtype)
+        check(self.lib.gh_get_field_blocks(self.h, field, _ptr(out)))
+        return out
+
+    def uq_resample(self, gpd: int) -> np.ndarray:
+        out = np.zeros((gpd, gpd, gpd), np.float64)
+        check(self.lib.gh_uq_resample(self.h, gpd, _ptr(out)))
+        return out
+
     # ---- fields ----
     def set_anatomy(self, gm, wm, csf):
         n = gm.shape[0]

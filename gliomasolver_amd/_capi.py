@@ -36,6 +36,14 @@ SIGNATURES = {
     "gh_grid_init_uniform": (_i, [_vp, _i]),
     "gh_grid_nblocks": (_i, [_vp, _pi]),
     "gh_grid_blocks": (_i, [_vp, _vp, _vp]),
+    "gh_grid_configure": (_i, [_vp, _i, _i, _i]),
+    "gh_grid_init_leaves": (_i, [_vp, _i, _vp, _vp]),
+    "gh_fwt_details": (_i, [_vp, _vp]),
+    "gh_refine": (_i, [_vp, _d, _i, _pi]),
+    "gh_compress": (_i, [_vp, _d, _i, _pi]),
+    "gh_set_field_blocks": (_i, [_vp, _i, _vp]),
+    "gh_get_field_blocks": (_i, [_vp, _i, _vp]),
+    "gh_uq_resample": (_i, [_vp, _i, _vp]),
     "gh_set_anatomy": (_i, [_vp, _i, _vp, _vp, _vp]),
     "gh_set_field_dense": (_i, [_vp, _i, _i, _vp]),
     "gh_get_field_dense": (_i, [_vp, _i, _i, _vp]),

@@ -1,0 +1,970 @@
+// gh_adapt.hip -- wavelet-adaptive grid on the device: ghost fill across
+// levels, FWT detail per block, refinement / compression with data movement,
+// the RD step on a multi-level grid, and the uniform resample for the
+// likelihood.
+//
+// Reference:
+//   ghost fill       MRAG/MRAGcore/MRAGBlockLab.h:154-301 (ghost = sum w*value)
+//   FWT details      MRAG/MRAGcore/MRAGBlockFWT.h:109-120, 219-298
+//   refine/compress  MRAG/MRAGscience/MRAGScienceCore.h:42-177
+//   split            MRAG/MRAGcore/MRAGBlockSplitter.h:24-99
+//   collapse         MRAG/MRAGcore/MRAGBlockCollapser.h:24-85
+//   RD on levels     Glioma/Operators/ReactionDiffusionOperator.h:47-139 (h per block)
+//   UQ resample      Glioma/Glioma_ReactionDiffusion.cpp:208-267
+//
+// Ghost values: a ghost lattice point at the block's level is a same-level
+// copy when a same-level leaf covers it, else a linear combination of leaf
+// cells built on the host (Octree::value_terms): injection from finer regions
+// (Ha = {1}), Hs synthesis from coarser ones.  Terms are accumulated in Real
+// in ascending (source cell, weight) order; the weights are dyadic, exact in
+// float and double.
+#include <algorithm>
+#include <array>
+#include <cmath>
+#include <unordered_map>
+#include <unordered_set>
+#include <vector>
+
+#include "gh_ctx.h"
+#include "gh_rd_common.h"
+
+#pragma clang fp contract(off)
+
+namespace gh {
+namespace {
+
+// interpolating wavelet taps in Real (MRAGWavelets_StaticData.cpp:13-21, 31-39)
+template <typename Real>
+struct WTaps {
+    int gs0, ge0;   // Ga support [gs0, ge0)
+    Real ga[7];     // Ga[i - gs0]
+    int hs0, he0;   // Hs support
+    Real hs[7];
+};
+template <typename Real>
+WTaps<Real> taps(int order) {
+    WTaps<Real> t{};
+    if (order == 2) {
+        t.gs0 = -2; t.ge0 = 1;
+        t.ga[0] = (Real)-0.5; t.ga[1] = (Real)1.; t.ga[2] = (Real)-0.5;
+        t.hs0 = -1; t.he0 = 2;
+        t.hs[0] = (Real)0.5; t.hs[1] = (Real)1.; t.hs[2] = (Real)0.5;
+    } else {
+        t.gs0 = -4; t.ge0 = 3;
+        const double g[7] = {1 / 16., 0, -9 / 16., 1, -9 / 16., 0, 1 / 16.};
+        const double h[7] = {-1 / 16., 0, 9 / 16., 1, 9 / 16., 0, -1 / 16.};
+        for (int i = 0; i < 7; ++i) { t.ga[i] = (Real)g[i]; t.hs[i] = (Real)h[i]; }
+        t.hs0 = -3; t.he0 = 4;
+    }
+    return t;
+}
+
+// lab geometry: positions [LS, LS+LAB) per axis (MRAGrid.inl:761-789 with the
+// RD stencil [-1,2)): order 4 -> [-2, 11), order 2 -> [-1, 9)
+inline int lab_start(int order) { return order == 4 ? -2 : -1; }
+inline int lab_size(int order) { return order == 4 ? 13 : 10; }
+
+__device__ inline int dir_code(int ix, int iy, int iz) {
+    const int dx = ix < 0 ? 0 : (ix < 8 ? 1 : 2), dy = iy < 0 ? 0 : (iy < 8 ? 1 : 2), dz = iz < 0 ? 0 : (iz < 8 ? 1 : 2);
+    return dx + 3 * dy + 9 * dz;
+}
+
+// interior and same-level ghosts of the lab of each listed slot
+template <typename Real>
+__global__ __launch_bounds__(256) void lab_fill_kernel(int nlist, const int32_t* __restrict__ slots, int LS, int LAB,
+                                                       const int32_t* __restrict__ nbr27, const Real* __restrict__ f,
+                                                       Real* __restrict__ lab) {
+    const int j = blockIdx.x;
+    if (j >= nlist) return;
+    const int s = slots[j];
+    const int lab3 = LAB * LAB * LAB;
+    for (int p = threadIdx.x; p < lab3; p += blockDim.x) {
+        const int ix = p % LAB + LS, iy = (p / LAB) % LAB + LS, iz = p / (LAB * LAB) + LS;
+        const int code = dir_code(ix, iy, iz);
+        const int src = code == 13 ? s : nbr27[27 * s + code];
+        if (src < 0) continue;  // interpolated ghost: written by ghost_entry_kernel
+        const int cx = (ix + 8) & 7, cy = (iy + 8) & 7, cz = (iz + 8) & 7;
+        lab[(size_t)j * lab3 + p] = f[(size_t)src * BCELLS + cx + 8 * cy + 64 * cz];
+    }
+}
+
+// out[dst] = sum of f[src] * w over the entry's terms, in table order
+template <typename Real>
+__global__ __launch_bounds__(256) void ghost_entry_kernel(int n, const GhostEntry* __restrict__ ent,
+                                                          const int32_t* __restrict__ tsrc,
+                                                          const Real* __restrict__ tw, const Real* __restrict__ f,
+                                                          Real* __restrict__ out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const GhostEntry e = ent[i];
+    Real acc = (Real)0;
+    for (int k = 0; k < e.count; ++k) {
+        const Real prod = f[tsrc[e.start + k]] * tw[e.start + k];
+        acc = acc + prod;
+    }
+    out[e.dst] = acc;
+}
+
+// BlockFWT::_fwt / _filter (MRAGBlockFWT.h:219-298): 64 coarse points x 8
+// tensor filters per block; the block's detail is the largest |value| over
+// the 7 wavelet codes (FWTReport::_getDetailMaxMag, :109-120).
+template <typename Real>
+__global__ __launch_bounds__(512) void fwt_kernel(int nlist, int LS, int LAB, WTaps<Real> tp,
+                                                  const Real* __restrict__ lab, double* __restrict__ detail) {
+    const int j = blockIdx.x;
+    if (j >= nlist) return;
+    __shared__ Real s_lab[13 * 13 * 13];
+    __shared__ double s_max[8];
+    const int lab3 = LAB * LAB * LAB;
+    for (int p = threadIdx.x; p < lab3; p += blockDim.x) s_lab[p] = lab[(size_t)j * lab3 + p];
+    __syncthreads();
+    const int code = threadIdx.x & 7, pt = threadIdx.x >> 3;
+    const int dx = pt & 3, dy = (pt >> 2) & 3, dz = pt >> 4;
+    const int bw[3] = {code & 1, (code >> 1) & 1, (code >> 2) & 1};
+    const int o[3] = {2 * dx, 2 * dy, 2 * dz};
+    int s[3], e[3];
+    for (int a = 0; a < 3; ++a) {
+        s[a] = bw[a] ? tp.gs0 : 0;  // Ha support [0,1)
+        e[a] = bw[a] ? tp.ge0 : 1;
+    }
+    Real result = (Real)0;
+    for (int iz = s[2]; iz < e[2]; ++iz) {
+        const Real wZ = bw[2] ? tp.ga[iz - tp.gs0] : (Real)1;
+        for (int iy = s[1]; iy < e[1]; ++iy) {
+            const Real wY = bw[1] ? tp.ga[iy - tp.gs0] : (Real)1;
+            const Real wZwY = wZ * wY;
+            const int base = (o[2] - iz - LS) * LAB * LAB + (o[1] - iy - LS) * LAB - LS;
+            for (int ix = s[0]; ix < e[0]; ++ix) {
+                const Real wX = bw[0] ? tp.ga[ix - tp.gs0] : (Real)1;
+                const Real w = wZwY * wX;
+                const Real prod = s_lab[base + o[0] - ix] * w;
+                result = result + prod;
+            }
+        }
+    }
+    double v = code == 0 ? 0.0 : (double)fabs(result);
+    for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
+    if ((threadIdx.x & 63) == 0) s_max[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double m = 0.0;
+        for (int w = 0; w < 8; ++w) m = fmax(m, s_max[w]);
+        detail[j] = m;
+    }
+}
+
+// new leaf from the old grid: copy, split (Hs synthesis from the parent's
+// lab, BlockSplitter::_splitBlocks) or collapse (Ha injection of the
+// children, BlockCollapser::_collapseBlocks)
+template <typename Real>
+__global__ __launch_bounds__(512) void remap_kernel(int nnew, const Remap* __restrict__ rm,
+                                                    const Real* __restrict__ old, const Real* __restrict__ lab,
+                                                    int LS, int LAB, WTaps<Real> tp, Real* __restrict__ out) {
+    const int ns = blockIdx.x;
+    if (ns >= nnew) return;
+    const Remap r = rm[ns];
+    const int t = threadIdx.x;
+    const int x = t & 7, y = (t >> 3) & 7, z = t >> 6;
+    Real v;
+    if (r.kind == 0) {
+        v = old[(size_t)r.src * BCELLS + t];
+    } else if (r.kind == 2) {
+        const int o = (x >> 2) | ((y >> 2) << 1) | ((z >> 2) << 2);
+        const int c = 2 * (x & 3) + 8 * (2 * (y & 3)) + 64 * (2 * (z & 3));
+        v = (Real)0 + old[(size_t)r.kids[o] * BCELLS + c];
+    } else {
+        // destination in the parent's doubled lattice
+        const int d[3] = {8 * (r.octant & 1) + x, 8 * ((r.octant >> 1) & 1) + y, 8 * ((r.octant >> 2) & 1) + z};
+        // source range [S0, E0) (MRAGBlockSplitter.h:49-59)
+        const int S0 = (int)ceil(tp.hs0 * 0.5), E0 = 1 + (int)floor((tp.he0 - 1) * 0.5 + 8);
+        const Real* L = lab + (size_t)r.src * LAB * LAB * LAB;
+        Real acc = (Real)0;
+        for (int sz = S0; sz < E0; ++sz) {
+            const int kz = 2 * sz - d[2];
+            if (!(d[2] >= max(0, 2 * sz + 1 - tp.he0) && d[2] < min(16, 2 * sz + 1 - tp.hs0))) continue;
+            const double wZ = (double)tp.hs[kz - tp.hs0];
+            for (int sy = S0; sy < E0; ++sy) {
+                const int ky = 2 * sy - d[1];
+                if (!(d[1] >= max(0, 2 * sy + 1 - tp.he0) && d[1] < min(16, 2 * sy + 1 - tp.hs0))) continue;
+                const double wZwY = wZ * (double)tp.hs[ky - tp.hs0];
+                for (int sx = S0; sx < E0; ++sx) {
+                    const int kx = 2 * sx - d[0];
+                    if (!(d[0] >= max(0, 2 * sx + 1 - tp.he0) && d[0] < min(16, 2 * sx + 1 - tp.hs0))) continue;
+                    const Real w = (Real)(wZwY * (double)tp.hs[kx - tp.hs0]);
+                    const Real prod = L[(sz - LS) * LAB * LAB + (sy - LS) * LAB + (sx - LS)] * w;
+                    acc = acc + prod;
+                }
+            }
+        }
+        v = acc;
+    }
+    out[(size_t)ns * BCELLS + t] = v;
+}
+
+// per-cell constants on a multi-level grid: as rd_statics_kernel, with the
+// face towards an interpolated ghost taken from the ghost's p_w, p_g; also the
+// ghosts' tau and the harmonic diffusivity of the minus faces towards ghosts
+template <typename Real>
+__global__ __launch_bounds__(256) void statics_ghosted_kernel(int nslots, const int32_t* __restrict__ nbr,
+                                                              const Real* __restrict__ pw, const Real* __restrict__ pg,
+                                                              const Real* __restrict__ gpw,
+                                                              const Real* __restrict__ gpg, Real Dw, Real Dg,
+                                                              Stat4<Real>* __restrict__ st, Real* __restrict__ sig,
+                                                              Real* __restrict__ fgTau, Real* __restrict__ fgDm) {
+    const size_t gid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (gid >= (size_t)nslots * BCELLS) return;
+    const int s = (int)(gid / BCELLS), l = (int)(gid % BCELLS);
+    const int x = l & 7, y = (l >> 3) & 7, z = l >> 6;
+    const Real pwc = pw[gid], pgc = pg[gid];
+    const Real dc = diffusivity(pwc, pgc, Dw, Dg);
+    auto nd = [&](int face, int inner_off, int wrap_off, int u, int v, bool at_edge) -> Real {
+        if (!at_edge) return diffusivity(pw[gid + inner_off], pg[gid + inner_off], Dw, Dg);
+        const int ns = nbr[6 * s + face];
+        if (ns >= 0) {
+            const size_t c = (size_t)ns * BCELLS + l + wrap_off;
+            return diffusivity(pw[c], pg[c], Dw, Dg);
+        }
+        const size_t gi = (size_t)s * 384 + face * 64 + v * 8 + u;
+        return diffusivity(gpw[gi], gpg[gi], Dw, Dg);
+    };
+    Stat4<Real> r;
+    r.dx = harmonic(nd(FXP, 1, -7, y, z, x == 7), dc);
+    r.dy = harmonic(nd(FYP, 8, -56, x, z, y == 7), dc);
+    r.dz = harmonic(nd(FZP, 64, -448, x, y, z == 7), dc);
+    r.tau = chf_threshold(pwc, pgc);
+    st[gid] = r;
+    sig[gid] = domain_threshold(pwc, pgc);
+    // minus faces towards interpolated ghosts
+    const int mfaces[3] = {FXM, FYM, FZM};
+    const bool atm[3] = {x == 0, y == 0, z == 0};
+    const int uu[3] = {y, x, x}, vv[3] = {z, z, y};
+    for (int a = 0; a < 3; ++a)
+        if (atm[a] && nbr[6 * s + mfaces[a]] < 0) {
+            const size_t gi = (size_t)s * 384 + mfaces[a] * 64 + vv[a] * 8 + uu[a];
+            fgDm[(size_t)s * 192 + a * 64 + vv[a] * 8 + uu[a]] = harmonic(diffusivity(gpw[gi], gpg[gi], Dw, Dg), dc);
+        }
+    // tau of the interpolated ghosts (one lane per ghost: the cells with x == 0 cover 64 per face)
+    if (l < 384) {
+        const int f = l >> 6;
+        if (nbr[6 * s + f] < 0) {
+            const size_t gi = (size_t)s * 384 + l;
+            fgTau[gi] = chf_threshold(gpw[gi], gpg[gi]);
+        }
+    }
+}
+
+__device__ inline int P(int x, int y, int z) { return (z + 1) * 100 + (y + 1) * 10 + (x + 1); }
+
+// RD step on a multi-level grid: one workgroup per block, h of the block's
+// level; a face's halo comes from the same-level neighbour or from the
+// interpolated face ghosts
+template <typename Real, bool FUSED>
+__global__ __launch_bounds__(512) void rd_step_ghosted_kernel(int nslots, int per_xcd, const int32_t* __restrict__ nbr,
+                                                              const int32_t* __restrict__ level,
+                                                              const Real* __restrict__ phi, Real* __restrict__ out,
+                                                              const Stat4<Real>* __restrict__ st,
+                                                              const Real* __restrict__ sig,
+                                                              const Real* __restrict__ fgPhi,
+                                                              const Real* __restrict__ fgTau,
+                                                              const Real* __restrict__ fgDm, Real rho, double dt) {
+    const int s = (int)(blockIdx.x % 8) * per_xcd + (int)(blockIdx.x / 8);
+    if (s >= nslots) return;
+    __shared__ Real s_phi[1000];
+    __shared__ unsigned char s_out[1000];
+    __shared__ Real s_Dx[576], s_Dy[576], s_Dz[576];
+    const int t = threadIdx.x;
+    const int x = t & 7, y = (t >> 3) & 7, z = t >> 6;
+    const size_t g = (size_t)s * BCELLS + t;
+    const Real pc = phi[g];
+    const Stat4<Real> sv = st[g];
+    const Real sc = sig[g];
+    s_phi[P(x, y, z)] = pc;
+    s_out[P(x, y, z)] = pc < sv.tau;
+    s_Dx[z * 72 + y * 9 + x + 1] = sv.dx;
+    s_Dy[z * 72 + (y + 1) * 8 + x] = sv.dy;
+    s_Dz[(z + 1) * 64 + y * 8 + x] = sv.dz;
+    if (t < 384) {
+        const int f = t >> 6, u = t & 7, v = (t >> 3) & 7;
+        int src, dst;
+        switch (f) {
+            case FXM: src = 7 + 8 * u + 64 * v; dst = P(-1, u, v); break;
+            case FXP: src = 0 + 8 * u + 64 * v; dst = P(8, u, v); break;
+            case FYM: src = u + 8 * 7 + 64 * v; dst = P(u, -1, v); break;
+            case FYP: src = u + 8 * 0 + 64 * v; dst = P(u, 8, v); break;
+            case FZM: src = u + 8 * v + 64 * 7; dst = P(u, v, -1); break;
+            default:  src = u + 8 * v + 64 * 0; dst = P(u, v, 8); break;
+        }
+        const int ns = nbr[6 * s + f];
+        Real ph, tau, dm = (Real)0;
+        if (ns >= 0) {
+            const size_t nb = (size_t)ns * BCELLS + src;
+            const Stat4<Real> hs = st[nb];
+            ph = phi[nb];
+            tau = hs.tau;
+            dm = f == FXM ? hs.dx : (f == FYM ? hs.dy : hs.dz);
+        } else {
+            ph = fgPhi[(size_t)s * 384 + t];
+            tau = fgTau[(size_t)s * 384 + t];
+            if (f == FXM || f == FYM || f == FZM) dm = fgDm[(size_t)s * 192 + (f >> 1) * 64 + v * 8 + u];
+        }
+        s_phi[dst] = ph;
+        s_out[dst] = ph < tau;
+        if (f == FXM) s_Dx[v * 72 + u * 9] = dm;
+        else if (f == FYM) s_Dy[v * 72 + u] = dm;
+        else if (f == FZM) s_Dz[v * 8 + u] = dm;
+    }
+    __syncthreads();
+    const double h = ldexp(1.0, -level[s]) / 8;  // info.h[0] (MRAGrid.inl:200-206)
+    const double ih2 = 1. / (h * h);
+    Real d0 = s_Dx[z * 72 + y * 9 + x], d1 = sv.dx, d2 = s_Dy[z * 72 + y * 8 + x], d3 = sv.dy;
+    Real d4 = s_Dz[z * 64 + y * 8 + x], d5 = sv.dz;
+    if (s_out[P(x - 1, y, z)]) d1 = (Real)((double)d1 * 2.0);
+    if (s_out[P(x + 1, y, z)]) d0 = (Real)((double)d0 * 2.0);
+    if (s_out[P(x, y - 1, z)]) d3 = (Real)((double)d3 * 2.0);
+    if (s_out[P(x, y + 1, z)]) d2 = (Real)((double)d2 * 2.0);
+    if (s_out[P(x, y, z - 1)]) d5 = (Real)((double)d5 * 2.0);
+    if (s_out[P(x, y, z + 1)]) d4 = (Real)((double)d4 * 2.0);
+    const Real p0 = s_phi[P(x - 1, y, z)], p1 = s_phi[P(x + 1, y, z)], p2 = s_phi[P(x, y - 1, z)];
+    const Real p3 = s_phi[P(x, y + 1, z)], p4 = s_phi[P(x, y, z - 1)], p5 = s_phi[P(x, y, z + 1)];
+    Real sin_ = d0 * p0;
+    sin_ = sin_ + d1 * p1;
+    sin_ = sin_ + d2 * p2;
+    sin_ = sin_ + d3 * p3;
+    sin_ = sin_ + d4 * p4;
+    sin_ = sin_ + d5 * p5;
+    const double fin = ih2 * (double)sin_;
+    Real sdf = d0 + d1;
+    sdf = sdf + d2;
+    sdf = sdf + d3;
+    sdf = sdf + d4;
+    sdf = sdf + d5;
+    const Real sdp = sdf * pc;
+    const double fout = -((double)sdp * ih2);
+    const Real rp = rho * pc;
+    const double react = (double)rp * (1. - (double)pc);
+    Real dphidt = (Real)((fout + fin) + react);
+    if (!(pc > sc)) dphidt = (Real)0.;
+    if (FUSED) {
+        Real p = (Real)((double)pc + dt * (double)dphidt);
+        p = ((Real)0. < p) ? p : (Real)0.;
+        p = (p < (Real)1.) ? p : (Real)1.;
+        out[g] = p;
+    } else {
+        out[g] = dphidt;
+    }
+}
+
+// _dumpUQoutput (Glioma_ReactionDiffusion.cpp:208-267): leaf cells onto the
+// finest uniform gpd^3 lattice; h = hf writes 1, 2hf writes 2^3, 3hf writes
+// 3^3 and anything coarser writes 4^3 cells (the reference's quirk is kept)
+template <typename Real>
+__global__ __launch_bounds__(512) void uq_resample_kernel(int nslots, const BlockRec* __restrict__ blocks,
+                                                          const Real* __restrict__ phi, int gpd,
+                                                          double* __restrict__ out) {
+    const int s = blockIdx.x;
+    if (s >= nslots) return;
+    const BlockRec b = blocks[s];
+    const int t = threadIdx.x;
+    const int ix = t & 7, iy = (t >> 3) & 7, iz = t >> 6;
+    const Real dil = (Real)ldexp(1.0, -b.level);
+    const Real h = dil / (Real)8;
+    const double hf = 1. / gpd, eps = hf * 0.5;
+    const double xs[3] = {(double)((Real)(b.index[0] * dil) + h * (Real)ix),
+                          (double)((Real)(b.index[1] * dil) + h * (Real)iy),
+                          (double)((Real)(b.index[2] * dil) + h * (Real)iz)};
+    const int m[3] = {(int)floor(xs[0] / hf), (int)floor(xs[1] / hf), (int)floor(xs[2] / hf)};
+    const double hd = (double)h;
+    const int k = hd < hf + eps ? 1 : (hd < 2. * hf + eps ? 2 : (hd < 3. * hf + eps ? 3 : 4));
+    const double v = (double)phi[(size_t)s * BCELLS + t];
+    for (int cz = 0; cz < k; ++cz)
+        for (int cy = 0; cy < k; ++cy)
+            for (int cx = 0; cx < k; ++cx) {
+                const int X = m[0] + cx, Y = m[1] + cy, Z = m[2] + cz;
+                if (X < gpd && Y < gpd && Z < gpd) out[(size_t)X + (size_t)gpd * (Y + (size_t)gpd * Z)] = v;
+            }
+}
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+
+// sort terms into the canonical accumulation order: ascending source cell,
+// then weight
+void canonical(std::vector<GhostTerm>& v) {
+    std::sort(v.begin(), v.end(), [](const GhostTerm& a, const GhostTerm& b) {
+        return a.src != b.src ? a.src < b.src : a.w < b.w;
+    });
+}
+
+struct HostTable {
+    std::vector<GhostEntry> ent;
+    std::vector<int32_t> src;
+    std::vector<double> w;
+    void add(int32_t dst, std::vector<GhostTerm>& terms) {
+        canonical(terms);
+        ent.push_back(GhostEntry{dst, (int32_t)src.size(), (int32_t)terms.size()});
+        for (auto& t : terms) {
+            src.push_back(t.src);
+            w.push_back(t.w);
+        }
+    }
+};
+
+template <typename Real>
+int upload_table(Ctx& c, const HostTable& h, DevBuf& ent, DevBuf& src, DevBuf& w) {
+    int rc;
+    if ((rc = ent.ensure(std::max<size_t>(1, h.ent.size()) * sizeof(GhostEntry))) ||
+        (rc = src.ensure(std::max<size_t>(1, h.src.size()) * sizeof(int32_t))) ||
+        (rc = w.ensure(std::max<size_t>(1, h.w.size()) * sizeof(Real))))
+        return rc;
+    std::vector<Real> wr(h.w.size());
+    for (size_t i = 0; i < wr.size(); ++i) wr[i] = (Real)h.w[i];
+    if (!h.ent.empty()) {
+        GH_HIP_CHECK(hipMemcpyAsync(ent.p, h.ent.data(), h.ent.size() * sizeof(GhostEntry), hipMemcpyHostToDevice,
+                                    c.stream));
+        GH_HIP_CHECK(hipMemcpyAsync(src.p, h.src.data(), h.src.size() * 4, hipMemcpyHostToDevice, c.stream));
+        GH_HIP_CHECK(hipMemcpyAsync(w.p, wr.data(), wr.size() * sizeof(Real), hipMemcpyHostToDevice, c.stream));
+    }
+    GH_HIP_CHECK(hipStreamSynchronize(c.stream));
+    return GH_OK;
+}
+
+// face ghosts of the RD stencil for faces without a same-level neighbour
+HostTable build_face_table(const Octree& t, const std::vector<int32_t>& nbr) {
+    HostTable h;
+    std::vector<GhostTerm> terms;
+    for (int s = 0; s < (int)t.leaves.size(); ++s) {
+        const BlockRec& b = t.leaves[s];
+        for (int f = 0; f < 6; ++f) {
+            if (nbr[6 * s + f] >= 0) continue;
+            for (int v = 0; v < 8; ++v)
+                for (int u = 0; u < 8; ++u) {
+                    int i[3];
+                    switch (f) {
+                        case FXM: i[0] = -1; i[1] = u; i[2] = v; break;
+                        case FXP: i[0] = 8; i[1] = u; i[2] = v; break;
+                        case FYM: i[0] = u; i[1] = -1; i[2] = v; break;
+                        case FYP: i[0] = u; i[1] = 8; i[2] = v; break;
+                        case FZM: i[0] = u; i[1] = v; i[2] = -1; break;
+                        default:  i[0] = u; i[1] = v; i[2] = 8; break;
+                    }
+                    const int64_t q[3] = {8LL * b.index[0] + i[0], 8LL * b.index[1] + i[1], 8LL * b.index[2] + i[2]};
+                    terms.clear();
+                    t.value_terms(b.level, q, 1.0, terms);
+                    h.add(s * 384 + f * 64 + v * 8 + u, terms);
+                }
+        }
+    }
+    return h;
+}
+
+// interpolated ghosts of the full labs of the listed slots (compact index j)
+HostTable build_lab_table(const Octree& t, const std::vector<int>& slots, const std::vector<int32_t>& nbr27,
+                          int order) {
+    HostTable h;
+    const int LS = lab_start(order), LAB = lab_size(order), lab3 = LAB * LAB * LAB;
+    std::vector<GhostTerm> terms;
+    for (int j = 0; j < (int)slots.size(); ++j) {
+        const int s = slots[j];
+        const BlockRec& b = t.leaves[s];
+        for (int p = 0; p < lab3; ++p) {
+            const int i[3] = {p % LAB + LS, (p / LAB) % LAB + LS, p / (LAB * LAB) + LS};
+            const int code = (i[0] < 0 ? 0 : (i[0] < 8 ? 1 : 2)) + 3 * (i[1] < 0 ? 0 : (i[1] < 8 ? 1 : 2)) +
+                             9 * (i[2] < 0 ? 0 : (i[2] < 8 ? 1 : 2));
+            if (code == 13 || nbr27[27 * s + code] >= 0) continue;
+            const int64_t q[3] = {8LL * b.index[0] + i[0], 8LL * b.index[1] + i[1], 8LL * b.index[2] + i[2]};
+            terms.clear();
+            t.value_terms(b.level, q, 1.0, terms);
+            h.add(j * lab3 + p, terms);
+        }
+    }
+    return h;
+}
+
+}  // namespace
+
+// topology derived from the octree: slots, same-level neighbours, levels,
+// face-ghost table; uploads everything and sizes the per-slot ghost buffers
+int apply_topology(Ctx& c) {
+    Octree& t = c.tree;
+    Topology topo;
+    topo.blocks = t.leaves;
+    const int nb = (int)t.leaves.size();
+    topo.nbr.resize(6 * (size_t)nb);
+    std::vector<int32_t> n27(27 * (size_t)nb), lev(nb);
+    static const int face_code[6] = {12, 14, 10, 16, 4, 22};
+    for (int s = 0; s < nb; ++s) {
+        int o[27];
+        t.same_level_neighbours(s, o);
+        for (int k = 0; k < 27; ++k) n27[27 * (size_t)s + k] = o[k];
+        for (int f = 0; f < 6; ++f) topo.nbr[6 * (size_t)s + f] = o[face_code[f]];
+        lev[s] = t.leaves[s].level;
+    }
+    topo.uniform_bpd = t.is_uniform() ? (1 << t.levels_min()) : 0;
+    topo.nbr27 = n27;
+    c.topo = std::move(topo);
+    int rc;
+    if ((rc = c.upload_topology())) return rc;
+    if ((rc = c.dNbr27.ensure(n27.size() * 4)) || (rc = c.dLevel.ensure(lev.size() * 4))) return rc;
+    GH_HIP_CHECK(hipMemcpyAsync(c.dNbr27.p, n27.data(), n27.size() * 4, hipMemcpyHostToDevice, c.stream));
+    GH_HIP_CHECK(hipMemcpyAsync(c.dLevel.p, lev.data(), lev.size() * 4, hipMemcpyHostToDevice, c.stream));
+    HostTable ft = build_face_table(t, c.topo.nbr);
+    c.nFaceEnt = (int)ft.ent.size();
+    rc = c.real_kind == GH_REAL_F32 ? upload_table<float>(c, ft, c.faceEnt, c.faceSrc, c.faceW)
+                                    : upload_table<double>(c, ft, c.faceEnt, c.faceSrc, c.faceW);
+    if (rc) return rc;
+    const size_t rs = c.real_size;
+    if ((rc = c.fgPhi.ensure(std::max(1, nb) * 384 * rs)) || (rc = c.fgTau.ensure(std::max(1, nb) * 384 * rs)) ||
+        (rc = c.fgDm.ensure(std::max(1, nb) * 192 * rs)) || (rc = c.fgPw.ensure(std::max(1, nb) * 384 * rs)) ||
+        (rc = c.fgPg.ensure(std::max(1, nb) * 384 * rs)))
+        return rc;
+    GH_HIP_CHECK(hipStreamSynchronize(c.stream));
+    return GH_OK;
+}
+
+namespace {
+
+template <typename Real>
+int face_ghosts(Ctx& c, const Real* field, Real* out) {
+    if (c.nFaceEnt == 0) return GH_OK;
+    hipLaunchKernelGGL(ghost_entry_kernel<Real>, dim3((c.nFaceEnt + 255) / 256), dim3(256), 0, c.stream, c.nFaceEnt,
+                       c.faceEnt.as<GhostEntry>(), c.faceSrc.as<int32_t>(), c.faceW.as<Real>(), field, out);
+    GH_HIP_CHECK(hipGetLastError());
+    return GH_OK;
+}
+
+template <typename Real>
+int prepare_ghosted_t(Ctx& c, double Dw, double Dg) {
+    int rc;
+    if ((rc = face_ghosts<Real>(c, c.pw.as<Real>(), c.fgPw.as<Real>())) ||
+        (rc = face_ghosts<Real>(c, c.pg.as<Real>(), c.fgPg.as<Real>())))
+        return rc;
+    const int ns = c.topo.nblocks();
+    const size_t nc = c.ncells();
+    hipLaunchKernelGGL(statics_ghosted_kernel<Real>, dim3((unsigned)((nc + 255) / 256)), dim3(256), 0, c.stream, ns,
+                       c.nbr.as<int32_t>(), c.pw.as<Real>(), c.pg.as<Real>(), c.fgPw.as<Real>(), c.fgPg.as<Real>(),
+                       (Real)Dw, (Real)Dg, c.statD.as<Stat4<Real>>(), c.statSig.as<Real>(), c.fgTau.as<Real>(),
+                       c.fgDm.as<Real>());
+    GH_HIP_CHECK(hipGetLastError());
+    return GH_OK;
+}
+
+template <typename Real, bool FUSED>
+int step_ghosted_t(Ctx& c, const Real* in, Real* out, double dt) {
+    int rc = face_ghosts<Real>(c, in, c.fgPhi.as<Real>());
+    if (rc) return rc;
+    const int ns = c.topo.nblocks();
+    const int per_xcd = (ns + 7) / 8;
+    hipLaunchKernelGGL((rd_step_ghosted_kernel<Real, FUSED>), dim3(8 * per_xcd), dim3(512), 0, c.stream, ns, per_xcd,
+                       c.nbr.as<int32_t>(), c.dLevel.as<int32_t>(), in, out, c.statD.as<Stat4<Real>>(),
+                       c.statSig.as<Real>(), c.fgPhi.as<Real>(), c.fgTau.as<Real>(), c.fgDm.as<Real>(), (Real)c.rho,
+                       dt);
+    GH_HIP_CHECK(hipGetLastError());
+    return GH_OK;
+}
+
+// full labs of `slots` for `field` into `out` (compact, one lab per listed
+// slot); the interpolated ghosts come from the uploaded lab table `tab`
+template <typename Real>
+int build_labs(Ctx& c, const std::vector<int>& slots, const Real* field, const HostTable& tab, DevBuf& out) {
+    const int order = c.tree.order;
+    const int LS = lab_start(order), LAB = lab_size(order), lab3 = LAB * LAB * LAB;
+    int rc;
+    if ((rc = out.ensure(std::max<size_t>(1, slots.size()) * lab3 * sizeof(Real))) ||
+        (rc = c.labSlots.ensure(std::max<size_t>(1, slots.size()) * 4)))
+        return rc;
+    if (slots.empty()) return GH_OK;
+    GH_HIP_CHECK(hipMemcpyAsync(c.labSlots.p, slots.data(), slots.size() * 4, hipMemcpyHostToDevice, c.stream));
+    hipLaunchKernelGGL(lab_fill_kernel<Real>, dim3((unsigned)slots.size()), dim3(256), 0, c.stream, (int)slots.size(),
+                       c.labSlots.as<int32_t>(), LS, LAB, c.dNbr27.as<int32_t>(), field, out.as<Real>());
+    GH_HIP_CHECK(hipGetLastError());
+    if (!tab.ent.empty()) {
+        hipLaunchKernelGGL(ghost_entry_kernel<Real>, dim3((unsigned)((tab.ent.size() + 255) / 256)), dim3(256), 0,
+                           c.stream, (int)tab.ent.size(), c.labEnt.as<GhostEntry>(), c.labSrc.as<int32_t>(),
+                           c.labW.as<Real>(), field, out.as<Real>());
+        GH_HIP_CHECK(hipGetLastError());
+    }
+    return GH_OK;
+}
+
+template <typename Real>
+int fwt_details_t(Ctx& c, const std::vector<int>& slots, std::vector<double>& det) {
+    det.assign(slots.size(), 0.0);
+    if (slots.empty()) return GH_OK;
+    const int order = c.tree.order;
+    HostTable tab = build_lab_table(c.tree, slots, c.topo.nbr27, order);
+    int rc;
+    if ((rc = upload_table<Real>(c, tab, c.labEnt, c.labSrc, c.labW))) return rc;
+    if ((rc = build_labs<Real>(c, slots, c.phi[c.cur].as<Real>(), tab, c.labBuf))) return rc;
+    DevBuf d;
+    if ((rc = d.ensure(slots.size() * sizeof(double)))) return rc;
+    hipLaunchKernelGGL(fwt_kernel<Real>, dim3((unsigned)slots.size()), dim3(512), 0, c.stream, (int)slots.size(),
+                       lab_start(order), lab_size(order), taps<Real>(order), c.labBuf.as<Real>(), d.as<double>());
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpyAsync(det.data(), d.p, det.size() * 8, hipMemcpyDeviceToHost, c.stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c.stream);
+    d.release();
+    if (e != hipSuccess) return set_error(GH_EHIP, std::string("fwt: ") + hipGetErrorString(e));
+    return GH_OK;
+}
+
+// replace the grid by `nt`: every new leaf is a copy, a split child (from the
+// old parent's lab) or a collapse of 8 old leaves; field arrays are rebuilt
+// on the device and the per-solve constants recomputed
+template <typename Real>
+int apply_new_tree(Ctx& c, const Octree& nt, const std::vector<Remap>& rm, const std::vector<int>& split_parents) {
+    const int order = c.tree.order;
+    const int LS = lab_start(order), LAB = lab_size(order);
+    int rc;
+    DevBuf* fields[4] = {&c.phi[c.cur], &c.pw, &c.pg, &c.pcsf};
+    DevBuf labs[4];
+    if (!split_parents.empty()) {
+        HostTable tab = build_lab_table(c.tree, split_parents, c.topo.nbr27, order);
+        if ((rc = upload_table<Real>(c, tab, c.labEnt, c.labSrc, c.labW))) return rc;
+        for (int f = 0; f < 4; ++f)
+            if ((rc = build_labs<Real>(c, split_parents, fields[f]->as<Real>(), tab, labs[f]))) return rc;
+    }
+    const int nnew = (int)nt.leaves.size();
+    DevBuf drm;
+    if ((rc = drm.ensure(rm.size() * sizeof(Remap)))) return rc;
+    GH_HIP_CHECK(hipMemcpyAsync(drm.p, rm.data(), rm.size() * sizeof(Remap), hipMemcpyHostToDevice, c.stream));
+    DevBuf nf[4];
+    for (int f = 0; f < 4; ++f) {
+        if ((rc = nf[f].ensure((size_t)nnew * BCELLS * sizeof(Real)))) return rc;
+        hipLaunchKernelGGL(remap_kernel<Real>, dim3(nnew), dim3(512), 0, c.stream, nnew, drm.as<Remap>(),
+                           fields[f]->as<Real>(), labs[f].as<Real>(), LS, LAB, taps<Real>(order), nf[f].as<Real>());
+        GH_HIP_CHECK(hipGetLastError());
+    }
+    GH_HIP_CHECK(hipStreamSynchronize(c.stream));
+    for (int f = 0; f < 4; ++f) {
+        fields[f]->release();
+        *fields[f] = nf[f];
+        nf[f].p = nullptr;
+        nf[f].bytes = 0;
+        labs[f].release();
+    }
+    drm.release();
+    c.tree = nt;
+    if ((rc = apply_topology(c))) return rc;
+    // the other state buffers follow the new size
+    const size_t n = c.ncells() * c.real_size;
+    if ((rc = c.phi[1 - c.cur].ensure(n)) || (rc = c.dphidt.ensure(n)) || (rc = c.statD.ensure(4 * n)) ||
+        (rc = c.statSig.ensure(n)))
+        return rc;
+    if (c.prepared) return rd_prepare(c, c.Dw, c.Dg, c.rho);
+    return GH_OK;
+}
+
+}  // namespace
+
+int rd_prepare_ghosted(Ctx& c, double Dw, double Dg) {
+    return c.real_kind == GH_REAL_F32 ? prepare_ghosted_t<float>(c, Dw, Dg) : prepare_ghosted_t<double>(c, Dw, Dg);
+}
+
+int rd_step_ghosted(Ctx& c, const void* in, void* out, double dt, bool fused) {
+    if (c.real_kind == GH_REAL_F32)
+        return fused ? step_ghosted_t<float, true>(c, (const float*)in, (float*)out, dt)
+                     : step_ghosted_t<float, false>(c, (const float*)in, (float*)out, dt);
+    return fused ? step_ghosted_t<double, true>(c, (const double*)in, (double*)out, dt)
+                 : step_ghosted_t<double, false>(c, (const double*)in, (double*)out, dt);
+}
+
+int fwt_details(Ctx& c, const std::vector<int>& slots, std::vector<double>& det) {
+    return c.real_kind == GH_REAL_F32 ? fwt_details_t<float>(c, slots, det) : fwt_details_t<double>(c, slots, det);
+}
+
+// Grid::refine with Refiner_SpaceExtension (MRAGrid.inl:247-355)
+int grid_refine(Ctx& c, const std::vector<char>& flagged, int* nrefined) {
+    const Octree& t = c.tree;
+    std::vector<int> victims = t.refinement_victims(flagged);
+    if (nrefined) *nrefined = (int)victims.size();
+    if (victims.empty()) return GH_OK;
+    Octree nt = t;
+    std::vector<char> is_victim(t.leaves.size(), 0);
+    for (int v : victims) is_victim[v] = 1;
+    nt.leaves.clear();
+    std::unordered_map<uint64_t, int> parent_of;  // child key -> compact parent index
+    std::unordered_map<uint64_t, int> octant_of;
+    for (int s = 0; s < (int)t.leaves.size(); ++s) {
+        const BlockRec& b = t.leaves[s];
+        if (!is_victim[s]) {
+            nt.leaves.push_back(b);
+            continue;
+        }
+        nt.interior.insert(Octree::key(b));
+    }
+    for (int j = 0; j < (int)victims.size(); ++j) {
+        const BlockRec& b = t.leaves[victims[j]];
+        for (int o = 0; o < 8; ++o) {
+            BlockRec ch{b.level + 1, {2 * b.index[0] + (o & 1), 2 * b.index[1] + ((o >> 1) & 1),
+                                      2 * b.index[2] + ((o >> 2) & 1)}};
+            nt.leaves.push_back(ch);
+            parent_of[Octree::key(ch)] = j;
+            octant_of[Octree::key(ch)] = o;
+        }
+    }
+    nt.rebuild_index();
+    std::vector<Remap> rm(nt.leaves.size());
+    for (int s = 0; s < (int)nt.leaves.size(); ++s) {
+        const uint64_t k = Octree::key(nt.leaves[s]);
+        Remap r{};
+        auto it = t.leaf_slot.find(k);
+        if (it != t.leaf_slot.end()) {
+            r.kind = 0;
+            r.src = it->second;
+        } else {
+            r.kind = 1;
+            r.src = parent_of.at(k);
+            r.octant = octant_of.at(k);
+        }
+        rm[s] = r;
+    }
+    return c.real_kind == GH_REAL_F32 ? apply_new_tree<float>(c, nt, rm, victims)
+                                      : apply_new_tree<double>(c, nt, rm, victims);
+}
+
+// Grid::compress with Compressor (MRAGrid.inl:383-486)
+int grid_compress(Ctx& c, const std::vector<char>& flagged, int* ncollapsed) {
+    const Octree& t = c.tree;
+    std::vector<uint64_t> parents = t.collapse_parents(flagged);
+    if (ncollapsed) *ncollapsed = (int)parents.size();
+    if (parents.empty()) return GH_OK;
+    Octree nt = t;
+    std::unordered_map<uint64_t, std::array<int, 8>> kids;
+    std::unordered_set<int> gone;
+    for (uint64_t pk : parents) {
+        const int L = (int)(pk >> 48);
+        const int px = (int)((pk >> 32) & 0xffff), py = (int)((pk >> 16) & 0xffff), pz = (int)(pk & 0xffff);
+        std::array<int, 8> k{};
+        for (int o = 0; o < 8; ++o) {
+            int slot = -1;
+            t.kind(L + 1, 2 * px + (o & 1), 2 * py + ((o >> 1) & 1), 2 * pz + ((o >> 2) & 1), &slot);
+            k[o] = slot;
+            gone.insert(slot);
+        }
+        kids[pk] = k;
+        nt.interior.erase(pk);
+    }
+    nt.leaves.clear();
+    for (int s = 0; s < (int)t.leaves.size(); ++s)
+        if (!gone.count(s)) nt.leaves.push_back(t.leaves[s]);
+    for (uint64_t pk : parents)
+        nt.leaves.push_back(BlockRec{(int)(pk >> 48), {(int)((pk >> 32) & 0xffff), (int)((pk >> 16) & 0xffff),
+                                                       (int)(pk & 0xffff)}});
+    nt.rebuild_index();
+    std::vector<Remap> rm(nt.leaves.size());
+    for (int s = 0; s < (int)nt.leaves.size(); ++s) {
+        const uint64_t k = Octree::key(nt.leaves[s]);
+        Remap r{};
+        auto it = t.leaf_slot.find(k);
+        if (it != t.leaf_slot.end()) {
+            r.kind = 0;
+            r.src = it->second;
+        } else {
+            r.kind = 2;
+            const auto& kk = kids.at(k);
+            for (int o = 0; o < 8; ++o) r.kids[o] = kk[o];
+        }
+        rm[s] = r;
+    }
+    return c.real_kind == GH_REAL_F32 ? apply_new_tree<float>(c, nt, rm, {})
+                                      : apply_new_tree<double>(c, nt, rm, {});
+}
+
+}  // namespace gh
+
+using namespace gh;
+
+extern "C" {
+
+int gh_grid_configure(gh_ctx* ctx, int wavelet_order, int max_level, int min_level) {
+    Ctx* c = as_ctx(ctx);
+    if (!c) return set_error(GH_EINVAL, "null context");
+    if ((wavelet_order != 2 && wavelet_order != 4) || max_level < 0 || max_level > 9 || min_level < 0 ||
+        min_level > max_level)
+        return set_error(GH_EINVAL, "gh_grid_configure: order 2|4, 0 <= min_level <= max_level <= 9");
+    c->tree.order = wavelet_order;
+    c->tree.max_level = max_level;
+    c->tree.min_level = min_level;
+    return GH_OK;
+}
+
+int gh_grid_init_leaves(gh_ctx* ctx, int nleaves, const int* level, const int* index) {
+    Ctx* c = as_ctx(ctx);
+    if (!c || nleaves <= 0 || !level || !index) return set_error(GH_EINVAL, "gh_grid_init_leaves: bad argument");
+    GH_HIP_CHECK(hipSetDevice(c->device));
+    Octree t = c->tree;
+    t.leaves.clear();
+    t.interior.clear();
+    double vol = 0;
+    for (int i = 0; i < nleaves; ++i) {
+        const int L = level[i];
+        if (L < 0 || L > 9) return set_error(GH_EINVAL, "gh_grid_init_leaves: level out of range");
+        for (int a = 0; a < 3; ++a)
+            if (index[3 * i + a] < 0 || index[3 * i + a] >= (1 << L))
+                return set_error(GH_EINVAL, "gh_grid_init_leaves: index out of range");
+        t.leaves.push_back(BlockRec{L, {index[3 * i], index[3 * i + 1], index[3 * i + 2]}});
+        vol += std::ldexp(1.0, -3 * L);
+        for (int l = L - 1; l >= 0; --l)
+            t.interior.insert(Octree::key(l, index[3 * i] >> (L - l), index[3 * i + 1] >> (L - l),
+                                          index[3 * i + 2] >> (L - l)));
+    }
+    t.rebuild_index();
+    if ((int)t.leaf_slot.size() != nleaves) return set_error(GH_EINVAL, "gh_grid_init_leaves: duplicate leaf");
+    for (auto& b : t.leaves)
+        if (t.interior.count(Octree::key(b)))
+            return set_error(GH_EINVAL, "gh_grid_init_leaves: a leaf contains another leaf");
+    if (vol != 1.0) return set_error(GH_EINVAL, "gh_grid_init_leaves: leaves do not tile the domain");
+    c->tree = t;
+    int rc = apply_topology(*c);
+    if (rc) return rc;
+    return c->alloc_fields();
+}
+
+int gh_fwt_details(gh_ctx* ctx, double* detail) {
+    Ctx* c = as_ctx(ctx);
+    if (!c || !detail) return set_error(GH_EINVAL, "null argument");
+    GH_HIP_CHECK(hipSetDevice(c->device));
+    std::vector<int> all(c->topo.nblocks());
+    for (size_t i = 0; i < all.size(); ++i) all[i] = (int)i;
+    std::vector<double> d;
+    int rc = fwt_details(*c, all, d);
+    if (rc) return rc;
+    std::copy(d.begin(), d.end(), detail);
+    return GH_OK;
+}
+
+// Science::AutomaticRefinement<0,0>(grid, fwt, tol, maxLevel, loops) (MRAGScienceCore.h:42-108)
+int gh_refine(gh_ctx* ctx, double tol, int max_loops, int* nrefined) {
+    Ctx* c = as_ctx(ctx);
+    if (!c) return set_error(GH_EINVAL, "null context");
+    GH_HIP_CHECK(hipSetDevice(c->device));
+    std::unordered_set<uint64_t> nice;
+    int loops = 0, total = 0;
+    while (true) {
+        std::vector<int> cand;
+        for (int s = 0; s < c->topo.nblocks(); ++s) {
+            const BlockRec& b = c->tree.leaves[s];
+            if (c->tree.max_level >= 0 && c->tree.max_level <= b.level) continue;
+            if (!nice.count(Octree::key(b))) cand.push_back(s);
+        }
+        if (cand.empty()) break;
+        std::vector<double> det;
+        int rc = fwt_details(*c, cand, det);
+        if (rc) return rc;
+        std::vector<char> flagged(c->topo.nblocks(), 0);
+        int nflag = 0;
+        for (size_t i = 0; i < cand.size(); ++i) {
+            if (det[i] >= tol) {
+                flagged[cand[i]] = 1;
+                ++nflag;
+            } else {
+                nice.insert(Octree::key(c->tree.leaves[cand[i]]));
+            }
+        }
+        if (nflag == 0) break;
+        int nr = 0;
+        if ((rc = grid_refine(*c, flagged, &nr))) return rc;
+        total += nr;
+        ++loops;
+        if (max_loops > 0 && loops >= max_loops) break;
+    }
+    if (nrefined) *nrefined = total;
+    return GH_OK;
+}
+
+// Science::AutomaticCompression<0,0>(grid, fwt, tol, loops) (MRAGScienceCore.h:117-177)
+int gh_compress(gh_ctx* ctx, double tol, int max_loops, int* ncollapsed) {
+    Ctx* c = as_ctx(ctx);
+    if (!c) return set_error(GH_EINVAL, "null context");
+    GH_HIP_CHECK(hipSetDevice(c->device));
+    std::unordered_set<uint64_t> nice;
+    int loops = 0, total = 0;
+    while (true) {
+        std::vector<int> cand;
+        for (int s = 0; s < c->topo.nblocks(); ++s)
+            if (!nice.count(Octree::key(c->tree.leaves[s]))) cand.push_back(s);
+        std::vector<double> det;
+        int rc = fwt_details(*c, cand, det);
+        if (rc) return rc;
+        std::vector<char> flagged(c->topo.nblocks(), 0);
+        int nflag = 0;
+        for (size_t i = 0; i < cand.size(); ++i) {
+            if (det[i] < tol) {
+                flagged[cand[i]] = 1;
+                ++nflag;
+            } else {
+                nice.insert(Octree::key(c->tree.leaves[cand[i]]));
+            }
+        }
+        if (nflag == 0) break;
+        int nc = 0;
+        if ((rc = grid_compress(*c, flagged, &nc))) return rc;
+        if (nc == 0) break;
+        total += nc;
+        ++loops;
+        if (max_loops > 0 && loops >= max_loops) break;
+    }
+    if (ncollapsed) *ncollapsed = total;
+    return GH_OK;
+}
+
+int gh_set_field_blocks(gh_ctx* ctx, int field, const void* values) {
+    Ctx* c = as_ctx(ctx);
+    if (!c || !values) return set_error(GH_EINVAL, "null argument");
+    DevBuf* b = field == GH_FIELD_PHI ? &c->phi[c->cur]
+              : field == GH_FIELD_PW  ? &c->pw
+              : field == GH_FIELD_PG  ? &c->pg
+              : field == GH_FIELD_PCSF ? &c->pcsf
+              : field == GH_FIELD_DPHIDT ? &c->dphidt : nullptr;
+    if (!b) return set_error(GH_EINVAL, "gh_set_field_blocks: bad field");
+    GH_HIP_CHECK(hipSetDevice(c->device));
+    GH_HIP_CHECK(hipMemcpyAsync(b->p, values, c->ncells() * c->real_size, hipMemcpyHostToDevice, c->stream));
+    GH_HIP_CHECK(hipStreamSynchronize(c->stream));
+    if (field == GH_FIELD_PW || field == GH_FIELD_PG) c->prepared = false;
+    return GH_OK;
+}
+
+int gh_get_field_blocks(gh_ctx* ctx, int field, void* values) {
+    Ctx* c = as_ctx(ctx);
+    if (!c || !values) return set_error(GH_EINVAL, "null argument");
+    DevBuf* b = field == GH_FIELD_PHI ? &c->phi[c->cur]
+              : field == GH_FIELD_PW  ? &c->pw
+              : field == GH_FIELD_PG  ? &c->pg
+              : field == GH_FIELD_PCSF ? &c->pcsf
+              : field == GH_FIELD_DPHIDT ? &c->dphidt : nullptr;
+    if (!b) return set_error(GH_EINVAL, "gh_get_field_blocks: bad field");
+    GH_HIP_CHECK(hipSetDevice(c->device));
+    GH_HIP_CHECK(hipMemcpyAsync(values, b->p, c->ncells() * c->real_size, hipMemcpyDeviceToHost, c->stream));
+    GH_HIP_CHECK(hipStreamSynchronize(c->stream));
+    return GH_OK;
+}
+
+int gh_uq_resample(gh_ctx* ctx, int gpd, double* out) {
+    Ctx* c = as_ctx(ctx);
+    if (!c || !out || gpd <= 0 || gpd > 4096) return set_error(GH_EINVAL, "gh_uq_resample: bad argument");
+    GH_HIP_CHECK(hipSetDevice(c->device));
+    const size_t n = (size_t)gpd * gpd * gpd;
+    DevBuf d, blk;
+    int rc;
+    if ((rc = d.ensure(n * 8)) || (rc = blk.ensure(c->topo.blocks.size() * sizeof(BlockRec)))) return rc;
+    GH_HIP_CHECK(hipMemsetAsync(d.p, 0, n * 8, c->stream));
+    GH_HIP_CHECK(hipMemcpyAsync(blk.p, c->topo.blocks.data(), c->topo.blocks.size() * sizeof(BlockRec),
+                                hipMemcpyHostToDevice, c->stream));
+    if (c->real_kind == GH_REAL_F32)
+        hipLaunchKernelGGL(uq_resample_kernel<float>, dim3(c->topo.nblocks()), dim3(512), 0, c->stream,
+                           c->topo.nblocks(), blk.as<BlockRec>(), c->phi[c->cur].as<float>(), gpd, d.as<double>());
+    else
+        hipLaunchKernelGGL(uq_resample_kernel<double>, dim3(c->topo.nblocks()), dim3(512), 0, c->stream,
+                           c->topo.nblocks(), blk.as<BlockRec>(), c->phi[c->cur].as<double>(), gpd, d.as<double>());
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpyAsync(out, d.p, n * 8, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    d.release();
+    blk.release();
+    if (e != hipSuccess) return set_error(GH_EHIP, std::string("gh_uq_resample: ") + hipGetErrorString(e));
+    return GH_OK;
+}
+
+}  // extern "C"
+

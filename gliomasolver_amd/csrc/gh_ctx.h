@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include "gh_internal.h"
+#include "gh_octree.h"
 
 namespace gh {
 
@@ -31,6 +32,18 @@ struct Ctx {
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     Topology topo;
+    Octree tree;  // host octree the topology is derived from
+
+    // adaptive grid: per-slot level, same-level neighbours in 27 directions,
+    // interpolated face ghosts of the RD stencil (jump faces only)
+    DevBuf dLevel, dNbr27;
+    DevBuf faceEnt, faceSrc, faceW;  // GhostEntry / int32 / Real
+    int nFaceEnt = 0;
+    DevBuf fgPhi, fgTau, fgDm;       // per slot: 6*64 ghost phi, 6*64 ghost tau, 3*64 minus-face D
+    DevBuf fgPw, fgPg;               // per slot: 6*64 ghost p_w, p_g (statics)
+    // FWT / split labs (compact, one per listed slot)
+    DevBuf labBuf, labSlots, labEnt, labSrc, labW;
+    double last_fwt_ms = 0;
 
     DevBuf phi[2];      // ping-pong tumor density
     int cur = 0;        // phi[cur] holds the current state
@@ -67,6 +80,14 @@ int rd_steps(Ctx& c, int64_t nsteps, double dt);
 int rd_step_buffers(Ctx& c, const void* in, void* out, double dt);
 double block_h_of(const Ctx& c);
 int rd_autotune(Ctx& c, double dt, int trial_steps, int* best);
+
+// adaptive grid (gh_adapt.hip)
+int apply_topology(Ctx& c);
+int rd_prepare_ghosted(Ctx& c, double Dw, double Dg);
+int rd_step_ghosted(Ctx& c, const void* in, void* out, double dt, bool fused);
+int fwt_details(Ctx& c, const std::vector<int>& slots, std::vector<double>& det);
+int grid_refine(Ctx& c, const std::vector<char>& flagged, int* nrefined);
+int grid_compress(Ctx& c, const std::vector<char>& flagged, int* ncollapsed);
 
 // likelihood helpers (gh_lik.hip): -logL of the current phi; dJ/du of a state buffer
 int lik_eval_current(Ctx& c, const double* params, int stepPET, int stepTi, int use_roi, double* out,

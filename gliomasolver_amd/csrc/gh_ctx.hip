@@ -153,30 +153,12 @@ static int init_uniform(Ctx* c, int bpd) {
         return set_error(GH_EINVAL, "bpd must be a power of two in [1,64]");
     int level = 0;
     while ((1 << level) < bpd) ++level;
-    Topology t;
-    t.uniform_bpd = bpd;
-    t.blocks.resize((size_t)bpd * bpd * bpd);
-    t.nbr.resize(6 * t.blocks.size());
-    for (int bz = 0; bz < bpd; ++bz)
-        for (int by = 0; by < bpd; ++by)
-            for (int bx = 0; bx < bpd; ++bx) {
-                const size_t s = bx + (size_t)bpd * (by + (size_t)bpd * bz);
-                t.blocks[s] = BlockRec{level, {bx, by, bz}};
-                auto id = [bpd](int x, int y, int z) {
-                    x = (x + bpd) % bpd; y = (y + bpd) % bpd; z = (z + bpd) % bpd;
-                    return (int32_t)(x + bpd * (y + bpd * z));
-                };
-                t.nbr[6 * s + FXM] = id(bx - 1, by, bz);
-                t.nbr[6 * s + FXP] = id(bx + 1, by, bz);
-                t.nbr[6 * s + FYM] = id(bx, by - 1, bz);
-                t.nbr[6 * s + FYP] = id(bx, by + 1, bz);
-                t.nbr[6 * s + FZM] = id(bx, by, bz - 1);
-                t.nbr[6 * s + FZP] = id(bx, by, bz + 1);
-            }
-    c->topo = std::move(t);
-    int rc = c->alloc_fields();
+    // slot order (level, z, y, x) = bx + bpd*(by + bpd*bz) on one level
+    c->tree.init_uniform(level);
+    if (c->tree.max_level < level) c->tree.max_level = level;
+    int rc = apply_topology(*c);
     if (rc) return rc;
-    return c->upload_topology();
+    return c->alloc_fields();
 }
 
 template <typename Real>

@@ -1,0 +1,258 @@
+// gh_octree.cpp -- host octree of leaf blocks: topology, neighbourhood,
+// refinement / compression plans, ghost recursion.  See gh_octree.h.
+#include "gh_octree.h"
+
+#include <algorithm>
+#include <tuple>
+
+namespace gh {
+
+static inline int wrapi(int64_t v, int64_t n) {
+    int64_t r = v % n;
+    return (int)(r < 0 ? r + n : r);
+}
+
+Octree::Kind Octree::kind(int L, int x, int y, int z, int* slot) const {
+    const int n = 1 << L;
+    x = wrapi(x, n);
+    y = wrapi(y, n);
+    z = wrapi(z, n);
+    const uint64_t k = key(L, x, y, z);
+    auto it = leaf_slot.find(k);
+    if (it != leaf_slot.end()) {
+        if (slot) *slot = it->second;
+        return LEAF;
+    }
+    if (interior.count(k)) return FINER;
+    return COARSER;
+}
+
+void Octree::init_uniform(int level) {
+    leaves.clear();
+    interior.clear();
+    const int n = 1 << level;
+    for (int z = 0; z < n; ++z)
+        for (int y = 0; y < n; ++y)
+            for (int x = 0; x < n; ++x) leaves.push_back(BlockRec{level, {x, y, z}});
+    for (int l = 0; l < level; ++l) {
+        const int m = 1 << l;
+        for (int z = 0; z < m; ++z)
+            for (int y = 0; y < m; ++y)
+                for (int x = 0; x < m; ++x) interior.insert(key(l, x, y, z));
+    }
+    rebuild_index();
+}
+
+// canonical slot order: (level, z, y, x) ascending
+void Octree::rebuild_index() {
+    std::sort(leaves.begin(), leaves.end(), [](const BlockRec& a, const BlockRec& b) {
+        return std::make_tuple(a.level, a.index[2], a.index[1], a.index[0]) <
+               std::make_tuple(b.level, b.index[2], b.index[1], b.index[0]);
+    });
+    leaf_slot.clear();
+    for (size_t s = 0; s < leaves.size(); ++s) leaf_slot[key(leaves[s])] = (int)s;
+}
+
+int Octree::levels_min() const {
+    int m = MAXL;
+    for (auto& b : leaves) m = std::min(m, b.level);
+    return m;
+}
+int Octree::levels_max() const {
+    int m = 0;
+    for (auto& b : leaves) m = std::max(m, b.level);
+    return m;
+}
+
+// leaves of the subtree at (L, P) that touch the face/edge/corner facing
+// direction -d (the side adjacent to the querying block)
+static void touching_children(const Octree& t, int L, const int P[3], const int d[3], bool wrapped,
+                              std::vector<Octree::Nbr>& out) {
+    int slot;
+    const Octree::Kind k = t.kind(L, P[0], P[1], P[2], &slot);
+    if (k == Octree::LEAF) {
+        out.push_back({slot, wrapped});
+        return;
+    }
+    if (k != Octree::FINER) return;
+    for (int o = 0; o < 8; ++o) {
+        const int off[3] = {o & 1, (o >> 1) & 1, (o >> 2) & 1};
+        bool ok = true;
+        for (int a = 0; a < 3; ++a) {
+            if (d[a] < 0 && off[a] != 1) ok = false;  // child on the max side faces the block
+            if (d[a] > 0 && off[a] != 0) ok = false;
+        }
+        if (!ok) continue;
+        const int C[3] = {2 * P[0] + off[0], 2 * P[1] + off[1], 2 * P[2] + off[2]};
+        touching_children(t, L + 1, C, d, wrapped, out);
+    }
+}
+
+std::vector<Octree::Nbr> Octree::neighbours(int s) const {
+    const BlockRec& b = leaves[s];
+    const int L = b.level, n = 1 << L;
+    std::vector<Nbr> out;
+    for (int code = 0; code < 27; ++code) {
+        if (code == 13) continue;
+        const int d[3] = {code % 3 - 1, (code / 3) % 3 - 1, code / 9 - 1};
+        int P[3];
+        bool wrapped = false;
+        for (int a = 0; a < 3; ++a) {
+            P[a] = b.index[a] + d[a];
+            if (P[a] < 0 || P[a] >= n) wrapped = true;
+            P[a] = wrapi(P[a], n);
+        }
+        int slot;
+        const Kind k = kind(L, P[0], P[1], P[2], &slot);
+        if (k == LEAF) {
+            out.push_back({slot, wrapped});
+        } else if (k == FINER) {
+            touching_children(*this, L, P, d, wrapped, out);
+        } else {
+            for (int l = L - 1; l >= 0; --l) {
+                const int sh = L - l;
+                if (kind(l, P[0] >> sh, P[1] >> sh, P[2] >> sh, &slot) == LEAF) {
+                    // a coarser leaf reached through the wrap is a periodic image
+                    // unless its box also touches the block directly
+                    out.push_back({slot, wrapped});
+                    break;
+                }
+            }
+        }
+    }
+    std::sort(out.begin(), out.end(), [](const Nbr& a, const Nbr& b) {
+        return a.slot != b.slot ? a.slot < b.slot : a.wrapped < b.wrapped;
+    });
+    out.erase(std::unique(out.begin(), out.end(),
+                          [](const Nbr& a, const Nbr& b) { return a.slot == b.slot && a.wrapped == b.wrapped; }),
+              out.end());
+    return out;
+}
+
+void Octree::same_level_neighbours(int s, int out[27]) const {
+    const BlockRec& b = leaves[s];
+    for (int code = 0; code < 27; ++code) {
+        const int d[3] = {code % 3 - 1, (code / 3) % 3 - 1, code / 9 - 1};
+        int slot = -1;
+        if (kind(b.level, b.index[0] + d[0], b.index[1] + d[1], b.index[2] + d[2], &slot) != LEAF) slot = -1;
+        out[code] = slot;
+    }
+}
+
+// Interp wavelet synthesis taps Hs at the fine point q: even q -> the coarse
+// point q/2 with weight 1; odd q -> neighbours with Hs(+-1) (and Hs(+-3))
+// (MRAG/MRAGcore/MRAGWavelets_StaticData.cpp:13-21, 31-39)
+static int synth_taps(int order, int64_t q, int64_t s[4], double w[4]) {
+    if ((q & 1) == 0) {
+        s[0] = q / 2;
+        w[0] = 1.0;
+        return 1;
+    }
+    if (order == 2) {
+        s[0] = (q - 1) / 2; w[0] = 0.5;
+        s[1] = (q + 1) / 2; w[1] = 0.5;
+        return 2;
+    }
+    s[0] = (q - 3) / 2; w[0] = -1.0 / 16;
+    s[1] = (q - 1) / 2; w[1] = 9.0 / 16;
+    s[2] = (q + 1) / 2; w[2] = 9.0 / 16;
+    s[3] = (q + 3) / 2; w[3] = -1.0 / 16;
+    return 4;
+}
+
+void Octree::value_terms(int L, const int64_t q_in[3], double w, std::vector<GhostTerm>& out) const {
+    const int64_t n = (int64_t)BS << L;
+    int64_t q[3];
+    for (int a = 0; a < 3; ++a) q[a] = wrapi(q_in[a], n);
+    int slot;
+    const Kind k = kind(L, (int)(q[0] >> 3), (int)(q[1] >> 3), (int)(q[2] >> 3), &slot);
+    if (k == LEAF) {
+        out.push_back({slot * BCELLS + (int)((q[0] & 7) + 8 * (q[1] & 7) + 64 * (q[2] & 7)), w});
+        return;
+    }
+    if (k == FINER) {  // Ha = {1}: the coincident fine vertex
+        const int64_t f[3] = {2 * q[0], 2 * q[1], 2 * q[2]};
+        value_terms(L + 1, f, w, out);
+        return;
+    }
+    int64_t s[3][4];
+    double ws[3][4];
+    int nt[3];
+    for (int a = 0; a < 3; ++a) nt[a] = synth_taps(order, q[a], s[a], ws[a]);
+    for (int kz = 0; kz < nt[2]; ++kz)
+        for (int ky = 0; ky < nt[1]; ++ky)
+            for (int kx = 0; kx < nt[0]; ++kx) {
+                const int64_t c[3] = {s[0][kx], s[1][ky], s[2][kz]};
+                value_terms(L - 1, c, w * (ws[2][kz] * ws[1][ky] * ws[0][kx]), out);
+            }
+}
+
+std::vector<int> Octree::refinement_victims(const std::vector<char>& flagged) const {
+    const int nl = (int)leaves.size();
+    std::vector<char> flag(flagged.begin(), flagged.end());
+    flag.resize(nl, 0);
+    std::vector<std::vector<Nbr>> nb(nl);
+    for (int s = 0; s < nl; ++s) nb[s] = neighbours(s);
+    // space extension: a leaf with a finer non-periodic-image neighbour is refined
+    // (MRAGRefiner_SpaceExtension.h:35-51)
+    for (int s = 0; s < nl; ++s)
+        for (const Nbr& n : nb[s])
+            if (!n.wrapped && leaves[n.slot].level > leaves[s].level) {
+                flag[s] = 1;
+                break;
+            }
+    // nothing at the maximum level is refined (:54-63)
+    for (int s = 0; s < nl; ++s)
+        if (leaves[s].level >= max_level) flag[s] = 0;
+    // chain reaction: neighbours that would end up more than one level coarser
+    // than a refined block are refined too (MRAGRefiner.h:65-152)
+    std::set<int> victims;
+    for (int s = 0; s < nl; ++s)
+        if (flag[s]) victims.insert(s);
+    std::set<int> front = victims;
+    while (!front.empty()) {
+        std::set<int> next;
+        for (int v : front) {
+            const int new_level = leaves[v].level + 1;
+            for (const Nbr& n : nb[v])
+                if (std::abs(leaves[n.slot].level - new_level) > 1 && !victims.count(n.slot)) {
+                    victims.insert(n.slot);
+                    next.insert(n.slot);
+                }
+        }
+        front.swap(next);
+    }
+    return std::vector<int>(victims.begin(), victims.end());
+}
+
+std::vector<uint64_t> Octree::collapse_parents(const std::vector<char>& flagged) const {
+    std::vector<uint64_t> out;
+    // candidate parents: interior nodes whose 8 children are leaves
+    std::set<uint64_t> cand;
+    for (const BlockRec& b : leaves)
+        if (b.level > 0) cand.insert(key(b.level - 1, b.index[0] >> 1, b.index[1] >> 1, b.index[2] >> 1));
+    for (uint64_t pk : cand) {
+        const int L = (int)(pk >> 48);
+        if (L < min_level) continue;
+        const int px = (int)((pk >> 32) & 0xffff), py = (int)((pk >> 16) & 0xffff), pz = (int)(pk & 0xffff);
+        bool ok = true;
+        int kids[8];
+        for (int o = 0; o < 8 && ok; ++o) {
+            int slot;
+            if (kind(L + 1, 2 * px + (o & 1), 2 * py + ((o >> 1) & 1), 2 * pz + ((o >> 2) & 1), &slot) != LEAF) ok = false;
+            else if (!(slot < (int)flagged.size() && flagged[slot])) ok = false;
+            else kids[o] = slot;
+        }
+        // every neighbour of every child within one level of the parent (MRAGCompressor.h:95-107)
+        for (int o = 0; o < 8 && ok; ++o)
+            for (const Nbr& n : neighbours(kids[o]))
+                if (std::abs(leaves[n.slot].level - L) > 1) {
+                    ok = false;
+                    break;
+                }
+        if (ok) out.push_back(pk);
+    }
+    return out;
+}
+
+}  // namespace gh

@@ -1,0 +1,92 @@
+// gh_octree.h -- host-side block octree of the wavelet-adaptive grid:
+// topology, neighbourhood, refinement/compression plans and the ghost
+// instruction tables consumed by the HIP kernels.
+//
+// Reference: MRAG/MRAGcore/MRAGrid.inl (hierarchy, periodic neighbourhood
+// :840-1149, refine :247-355, compress :383-486), MRAG/MRAGcore/MRAGRefiner.h
+// (:43-197), MRAG/MRAGscience/MRAGRefiner_SpaceExtension.h (:25-67),
+// MRAG/MRAGcore/MRAGCompressor.h (:67-146), MRAG/MRAGscience/MRAGScienceCore.h
+// (:42-177), MRAG/MRAGcore/MRAG_BBInfoCreator.inl (ghost instructions).
+#pragma once
+#include <cstdint>
+#include <map>
+#include <set>
+#include <unordered_map>
+#include <unordered_set>
+#include <vector>
+
+#include "gh_internal.h"
+
+namespace gh {
+
+// one contribution to a ghost value: value(src cell) * w
+struct GhostTerm {
+    int32_t src;  // slot * 512 + cell
+    double w;
+};
+
+// a ghost whose value needs interpolation (neighbour not at the same level)
+struct GhostEntry {
+    int32_t dst;    // index into the output buffer
+    int32_t start;  // first term
+    int32_t count;  // number of terms
+};
+
+struct Octree {
+    static constexpr int MAXL = 10;
+    int order = 4;         // interpolating wavelet order: 2 or 4 (MRAGWavelets_Interp{2nd,4th}Order)
+    int max_level = 4;     // Refiner_SpaceExtension maxLevel
+    int min_level = 0;     // compression floor (0 = MRAG: no floor)
+    std::vector<BlockRec> leaves;                 // slot order
+    std::unordered_map<uint64_t, int> leaf_slot;  // key -> slot
+    std::unordered_set<uint64_t> interior;        // keys of nodes that have children
+
+    static uint64_t key(int L, int x, int y, int z) {
+        return ((uint64_t)L << 48) | ((uint64_t)(uint32_t)x << 32) | ((uint64_t)(uint32_t)y << 16) | (uint64_t)(uint32_t)z;
+    }
+    static uint64_t key(const BlockRec& b) { return key(b.level, b.index[0], b.index[1], b.index[2]); }
+
+    enum Kind { LEAF = 0, FINER = 1, COARSER = 2 };
+    // what covers block position (L, x, y, z) (wrapped periodically): a leaf, a
+    // subdivided node, or nothing (a coarser leaf covers it)
+    Kind kind(int L, int x, int y, int z, int* slot = nullptr) const;
+
+    void init_uniform(int level);
+    void rebuild_index();  // canonical slot order (level, z, y, x) + maps
+    int levels_min() const;
+    int levels_max() const;
+    bool is_uniform() const { return levels_min() == levels_max(); }
+
+    // neighbourhood of a leaf: leaves whose finest-level boxes touch its box,
+    // with periodic images (MRAGrid.inl:840-1149, MRAGridHelpers.h:77-84).
+    // wrapped = reached only through the periodic boundary (a "ghost node").
+    struct Nbr {
+        int slot;
+        bool wrapped;
+    };
+    std::vector<Nbr> neighbours(int slot) const;
+    // same-level neighbour slot in each of the 27 directions (-1 if none)
+    void same_level_neighbours(int slot, int out[27]) const;
+
+    // ghost value at level-L lattice point q as a linear combination of leaf
+    // cell values: same-level leaf -> the cell; finer region -> injection
+    // (Ha = {1}); coarser region -> Hs synthesis from level L-1.
+    void value_terms(int L, const int64_t q[3], double w, std::vector<GhostTerm>& out) const;
+
+    // --- plans (return the new leaf list; data movement is done by the caller)
+    // Science::AutomaticRefinement's refine(set): space extension + 2:1 chain reaction
+    std::vector<int> refinement_victims(const std::vector<char>& flagged) const;
+    // Compressor::createPlan: parents whose 8 children are flagged leaves and whose
+    // children's neighbours are all within one level of the parent
+    std::vector<uint64_t> collapse_parents(const std::vector<char>& flagged) const;
+};
+
+// how a leaf of the new grid is produced from the old one
+struct Remap {
+    int kind;      // 0 copy, 1 split (child of an old leaf), 2 collapse (8 old leaves)
+    int src;       // copy / split: old slot
+    int octant;    // split: child position (x | y<<1 | z<<2)
+    int kids[8];   // collapse: old slots of the children by octant
+};
+
+}  // namespace gh

@@ -249,11 +249,20 @@ int step_t(Ctx& c, Real* out) {
 }  // namespace
 
 int rd_prepare(Ctx& c, double Dw, double Dg, double rho) {
-    if (c.topo.uniform_bpd <= 0) return set_error(GH_ESTATE, "rd_prepare: grid has blocks on several levels");
+    if (c.topo.uniform_bpd <= 0) {  // blocks on several levels: ghosts across level jumps
+        int rc = rd_prepare_ghosted(c, Dw, Dg);
+        if (rc) return rc;
+        c.Dw = Dw;
+        c.Dg = Dg;
+        c.rho = rho;
+        c.prepared = true;
+        return GH_OK;
+    }
     return c.real_kind == GH_REAL_F32 ? prepare_t<float>(c, Dw, Dg, rho) : prepare_t<double>(c, Dw, Dg, rho);
 }
 
 int rd_rhs(Ctx& c) {
+    if (c.topo.uniform_bpd <= 0) return rd_step_ghosted(c, c.phi[c.cur].p, c.dphidt.p, 0.0, false);
     return c.real_kind == GH_REAL_F32 ? step_t<float, false>(c, c.dphidt.as<float>())
                                       : step_t<double, false>(c, c.dphidt.as<double>());
 }
@@ -276,6 +285,10 @@ int rd_update(Ctx& c, double dt) {
 // fastest.
 int rd_autotune(Ctx& c, double dt, int trial_steps, int* best_out) {
     static const int cand[3] = {1, 2, 4};
+    if (c.topo.uniform_bpd <= 0) {  // the multi-level kernel has a single tile
+        if (best_out) *best_out = c.rd_bpw;
+        return GH_OK;
+    }
     const size_t nb = c.ncells() * c.real_size;
     GH_HIP_CHECK(hipMemcpyAsync(c.dphidt.p, c.phi[c.cur].p, nb, hipMemcpyDeviceToDevice, c.stream));
     const int cur0 = c.cur;
@@ -310,9 +323,11 @@ int rd_step_buffers(Ctx& c, const void* in, void* out, double dt) {
 
 int rd_steps(Ctx& c, int64_t nsteps, double dt) {
     c.dt_last = dt;
+    const bool ghosted = c.topo.uniform_bpd <= 0;
     for (int64_t k = 0; k < nsteps; ++k) {
-        int rc = c.real_kind == GH_REAL_F32 ? step_t<float, true>(c, c.phi[1 - c.cur].as<float>())
-                                            : step_t<double, true>(c, c.phi[1 - c.cur].as<double>());
+        int rc = ghosted ? rd_step_ghosted(c, c.phi[c.cur].p, c.phi[1 - c.cur].p, dt, true)
+                 : c.real_kind == GH_REAL_F32 ? step_t<float, true>(c, c.phi[1 - c.cur].as<float>())
+                                              : step_t<double, true>(c, c.phi[1 - c.cur].as<double>());
         if (rc) return rc;
         c.cur = 1 - c.cur;
     }

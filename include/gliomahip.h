@@ -51,6 +51,36 @@ int gh_grid_init_uniform(gh_ctx* ctx, int bpd);
 int gh_grid_nblocks(gh_ctx* ctx, int* nblocks);
 int gh_grid_blocks(gh_ctx* ctx, int* level, int* index);
 
+/* ---- wavelet-adaptive grid ---------------------------------------------------
+ * gh_grid_configure: wavelet order (2: Wavelets_Interp2ndOrder, 4:
+ *   Wavelets_Interp4thOrder, MRAG/MRAGcore/MRAGWavelets_StaticData.cpp:13-39),
+ *   maximum level (Refiner_SpaceExtension(resJump, maxLevel)) and a compression
+ *   floor min_level (0 = none, the reference's behaviour).
+ * gh_grid_init_leaves: any 2:1-free tiling by leaf blocks (level[i],
+ *   index[3i..3i+2]); fields are zeroed.
+ * gh_fwt_details: per block max |detail| of the projector phi
+ *   (BlockFWT::multichannel_fwt + FWTReport::getOverAll_DetailMaxMag,
+ *   MRAG/MRAGcore/MRAGBlockFWT.h:109-120,219-298,447-478).
+ * gh_refine: Science::AutomaticRefinement<0,0>(grid, fwt, tol, maxLevel,
+ *   max_loops) (MRAG/MRAGscience/MRAGScienceCore.h:42-108) with
+ *   Refiner_SpaceExtension; *nrefined = refined blocks.
+ * gh_compress: Science::AutomaticCompression<0,0>(grid, fwt, tol, max_loops)
+ *   (:117-177); *ncollapsed = collapsed parents.  max_loops <= 0: until no change.
+ * Both move phi, p_w, p_g, p_csf (split: Hs synthesis, MRAGBlockSplitter.h:24-99;
+ * collapse: Ha injection, MRAGBlockCollapser.h:24-85) and recompute the
+ * per-solve constants if gh_rd_prepare was called.
+ * gh_set/get_field_blocks: block-major arrays in gh_grid_blocks() order.
+ * gh_uq_resample: _dumpUQoutput (Glioma/Glioma_ReactionDiffusion.cpp:208-267),
+ *   phi onto the uniform gpd^3 lattice (x-fastest, FP64). */
+int gh_grid_configure(gh_ctx* ctx, int wavelet_order, int max_level, int min_level);
+int gh_grid_init_leaves(gh_ctx* ctx, int nleaves, const int* level, const int* index);
+int gh_fwt_details(gh_ctx* ctx, double* detail);
+int gh_refine(gh_ctx* ctx, double tol, int max_loops, int* nrefined);
+int gh_compress(gh_ctx* ctx, double tol, int max_loops, int* ncollapsed);
+int gh_set_field_blocks(gh_ctx* ctx, int field, const void* values);
+int gh_get_field_blocks(gh_ctx* ctx, int field, void* values);
+int gh_uq_resample(gh_ctx* ctx, int gpd, double* out);
+
 /* ---- fields ----------------------------------------------------------------
  * gh_set_anatomy: Glioma_ReactionDiffusion::_ic anatomy part
  * (Glioma/Glioma_ReactionDiffusion.cpp:79-152): n^3 GM/WM/CSF fractions as
