@@ -79,6 +79,38 @@ def rd_thresholds(real, p_w: np.ndarray, p_g: np.ndarray):
     return tau, sig
 
 
+def canonical_leaves(rows) -> np.ndarray:
+    """Leaf rows (level, ix, iy, iz) sorted into the canonical (level, iz, iy, ix) order."""
+    r = np.asarray(rows, np.int32).reshape(-1, 4)
+    return np.ascontiguousarray(r[np.lexsort((r[:, 1], r[:, 2], r[:, 3], r[:, 0]))])
+
+
+def plan_neighbours(rows, leaf: int):
+    r = canonical_leaves(rows)
+    out = np.zeros((128, 2), np.int32)
+    n = C.c_int()
+    check(load().gh_plan_neighbours(len(r), _ptr(r), leaf, C.byref(n), _ptr(out)))
+    return [(int(a), bool(b)) for a, b in out[:n.value]]
+
+
+def plan_refine(rows, flagged, max_level: int):
+    r = canonical_leaves(rows)
+    f = np.ascontiguousarray(flagged, np.uint8)
+    out = np.zeros(len(r), np.int32)
+    n = C.c_int()
+    check(load().gh_plan_refine(len(r), _ptr(r), _ptr(f), max_level, C.byref(n), _ptr(out)))
+    return sorted(int(v) for v in out[:n.value])
+
+
+def plan_compress(rows, flagged, min_level: int = 0):
+    r = canonical_leaves(rows)
+    f = np.ascontiguousarray(flagged, np.uint8)
+    out = np.zeros((len(r) // 8 + 1, 4), np.int32)
+    n = C.c_int()
+    check(load().gh_plan_compress(len(r), _ptr(r), _ptr(f), min_level, C.byref(n), _ptr(out)))
+    return sorted(tuple(int(v) for v in row) for row in out[:n.value])
+
+
 class Context:
     """One device context (gh_ctx): grid topology plus block-major fields in HBM."""
 

@@ -44,6 +44,9 @@ SIGNATURES = {
     "gh_set_field_blocks": (_i, [_vp, _i, _vp]),
     "gh_get_field_blocks": (_i, [_vp, _i, _vp]),
     "gh_uq_resample": (_i, [_vp, _i, _vp]),
+    "gh_plan_neighbours": (_i, [_i, _vp, _i, _pi, _vp]),
+    "gh_plan_refine": (_i, [_i, _vp, _vp, _i, _pi, _vp]),
+    "gh_plan_compress": (_i, [_i, _vp, _vp, _i, _pi, _vp]),
     "gh_set_anatomy": (_i, [_vp, _i, _vp, _vp, _vp]),
     "gh_set_field_dense": (_i, [_vp, _i, _i, _vp]),
     "gh_get_field_dense": (_i, [_vp, _i, _i, _vp]),

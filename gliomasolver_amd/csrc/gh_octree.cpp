@@ -255,4 +255,76 @@ std::vector<uint64_t> Octree::collapse_parents(const std::vector<char>& flagged)
     return out;
 }
 
+// octree from leaf rows (level, ix, iy, iz) given in canonical order
+static int tree_from_rows(int nleaves, const int* rows, Octree& t) {
+    if (nleaves <= 0 || !rows) return set_error(GH_EINVAL, "plan: no leaves");
+    t.leaves.clear();
+    t.interior.clear();
+    for (int i = 0; i < nleaves; ++i) {
+        const int L = rows[4 * i];
+        if (L < 0 || L > 9) return set_error(GH_EINVAL, "plan: level out of range");
+        BlockRec b{L, {rows[4 * i + 1], rows[4 * i + 2], rows[4 * i + 3]}};
+        for (int a = 0; a < 3; ++a)
+            if (b.index[a] < 0 || b.index[a] >= (1 << L)) return set_error(GH_EINVAL, "plan: index out of range");
+        t.leaves.push_back(b);
+        for (int l = L - 1; l >= 0; --l)
+            t.interior.insert(Octree::key(l, b.index[0] >> (L - l), b.index[1] >> (L - l), b.index[2] >> (L - l)));
+    }
+    std::vector<BlockRec> given = t.leaves;
+    t.rebuild_index();
+    for (int i = 0; i < nleaves; ++i)
+        if (Octree::key(given[i]) != Octree::key(t.leaves[i]))
+            return set_error(GH_EINVAL, "plan: leaves must be in canonical (level, z, y, x) order");
+    return GH_OK;
+}
+
 }  // namespace gh
+
+using namespace gh;
+
+extern "C" int gh_plan_neighbours(int nleaves, const int* leaves, int leaf, int* nnbr, int* nbr) {
+    Octree t;
+    int rc = tree_from_rows(nleaves, leaves, t);
+    if (rc) return rc;
+    if (leaf < 0 || leaf >= nleaves || !nnbr || !nbr) return set_error(GH_EINVAL, "gh_plan_neighbours: bad argument");
+    std::vector<Octree::Nbr> v = t.neighbours(leaf);
+    *nnbr = (int)v.size();
+    for (size_t i = 0; i < v.size() && i < 128; ++i) {
+        nbr[2 * i] = v[i].slot;
+        nbr[2 * i + 1] = v[i].wrapped;
+    }
+    return GH_OK;
+}
+
+extern "C" int gh_plan_refine(int nleaves, const int* leaves, const unsigned char* flagged, int max_level,
+                              int* nvictims, int* victims) {
+    Octree t;
+    int rc = tree_from_rows(nleaves, leaves, t);
+    if (rc) return rc;
+    if (!flagged || !nvictims || !victims) return set_error(GH_EINVAL, "gh_plan_refine: null argument");
+    t.max_level = max_level;
+    std::vector<char> f(flagged, flagged + nleaves);
+    std::vector<int> v = t.refinement_victims(f);
+    *nvictims = (int)v.size();
+    std::copy(v.begin(), v.end(), victims);
+    return GH_OK;
+}
+
+extern "C" int gh_plan_compress(int nleaves, const int* leaves, const unsigned char* flagged, int min_level,
+                                int* nparents, int* parents) {
+    Octree t;
+    int rc = tree_from_rows(nleaves, leaves, t);
+    if (rc) return rc;
+    if (!flagged || !nparents || !parents) return set_error(GH_EINVAL, "gh_plan_compress: null argument");
+    t.min_level = min_level;
+    std::vector<char> f(flagged, flagged + nleaves);
+    std::vector<uint64_t> p = t.collapse_parents(f);
+    *nparents = (int)p.size();
+    for (size_t i = 0; i < p.size(); ++i) {
+        parents[4 * i] = (int)(p[i] >> 48);
+        parents[4 * i + 1] = (int)((p[i] >> 32) & 0xffff);
+        parents[4 * i + 2] = (int)((p[i] >> 16) & 0xffff);
+        parents[4 * i + 3] = (int)(p[i] & 0xffff);
+    }
+    return GH_OK;
+}

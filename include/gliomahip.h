@@ -80,6 +80,19 @@ int gh_compress(gh_ctx* ctx, double tol, int max_loops, int* ncollapsed);
 int gh_set_field_blocks(gh_ctx* ctx, int field, const void* values);
 int gh_get_field_blocks(gh_ctx* ctx, int field, void* values);
 int gh_uq_resample(gh_ctx* ctx, int gpd, double* out);
+/* host-only plan queries on a leaf list (rows of level, ix, iy, iz in the
+ * canonical order level, iz, iy, ix):
+ * gh_plan_neighbours: leaves touching `leaf` (MRAGrid.inl:840-1149), rows of
+ *   (leaf, wrapped) -- wrapped = a periodic image (capacity 128 rows).
+ * gh_plan_refine: blocks refined for the flags (Refiner_SpaceExtension +
+ *   Refiner::createPlan, MRAGRefiner_SpaceExtension.h:25-67, MRAGRefiner.h:43-197).
+ * gh_plan_compress: parents collapsed for the flags (Compressor::createPlan,
+ *   MRAGCompressor.h:67-146) as rows of (level, ix, iy, iz). */
+int gh_plan_neighbours(int nleaves, const int* leaves, int leaf, int* nnbr, int* nbr);
+int gh_plan_refine(int nleaves, const int* leaves, const unsigned char* flagged, int max_level, int* nvictims,
+                   int* victims);
+int gh_plan_compress(int nleaves, const int* leaves, const unsigned char* flagged, int min_level, int* nparents,
+                     int* parents);
 
 /* ---- fields ----------------------------------------------------------------
  * gh_set_anatomy: Glioma_ReactionDiffusion::_ic anatomy part

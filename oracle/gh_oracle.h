@@ -68,6 +68,21 @@ double gho_tlm_f64(int n, const double* phi0, const double* p_w, const double* p
                    double rho, double h, double T, int64_t N, double sigma, const double* x0, const double* dir,
                    const double* t1c, const double* flair, const double* pet, const double* lik, int stepPET,
                    int stepTi, double* loss);
+/* ---- wavelet-adaptive grid (gh_oracle_amr.c).  leaves: nleaves rows of
+ * (level, ix, iy, iz) in canonical order (level, iz, iy, ix) ascending; fields
+ * block-major in that order.  order = 2 or 4 (interpolating wavelets). */
+void gho_amr_lab_f32(int nleaves, const int* leaves, int order, const float* field, int leaf, float* lab);
+void gho_amr_lab_f64(int nleaves, const int* leaves, int order, const double* field, int leaf, double* lab);
+double gho_amr_fwt_f32(int order, const float* lab);
+double gho_amr_fwt_f64(int order, const double* lab);
+void gho_amr_split_f32(int order, const float* lab, int octant, float* child);
+void gho_amr_split_f64(int order, const double* lab, int octant, double* child);
+void gho_amr_rd_steps_f32(int nleaves, const int* leaves, int order, float* phi, const float* p_w,
+                          const float* p_g, float Dw, float Dg, float rho, double dt, int64_t nsteps);
+void gho_amr_rd_steps_f64(int nleaves, const int* leaves, int order, double* phi, const double* p_w,
+                          const double* p_g, double Dw, double Dg, double rho, double dt, int64_t nsteps);
+void gho_amr_resample(int nleaves, const int* leaves, const double* phi, int is_f32, int gpd, double* out);
+
 /* dJ/du of J = -logL (HGG_Likelihood.cpp:40-156 differentiated), full grid */
 void gho_dloss_du(int n, const double* u, const double* t1c, const double* flair, const double* pet,
                   const double* lik, int stepPET, int stepTi, double* g);
