@@ -44,6 +44,15 @@ def _load_oracle():
         "gho_tlm_f64": (_d, [_i, _vp, _vp, _vp, _d, _d, _d, _d, _d, _i64, _d, _vp, _vp, _vp, _vp, _vp, _vp, _i, _i,
                              _vp]),
         "gho_dloss_du": (None, [_i, _vp, _vp, _vp, _vp, _vp, _i, _i, _vp]),
+        "gho_amr_lab_f32": (None, [_i, _vp, _i, _vp, _i, _vp]),
+        "gho_amr_lab_f64": (None, [_i, _vp, _i, _vp, _i, _vp]),
+        "gho_amr_fwt_f32": (_d, [_i, _vp]),
+        "gho_amr_fwt_f64": (_d, [_i, _vp]),
+        "gho_amr_split_f32": (None, [_i, _vp, _i, _vp]),
+        "gho_amr_split_f64": (None, [_i, _vp, _i, _vp]),
+        "gho_amr_rd_steps_f32": (None, [_i, _vp, _i, _vp, _vp, _vp, C.c_float, C.c_float, C.c_float, _d, _i64]),
+        "gho_amr_rd_steps_f64": (None, [_i, _vp, _i, _vp, _vp, _vp, _d, _d, _d, _d, _i64]),
+        "gho_amr_resample": (None, [_i, _vp, _vp, _i, _i, _vp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -118,6 +127,48 @@ class Oracle:
         self.lib.gho_dloss_du(u.shape[0], self.p(u), self.p(t1c), self.p(flair), self.p(pet), self.p(la), stepPET,
                               stepTi, self.p(g))
         return g
+
+
+    # ---- wavelet-adaptive grid: leaves = canonical rows (level, ix, iy, iz) ----
+    @staticmethod
+    def _rows(leaves):
+        return np.ascontiguousarray(leaves, np.int32).reshape(-1, 4)
+
+    def amr_lab(self, leaves, order, field, leaf):
+        r = self._rows(leaves)
+        f = np.ascontiguousarray(field)
+        n = 13 if order == 4 else 10
+        lab = np.zeros(n ** 3, f.dtype)
+        fn = self.lib.gho_amr_lab_f32 if f.dtype == np.float32 else self.lib.gho_amr_lab_f64
+        fn(len(r), self.p(r), order, self.p(f), leaf, self.p(lab))
+        return lab.reshape(n, n, n)
+
+    def amr_fwt(self, order, lab):
+        lab = np.ascontiguousarray(lab)
+        fn = self.lib.gho_amr_fwt_f32 if lab.dtype == np.float32 else self.lib.gho_amr_fwt_f64
+        return fn(order, self.p(lab))
+
+    def amr_split(self, order, lab, octant):
+        lab = np.ascontiguousarray(lab)
+        out = np.zeros(512, lab.dtype)
+        fn = self.lib.gho_amr_split_f32 if lab.dtype == np.float32 else self.lib.gho_amr_split_f64
+        fn(order, self.p(lab), octant, self.p(out))
+        return out.reshape(8, 8, 8)
+
+    def amr_rd_steps(self, leaves, order, phi, pw, pg, Dw, Dg, rho, dt, nsteps):
+        r = self._rows(leaves)
+        phi = np.ascontiguousarray(phi).copy()
+        fn = self.lib.gho_amr_rd_steps_f32 if phi.dtype == np.float32 else self.lib.gho_amr_rd_steps_f64
+        fn(len(r), self.p(r), order, self.p(phi), self.p(np.ascontiguousarray(pw)), self.p(np.ascontiguousarray(pg)),
+           Dw, Dg, rho, dt, nsteps)
+        return phi
+
+    def amr_resample(self, leaves, phi, gpd):
+        r = self._rows(leaves)
+        out = np.zeros((gpd, gpd, gpd))
+        ph = np.ascontiguousarray(phi, np.float64)
+        self.lib.gho_amr_resample(len(r), self.p(r), self.p(ph), int(phi.dtype == np.float32), gpd, self.p(out))
+        return out
 
 
 @pytest.fixture(scope="session")
