@@ -145,32 +145,52 @@ static void collect(const tree_t* t, int order, int L, int64_t qx, int64_t qy, i
 static int lab_lo(int order) { return order == 4 ? -2 : -1; }
 static int lab_n(int order) { return order == 4 ? 13 : 10; }
 
-#define LAB_BODY(REAL)                                                                               \
-    tree_t t;                                                                                        \
-    tree_build(&t, nleaves, leaves);                                                                 \
-    const int L = leaves[4 * leaf], B[3] = {leaves[4 * leaf + 1], leaves[4 * leaf + 2], leaves[4 * leaf + 3]}; \
-    const int lo = lab_lo(order), N = lab_n(order);                                                  \
-    terms_t T = {0, 0, 0};                                                                           \
-    for (int p = 0; p < N * N * N; ++p) {                                                            \
-        const int i[3] = {p % N + lo, (p / N) % N + lo, p / (N * N) + lo};                           \
-        T.n = 0;                                                                                     \
-        collect(&t, order, L, 8LL * B[0] + i[0], 8LL * B[1] + i[1], 8LL * B[2] + i[2], 1.0, &T);     \
-        qsort(T.v, T.n, sizeof(term_t), cmp_term);                                                   \
-        REAL acc = (REAL)0;                                                                          \
-        for (int k = 0; k < T.n; ++k) {                                                              \
-            REAL prod = field[T.v[k].src] * (REAL)T.v[k].w;                                          \
-            acc = acc + prod;                                                                        \
+/* lab of `leaf`; faces_only: just the interior and the six face layers at
+ * distance 1 (what the 7-point stencil reads) */
+#define LAB_FILL(NAME, REAL)                                                                         \
+    static void NAME(const tree_t* t, const int* leaves, int order, const REAL* field, int leaf, REAL* lab, \
+                     int faces_only, terms_t* T) {                                                   \
+        const int L = leaves[4 * leaf], B[3] = {leaves[4 * leaf + 1], leaves[4 * leaf + 2], leaves[4 * leaf + 3]}; \
+        const int lo = lab_lo(order), N = lab_n(order);                                              \
+        for (int p = 0; p < N * N * N; ++p) {                                                        \
+            const int i[3] = {p % N + lo, (p / N) % N + lo, p / (N * N) + lo};                       \
+            if (faces_only) {                                                                        \
+                int out = 0, far = 0;                                                                \
+                for (int a = 0; a < 3; ++a) {                                                        \
+                    if (i[a] < 0 || i[a] > 7) ++out;                                                 \
+                    if (i[a] < -1 || i[a] > 8) far = 1;                                              \
+                }                                                                                    \
+                if (out > 1 || far) { lab[p] = (REAL)0; continue; }                                  \
+            }                                                                                        \
+            T->n = 0;                                                                                \
+            collect(t, order, L, 8LL * B[0] + i[0], 8LL * B[1] + i[1], 8LL * B[2] + i[2], 1.0, T);   \
+            qsort(T->v, T->n, sizeof(term_t), cmp_term);                                             \
+            REAL acc = (REAL)0;                                                                      \
+            for (int k = 0; k < T->n; ++k) {                                                         \
+                REAL prod = field[T->v[k].src] * (REAL)T->v[k].w;                                    \
+                acc = acc + prod;                                                                    \
+            }                                                                                        \
+            lab[p] = acc;                                                                            \
         }                                                                                            \
-        lab[p] = acc;                                                                                \
-    }                                                                                                \
-    free(T.v);                                                                                       \
-    tree_free(&t);
+    }
+LAB_FILL(lab_fill_f32, float)
+LAB_FILL(lab_fill_f64, double)
 
 void gho_amr_lab_f32(int nleaves, const int* leaves, int order, const float* field, int leaf, float* lab) {
-    LAB_BODY(float)
+    tree_t t;
+    terms_t T = {0, 0, 0};
+    tree_build(&t, nleaves, leaves);
+    lab_fill_f32(&t, leaves, order, field, leaf, lab, 0, &T);
+    free(T.v);
+    tree_free(&t);
 }
 void gho_amr_lab_f64(int nleaves, const int* leaves, int order, const double* field, int leaf, double* lab) {
-    LAB_BODY(double)
+    tree_t t;
+    terms_t T = {0, 0, 0};
+    tree_build(&t, nleaves, leaves);
+    lab_fill_f64(&t, leaves, order, field, leaf, lab, 0, &T);
+    free(T.v);
+    tree_free(&t);
 }
 
 /* BlockFWT::_filter / _fwt (MRAG/MRAGcore/MRAGBlockFWT.h:219-298) in Real;
@@ -260,9 +280,12 @@ void gho_amr_split_f64(int order, const double* lab, int octant, double* child) 
 /* RD steps on a multi-level grid: labs of phi, p_w, p_g per leaf (face ghosts
  * from the multiresolution), operator of ReactionDiffusionOperator.h:98-139
  * with h = 2^-level/8 per block, then UpdateTumor (:215-222) */
-#define AMR_RD_BODY(REAL, LABFN)                                                                     \
+#define AMR_RD_BODY(REAL, FILL)                                                                      \
     const int lo = lab_lo(order), N = lab_n(order), N3 = N * N * N;                                  \
     const int64_t nc = (int64_t)nleaves * 512;                                                       \
+    tree_t tr;                                                                                       \
+    terms_t TT = {0, 0, 0};                                                                          \
+    tree_build(&tr, nleaves, leaves);                                                                \
     REAL* lp = (REAL*)malloc(sizeof(REAL) * N3);                                                     \
     REAL* lw = (REAL*)malloc(sizeof(REAL) * N3);                                                     \
     REAL* lg = (REAL*)malloc(sizeof(REAL) * N3);                                                     \
@@ -270,9 +293,9 @@ void gho_amr_split_f64(int order, const double* lab, int octant, double* child) 
     const REAL eps_h = (REAL)1.0e-08, eps_bc = (REAL)0.1;                                            \
     for (int64_t st = 0; st < nsteps; ++st) {                                                        \
         for (int b = 0; b < nleaves; ++b) {                                                          \
-            LABFN(nleaves, leaves, order, phi, b, lp);                                               \
-            LABFN(nleaves, leaves, order, p_w, b, lw);                                               \
-            LABFN(nleaves, leaves, order, p_g, b, lg);                                               \
+            FILL(&tr, leaves, order, phi, b, lp, 1, &TT);                                            \
+            FILL(&tr, leaves, order, p_w, b, lw, 1, &TT);                                            \
+            FILL(&tr, leaves, order, p_g, b, lg, 1, &TT);                                            \
             const double hh = ldexp(1.0, -leaves[4 * b]) / 8;                                        \
             const double ih2 = 1. / (hh * hh);                                                       \
             for (int z = 0; z < 8; ++z) for (int y = 0; y < 8; ++y) for (int x = 0; x < 8; ++x) {    \
@@ -311,15 +334,16 @@ void gho_amr_split_f64(int order, const double* lab, int octant, double* child) 
             phi[c] = p;                                                                              \
         }                                                                                            \
     }                                                                                                \
-    free(lp); free(lw); free(lg); free(dphi);
+    free(lp); free(lw); free(lg); free(dphi); free(TT.v);                                            \
+    tree_free(&tr);
 
 void gho_amr_rd_steps_f32(int nleaves, const int* leaves, int order, float* phi, const float* p_w, const float* p_g,
                           float Dw, float Dg, float rho, double dt, int64_t nsteps) {
-    AMR_RD_BODY(float, gho_amr_lab_f32)
+    AMR_RD_BODY(float, lab_fill_f32)
 }
 void gho_amr_rd_steps_f64(int nleaves, const int* leaves, int order, double* phi, const double* p_w,
                           const double* p_g, double Dw, double Dg, double rho, double dt, int64_t nsteps) {
-    AMR_RD_BODY(double, gho_amr_lab_f64)
+    AMR_RD_BODY(double, lab_fill_f64)
 }
 
 /* _dumpUQoutput (Glioma/Glioma_ReactionDiffusion.cpp:208-267) with Real =

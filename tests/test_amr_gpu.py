@@ -1,0 +1,270 @@
+"""Wavelet-adaptive grid on the GPU (through the C ABI) against the oracle:
+FWT details, multi-level RD steps, refine / compress with data movement, the
+UQ resample and an end-to-end adaptive run.  Bar: bit-exact (identical
+Real arithmetic in a canonical accumulation order)."""
+from __future__ import annotations
+
+import sys
+
+import numpy as np
+import pytest
+
+import gliomasolver_amd as gh
+from gliomasolver_amd import FIELD_PCSF, FIELD_PG, FIELD_PHI, FIELD_PW
+from amr_util import (apply_collapse, apply_refine, balanced_random_grid, canonical, cell_positions,
+                      collapse_parents, refine_victims)
+from test_amr_host import central_two_level_grid
+
+pytestmark = pytest.mark.gpu
+FIELDS = (FIELD_PHI, FIELD_PW, FIELD_PG, FIELD_PCSF)
+
+
+def say(*a):
+    print("[gpu]", *a, flush=True)
+    sys.stdout.flush()
+
+
+def bits_equal(a, b):
+    return a.dtype == b.dtype and a.shape == b.shape and np.array_equal(a.view(np.uint8), b.view(np.uint8))
+
+
+def grids():
+    return {"central": central_two_level_grid(),
+            "random": balanced_random_grid(np.random.default_rng(11), base_level=1, max_level=3, nrefine=4)}
+
+
+def random_fields(rows, t, seed):
+    rng = np.random.default_rng(seed)
+    pos = cell_positions(rows)
+    smooth = (0.5 + 0.4 * np.sin(6 * pos[..., 0] + 1) * np.cos(5 * pos[..., 1]) * np.sin(4 * pos[..., 2] + 2))
+    phi = (smooth * (rng.uniform(0, 1, smooth.shape) < 0.95)).reshape(len(rows), 512).astype(t)
+    pw = rng.uniform(0, 1, (len(rows), 512))
+    pg = (1 - pw) * rng.uniform(0, 1, pw.shape)
+    csf = rng.uniform(0, 1, pw.shape) < 0.1
+    pw[csf] = 0
+    pg[csf] = 0
+    return {FIELD_PHI: phi, FIELD_PW: pw.astype(t), FIELD_PG: pg.astype(t), FIELD_PCSF: csf.astype(t)}
+
+
+def make_ctx(real, rows, order, fields, max_level=4, min_level=0):
+    ctx = gh.Context(0, real)
+    ctx.grid_configure(order, max_level, min_level)
+    ctx.grid_init_leaves(rows[:, 0], rows[:, 1:])
+    lev, idx = ctx.blocks()
+    assert np.array_equal(np.c_[lev, idx], rows)
+    for f, v in fields.items():
+        ctx.set_field_blocks(f, v)
+    return ctx
+
+
+def rows_of(ctx):
+    lev, idx = ctx.blocks()
+    return np.c_[lev, idx].astype(np.int32)
+
+
+@pytest.mark.parametrize("real", ["f32", "f64"])
+@pytest.mark.parametrize("order", [4, 2])
+@pytest.mark.parametrize("grid", ["central", "random"])
+def test_gpu_fwt_details_bit_exact(oracle, real, order, grid):
+    rows = grids()[grid]
+    t = np.float32 if real == "f32" else np.float64
+    fields = random_fields(rows, t, 1)
+    say("fwt", grid, real, order, len(rows), "blocks")
+    ctx = make_ctx(real, rows, order, fields)
+    det = ctx.fwt_details()
+    ctx.close()
+    ref = np.array([oracle.amr_fwt(order, oracle.amr_lab(rows, order, fields[FIELD_PHI], b)) for b in range(len(rows))])
+    assert np.array_equal(det, ref), "max |diff| %g" % np.abs(det - ref).max()
+
+
+@pytest.mark.parametrize("real", ["f32", "f64"])
+@pytest.mark.parametrize("order", [4, 2])
+@pytest.mark.parametrize("grid", ["central", "random"])
+def test_gpu_multilevel_rd_steps_bit_exact(oracle, real, order, grid):
+    rows = grids()[grid]
+    t = np.float32 if real == "f32" else np.float64
+    fields = random_fields(rows, t, 2)
+    say("multilevel rd", grid, real, order)
+    ctx = make_ctx(real, rows, order, fields)
+    Dw, Dg, rho, dt, k = 1.5e-4, 1.5e-5, 0.08, 0.02, 7
+    ctx.rd_prepare(Dw, Dg, rho)
+    ctx.rd_steps(k, dt)
+    out = ctx.get_field_blocks(FIELD_PHI).reshape(len(rows), 512)
+    ctx.close()
+    ref = oracle.amr_rd_steps(rows, order, fields[FIELD_PHI], fields[FIELD_PW], fields[FIELD_PG], t(Dw), t(Dg),
+                              t(rho), dt, k)
+    assert bits_equal(out, ref), "max |diff| %g" % np.abs(out.astype(float) - ref).max()
+
+
+# --- oracle-driven simulation of AutomaticRefinement / AutomaticCompression --
+
+def sim_details(oracle, rows, order, phi, subset):
+    return {b: oracle.amr_fwt(order, oracle.amr_lab(rows, order, phi, b)) for b in subset}
+
+
+def sim_refine(oracle, rows, fields, order, tol, max_level, max_loops=1):
+    nice = set()
+    loops = total = 0
+    while True:
+        cand = [b for b in range(len(rows)) if rows[b, 0] < max_level and tuple(rows[b]) not in nice]
+        if not cand:
+            break
+        det = sim_details(oracle, rows, order, fields[FIELD_PHI], cand)
+        flags = np.zeros(len(rows), np.uint8)
+        for b in cand:
+            if det[b] >= tol:
+                flags[b] = 1
+            else:
+                nice.add(tuple(rows[b]))
+        if not flags.any():
+            break
+        victims = refine_victims(rows, flags, max_level)
+        labs = {f: {v: oracle.amr_lab(rows, order, fields[f], v) for v in victims} for f in FIELDS}
+        new = apply_refine(rows, victims)
+        old_index = {tuple(r): i for i, r in enumerate(rows.tolist())}
+        nf = {f: np.zeros((len(new), 512), fields[f].dtype) for f in FIELDS}
+        for s, (L, x, y, z) in enumerate(new.tolist()):
+            if (L, x, y, z) in old_index:
+                for f in FIELDS:
+                    nf[f][s] = fields[f][old_index[(L, x, y, z)]]
+            else:
+                parent = old_index[(L - 1, x >> 1, y >> 1, z >> 1)]
+                o = (x & 1) | ((y & 1) << 1) | ((z & 1) << 2)
+                for f in FIELDS:
+                    nf[f][s] = oracle.amr_split(order, labs[f][parent], o).reshape(512)
+        rows, fields = new, nf
+        total += len(victims)
+        loops += 1
+        if max_loops > 0 and loops >= max_loops:
+            break
+    return rows, fields, total
+
+
+def sim_compress(oracle, rows, fields, order, tol, min_level, max_loops=-1):
+    nice = set()
+    loops = total = 0
+    while True:
+        cand = [b for b in range(len(rows)) if tuple(rows[b]) not in nice]
+        det = sim_details(oracle, rows, order, fields[FIELD_PHI], cand)
+        flags = np.zeros(len(rows), np.uint8)
+        for b in cand:
+            if det[b] < tol:
+                flags[b] = 1
+            else:
+                nice.add(tuple(rows[b]))
+        if not flags.any():
+            break
+        parents = collapse_parents(rows, flags, min_level)
+        if not parents:
+            break
+        new = apply_collapse(rows, parents)
+        old_index = {tuple(r): i for i, r in enumerate(rows.tolist())}
+        nf = {f: np.zeros((len(new), 512), fields[f].dtype) for f in FIELDS}
+        for s, (L, x, y, z) in enumerate(new.tolist()):
+            if (L, x, y, z) in old_index:
+                for f in FIELDS:
+                    nf[f][s] = fields[f][old_index[(L, x, y, z)]]
+            else:
+                for f in FIELDS:
+                    blk = np.zeros(512, fields[f].dtype)
+                    for c in range(512):
+                        cx, cy, cz = c & 7, (c >> 3) & 7, c >> 6
+                        o = (cx >> 2) | ((cy >> 2) << 1) | ((cz >> 2) << 2)
+                        kid = old_index[(L + 1, 2 * x + (o & 1), 2 * y + ((o >> 1) & 1), 2 * z + ((o >> 2) & 1))]
+                        src = 2 * (cx & 3) + 8 * 2 * (cy & 3) + 64 * 2 * (cz & 3)
+                        blk[c] = fields[f].dtype.type(0) + fields[f][kid, src]
+                    nf[f][s] = blk
+        rows, fields = new, nf
+        total += len(parents)
+        loops += 1
+        if max_loops > 0 and loops >= max_loops:
+            break
+    return rows, fields, total
+
+
+def blob_fields(rows, t, centre=(0.45, 0.55, 0.5), width=0.08):
+    pos = cell_positions(rows)
+    r2 = sum((pos[..., a] - centre[a]) ** 2 for a in range(3))
+    phi = np.exp(-r2 / (2 * width ** 2)).reshape(len(rows), 512).astype(t)
+    pw = np.full((len(rows), 512), 0.7, t)
+    pg = np.full((len(rows), 512), 0.3, t)
+    return {FIELD_PHI: phi, FIELD_PW: pw, FIELD_PG: pg, FIELD_PCSF: np.zeros_like(pw)}
+
+
+@pytest.mark.parametrize("real", ["f32", "f64"])
+def test_gpu_refine_and_compress_bit_exact(oracle, real):
+    t = np.float32 if real == "f32" else np.float64
+    rows = canonical([[2, x, y, z] for z in range(4) for y in range(4) for x in range(4)])
+    fields = blob_fields(rows, t)
+    ctx = make_ctx(real, rows, 4, fields, max_level=4, min_level=1)
+    say("refine", real)
+    n = ctx.refine(1e-3, 1)
+    srows, sfields, sn = sim_refine(oracle, rows, fields, 4, 1e-3, 4, 1)
+    assert n == sn and n > 0
+    assert np.array_equal(rows_of(ctx), srows)
+    for f in FIELDS:
+        assert bits_equal(ctx.get_field_blocks(f).reshape(len(srows), 512), sfields[f]), f
+    say("compress", real)
+    n = ctx.compress(2e-2, -1)
+    crows, cfields, cn = sim_compress(oracle, srows, sfields, 4, 2e-2, 1, -1)
+    assert n == cn and n > 0
+    assert np.array_equal(rows_of(ctx), crows)
+    for f in FIELDS:
+        assert bits_equal(ctx.get_field_blocks(f).reshape(len(crows), 512), cfields[f]), f
+    ctx.close()
+
+
+@pytest.mark.parametrize("real", ["f32", "f64"])
+def test_gpu_uq_resample(oracle, real):
+    rows = grids()["central"]
+    t = np.float32 if real == "f32" else np.float64
+    fields = random_fields(rows, t, 3)
+    ctx = make_ctx(real, rows, 4, fields)
+    for gpd in (64, 128):
+        out = ctx.uq_resample(gpd)
+        assert np.array_equal(out, oracle.amr_resample(rows, fields[FIELD_PHI], gpd))
+    ctx.close()
+
+
+def test_gpu_adaptive_run_matches_oracle_driven_simulation(oracle):
+    """Uniform level-2 start on a phantom, refine (1 loop) + compress (all
+    loops) at start, then 30 steps with refinement every 10 steps; block lists
+    and phi bit-exact at every epoch (FP32 state, 4th-order wavelets)."""
+    t = np.float32
+    n = 32
+    case_gm, case_wm, case_csf, sv = gh.phantom(n, 21)
+    pw, pg, _ = oracle.anatomy(case_gm, case_wm, case_csf, "f32")
+    phi = gh.gaussian_ic("f32", pw, pg, sv, 1.5, 0.6)
+    rows = canonical([[2, x, y, z] for z in range(4) for y in range(4) for x in range(4)])
+
+    def to_blocks(a):
+        return np.ascontiguousarray(a.reshape(4, 8, 4, 8, 4, 8).transpose(0, 2, 4, 1, 3, 5).reshape(64, 512))
+
+    fields = {FIELD_PHI: to_blocks(phi), FIELD_PW: to_blocks(pw), FIELD_PG: to_blocks(pg),
+              FIELD_PCSF: np.zeros((64, 512), t)}
+    Dw, Dg, dt, _ = gh.schedule("f32", 0.013, 17.6, 100.0, 8 << 3, 0.9)
+    rho = 0.05
+    ctx = make_ctx("f32", rows, 4, fields, max_level=3, min_level=1)
+    ctx.rd_prepare(Dw, Dg, rho)
+    say("adaptive run: initial adaptation")
+    ctx.refine(1e-4, 1)
+    ctx.compress(1e-6, -1)
+    rows, fields, _ = sim_refine(oracle, rows, fields, 4, 1e-4, 3, 1)
+    rows, fields, _ = sim_compress(oracle, rows, fields, 4, 1e-6, 1, -1)
+    assert np.array_equal(rows_of(ctx), rows)
+    assert bits_equal(ctx.get_field_blocks(FIELD_PHI).reshape(len(rows), 512), fields[FIELD_PHI])
+    levels_seen = set(rows[:, 0].tolist())
+    for epoch in range(3):
+        ctx.rd_steps(10, dt)
+        fields[FIELD_PHI] = oracle.amr_rd_steps(rows, 4, fields[FIELD_PHI], fields[FIELD_PW], fields[FIELD_PG], t(Dw),
+                                                t(Dg), t(rho), dt, 10)
+        say("epoch", epoch, "blocks", len(rows))
+        assert bits_equal(ctx.get_field_blocks(FIELD_PHI).reshape(len(rows), 512), fields[FIELD_PHI])
+        ctx.refine(1e-4, 1)
+        rows, fields, _ = sim_refine(oracle, rows, fields, 4, 1e-4, 3, 1)
+        assert np.array_equal(rows_of(ctx), rows)
+        for f in FIELDS:
+            assert bits_equal(ctx.get_field_blocks(f).reshape(len(rows), 512), fields[f]), f
+        levels_seen |= set(rows[:, 0].tolist())
+    ctx.close()
+    assert len(levels_seen) >= 2
