@@ -202,6 +202,13 @@ class Context:
         arrs = [np.ascontiguousarray(a, np.float64) for a in (gm, wm, csf)]
         check(self.lib.gh_set_anatomy(self.h, n, *[_ptr(a) for a in arrs]))
 
+    def ic_from_anatomy(self, gm, wm, csf, tumor_ic=None):
+        """Glioma_ReactionDiffusion::_ic for anatomy arrays of shape (nz, ny, nx)."""
+        arrs = [np.ascontiguousarray(a, np.float64) for a in (gm, wm, csf)]
+        nz, ny, nx = arrs[0].shape
+        ic = None if tumor_ic is None else (C.c_double * 3)(*[float(v) for v in tumor_ic])
+        check(self.lib.gh_ic_from_anatomy(self.h, nx, ny, nz, *[_ptr(a) for a in arrs], ic))
+
     def set_field(self, field: int, values: np.ndarray):
         v = np.ascontiguousarray(values, self.dtype)
         check(self.lib.gh_set_field_dense(self.h, field, v.shape[0], _ptr(v)))
@@ -283,3 +290,85 @@ class Context:
 
     def sync(self):
         check(self.lib.gh_sync(self.h))
+
+
+class Batch:
+    """A population of independent uniform solves (gh_batch): one device solve
+    of many samples, each bit-identical to solving it alone with Context."""
+
+    def __init__(self, device: int = 0, real="f32"):
+        self.lib = load()
+        self.kind = real_kind(real)
+        self.dtype = np_real(self.kind)
+        h = C.c_void_p()
+        check(self.lib.gh_batch_create(device, self.kind, C.byref(h)))
+        self.h = h
+        self.sizes = []
+        self.ns = 0
+        self.anat_of = []
+
+    def close(self):
+        if getattr(self, "h", None) is not None and self.h.value:
+            self.lib.gh_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def add_anatomy(self, bpd: int, gm, wm, csf) -> int:
+        """GM/WM/CSF of shape (nz, ny, nx), any size; grid of bpd^3 blocks."""
+        arrs = [np.ascontiguousarray(a, np.float64) for a in (gm, wm, csf)]
+        nz, ny, nx = arrs[0].shape
+        aid = C.c_int()
+        check(self.lib.gh_batch_add_anatomy(self.h, bpd, nx, ny, nz, *[_ptr(a) for a in arrs], C.byref(aid)))
+        self.sizes.append(8 * bpd)
+        return aid.value
+
+    def set_observations(self, anatomy: int, t1c, flair, pet, roi=None):
+        arrs = [np.ascontiguousarray(a, np.float64) for a in (t1c, flair, pet)]
+        r = None if roi is None else np.ascontiguousarray(np.asarray(roi, np.float64).T)
+        check(self.lib.gh_batch_set_observations(self.h, anatomy, *[_ptr(a) for a in arrs],
+                                                 None if r is None else _ptr(r), 0 if roi is None else len(roi)))
+
+    def set_samples(self, anatomy, params, cfl: float = 0.99):
+        """params: (N, 6) rows {Dw, rho, Tend, icx, icy, icz}."""
+        a = np.ascontiguousarray(anatomy, np.int32)
+        p = np.ascontiguousarray(params, np.float64).reshape(len(a), 6)
+        check(self.lib.gh_batch_set_samples(self.h, len(a), _ptr(a), _ptr(p), cfl))
+        self.ns = len(a)
+        self.anat_of = [int(v) for v in a]
+
+    def run(self) -> float:
+        ms = C.c_double()
+        check(self.lib.gh_batch_run(self.h, C.byref(ms)))
+        return ms.value
+
+    def likelihood(self, lik_params, stepPET=1, stepTi=1, use_roi=False) -> np.ndarray:
+        """(N, 4) rows {-logL, L_PET, L_T1, L_T2}; lik_params (N, 5) or one row for all."""
+        lp = np.asarray(lik_params, np.float64)
+        if lp.ndim == 1:
+            lp = np.tile(lp, (self.ns, 1))
+        lp = np.ascontiguousarray(lp)
+        out = np.zeros((self.ns, 4))
+        check(self.lib.gh_batch_likelihood(self.h, _ptr(lp), stepPET, stepTi, int(use_roi), _ptr(out)))
+        return out
+
+    def sample_info(self, i: int):
+        Dw, Dg, dt, ns = C.c_double(), C.c_double(), C.c_double(), C.c_int64()
+        check(self.lib.gh_batch_sample_info(self.h, i, C.byref(Dw), C.byref(Dg), C.byref(dt), C.byref(ns)))
+        return Dw.value, Dg.value, dt.value, ns.value
+
+    def get_phi(self, i: int) -> np.ndarray:
+        n = self.sizes[self.anat_of[i]]
+        out = np.zeros((n, n, n), self.dtype)
+        check(self.lib.gh_batch_get_phi(self.h, i, _ptr(out)))
+        return out

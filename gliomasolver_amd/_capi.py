@@ -48,6 +48,7 @@ SIGNATURES = {
     "gh_plan_refine": (_i, [_i, _vp, _vp, _i, _pi, _vp]),
     "gh_plan_compress": (_i, [_i, _vp, _vp, _i, _pi, _vp]),
     "gh_set_anatomy": (_i, [_vp, _i, _vp, _vp, _vp]),
+    "gh_ic_from_anatomy": (_i, [_vp, _i, _i, _i, _vp, _vp, _vp, _pd]),
     "gh_set_field_dense": (_i, [_vp, _i, _i, _vp]),
     "gh_get_field_dense": (_i, [_vp, _i, _i, _vp]),
     "gh_schedule": (_i, [_i, _d, _d, _d, _i, _d, _pd, _pd, _pd, _pi64]),
@@ -64,6 +65,15 @@ SIGNATURES = {
     "gh_rd_adjoint": (_i, [_vp, _i64, _d, _i, _pd, _i, _i, _i, _pd, _d, _pd, _pd]),
     "gh_checkpoint_save": (_i, [_vp, _i]),
     "gh_checkpoint_load": (_i, [_vp, _i]),
+    "gh_batch_create": (_i, [_i, _i, C.POINTER(_vp)]),
+    "gh_batch_destroy": (_i, [_vp]),
+    "gh_batch_add_anatomy": (_i, [_vp, _i, _i, _i, _i, _vp, _vp, _vp, _pi]),
+    "gh_batch_set_observations": (_i, [_vp, _i, _vp, _vp, _vp, _vp, _i]),
+    "gh_batch_set_samples": (_i, [_vp, _i, _vp, _vp, _d]),
+    "gh_batch_run": (_i, [_vp, _pd]),
+    "gh_batch_likelihood": (_i, [_vp, _vp, _i, _i, _i, _vp]),
+    "gh_batch_sample_info": (_i, [_vp, _i, _pd, _pd, _pd, _pi64]),
+    "gh_batch_get_phi": (_i, [_vp, _i, _vp]),
     "gh_phantom": (_i, [_i, _u64, _vp, _vp, _vp, _pi]),
     "gh_gaussian_ic": (_i, [_i, _i, _vp, _vp, _pd, _d, _d, _vp]),
 }

@@ -1,4 +1,5 @@
-"""Build libgliomahip.so in-tree for gfx950 (hipcc), plus the oracle checker.
+"""Build libgliomahip.so in-tree for gfx950 (hipcc), the host executables in
+gliomasolver_amd/bin, and the oracle checker.
 
 Usage: python -m gliomasolver_amd.build [--jobs N]
 """
@@ -61,6 +62,31 @@ def build_library(jobs: int = 8) -> str:
     return LIB
 
 
+APPS = os.path.join(PKG, "apps")
+BIN = os.path.join(PKG, "bin")
+
+
+def build_apps() -> list:
+    """Host executables over the C ABI (brain, likelihood, rd_pipeline), linked
+    against the in-tree library with an $ORIGIN-relative rpath."""
+    os.makedirs(BIN, exist_ok=True)
+    cxx = shutil.which("g++") or "g++"
+    out = []
+    for name in sorted(os.listdir(APPS)):
+        if not name.endswith(".cpp"):
+            continue
+        exe = os.path.join(BIN, name[:-4])
+        cmd = [cxx, "-O2", "-std=c++17", "-Wall", "-I" + os.path.join(ROOT, "include"), "-I" + APPS,
+               os.path.join(APPS, name), "-o", exe + ".tmp", "-L" + PKG, "-lgliomahip",
+               "-Wl,-rpath,$ORIGIN/.."]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError("app build failed: %s\n%s\n%s" % (" ".join(cmd), r.stdout, r.stderr))
+        os.replace(exe + ".tmp", exe)
+        out.append(exe)
+    return out
+
+
 def build_oracle() -> None:
     """Checker only (oracle/): liboracle.so, and oracle/_ref when the reference tree exists."""
     make = shutil.which("make")
@@ -77,6 +103,8 @@ def main(argv=None) -> int:
     ap.add_argument("--no-oracle", action="store_true")
     a = ap.parse_args(argv)
     print(build_library(a.jobs))
+    for exe in build_apps():
+        print(exe)
     if not a.no_oracle:
         build_oracle()
     return 0

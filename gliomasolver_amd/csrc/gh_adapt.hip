@@ -253,8 +253,6 @@ __global__ __launch_bounds__(256) void statics_ghosted_kernel(int nslots, const 
     }
 }
 
-__device__ inline int P(int x, int y, int z) { return (z + 1) * 100 + (y + 1) * 10 + (x + 1); }
-
 // RD step on a multi-level grid: one workgroup per block, h of the block's
 // level; a face's halo comes from the same-level neighbour or from the
 // interpolated face ghosts

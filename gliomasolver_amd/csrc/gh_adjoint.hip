@@ -31,7 +31,6 @@ namespace gh {
 namespace {
 
 __device__ inline int Q(int x, int y, int z) { return (z + 2) * 144 + (y + 2) * 12 + (x + 2); }  // [-2,10)^3
-__device__ inline int P(int x, int y, int z) { return (z + 1) * 100 + (y + 1) * 10 + (x + 1); }  // [-1,9)^3
 
 __device__ inline double block_sum(double v, double* red) {
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);

@@ -4,6 +4,7 @@
 #include <cstring>
 #include <new>
 
+#include "gh_anatomy.h"
 #include "gh_ctx.h"
 
 namespace gh {
@@ -146,6 +147,36 @@ static int set_anatomy_t(Ctx* c, int n, const double* gm, const double* wm, cons
     return GH_OK;
 }
 
+// Glioma_ReactionDiffusion::_ic (Glioma/Glioma_ReactionDiffusion.cpp:79-172) on
+// a uniform grid for anatomy data of any size: the cell at x = origin + h*i
+// (Real) samples voxel floor(x / (1/brainSizeMax)); majority vote and
+// normalisation as above; with tumor_ic, the reference's smoothed ball of
+// radius 0.005 and width 2/128 gated by the voted tissue fractions.
+template <typename Real>
+static int ic_from_anatomy_t(Ctx* c, int nx, int ny, int nz, const double* gm, const double* wm, const double* csf,
+                             const double* tumor_ic) {
+    const int bpd = c->topo.uniform_bpd;
+    const size_t nc = c->ncells();
+    const int n = BS * bpd;
+    std::vector<Real> pw(nc), pg(nc), pc(nc), tis(nc), phi(nc, (Real)0);
+    anatomy_from_data<Real>(bpd, nx, ny, nz, gm, wm, csf, pw.data(), pg.data(), pc.data(), tis.data());
+    if (tumor_ic) {
+        std::vector<std::pair<size_t, Real>> cells;
+        tumour_cells<Real>(bpd, tis.data(), tumor_ic, cells);
+        for (const auto& e : cells) phi[e.first] = e.second;
+    }
+    std::vector<Real> bm(nc);
+    const std::vector<Real>* src[4] = {&pw, &pg, &pc, &phi};
+    DevBuf* dst[4] = {&c->pw, &c->pg, &c->pcsf, &c->phi[c->cur]};
+    for (int f = 0; f < (tumor_ic ? 4 : 3); ++f) {
+        dense_to_blocks<Real>(n, src[f]->data(), bm.data());
+        GH_HIP_CHECK(hipMemcpyAsync(dst[f]->p, bm.data(), nc * sizeof(Real), hipMemcpyHostToDevice, c->stream));
+        GH_HIP_CHECK(hipStreamSynchronize(c->stream));
+    }
+    c->prepared = false;
+    return GH_OK;
+}
+
 // Grid(bpd,bpd,bpd): blocks at level log2(bpd), index (bx,by,bz); periodic
 // same-level neighbours (MRAG/MRAGcore/MRAGrid.inl:37-39, 940-966).
 static int init_uniform(Ctx* c, int bpd) {
@@ -276,6 +307,18 @@ int gh_set_anatomy(gh_ctx* ctx, int n, const double* gm, const double* wm, const
                                        : set_anatomy_t<double>(c, n, gm, wm, csf);
 }
 
+int gh_ic_from_anatomy(gh_ctx* ctx, int nx, int ny, int nz, const double* gm, const double* wm, const double* csf,
+                       const double* tumor_ic) {
+    Ctx* c = as_ctx(ctx);
+    if (!c) return set_error(GH_EINVAL, "null context");
+    if (c->topo.uniform_bpd <= 0) return set_error(GH_ESTATE, "gh_ic_from_anatomy: needs a uniform grid");
+    if (nx <= 0 || ny <= 0 || nz <= 0 || !gm || !wm || !csf)
+        return set_error(GH_EINVAL, "gh_ic_from_anatomy: bad argument");
+    GH_HIP_CHECK(hipSetDevice(c->device));
+    return c->real_kind == GH_REAL_F32 ? ic_from_anatomy_t<float>(c, nx, ny, nz, gm, wm, csf, tumor_ic)
+                                       : ic_from_anatomy_t<double>(c, nx, ny, nz, gm, wm, csf, tumor_ic);
+}
+
 int gh_set_field_dense(gh_ctx* ctx, int field, int n, const void* values) {
     Ctx* c = as_ctx(ctx);
     int rc = check_dense(c, n);
@@ -331,6 +374,8 @@ int gh_rd_prepare(gh_ctx* ctx, double Dw, double Dg, double rho) {
     Ctx* c = as_ctx(ctx);
     if (!c) return set_error(GH_EINVAL, "null context");
     if (c->topo.nblocks() == 0) return set_error(GH_ESTATE, "gh_rd_prepare: no grid");
+    // statics are invalidated by every change of p_w, p_g or the grid
+    if (c->prepared && c->Dw == Dw && c->Dg == Dg && c->rho == rho) return GH_OK;
     GH_HIP_CHECK(hipSetDevice(c->device));
     return rd_prepare(*c, Dw, Dg, rho);
 }

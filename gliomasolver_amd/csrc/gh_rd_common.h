@@ -12,6 +12,8 @@
 #include <cmath>
 #include <cstdint>
 
+#include "gh_internal.h"
+
 #pragma clang fp contract(off)
 
 namespace gh {
@@ -95,6 +97,151 @@ template <typename Real>
 __host__ __device__ inline Real domain_threshold(Real pw, Real pg) {
     const Real s = pw + pg;
     return -s;
+}
+
+// LDS index of cell (x, y, z) in a 10^3 block tile with a one-cell halo
+__device__ inline int P(int x, int y, int z) { return (z + 1) * 100 + (y + 1) * 10 + (x + 1); }
+
+// dphidt of one cell (ReactionDiffusionOperator.h:103-138) from its face
+// coefficients d0..d5 (-x,+x,-y,+y,-z,+z, before the no-flux rule), the
+// neighbour values p0..p5 with their outside flags o0..o5, its own value pc
+// and domain threshold sc.
+template <typename Real>
+__device__ inline Real rd_cell_rhs(Real d0, Real d1, Real d2, Real d3, Real d4, Real d5, Real p0, Real p1, Real p2,
+                                   Real p3, Real p4, Real p5, bool o0, bool o1, bool o2, bool o3, bool o4, bool o5,
+                                   Real pc, Real sc, Real rho, double ih2) {
+    // _applyNoFluxBC: an outside neighbour doubles the opposite face (exact)
+    if (o0) d1 = (Real)((double)d1 * 2.0);
+    if (o1) d0 = (Real)((double)d0 * 2.0);
+    if (o2) d3 = (Real)((double)d3 * 2.0);
+    if (o3) d2 = (Real)((double)d2 * 2.0);
+    if (o4) d5 = (Real)((double)d5 * 2.0);
+    if (o5) d4 = (Real)((double)d4 * 2.0);
+    // diffusionFluxIn = ih2 * (Real sum of df_i*phi_i, left to right)
+    Real sin_ = d0 * p0;
+    sin_ = sin_ + d1 * p1;
+    sin_ = sin_ + d2 * p2;
+    sin_ = sin_ + d3 * p3;
+    sin_ = sin_ + d4 * p4;
+    sin_ = sin_ + d5 * p5;
+    const double fin = ih2 * (double)sin_;
+    // diffusionFluxOut = -((Real sum of df_i) * phi * ih2)
+    Real sdf = d0 + d1;
+    sdf = sdf + d2;
+    sdf = sdf + d3;
+    sdf = sdf + d4;
+    sdf = sdf + d5;
+    const Real sdp = sdf * pc;
+    const double fout = -((double)sdp * ih2);
+    // reactionFlux = (Real)(rho*phi) * (1. - phi)
+    const Real rp = rho * pc;
+    const double react = (double)rp * (1. - (double)pc);
+    const Real dphidt = (Real)((fout + fin) + react);
+    return (pc > sc) ? dphidt : (Real)0.;  // outside the brain domain (:103, :137-138)
+}
+
+// UpdateTumor (ReactionDiffusionOperator.h:219-221): phi += dt*dphidt in
+// double, then std::max((Real)0, .), std::min((Real)1, .)
+template <typename Real>
+__device__ inline Real rd_update(Real pc, Real dphidt, double dt) {
+    Real p = (Real)((double)pc + dt * (double)dphidt);
+    p = ((Real)0. < p) ? p : (Real)0.;
+    return (p < (Real)1.) ? p : (Real)1.;
+}
+
+// Step parameters of a slot: one solve (the same rho, 1/h^2, dt for every
+// slot) or many independent solves packed slot range after slot range.
+template <typename Real>
+struct UniformStepParams {
+    Real rho;
+    double ih2, dt;
+    __device__ inline void get(int, Real& r, double& i, double& d) const {
+        r = rho;
+        i = ih2;
+        d = dt;
+    }
+};
+
+// BPW blocks per workgroup of 512 lanes; lane = cell (x fastest) of each
+// block.  All loads of a lane (BPW cells, one vector load of constants each)
+// are issued before the first wait.  Group mapping is XCD-aware: consecutive
+// workgroups land on different XCDs, so workgroup w serves group
+// (w % 8) * per_xcd + w / 8 and each XCD sweeps a contiguous range of blocks
+// whose halos stay in its own L2.
+template <typename Real, bool FUSED, int BPW, typename Params>
+__global__ __launch_bounds__(512) void rd_step_kernel(int nslots, int per_xcd, const int32_t* __restrict__ nbr,
+                                                      const Real* __restrict__ phi, Real* __restrict__ out,
+                                                      const Stat4<Real>* __restrict__ st,
+                                                      const Real* __restrict__ sig, Params prm) {
+    const int grp = (int)(blockIdx.x % 8) * per_xcd + (int)(blockIdx.x / 8);
+    if (grp * BPW >= nslots) return;
+    __shared__ Real s_phi[BPW][1000];
+    __shared__ unsigned char s_out[BPW][1000];
+    __shared__ Real s_Dx[BPW][576], s_Dy[BPW][576], s_Dz[BPW][576];
+
+    const int t = threadIdx.x;
+    const int x = t & 7, y = (t >> 3) & 7, z = t >> 6;
+    Real pc[BPW], sc[BPW];
+    Stat4<Real> sv[BPW];
+#pragma unroll
+    for (int j = 0; j < BPW; ++j) {
+        const int s = grp * BPW + j;
+        if (s >= nslots) break;
+        const size_t g = (size_t)s * BCELLS + t;
+        pc[j] = phi[g];
+        sv[j] = st[g];
+        sc[j] = sig[g];
+    }
+    if (t < 384) {  // six face halos of 64 cells, one lane each
+        const int f = t >> 6, u = t & 7, v = (t >> 3) & 7;
+        int src, dst;
+        switch (f) {
+            case FXM: src = 7 + 8 * u + 64 * v; dst = P(-1, u, v); break;
+            case FXP: src = 0 + 8 * u + 64 * v; dst = P(8, u, v); break;
+            case FYM: src = u + 8 * 7 + 64 * v; dst = P(u, -1, v); break;
+            case FYP: src = u + 8 * 0 + 64 * v; dst = P(u, 8, v); break;
+            case FZM: src = u + 8 * v + 64 * 7; dst = P(u, v, -1); break;
+            default:  src = u + 8 * v + 64 * 0; dst = P(u, v, 8); break;
+        }
+#pragma unroll
+        for (int j = 0; j < BPW; ++j) {
+            const int s = grp * BPW + j;
+            if (s >= nslots) break;
+            const size_t nb = (size_t)nbr[6 * s + f] * BCELLS + src;
+            const Real ph = phi[nb];
+            const Stat4<Real> hs = st[nb];
+            s_phi[j][dst] = ph;
+            s_out[j][dst] = ph < hs.tau;
+            if (f == FXM) s_Dx[j][v * 72 + u * 9] = hs.dx;
+            else if (f == FYM) s_Dy[j][v * 72 + u] = hs.dy;
+            else if (f == FZM) s_Dz[j][v * 8 + u] = hs.dz;
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < BPW; ++j) {
+        if (grp * BPW + j >= nslots) break;
+        s_phi[j][P(x, y, z)] = pc[j];
+        s_out[j][P(x, y, z)] = pc[j] < sv[j].tau;
+        s_Dx[j][z * 72 + y * 9 + x + 1] = sv[j].dx;
+        s_Dy[j][z * 72 + (y + 1) * 8 + x] = sv[j].dy;
+        s_Dz[j][(z + 1) * 64 + y * 8 + x] = sv[j].dz;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < BPW; ++j) {
+        const int s = grp * BPW + j;
+        if (s >= nslots) break;
+        Real rho;
+        double ih2, dt;
+        prm.get(s, rho, ih2, dt);
+        const Real d = rd_cell_rhs<Real>(
+            s_Dx[j][z * 72 + y * 9 + x], sv[j].dx, s_Dy[j][z * 72 + y * 8 + x], sv[j].dy,
+            s_Dz[j][z * 64 + y * 8 + x], sv[j].dz, s_phi[j][P(x - 1, y, z)], s_phi[j][P(x + 1, y, z)],
+            s_phi[j][P(x, y - 1, z)], s_phi[j][P(x, y + 1, z)], s_phi[j][P(x, y, z - 1)], s_phi[j][P(x, y, z + 1)],
+            s_out[j][P(x - 1, y, z)], s_out[j][P(x + 1, y, z)], s_out[j][P(x, y - 1, z)],
+            s_out[j][P(x, y + 1, z)], s_out[j][P(x, y, z - 1)], s_out[j][P(x, y, z + 1)], pc[j], sc[j], rho, ih2);
+        out[(size_t)s * BCELLS + t] = FUSED ? rd_update(pc[j], d, dt) : d;
+    }
 }
 
 }  // namespace gh

@@ -100,6 +100,13 @@ int gh_plan_compress(int nleaves, const int* leaves, const unsigned char* flagge
  * read from GM.dat/WM.dat/CSF.dat, majority vote, normalised p_w, p_g, p_csf.
  * Requires a uniform grid with n = 8*bpd (voxel (i,j,k) -> cell (i,j,k)). */
 int gh_set_anatomy(gh_ctx* ctx, int n, const double* gm, const double* wm, const double* csf);
+/* gh_ic_from_anatomy: all of _ic (Glioma_ReactionDiffusion.cpp:79-172) for
+ * anatomy arrays of any nx*ny*nz (1 mm voxels; the cell at x samples voxel
+ * floor(x*max(nx,ny,nz)), cells outside the data stay 0) and, when tumor_ic
+ * is not NULL, the reference's initial tumour (smoothed ball of radius 0.005
+ * centred at tumor_ic in the unit cube, -icx/-icy/-icz or TumorIC.txt). */
+int gh_ic_from_anatomy(gh_ctx* ctx, int nx, int ny, int nz, const double* gm, const double* wm,
+                       const double* csf, const double* tumor_ic);
 /* dense n^3 real arrays <-> the uniform grid's cells */
 #define GH_FIELD_PHI 0
 #define GH_FIELD_PW 1
@@ -120,7 +127,9 @@ int gh_schedule(int real_kind, double Dw_phys, double L, double tend, int gpd, d
  * gh_rd_prepare: per-solve constants of ReactionDiffusionOperator(Dw,Dg,rho)
  * (Glioma/Operators/ReactionDiffusionOperator.h:29-35): harmonic face
  * diffusivities (:148-163) and the per-cell no-flux / in-domain thresholds
- * (:103, :166-181).  Must be called after the anatomy changes.
+ * (:103, :166-181).  Must be called after the anatomy changes; returns at
+ * once when called again with the same Dw, Dg, rho on unchanged p_w, p_g and
+ * grid, so per-step callers pay nothing for it.
  * gh_rd_rhs: pipeline_process(vInfo, coll, bInfo, rhs) -> dphidt
  *   (Glioma_ReactionDiffusion.cpp:185, ReactionDiffusionOperator.h:98-139).
  * gh_rd_update: BlockProcessing::process(vInfo, coll, UpdateTumor(dt))
@@ -189,6 +198,44 @@ int gh_rd_adjoint(gh_ctx* ctx, int64_t nsteps, double T, int ckpt_every, const d
  * allocated on first save (0 <= slot <= 4096). */
 int gh_checkpoint_save(gh_ctx* ctx, int slot);
 int gh_checkpoint_load(gh_ctx* ctx, int slot);
+
+/* ---- batched evaluation of a TMCMC population -------------------------------
+ * Replaces the per-sample process chain of the reference's inference
+ * (tools/pi4u_lite/Inference/fitfun_glioma.c:118-152: write
+ * InputParameters.txt / TumorIC.txt / LikelihoodInput.txt, run runAll.sh =
+ * `brain -UQ 1` + `likelihood`, read Likelihood.txt) by one device solve of
+ * all samples.  Each sample is a uniform periodic grid of bpd^3 blocks
+ * (bpd in [1,64], any value) -- the finest resolution of the reference's
+ * adaptive run, at which HGG_data.dat is written -- with the schedule of
+ * gh_schedule and the initial condition of gh_ic_from_anatomy; results are
+ * bit-identical to solving each sample alone through gh_ctx.
+ * gh_batch_add_anatomy: GM/WM/CSF data of any nx*ny*nz (the _ic mapping);
+ *   L = 0.1*max(nx,ny,nz) in Real.  Returns its id.
+ * gh_batch_set_observations: tumT1c / tumFLAIR / tumPET at the grid
+ *   resolution n = 8*bpd (n^3, x fastest) and optional ROI points (as
+ *   gh_lik_set_data).
+ * gh_batch_set_samples: params = 6 doubles per sample {Dw (cm^2/day), rho
+ *   (1/day), Tend (days), icx, icy, icz}; cfl 0.99 is the reference's.
+ *   Allocates 2 states + statics per cell (28 B FP32, 56 B FP64); split a
+ *   larger population into several batches.
+ * gh_batch_run: all Euler steps of all samples; elapsed_ms = device time.
+ * gh_batch_likelihood: lik_params = 5 doubles per sample (LikelihoodInput.txt
+ *   order); out = 4 doubles per sample as gh_likelihood.
+ * gh_batch_sample_info / gh_batch_get_phi: a sample's schedule and state
+ *   (n^3 Real, x fastest). */
+typedef struct gh_batch gh_batch;
+int gh_batch_create(int device, int real_kind, gh_batch** out);
+int gh_batch_destroy(gh_batch* b);
+int gh_batch_add_anatomy(gh_batch* b, int bpd, int nx, int ny, int nz, const double* gm, const double* wm,
+                         const double* csf, int* anatomy_id);
+int gh_batch_set_observations(gh_batch* b, int anatomy_id, const double* t1c, const double* flair,
+                              const double* pet, const double* roi, int nroi);
+int gh_batch_set_samples(gh_batch* b, int nsamples, const int* anatomy_id, const double* params, double cfl);
+int gh_batch_run(gh_batch* b, double* elapsed_ms);
+int gh_batch_likelihood(gh_batch* b, const double* lik_params, int stepPET, int stepTi, int use_roi,
+                        double* out);
+int gh_batch_sample_info(gh_batch* b, int sample, double* Dw, double* Dg, double* dt, int64_t* nsteps);
+int gh_batch_get_phi(gh_batch* b, int sample, void* dense);
 
 /* ---- synthetic inputs (no reference counterpart: the reference reads
  * patient .dat files) -------------------------------------------------------

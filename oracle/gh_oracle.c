@@ -38,6 +38,67 @@ void gho_anatomy_f64(int64_t ncells, const double* gm, const double* wm, const d
                      double* p_w, double* p_g, double* p_csf) { ANATOMY_BODY(double) }
 
 /* ------------------------------------------------------------------------ */
+/* initial condition: Glioma_ReactionDiffusion.cpp:79-172.                   */
+/*   cell position x = origin + h*i (MRAG/MRAGcore/MRAGCommon.h:199-204),     */
+/*   origin = block index * 1/bpd, h = (1/bpd)/8, all Real;                   */
+/*   voxel = floor(x / (1./max(nx,ny,nz))) in double; cells outside stay 0;   */
+/*   tumour: psi = (|x - ic| - 0.005) * (1./(2.*(1./128))) in Real;           */
+/*   phi = 1 inside psi<-1 (tissue > 0.001), 0.5*(1-psi - sin(pi psi)/pi)     */
+/*   on the smoothing shell (tissue > 0), else 0.  (1 - psi) is Real.         */
+/* ------------------------------------------------------------------------ */
+#define IC_BODY(REAL)                                                                        \
+    const int n = 8 * bpd;                                                                   \
+    const int bmax = nx > ny ? (nx > nz ? nx : nz) : (ny > nz ? ny : nz);                    \
+    const double bh = 1.0 / (double)bmax;                                                    \
+    const REAL dil = (REAL)(1.0 / bpd), h = dil / (REAL)8;                                   \
+    const REAL rad = (REAL)0.005, sup = (REAL)2., hh = (REAL)(1. / 128);                    \
+    const REAL iw = (REAL)(1. / (double)(REAL)(sup * hh));                                   \
+    for (int k = 0; k < n; ++k)                                                              \
+        for (int j = 0; j < n; ++j)                                                          \
+            for (int i = 0; i < n; ++i) {                                                    \
+                const int64_t c = i + (int64_t)n * (j + (int64_t)n * k);                     \
+                p_w[c] = p_g[c] = p_csf[c] = phi[c] = (REAL)0;                               \
+                REAL x[3];                                                                   \
+                const int ii[3] = {i, j, k};                                                 \
+                for (int d = 0; d < 3; ++d) {                                                \
+                    REAL org = (REAL)(ii[d] / 8) * dil;                                      \
+                    x[d] = org + h * (REAL)(ii[d] % 8);                                      \
+                }                                                                            \
+                const int mx = (int)floor((double)x[0] / bh);                                \
+                const int my = (int)floor((double)x[1] / bh);                                \
+                const int mz = (int)floor((double)x[2] / bh);                                \
+                if (mx < 0 || mx >= nx || my < 0 || my >= ny || mz < 0 || mz >= nz) continue; \
+                const int64_t v = mx + (int64_t)nx * (my + (int64_t)ny * mz);                \
+                REAL pGM = (REAL)gm[v], pWM = (REAL)wm[v], pCSF = (REAL)csf[v];              \
+                double tissue = (double)(REAL)(pWM + pGM);                                   \
+                pCSF = ((double)pCSF > tissue) ? (REAL)1. : (REAL)0.;                        \
+                pWM = ((double)pCSF > tissue) ? (REAL)0. : pWM;                              \
+                pGM = ((double)pCSF > tissue) ? (REAL)0. : pGM;                              \
+                tissue = (double)(REAL)(pWM + pGM);                                          \
+                p_w[c] = (tissue > 0.) ? (REAL)((double)pWM / tissue) : (REAL)0.;            \
+                p_g[c] = (tissue > 0.) ? (REAL)((double)pGM / tissue) : (REAL)0.;            \
+                p_csf[c] = pCSF;                                                             \
+                if (!tumor_ic) continue;                                                     \
+                REAL p0 = x[0] - (REAL)tumor_ic[0], p1 = x[1] - (REAL)tumor_ic[1],           \
+                     p2 = x[2] - (REAL)tumor_ic[2];                                          \
+                REAL r2 = p0 * p0;                                                           \
+                r2 = r2 + p1 * p1;                                                           \
+                r2 = r2 + p2 * p2;                                                           \
+                const REAL dist = (REAL)sqrt((double)r2);                                    \
+                const REAL psi = (dist - rad) * iw;                                          \
+                const REAL tis = pGM + pWM;                                                  \
+                if (psi < -1 && tis > 0.001)                                                 \
+                    phi[c] = (REAL)1.0;                                                      \
+                else if (-1 <= psi && psi <= 1 && tis > 0)                                   \
+                    phi[c] = (REAL)(0.5 * ((double)((REAL)1 - psi) - sin(M_PI * psi) / M_PI)); \
+            }
+
+void gho_ic_f32(int bpd, int nx, int ny, int nz, const double* gm, const double* wm, const double* csf,
+                const double* tumor_ic, float* p_w, float* p_g, float* p_csf, float* phi) { IC_BODY(float) }
+void gho_ic_f64(int bpd, int nx, int ny, int nz, const double* gm, const double* wm, const double* csf,
+                const double* tumor_ic, double* p_w, double* p_g, double* p_csf, double* phi) { IC_BODY(double) }
+
+/* ------------------------------------------------------------------------ */
 /* schedule: Glioma_ReactionDiffusion.cpp:296-320                            */
 /*   Dw = Dw/(L*L) (Real); Dg = 0.1*Dw; h = 1./(bs*bpd) (Real);              */
 /*   dt = cfl*h*h/(2.*_DIM*max(Dw,Dg)) (Real); while(t<=tend){...; t+=dt;}    */

@@ -22,6 +22,14 @@ void gho_anatomy_f32(int64_t ncells, const double* gm, const double* wm, const d
 void gho_anatomy_f64(int64_t ncells, const double* gm, const double* wm, const double* csf,
                      double* p_w, double* p_g, double* p_csf);
 
+/* ---- whole initial condition: Glioma/Glioma_ReactionDiffusion.cpp:79-172.
+ * Grid of bpd^3 blocks of 8^3 cells; anatomy of nx*ny*nz voxels (x fastest);
+ * outputs are dense n^3 arrays, n = 8*bpd.  tumor_ic NULL leaves phi at 0. */
+void gho_ic_f32(int bpd, int nx, int ny, int nz, const double* gm, const double* wm, const double* csf,
+                const double* tumor_ic, float* p_w, float* p_g, float* p_csf, float* phi);
+void gho_ic_f64(int bpd, int nx, int ny, int nz, const double* gm, const double* wm, const double* csf,
+                const double* tumor_ic, double* p_w, double* p_g, double* p_csf, double* phi);
+
 /* ---- time step and step count: Glioma/Glioma_ReactionDiffusion.cpp:302-320 */
 /* outputs: Dw, Dg (rescaled by L^2), dt, nsteps (iterations of while(t<=tend)) */
 void gho_schedule_f32(double Dw_phys, double L, double tend, int gpd, double cfl,

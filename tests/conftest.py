@@ -33,6 +33,8 @@ def _load_oracle():
     sig = {
         "gho_anatomy_f32": (None, [_i64, _vp, _vp, _vp, _vp, _vp, _vp]),
         "gho_anatomy_f64": (None, [_i64, _vp, _vp, _vp, _vp, _vp, _vp]),
+        "gho_ic_f32": (None, [_i, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+        "gho_ic_f64": (None, [_i, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
         "gho_schedule_f32": (None, [_d, _d, _d, _i, _d, _vp, _vp, _vp, _vp]),
         "gho_schedule_f64": (None, [_d, _d, _d, _i, _d, _vp, _vp, _vp, _vp]),
         "gho_rd_steps_f32": (None, [_i, _vp, _vp, _vp, _vp, C.c_float, C.c_float, C.c_float, C.c_float, _d, _i64]),
@@ -77,6 +79,18 @@ class Oracle:
         f = self.lib.gho_anatomy_f32 if real == "f32" else self.lib.gho_anatomy_f64
         f(gm.size, self.p(gm), self.p(wm), self.p(csf), self.p(pw), self.p(pg), self.p(pc))
         return pw, pg, pc
+
+    def ic(self, bpd, gm, wm, csf, tumor_ic, real):
+        """_ic on a bpd^3-block grid from (nz, ny, nx) anatomy; returns pw, pg, pcsf, phi (n^3)."""
+        t = np.float32 if real == "f32" else np.float64
+        n = 8 * bpd
+        out = [np.zeros((n, n, n), t) for _ in range(4)]
+        gm, wm, csf = (np.ascontiguousarray(a, np.float64) for a in (gm, wm, csf))
+        nz, ny, nx = gm.shape
+        ic = None if tumor_ic is None else (C.c_double * 3)(*tumor_ic)
+        f = self.lib.gho_ic_f32 if real == "f32" else self.lib.gho_ic_f64
+        f(bpd, nx, ny, nz, self.p(gm), self.p(wm), self.p(csf), ic, *[self.p(a) for a in out])
+        return tuple(out)
 
     def schedule(self, real, Dw_phys, L, tend, gpd, cfl):
         t = C.c_float if real == "f32" else C.c_double

@@ -1,0 +1,116 @@
+"""GPU parity of the batched population solve (gh_batch):
+
+every sample of a mixed batch (three anatomies, grids of 16^3, 24^3 and 32^3
+cells, different Dw, rho, Tend and tumour centres, so different step counts)
+must equal
+  * the single-solve path through gh_ctx (gh_ic_from_anatomy, gh_schedule with
+    cfl 0.99, gh_rd_prepare, gh_rd_steps, gh_likelihood) bit for bit, for the
+    power-of-two grids that path supports, and
+  * the oracle (its _ic restatement, uniform RD steps and likelihood) bit for
+    bit on the state, for every grid including the 24^3 one.
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import gliomasolver_amd as gh
+from gliomasolver_amd import FIELD_PG, FIELD_PHI, FIELD_PW
+
+pytestmark = pytest.mark.gpu
+LIK = (0.000361, 0.85, 0.7, 0.25, 2.0)
+
+
+def say(*a):
+    print("[gpu]", *a, flush=True)
+
+
+def anatomy(shape, seed):
+    m = max(shape)
+    gm, wm, csf, _ = gh.phantom(m, seed)
+    nz, ny, nx = shape
+    return gm[:nz, :ny, :nx].copy(), wm[:nz, :ny, :nx].copy(), csf[:nz, :ny, :nx].copy()
+
+
+def observations(n, seed):
+    rng = np.random.default_rng(seed)
+    z, y, x = np.meshgrid(*(np.arange(n),) * 3, indexing="ij")
+    u = np.clip(1.1 - np.sqrt((x - 0.45 * n) ** 2 + (y - 0.5 * n) ** 2 + (z - 0.4 * n) ** 2) / (0.2 * n), 0, 1)
+    t1 = (u >= 0.7).astype(np.float64)
+    fl = (u >= 0.25).astype(np.float64)
+    for m in (t1, fl):
+        f = rng.uniform(size=m.shape) < 0.05
+        m[f] = 1 - m[f]
+    pet = np.where(t1 > 0, np.clip(u + rng.normal(0, 0.05, u.shape), 0, None), 0.0)
+    return t1, fl, pet
+
+
+ANAT = [(4, (36, 40, 44), 21), (3, (30, 24, 28), 22), (2, (16, 16, 16), 23)]   # (bpd, data shape, seed)
+SAMPLES = [  # anatomy, Dw, rho, Tend, icx, icy, icz
+    (0, 0.05, 0.2, 6.0, 0.45, 0.5, 0.42),
+    (1, 0.03, 0.1, 9.0, 0.5, 0.45, 0.5),
+    (0, 0.12, 0.3, 3.0, 0.4, 0.55, 0.45),
+    (2, 0.02, 0.4, 12.0, 0.5, 0.5, 0.5),
+    (1, 0.08, 0.05, 2.0, 0.45, 0.5, 0.45),
+    (0, 0.01, 0.25, 10.0, 0.47, 0.47, 0.47),
+]
+
+
+@pytest.fixture(scope="module", params=["f32", "f64"])
+def batch_run(request):
+    real = request.param
+    say("batch", real, "%d samples on 32^3/24^3/16^3" % len(SAMPLES))
+    b = gh.Batch(0, real)
+    data = []
+    for bpd, shape, seed in ANAT:
+        gm, wm, csf = anatomy(shape, seed)
+        aid = b.add_anatomy(bpd, gm, wm, csf)
+        obs = observations(8 * bpd, seed + 100)
+        b.set_observations(aid, *obs)
+        data.append((bpd, gm, wm, csf, obs))
+    anat = [s[0] for s in SAMPLES]
+    params = np.array([s[1:] for s in SAMPLES], np.float64)
+    b.set_samples(anat, params)
+    ms = b.run()
+    lik = b.likelihood(LIK, stepPET=2, stepTi=1)
+    phis = [b.get_phi(i) for i in range(len(SAMPLES))]
+    infos = [b.sample_info(i) for i in range(len(SAMPLES))]
+    b.close()
+    say("batch device time %.2f ms, steps" % ms, [i[3] for i in infos])
+    return real, data, params, phis, lik, infos
+
+
+def test_batch_matches_single_solve(batch_run):
+    real, data, params, phis, lik, infos = batch_run
+    for i, (a, *p) in enumerate(SAMPLES):
+        bpd, gm, wm, csf, obs = data[a]
+        if bpd & (bpd - 1):
+            continue                      # the single-solve grid needs a power of two
+        with gh.Context(0, real) as ctx:
+            ctx.grid_init_uniform(bpd)
+            ctx.ic_from_anatomy(gm, wm, csf, tuple(p[3:6]))
+            L = float(np.float32(max(gm.shape) * 0.1)) if real == "f32" else max(gm.shape) * 0.1
+            Dw, Dg, dt, ns = gh.schedule(real, p[0], L, p[2], 8 * bpd, 0.99)
+            assert (Dw, Dg, dt, ns) == infos[i]
+            ctx.rd_prepare(Dw, Dg, float(np.float32(p[1])) if real == "f32" else p[1])
+            ctx.rd_steps(ns, dt)
+            single = ctx.get_field(FIELD_PHI)
+            ctx.lik_set_data(*obs)
+            tot, comps, _ = ctx.likelihood(LIK, stepPET=2, stepTi=1)
+        assert np.array_equal(phis[i], single), "sample %d" % i
+        assert lik[i, 0] == tot and tuple(lik[i, 1:]) == comps, "sample %d" % i
+
+
+def test_batch_matches_oracle(batch_run, oracle):
+    real, data, params, phis, lik, infos = batch_run
+    t = np.float32 if real == "f32" else np.float64
+    for i, (a, *p) in enumerate(SAMPLES):
+        bpd, gm, wm, csf, obs = data[a]
+        n = 8 * bpd
+        pw, pg, _, phi0 = oracle.ic(bpd, gm, wm, csf, tuple(p[3:6]), real)
+        Dw, Dg, dt, ns = infos[i]
+        ref = oracle.rd_steps(phi0, pw, pg, t(Dw), t(Dg), t(p[1]), t(1.0 / n), dt, ns)
+        assert np.array_equal(phis[i], ref), "sample %d (n=%d)" % (i, n)
+        tot, comps, _ = oracle.likelihood(ref.astype(np.float64), obs[2], obs[0], obs[1], stepPET=2, stepTi=1)
+        assert abs(lik[i, 0] - tot) <= 1e-9 * max(1.0, abs(tot)), "sample %d" % i
+    assert len({i[3] for i in infos}) > 3      # the samples really had different step counts
