@@ -372,3 +372,81 @@ class Batch:
         out = np.zeros((n, n, n), self.dtype)
         check(self.lib.gh_batch_get_phi(self.h, i, _ptr(out)))
         return out
+
+
+class Tmcmc:
+    """The TMCMC sampler (gh_tmcmc): ask for candidates, evaluate them (e.g.
+    with Batch), submit their log-likelihoods, until finished."""
+
+    def __init__(self, lo, hi, pop_size, max_stages=1000, min_chain=0, max_chain=1, tol_cov=1.0, bbeta=0.2,
+                 seed=280675):
+        self.lib = load()
+        self.lo = np.ascontiguousarray(lo, np.float64)
+        self.hi = np.ascontiguousarray(hi, np.float64)
+        self.nth = len(self.lo)
+        h = C.c_void_p()
+        check(self.lib.gh_tmcmc_create(self.nth, _ptr(self.lo), _ptr(self.hi), pop_size, max_stages, min_chain,
+                                       max_chain, tol_cov, bbeta, seed, C.byref(h)))
+        self.h = h
+        self.pop = pop_size
+
+    def close(self):
+        if getattr(self, "h", None) is not None and self.h.value:
+            self.lib.gh_tmcmc_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def candidates(self) -> np.ndarray:
+        n = C.c_int()
+        check(self.lib.gh_tmcmc_candidates(self.h, C.byref(n), None))
+        out = np.zeros((n.value, self.nth))
+        check(self.lib.gh_tmcmc_candidates(self.h, C.byref(n), _ptr(out)))
+        return out
+
+    def submit(self, loglik):
+        f = np.ascontiguousarray(loglik, np.float64)
+        check(self.lib.gh_tmcmc_submit(self.h, len(f), _ptr(f)))
+
+    def state(self):
+        g, p, cov, ev, done = C.c_int(), C.c_double(), C.c_double(), C.c_double(), C.c_int()
+        check(self.lib.gh_tmcmc_state(self.h, C.byref(g), C.byref(p), C.byref(cov), C.byref(ev), C.byref(done)))
+        return dict(gen=g.value, p=p.value, cov=cov.value, log_evidence=ev.value, done=bool(done.value))
+
+    def population(self):
+        n = C.c_int()
+        check(self.lib.gh_tmcmc_population(self.h, C.byref(n), None, None))
+        x = np.zeros((n.value, self.nth))
+        f = np.zeros(n.value)
+        check(self.lib.gh_tmcmc_population(self.h, C.byref(n), _ptr(x), _ptr(f)))
+        return x, f
+
+
+def tmcmc_next_exponent(loglik, p: float, tol_cov: float = 1.0):
+    f = np.ascontiguousarray(loglik, np.float64)
+    pn, cov = C.c_double(), C.c_double()
+    check(load().gh_tmcmc_next_exponent(len(f), _ptr(f), p, tol_cov, C.byref(pn), C.byref(cov)))
+    return pn.value, cov.value
+
+
+def glioma_params(x, real="f32"):
+    """fitfun_glioma.c's map of 11 log-space parameters to the solver's
+    {Dw, rho, Tend, icx, icy, icz} and the likelihood's 5 inputs, through the
+    reference's text files.  x: (N, 11)."""
+    xa = np.ascontiguousarray(np.atleast_2d(x), np.float64)
+    s = np.zeros((len(xa), 6))
+    lk = np.zeros((len(xa), 5))
+    check(load().gh_glioma_params(len(xa), _ptr(xa), real_kind(real), _ptr(s), _ptr(lk)))
+    return s, lk
+
+
+def glioma_fitness(neg_loglik):
+    """-logL -> the log-likelihood TMCMC sees (Likelihood.txt round trip, 1e12 penalty)."""
+    v = np.ascontiguousarray(neg_loglik, np.float64)
+    out = np.zeros_like(v)
+    check(load().gh_glioma_fitness(len(v), _ptr(v), _ptr(out)))
+    return out

@@ -74,6 +74,15 @@ SIGNATURES = {
     "gh_batch_likelihood": (_i, [_vp, _vp, _i, _i, _i, _vp]),
     "gh_batch_sample_info": (_i, [_vp, _i, _pd, _pd, _pd, _pi64]),
     "gh_batch_get_phi": (_i, [_vp, _i, _vp]),
+    "gh_tmcmc_create": (_i, [_i, _vp, _vp, _i, _i, _i, _i, _d, _d, _u64, C.POINTER(_vp)]),
+    "gh_tmcmc_destroy": (_i, [_vp]),
+    "gh_tmcmc_candidates": (_i, [_vp, _pi, _vp]),
+    "gh_tmcmc_submit": (_i, [_vp, _i, _vp]),
+    "gh_tmcmc_state": (_i, [_vp, _pi, _pd, _pd, _pd, _pi]),
+    "gh_tmcmc_population": (_i, [_vp, _pi, _vp, _vp]),
+    "gh_tmcmc_next_exponent": (_i, [_i, _vp, _d, _d, _pd, _pd]),
+    "gh_glioma_params": (_i, [_i, _vp, _i, _vp, _vp]),
+    "gh_glioma_fitness": (_i, [_i, _vp, _vp]),
     "gh_phantom": (_i, [_i, _u64, _vp, _vp, _vp, _pi]),
     "gh_gaussian_ic": (_i, [_i, _i, _vp, _vp, _pd, _d, _d, _vp]),
 }

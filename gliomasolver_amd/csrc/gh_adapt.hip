@@ -355,10 +355,10 @@ __global__ __launch_bounds__(512) void rd_step_ghosted_kernel(int nslots, int pe
 // _dumpUQoutput (Glioma_ReactionDiffusion.cpp:208-267): leaf cells onto the
 // finest uniform gpd^3 lattice; h = hf writes 1, 2hf writes 2^3, 3hf writes
 // 3^3 and anything coarser writes 4^3 cells (the reference's quirk is kept)
-template <typename Real>
+template <typename Real, bool WRITE>
 __global__ __launch_bounds__(512) void uq_resample_kernel(int nslots, const BlockRec* __restrict__ blocks,
                                                           const Real* __restrict__ phi, int gpd,
-                                                          double* __restrict__ out) {
+                                                          int* __restrict__ owner, double* __restrict__ out) {
     const int s = blockIdx.x;
     if (s >= nslots) return;
     const BlockRec b = blocks[s];
@@ -373,12 +373,18 @@ __global__ __launch_bounds__(512) void uq_resample_kernel(int nslots, const Bloc
     const int m[3] = {(int)floor(xs[0] / hf), (int)floor(xs[1] / hf), (int)floor(xs[2] / hf)};
     const double hd = (double)h;
     const int k = hd < hf + eps ? 1 : (hd < 2. * hf + eps ? 2 : (hd < 3. * hf + eps ? 3 : 4));
+    const int key = s * BCELLS + t;  // the reference's write order: blocks in order, cells z, y, x
     const double v = (double)phi[(size_t)s * BCELLS + t];
     for (int cz = 0; cz < k; ++cz)
         for (int cy = 0; cy < k; ++cy)
             for (int cx = 0; cx < k; ++cx) {
                 const int X = m[0] + cx, Y = m[1] + cy, Z = m[2] + cz;
-                if (X < gpd && Y < gpd && Z < gpd) out[(size_t)X + (size_t)gpd * (Y + (size_t)gpd * Z)] = v;
+                if (X >= gpd || Y >= gpd || Z >= gpd) continue;
+                const size_t o = (size_t)X + (size_t)gpd * (Y + (size_t)gpd * Z);
+                if (!WRITE)
+                    atomicMax(&owner[o], key);
+                else if (owner[o] == key)
+                    out[o] = v;
             }
 }
 
@@ -940,25 +946,40 @@ int gh_get_field_blocks(gh_ctx* ctx, int field, void* values) {
 
 int gh_uq_resample(gh_ctx* ctx, int gpd, double* out) {
     Ctx* c = as_ctx(ctx);
-    if (!c || !out || gpd <= 0 || gpd > 4096) return set_error(GH_EINVAL, "gh_uq_resample: bad argument");
+    if (!c || !out || gpd <= 0 || gpd > 2048) return set_error(GH_EINVAL, "gh_uq_resample: bad argument");
+    if ((size_t)c->topo.nblocks() * BCELLS >= (size_t)INT32_MAX)
+        return set_error(GH_EINVAL, "gh_uq_resample: grid too large");
     GH_HIP_CHECK(hipSetDevice(c->device));
     const size_t n = (size_t)gpd * gpd * gpd;
-    DevBuf d, blk;
+    DevBuf d, own, blk;
     int rc;
-    if ((rc = d.ensure(n * 8)) || (rc = blk.ensure(c->topo.blocks.size() * sizeof(BlockRec)))) return rc;
+    if ((rc = d.ensure(n * 8)) || (rc = own.ensure(n * sizeof(int))) ||
+        (rc = blk.ensure(c->topo.blocks.size() * sizeof(BlockRec))))
+        return rc;
     GH_HIP_CHECK(hipMemsetAsync(d.p, 0, n * 8, c->stream));
+    GH_HIP_CHECK(hipMemsetAsync(own.p, 0xff, n * sizeof(int), c->stream));  // -1: no writer
     GH_HIP_CHECK(hipMemcpyAsync(blk.p, c->topo.blocks.data(), c->topo.blocks.size() * sizeof(BlockRec),
                                 hipMemcpyHostToDevice, c->stream));
-    if (c->real_kind == GH_REAL_F32)
-        hipLaunchKernelGGL(uq_resample_kernel<float>, dim3(c->topo.nblocks()), dim3(512), 0, c->stream,
-                           c->topo.nblocks(), blk.as<BlockRec>(), c->phi[c->cur].as<float>(), gpd, d.as<double>());
-    else
-        hipLaunchKernelGGL(uq_resample_kernel<double>, dim3(c->topo.nblocks()), dim3(512), 0, c->stream,
-                           c->topo.nblocks(), blk.as<BlockRec>(), c->phi[c->cur].as<double>(), gpd, d.as<double>());
+    // where several cells land on one voxel (blocks finer than 1/gpd) the
+    // last writer of the reference's loop order wins: owner pass, then write pass
+    const dim3 g(c->topo.nblocks()), b(512);
+    const int ns = c->topo.nblocks();
+    if (c->real_kind == GH_REAL_F32) {
+        hipLaunchKernelGGL((uq_resample_kernel<float, false>), g, b, 0, c->stream, ns, blk.as<BlockRec>(),
+                           c->phi[c->cur].as<float>(), gpd, own.as<int>(), d.as<double>());
+        hipLaunchKernelGGL((uq_resample_kernel<float, true>), g, b, 0, c->stream, ns, blk.as<BlockRec>(),
+                           c->phi[c->cur].as<float>(), gpd, own.as<int>(), d.as<double>());
+    } else {
+        hipLaunchKernelGGL((uq_resample_kernel<double, false>), g, b, 0, c->stream, ns, blk.as<BlockRec>(),
+                           c->phi[c->cur].as<double>(), gpd, own.as<int>(), d.as<double>());
+        hipLaunchKernelGGL((uq_resample_kernel<double, true>), g, b, 0, c->stream, ns, blk.as<BlockRec>(),
+                           c->phi[c->cur].as<double>(), gpd, own.as<int>(), d.as<double>());
+    }
     hipError_t e = hipGetLastError();
     if (e == hipSuccess) e = hipMemcpyAsync(out, d.p, n * 8, hipMemcpyDeviceToHost, c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     d.release();
+    own.release();
     blk.release();
     if (e != hipSuccess) return set_error(GH_EHIP, std::string("gh_uq_resample: ") + hipGetErrorString(e));
     return GH_OK;

@@ -237,6 +237,42 @@ int gh_batch_likelihood(gh_batch* b, const double* lik_params, int stepPET, int 
 int gh_batch_sample_info(gh_batch* b, int sample, double* Dw, double* Dg, double* dt, int64_t* nsteps);
 int gh_batch_get_phi(gh_batch* b, int sample, void* dense);
 
+/* ---- TMCMC sampler (host) ----------------------------------------------------
+ * The sampler of tools/pi4u_lite/Inference/engine_tmcmc.c (main loop
+ * :1078-1290, prepare_newgen :754-1001, chaintask :683-738) and
+ * tmcmc_stats.c (calculate_statistics :545-760) as a state machine that hands
+ * out whole rounds of candidates, so one batched solve evaluates them:
+ *   gh_tmcmc_create: nth parameters uniform in [lo, hi] (tmcmc_glioma.par
+ *     B0..B10), PopSize, MaxStages, Min/MaxChainLength, TolCOV, bbeta, seed.
+ *   gh_tmcmc_candidates: the points to evaluate now (*n of them, nth doubles
+ *     each; points may be NULL to query n); *n == 0 once finished.
+ *   gh_tmcmc_submit: their log-likelihoods (non-finite values become the
+ *     reference's -1e12 penalty, fitfun_glioma.c:198-207), in the same order.
+ *   gh_tmcmc_state: generation, its annealing exponent p_j, coefficient of
+ *     variation, log evidence so far (sum of log-selections), finished flag.
+ *   gh_tmcmc_population: the last completed generation (curgen_db).
+ *   gh_tmcmc_next_exponent: p_{j+1} from log-likelihoods at p_j (stateless).
+ * The random stream is this library's own (xoshiro256**), not GSL's. */
+typedef struct gh_tmcmc gh_tmcmc;
+int gh_tmcmc_create(int nth, const double* lo, const double* hi, int pop_size, int max_stages, int min_chain,
+                    int max_chain, double tol_cov, double bbeta, uint64_t seed, gh_tmcmc** out);
+int gh_tmcmc_destroy(gh_tmcmc* t);
+int gh_tmcmc_candidates(gh_tmcmc* t, int* n, double* points);
+int gh_tmcmc_submit(gh_tmcmc* t, int n, const double* loglik);
+int gh_tmcmc_state(gh_tmcmc* t, int* gen, double* p, double* cov, double* log_evidence, int* done);
+int gh_tmcmc_population(gh_tmcmc* t, int* n, double* points, double* loglik);
+int gh_tmcmc_next_exponent(int n, const double* loglik, double p, double tol_cov, double* p_next, double* cov);
+/* gh_glioma_params: fitfun_glioma.c:118-152 for n points of 11 log-space
+ *   parameters: sample = {Dw, rho, Tend = sqrt(e^x2 / rho), icx, icy, icz}
+ *   and lik = {PETsigma2 = (e^x6)^2, PETscale, T1uc, T2uc, slope}, each
+ *   passed through the text the reference writes ("%.16lf", the centre
+ *   "%.4f") and read back as brain (Real) or likelihood (double) would.
+ * gh_glioma_fitness: -logL -> the value TMCMC receives: Likelihood.txt's
+ *   "%Lf" read back, negated; non-finite becomes -1e12 (fitfun_glioma.c:
+ *   190-210). */
+int gh_glioma_params(int n, const double* x, int real_kind, double* sample, double* lik);
+int gh_glioma_fitness(int n, const double* neg_loglik, double* F);
+
 /* ---- synthetic inputs (no reference counterpart: the reference reads
  * patient .dat files) -------------------------------------------------------
  * gh_phantom: seeded brain phantom on an n^3 vertex grid -- outer CSF shell,

@@ -26,10 +26,11 @@ def say(*a):
 
 
 def anatomy(shape, seed):
-    m = max(shape)
+    """Seeded phantom of twice the largest side, every second voxel, cropped."""
+    m = 2 * max(shape)
     gm, wm, csf, _ = gh.phantom(m, seed)
     nz, ny, nx = shape
-    return gm[:nz, :ny, :nx].copy(), wm[:nz, :ny, :nx].copy(), csf[:nz, :ny, :nx].copy()
+    return tuple(np.ascontiguousarray(a[::2, ::2, ::2][:nz, :ny, :nx]) for a in (gm, wm, csf))
 
 
 def observations(n, seed):
@@ -114,3 +115,51 @@ def test_batch_matches_oracle(batch_run, oracle):
         tot, comps, _ = oracle.likelihood(ref.astype(np.float64), obs[2], obs[0], obs[1], stepPET=2, stepTi=1)
         assert abs(lik[i, 0] - tot) <= 1e-9 * max(1.0, abs(tot)), "sample %d" % i
     assert len({i[3] for i in infos}) > 3      # the samples really had different step counts
+
+
+def test_tmcmc_glioma_app(tmp_path):
+    """The TMCMC driver end to end on a 16^3 grid: a few generations of a small
+    population, each generation one batched solve; generation 0's stored
+    log-likelihoods equal single solves of the same points through gh_ctx."""
+    import os
+    import subprocess
+
+    from conftest import ROOT
+    from gliomasolver_amd.datfile import read_dat, write_dat
+
+    gm, wm, csf = anatomy((20, 20, 20), 31)
+    for name, a in (("GM", gm), ("WM", wm), ("CSF", csf)):
+        write_dat(str(tmp_path / (name + ".dat")), a)
+    t1, fl, pet = observations(16, 32)
+    for name, a in (("tumT1c", t1), ("tumFLAIR", fl), ("tumPET", pet)):
+        write_dat(str(tmp_path / (name + ".dat")), a)
+    bounds = [(-3.0, -1.5), (-2.5, -1.0), (2.0, 3.0), (-0.9, -0.6), (-0.8, -0.6), (-0.9, -0.6), (-4.0, -1.4),
+              (-0.5, 0.0), (-0.5, -0.2), (-3.0, -0.5), (-3.0, -2.3)]
+    par = ["Nth 11", "MaxStages 3", "PopSize 48", "MinChainLength 0", "MaxChainLength 1", "Bdef 0 4"]
+    par += ["B%d %r %r" % (i, lo, hi) for i, (lo, hi) in enumerate(bounds)]
+    par += ["TolCOV 1", "bbeta 0.2", "seed 280675"]
+    (tmp_path / "tmcmc_glioma.par").write_text("\n".join(par) + "\n")
+    say("tmcmc_glioma 48 samples x 3 generations on 16^3")
+    r = subprocess.run([os.path.join(ROOT, "gliomasolver_amd", "bin", "tmcmc_glioma"), "-PatFileName",
+                        str(tmp_path) + "/", "-obs", str(tmp_path), "-bpd", "2", "-stepPET", "1"], cwd=tmp_path,
+                       capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    say(r.stdout.strip().splitlines()[-1])
+    db0 = np.loadtxt(tmp_path / "curgen_db_000.txt")
+    assert db0.shape == (48, 12)
+    assert (tmp_path / "curgen_db_002.txt").exists() and (tmp_path / "fitness.txt").exists()
+    for i in range(3):
+        x = db0[i, :11]
+        s, lk = gh.glioma_params(x[None], "f32")
+        s, lk = s[0], lk[0]
+        with gh.Context(0, "f32") as ctx:
+            ctx.grid_init_uniform(2)
+            ctx.ic_from_anatomy(gm, wm, csf, tuple(s[3:6]))
+            L = float(np.float32(20 * 0.1))
+            Dw, Dg, dt, ns = gh.schedule("f32", s[0], L, s[2], 16, 0.99)
+            ctx.rd_prepare(Dw, Dg, s[1])
+            ctx.rd_steps(ns, dt)
+            ctx.lik_set_data(t1, fl, pet)
+            nll, _, _ = ctx.likelihood(tuple(lk), stepPET=1, stepTi=1)
+        F = gh.glioma_fitness(np.array([nll]))[0]
+        assert abs(F - db0[i, 11]) <= 1e-9 * max(1.0, abs(F)), (i, F, db0[i, 11])
