@@ -9,9 +9,16 @@ the same initial condition, which is restored from a device-side checkpoint.
 Inputs are resident in HBM before the timed region starts.
 
 value      = leaf cells x Euler steps x solves / wall seconds (cell-updates/s)
-roofline   = 16 B per cell-update (read u, write u, read p_w, read p_g) x cells
-             per launch / average rd_step_kernel duration (HIP events on the
-             context stream), against 8 TB/s
+roofline   = 16 B per cell-update (read u, write u, read p_w, read p_g) x cell-
+             updates / device time of the step launches (HIP events on the
+             context stream), against 8 TB/s; per launch: 16 B x cells x the
+             launch's Euler steps (two for the temporally blocked kernel)
+batched    = one TMCMC stage (forward + likelihood + adjoint of every sample,
+             1,024 samples on 64^3/96^3/128^3 grids) solved by gh_batch in
+             memory-sized chunks; evals/s against the same samples one call
+             each (timed on the first samples and scaled by their share of
+             the stage's cell-steps); FP32 loss/gradient error against the
+             FP64 solve of the first samples
 cpu_baseline = the reference's ReactionDiffusionOperator + UpdateTumor
              (oracle/_ref/rd_ref_f32, built from the reference sources) on the
              host cores, on a bounded sample of the same workload; the oracle
@@ -84,6 +91,182 @@ def cpu_baseline(case, pw, pg, phi0, target_s=12.0):
             "sample": "%d Euler steps of the 128^3 FP32 workload (%.1f s), oracle restatement, 1 thread" % (k, el)}
 
 
+BATCH_LIK = (0.0025, 1.0, 0.7, 0.25, 2.0)  # PET noise sigma 0.05, PET scale 1; T1Gd/FLAIR thresholds 0.7/0.25
+
+
+def _stage_memory(cells, nsteps, K, real_size=4):
+    """Device bytes of one sample in gh_batch_adjoint: two states, statics (4+1 Real),
+    two FP64 adjoint states, the (K+1)-state tape and ceil(N/K) checkpoints."""
+    per_cell = real_size * (2 + 5 + (K + 1) + -(-nsteps // K)) + 16
+    return cells * per_cell
+
+
+def batched_leg(nsamples, K, budget_gb, seq_samples, acc_samples):
+    """One TMCMC stage of forward + likelihood + adjoint over a population of
+    64^3 / 96^3 / 128^3 samples (1:2:1): batched on the device (gh_batch), in
+    chunks that fit the memory budget, against the same samples solved one at a
+    time with the same kernels (a batch of one), and the FP32 results against
+    the FP64 solve of the first samples."""
+    import gliomasolver_amd as gh
+    from gliomasolver_amd import workloads as wl
+
+    st = wl.tmcmc_stage(nsamples)
+    cells = st.cells
+    log("batched stage: %d samples, sizes %s counts %s, %.3e forward cell-steps" %
+        (nsamples, st.sizes, np.bincount(st.anatomy, minlength=len(st.sizes)).tolist(), st.cell_steps()))
+
+    def make_batch(real):
+        bt = gh.Batch(0, real)
+        ids = [bt.add_anatomy(n // 8, *st.anatomies[a]) for a, n in enumerate(st.sizes)]
+        return bt, ids
+
+    b, aids = make_batch("f32")
+    # observations: thresholded, noisy copies of a ground-truth solve per anatomy
+    rng = np.random.default_rng(77)
+    truth_p, truth_n = [], []
+    for a, n in enumerate(st.sizes):
+        Dw = 0.006 / wl.L_CUBE ** 2
+        v = st.wm_voxels[a][len(st.wm_voxels[a]) // 2]
+        truth_p.append((Dw, 0.05, 300.0, float(v[0]), float(v[1]), float(v[2])))
+        truth_n.append(int(np.ceil(300.0 / (0.9 * (1.0 / n) ** 2 / (6 * Dw)))))
+    b.set_adjoint_samples(aids, truth_p, truth_n)
+    b.run()
+    obs = []
+    for a, n in enumerate(st.sizes):
+        u = b.get_phi(a).astype(np.float64)
+        t1 = (u >= 0.7).astype(np.float64)
+        fl = (u >= 0.25).astype(np.float64)
+        for m in (t1, fl):
+            f = rng.uniform(size=m.shape) < 0.05
+            m[f] = 1 - m[f]
+        pet = np.where(t1 > 0, u + rng.normal(0, 0.05, u.shape), 0.0)
+        obs.append((t1, fl, pet))
+        b.set_observations(aids[a], *obs[a])
+    # chunks under the budget
+    mem = np.array([_stage_memory(int(c), int(n), K) for c, n in zip(cells, st.nsteps)], np.float64)
+    chunks, cur, acc = [], [], 0.0
+    for i in range(nsamples):
+        if cur and acc + mem[i] > budget_gb * 1e9:
+            chunks.append(cur)
+            cur, acc = [], 0.0
+        cur.append(i)
+        acc += mem[i]
+    chunks.append(cur)
+    log("batched: %d chunks, largest %.1f GB" % (len(chunks), max(mem[c].sum() for c in chunks) / 1e9))
+
+    def solve(bt, idx):
+        bt.set_adjoint_samples(st.anatomy[idx], st.params[idx], st.nsteps[idx])
+        return bt.adjoint(BATCH_LIK, ckpt_every=K)
+
+    solve(b, [0, 1])  # warm-up (code objects, buffers)
+    b_loss = np.zeros((nsamples, 4))
+    b_grad = np.zeros((nsamples, 6))
+    dev_ms = 0.0
+    t0 = time.perf_counter()
+    for c in chunks:
+        loss, grad, ms = solve(b, c)
+        b_loss[c], b_grad[c] = loss, grad
+        dev_ms += ms
+    wall = time.perf_counter() - t0
+    log("batched: %d samples in %.2f s (device %.2f s)" % (nsamples, wall, dev_ms / 1e3))
+    # sequential: the first samples, one per call, same kernels
+    seq = list(range(min(seq_samples, nsamples)))
+    identical = True
+    t0 = time.perf_counter()
+    for i in seq:
+        loss, grad, _ = solve(b, [i])
+        identical &= bool(np.array_equal(loss[0], b_loss[i]) and np.array_equal(grad[0], b_grad[i]))
+    seq_wall = time.perf_counter() - t0
+    log("sequential: %d samples in %.2f s (identical to the batch: %s)" % (len(seq), seq_wall, identical))
+    b.close()
+    # FP32 state against the FP64 solve of the same samples
+    accs = list(range(min(acc_samples, nsamples)))
+    b64, ids64 = make_batch("f64")
+    for a in range(len(st.sizes)):
+        b64.set_observations(ids64[a], *obs[a])
+    loss64, grad64, _ = solve(b64, accs)
+    b64.close()
+    rel_loss = np.abs(b_loss[accs, 0] - loss64[:, 0]) / np.abs(loss64[:, 0])
+    rel_grad = np.abs(b_grad[accs] - grad64) / np.maximum(np.abs(grad64), 1e-300)
+    # work of the sequential sample against the whole stage's: evals/s of the plain path
+    seq_share = float((cells[seq] * st.nsteps[seq]).sum()) / float((cells * st.nsteps).sum())
+    seq_eval_s = len(seq) / seq_wall
+    seq_stage_s = seq_wall / seq_share  # the whole stage at the sequential rate per cell-step
+    out = {
+        "workload": "TMCMC stage: %d samples of (Dw, rho, T, x0, y0, z0), grids 64^3/96^3/128^3 = 1:2:1, "
+                    "T ~ U[100, 700] d, forward + likelihood + adjoint, FP32 state, FP64 likelihood and "
+                    "gradients, checkpoints every %d steps" % (nsamples, K),
+        "evals_per_s": nsamples / wall,
+        "stage_wall_s": wall,
+        "device_s": dev_ms / 1e3,
+        "chunks": len(chunks),
+        "forward_cell_steps": st.cell_steps(),
+        "sequential_evals_per_s": seq_eval_s,
+        "sequential_stage_s_est": seq_stage_s,
+        "speedup_vs_sequential": seq_stage_s / wall,
+        "sequential_sample": "first %d samples of the stage, one gh_batch call each (%.1f%% of its cell-steps), "
+                             "bit-identical results to the batch: %s" % (len(seq), 100 * seq_share, identical),
+        "accuracy_vs_fp64": {"samples": len(accs), "loss_rel_max": float(rel_loss.max()),
+                             "loss_rel_median": float(np.median(rel_loss)),
+                             "grad_rel_median": float(np.median(rel_grad))},
+    }
+    return out
+
+
+def adaptive_leg(tend, epoch_steps, max_epochs):
+    """The wavelet-adaptive workload: levels 3-5 of 8^3 blocks (256^3 at the
+    finest), 4th-order interpolating wavelets, FP32 state, seeded phantom,
+    Dw = 0.013 cm^2/day, rho = 0.025/day, Gaussian IC at a WM voxel; start on
+    the uniform finest level and compress (tolerance 1e-6), then every
+    epoch_steps Euler steps refine (1e-4, one pass) and compress (1e-6)."""
+    import gliomasolver_amd as gh
+    from gliomasolver_amd import FIELD_PG, FIELD_PHI, FIELD_PW
+    from gliomasolver_amd import workloads as wl
+
+    n = 256
+    gm, wm, csf, sv = gh.phantom(n, 3)
+    Dw, Dg, dt, nsteps = gh.schedule("f32", 0.013, wl.L_CUBE, tend, n, 0.9)
+    ctx = gh.Context(0, "f32")
+    ctx.grid_configure(4, 5, 3)
+    ctx.grid_init_uniform(n // 8)
+    ctx.set_anatomy(gm, wm, csf)
+    ctx.set_field(FIELD_PHI, gh.gaussian_ic("f32", ctx.get_field(FIELD_PW), ctx.get_field(FIELD_PG), sv, 1.0, 0.1))
+    ctx.rd_prepare(Dw, Dg, 0.025)
+    t0 = time.perf_counter()
+    ctx.compress(1e-6, -1)
+    ctx.sync()
+    t_init = time.perf_counter() - t0
+    log("adaptive: %d blocks after the initial compression (%.2f s)" % (ctx.nblocks(), t_init))
+    leaf_updates, steps_done, step_s, epoch_s, blocks = 0, 0, 0.0, 0.0, []
+    epochs = min(max_epochs, -(-nsteps // epoch_steps))
+    t0 = time.perf_counter()
+    for e in range(epochs):
+        k = min(epoch_steps, nsteps - steps_done)
+        nb = ctx.nblocks()
+        blocks.append(nb)
+        t1 = time.perf_counter()
+        ctx.rd_steps(k, dt)
+        ctx.sync()
+        t2 = time.perf_counter()
+        ctx.refine(1e-4, 1)
+        ctx.compress(1e-6, -1)
+        ctx.sync()
+        t3 = time.perf_counter()
+        step_s += t2 - t1
+        epoch_s += t3 - t2
+        leaf_updates += nb * 512 * k
+        steps_done += k
+    wall = time.perf_counter() - t0
+    ctx.close()
+    return {"metric": "adaptive RD leaf cell-updates/s", "value": leaf_updates / wall, "unit": "cell-updates/s",
+            "config": {"workload": "wavelet-adaptive levels 3-5 (256^3 finest), order-4 interpolating wavelets, "
+                                   "FP32, refine 1e-4 / compress 1e-6 every %d steps" % epoch_steps,
+                       "steps": steps_done, "of_total_steps": nsteps, "dt_days": dt, "epochs": epochs,
+                       "blocks_min_mean_max": [min(blocks), float(np.mean(blocks)), max(blocks)]},
+            "step_only_cell_updates_per_s": leaf_updates / step_s, "step_s": step_s, "epoch_s": epoch_s,
+            "epoch_ms_mean": 1e3 * epoch_s / max(1, epochs), "initial_compress_s": t_init, "wall_s": wall}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -91,11 +274,24 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--seed", type=int, default=2)
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched TMCMC-stage leg")
+    ap.add_argument("--batch-samples", type=int, default=1024)
+    ap.add_argument("--batch-ckpt", type=int, default=32, help="checkpoint interval of the batched adjoint")
+    ap.add_argument("--batch-budget-gb", type=float, default=120.0, help="device memory per batched chunk")
+    ap.add_argument("--batch-seq", type=int, default=64, help="samples timed one at a time")
+    ap.add_argument("--batch-acc", type=int, default=24, help="samples re-solved in FP64")
+    ap.add_argument("--workload", default="uniform128", choices=["uniform128", "adaptive"],
+                    help="adaptive: the levels 3-5 wavelet-adaptive run instead (its own JSON line)")
+    ap.add_argument("--adaptive-epochs", type=int, default=200)
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path (samples shard with no "
               "intra-stage traffic; see the design notes). Use --gpus 1.", file=sys.stderr)
         return 2
+
+    if a.workload == "adaptive":
+        print(json.dumps(adaptive_leg(500.0, 10, a.adaptive_epochs)), flush=True)
+        return 0
 
     import gliomasolver_amd as gh
     from gliomasolver_amd import FIELD_PG, FIELD_PHI, FIELD_PW
@@ -107,7 +303,8 @@ def main():
     pw, pg, phi0 = ctx.get_field(FIELD_PW), ctx.get_field(FIELD_PG), ctx.get_field(FIELD_PHI)
     ctx.checkpoint_save(0)
     tile = ctx.rd_autotune(case.dt, 100)
-    log("autotuned tile: %d blocks of 8^3 per workgroup" % tile)
+    log("autotuned schedule: %s" % ("two Euler steps per launch" if tile == 0 else
+                                    "%d blocks of 8^3 per workgroup" % tile))
     for i in range(a.warmup):
         ctx.checkpoint_load(0)
         ctx.rd_steps(case.nsteps, case.dt)
@@ -127,16 +324,21 @@ def main():
 
     updates = case.ncells * case.nsteps * a.steps
     value = updates / wall
-    launches = case.nsteps * a.steps
+    steps_per_launch = 2 if tile == 0 else 1
+    launches = (case.nsteps // steps_per_launch + case.nsteps % steps_per_launch) * a.steps
     avg_launch_s = kernel_ms * 1e-3 / launches
-    achieved = BYTES_PER_CELL_UPDATE_F32 * case.ncells / avg_launch_s / 1e9
+    # algorithmic bytes of all cell-updates over the device time of all launches
+    achieved = BYTES_PER_CELL_UPDATE_F32 * updates / (kernel_ms * 1e-3) / 1e9
     traffic = None
     if os.path.exists(TRAFFIC_FILE):
-        traffic = json.load(open(TRAFFIC_FILE)).get("bytes_per_launch")
+        tf = json.load(open(TRAFFIC_FILE))
+        if tf.get("kernel", "rd_step_kernel") == ("rd_step2_kernel" if tile == 0 else "rd_step_kernel"):
+            traffic = tf.get("bytes_per_launch")
+    kname = "rd_step2_kernel<float> (2 Euler steps per launch)" if tile == 0 else "rd_step_kernel<float,true,%d>" % tile
     roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-            "kernel": "rd_step_kernel<float,true,%d>" % tile, "avg_launch_us": round(avg_launch_s * 1e6, 3),
-            "algorithmic_bytes_per_launch": BYTES_PER_CELL_UPDATE_F32 * case.ncells}
+            "kernel": kname, "avg_launch_us": round(avg_launch_s * 1e6, 3), "steps_per_launch": steps_per_launch,
+            "algorithmic_bytes_per_launch": BYTES_PER_CELL_UPDATE_F32 * case.ncells * steps_per_launch}
     cpu = None
     if not a.no_cpu:
         log("cpu baseline ...")
@@ -152,6 +354,10 @@ def main():
             "roofline": roof, "cpu_baseline": cpu}
     if cpu:
         line["gpu_over_cpu"] = round(value / cpu["value"], 1)
+    if not a.no_batch:
+        log("batched TMCMC stage ...")
+        line["batched"] = batched_leg(a.batch_samples, a.batch_ckpt, a.batch_budget_gb, a.batch_seq, a.batch_acc)
+        log("batched", json.dumps(line["batched"]))
     print(json.dumps(line), flush=True)
     return 0
 

@@ -352,6 +352,28 @@ class Batch:
         check(self.lib.gh_batch_run(self.h, C.byref(ms)))
         return ms.value
 
+    def set_adjoint_samples(self, anatomy, params, nsteps, ic_sigma: float = 1.0, ic_peak: float = 0.1):
+        """params: (N, 6) rows {Dw (/L^2), rho, T, x0, y0, z0 (cells)}; nsteps: N per sample."""
+        a = np.ascontiguousarray(anatomy, np.int32)
+        p = np.ascontiguousarray(params, np.float64).reshape(len(a), 6)
+        n = np.ascontiguousarray(nsteps, np.int64)
+        check(self.lib.gh_batch_set_adjoint_samples(self.h, len(a), _ptr(a), _ptr(p), _ptr(n), ic_sigma, ic_peak))
+        self.ns = len(a)
+        self.anat_of = [int(v) for v in a]
+
+    def adjoint(self, lik_params, ckpt_every: int = 64, stepPET=1, stepTi=1, use_roi=False):
+        """Forward + loss + adjoint of every sample: (loss (N, 4), grad (N, 6), device ms)."""
+        lp = np.asarray(lik_params, np.float64)
+        if lp.ndim == 1:
+            lp = np.tile(lp, (self.ns, 1))
+        lp = np.ascontiguousarray(lp)
+        loss = np.zeros((self.ns, 4))
+        grad = np.zeros((self.ns, 6))
+        ms = C.c_double()
+        check(self.lib.gh_batch_adjoint(self.h, ckpt_every, _ptr(lp), stepPET, stepTi, int(use_roi), _ptr(loss),
+                                        _ptr(grad), C.byref(ms)))
+        return loss, grad, ms.value
+
     def likelihood(self, lik_params, stepPET=1, stepTi=1, use_roi=False) -> np.ndarray:
         """(N, 4) rows {-logL, L_PET, L_T1, L_T2}; lik_params (N, 5) or one row for all."""
         lp = np.asarray(lik_params, np.float64)

@@ -72,6 +72,8 @@ SIGNATURES = {
     "gh_batch_set_samples": (_i, [_vp, _i, _vp, _vp, _d]),
     "gh_batch_run": (_i, [_vp, _pd]),
     "gh_batch_likelihood": (_i, [_vp, _vp, _i, _i, _i, _vp]),
+    "gh_batch_set_adjoint_samples": (_i, [_vp, _i, _vp, _vp, _vp, _d, _d]),
+    "gh_batch_adjoint": (_i, [_vp, _i, _vp, _i, _i, _i, _vp, _vp, _pd]),
     "gh_batch_sample_info": (_i, [_vp, _i, _pd, _pd, _pd, _pi64]),
     "gh_batch_get_phi": (_i, [_vp, _i, _vp]),
     "gh_tmcmc_create": (_i, [_i, _vp, _vp, _i, _i, _i, _i, _d, _d, _u64, C.POINTER(_vp)]),

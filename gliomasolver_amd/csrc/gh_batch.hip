@@ -11,8 +11,9 @@
 // Layout: the samples' blocks are packed slot range after slot range, the
 // samples sorted by step count (descending), so at step k the samples still
 // running occupy a prefix of the slots and one launch of the single-solve step
-// kernel (rd_step_kernel with per-slot parameters) advances all of them.
-// Finished samples keep their final state in the buffer of their parity.
+// kernels (rd_step2_kernel / rd_step_kernel with per-slot parameters)
+// advances all of them by two (or one) Euler steps.  Finished samples keep
+// their final state in the buffer that was current when they ended.
 // The arithmetic per cell is the single-solve path's, so every sample is
 // bit-identical to solving it alone (gh_ic_from_anatomy + gh_rd_prepare +
 // gh_rd_steps + gh_likelihood); the likelihood uses the same fixed 1024-way
@@ -26,6 +27,7 @@
 #include "gh_anatomy.h"
 #include "gh_ctx.h"
 #include "gh_lik_common.h"
+#include "gh_adjoint_common.h"
 #include "gh_rd_common.h"
 
 #pragma clang fp contract(off)
@@ -39,6 +41,7 @@ struct BatchAnatomy {
     double L = 0;                // (Real)(0.1 * max(nx, ny, nz)), the characteristic length in cm
     DevBuf pw, pg;               // Real, block-major
     std::vector<unsigned char> tissue;  // host, Real per dense cell: voted pGM + pWM (gates the tumour)
+    std::vector<unsigned char> gate;    // host, per dense cell: p_w + p_g > 0 (gates a Gaussian IC)
     DevBuf t1, fl, pet, roi;     // observations (block-major), ROI {cell, ix, iy, iz}
     int nroi = 0;
     bool has_obs = false;
@@ -51,14 +54,61 @@ struct SlotRange {
     size_t aslot0;   // the anatomy's first slot is at aslot0 in its own arrays (always 0)
     Real rho, Dw, Dg;
     double ih2, dt;
+    double rho_d, invN, invDw;  // adjoint: rho and 1/Dw as given, 1/N
 };
 
 struct BatchSample {
     int anat;
-    double Dw, Dg, rho, dt;  // schedule values (Real, held in double)
+    double Dw, Dg, rho, dt;  // schedule values (Real, held in double); adjoint: as given, dt = T/N
     int64_t nsteps;
-    double ic[3];
+    double ic[3];            // reference IC: centre in the unit cube; adjoint: Gaussian centre in cells
+    double T;                // adjoint only
 };
+
+template <typename Real>
+struct BatchAdjParams {
+    const int32_t* __restrict__ slot_pos;
+    const SlotRange<Real>* __restrict__ range;
+    __device__ inline void get(int s, double& r, double& i, double& d, double& n, double& w) const {
+        const SlotRange<Real>& q = range[slot_pos[s]];
+        r = q.rho_d;
+        i = q.ih2;
+        d = q.dt;
+        n = q.invN;
+        w = q.invDw;
+    }
+};
+
+// one sample's reverse-sweep seed and initial-condition gradient inputs
+template <typename Real>
+struct AdjJob {
+    const Real* u;        // final state (seed) or initial state (IC gradient)
+    double* lam;          // the sample's adjoint range
+    const unsigned char* t1;
+    const unsigned char* fl;
+    const double* pet;
+    size_t ncells;
+    int bpd;
+    LikParams p;
+    double x0, y0, z0, is2;
+};
+
+template <typename Real>
+__global__ __launch_bounds__(256) void batch_seed_kernel(const AdjJob<Real>* __restrict__ jobs, int full_bern) {
+    const AdjJob<Real>& j = jobs[blockIdx.y];
+    const size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= j.ncells) return;
+    Layout lay{0, 0, 0, j.bpd};
+    lik_seed_cell<Real>(c, lay, j.u, j.t1, j.fl, j.pet, j.p, full_bern, j.lam);
+}
+
+template <typename Real>
+__global__ __launch_bounds__(256) void batch_ic_partials(const AdjJob<Real>* __restrict__ jobs,
+                                                         double* __restrict__ partial) {
+    const AdjJob<Real>& j = jobs[blockIdx.y];
+    adj_ic_body<Real>((int)blockIdx.x, (int)gridDim.x, j.ncells, j.bpd, j.lam, j.u, j.x0, j.y0, j.z0, j.is2,
+                      partial + (size_t)blockIdx.y * 3 * gridDim.x);
+}
 
 template <typename Real>
 struct BatchStepParams {
@@ -183,7 +233,10 @@ struct Batch {
     std::vector<int> pos_of;        // sample index -> position
     std::vector<size_t> slot0;      // per position, plus the total
     DevBuf phi[2], statD, statSig, nbr, slotPos, range, apw, apg, likPart, likOut, likJobs;
-    bool ran = false;
+    DevBuf ck, tape, lam[2], apart, icpart, adjJobs;  // adjoint
+    double ic_is2 = 1.0;                                 // adjoint: 1 / sigma^2 of the Gaussian IC
+    std::vector<int> final_buf;                          // per position: phi buffer of the final state
+    bool ran = false, adjoint = false;
     size_t nslots() const { return slot0.empty() ? 0 : slot0.back(); }
 };
 
@@ -200,6 +253,8 @@ int add_anatomy_t(Batch& b, BatchAnatomy& a, int nx, int ny, int nz, const doubl
     anatomy_from_data<Real>(a.bpd, nx, ny, nz, gm, wm, csf, pw.data(), pg.data(), pc.data(), tis.data());
     a.tissue.resize(nc * sizeof(Real));
     std::memcpy(a.tissue.data(), tis.data(), nc * sizeof(Real));
+    a.gate.resize(nc);
+    for (size_t c = 0; c < nc; ++c) a.gate[c] = ((double)pw[c] + (double)pg[c] > 0.) ? 1 : 0;
     const int bmax = std::max(nx, std::max(ny, nz));
     a.L = (double)(Real)(bmax * 0.1);  // Glioma_ReactionDiffusion.cpp:95
     int rc;
@@ -214,22 +269,11 @@ int add_anatomy_t(Batch& b, BatchAnatomy& a, int nx, int ny, int nz, const doubl
     return GH_OK;
 }
 
+// Positions (samples by step count, descending), neighbour tables, per-slot
+// parameters, statics, and phi[0] = 0.  b.smp must be complete.
 template <typename Real>
-int set_samples_t(Batch& b, const double* params, double cfl) {
+int build_layout_t(Batch& b) {
     const int ns = (int)b.smp.size();
-    // schedules: the reference's Real arithmetic (gh_schedule)
-    for (int i = 0; i < ns; ++i) {
-        BatchSample& q = b.smp[i];
-        const BatchAnatomy& a = *b.anat[q.anat];
-        const double* pp = params + 6 * i;
-        int64_t nsteps = 0;
-        int rc = gh_schedule(b.real_kind, pp[0], a.L, pp[2], BS * a.bpd, cfl, &q.Dw, &q.Dg, &q.dt, &nsteps);
-        if (rc) return rc;
-        q.nsteps = nsteps;
-        q.rho = (double)(Real)pp[1];
-        for (int d = 0; d < 3; ++d) q.ic[d] = pp[3 + d];
-    }
-    // positions: by step count, descending (ties by sample index)
     b.order.resize(ns);
     std::iota(b.order.begin(), b.order.end(), 0);
     std::stable_sort(b.order.begin(), b.order.end(),
@@ -241,8 +285,7 @@ int set_samples_t(Batch& b, const double* params, double cfl) {
         b.slot0[p + 1] = b.slot0[p] + b.anat[b.smp[b.order[p]].anat]->nslots;
     }
     const size_t nsl = b.nslots();
-    if (nsl * BCELLS > (size_t)1 << 40) return set_error(GH_EINVAL, "gh_batch_set_samples: batch too large");
-    if (nsl > (size_t)INT32_MAX / 8) return set_error(GH_EINVAL, "gh_batch_set_samples: too many blocks");
+    if (nsl > (size_t)INT32_MAX / 8) return set_error(GH_EINVAL, "gh_batch: too many blocks");
     const size_t nc = nsl * BCELLS;
     int rc;
     if ((rc = b.phi[0].ensure(nc * sizeof(Real))) || (rc = b.phi[1].ensure(nc * sizeof(Real))) ||
@@ -260,8 +303,7 @@ int set_samples_t(Batch& b, const double* params, double cfl) {
         const BatchAnatomy& a = *b.anat[q.anat];
         periodic_neighbours(a.bpd, b.slot0[p], nbr);
         for (size_t s = b.slot0[p]; s < b.slot0[p + 1]; ++s) spos[s] = p;
-        const double h = std::ldexp(1.0, 0) / (double)(BS * a.bpd);  // info.h[0] of a uniform grid
-        const Real hr = (Real)h;
+        const Real hr = (Real)(1.0 / (double)(BS * a.bpd));  // info.h[0] of a uniform grid
         rng[p].slot0 = b.slot0[p];
         rng[p].aslot0 = 0;
         rng[p].rho = (Real)q.rho;
@@ -269,6 +311,9 @@ int set_samples_t(Batch& b, const double* params, double cfl) {
         rng[p].Dg = (Real)q.Dg;
         rng[p].ih2 = 1. / ((double)hr * (double)hr);
         rng[p].dt = q.dt;
+        rng[p].rho_d = q.rho;
+        rng[p].invN = 1.0 / (double)q.nsteps;
+        rng[p].invDw = 1.0 / q.Dw;
         apw[p] = a.pw.p;
         apg[p] = a.pg.p;
     }
@@ -279,14 +324,51 @@ int set_samples_t(Batch& b, const double* params, double cfl) {
                                 b.stream));
     GH_HIP_CHECK(hipMemcpyAsync(b.apw.p, apw.data(), ns * sizeof(void*), hipMemcpyHostToDevice, b.stream));
     GH_HIP_CHECK(hipMemcpyAsync(b.apg.p, apg.data(), ns * sizeof(void*), hipMemcpyHostToDevice, b.stream));
-    // statics
     const unsigned g = (unsigned)((nc + 255) / 256);
     hipLaunchKernelGGL(batch_statics_kernel<Real>, dim3(g), dim3(256), 0, b.stream, nsl, b.nbr.as<int32_t>(),
                        b.slotPos.as<int32_t>(), b.range.as<SlotRange<Real>>(), b.apw.as<const Real* const>(),
                        b.apg.as<const Real* const>(), b.statD.as<Stat4<Real>>(), b.statSig.as<Real>());
     GH_HIP_CHECK(hipGetLastError());
-    // initial condition: zero, then the tumour cells of every sample
     GH_HIP_CHECK(hipMemsetAsync(b.phi[0].p, 0, nc * sizeof(Real), b.stream));
+    GH_HIP_CHECK(hipStreamSynchronize(b.stream));
+    return GH_OK;
+}
+
+// phi[0][idx] = val for the nonzero initial cells of every sample
+template <typename Real>
+int scatter_ic(Batch& b, const std::vector<uint64_t>& idx, const std::vector<Real>& val) {
+    if (idx.empty()) return GH_OK;
+    DevBuf di, dv;
+    int rc;
+    if ((rc = di.ensure(idx.size() * sizeof(uint64_t))) || (rc = dv.ensure(val.size() * sizeof(Real)))) return rc;
+    GH_HIP_CHECK(hipMemcpyAsync(di.p, idx.data(), idx.size() * sizeof(uint64_t), hipMemcpyHostToDevice, b.stream));
+    GH_HIP_CHECK(hipMemcpyAsync(dv.p, val.data(), val.size() * sizeof(Real), hipMemcpyHostToDevice, b.stream));
+    hipLaunchKernelGGL(scatter_kernel<Real>, dim3((unsigned)((idx.size() + 255) / 256)), dim3(256), 0, b.stream,
+                       idx.size(), di.as<uint64_t>(), dv.as<Real>(), b.phi[0].as<Real>());
+    GH_HIP_CHECK(hipGetLastError());
+    GH_HIP_CHECK(hipStreamSynchronize(b.stream));
+    return GH_OK;
+}
+
+template <typename Real>
+int set_samples_t(Batch& b, const double* params, double cfl) {
+    const int ns = (int)b.smp.size();
+    // schedules: the reference's Real arithmetic (gh_schedule)
+    for (int i = 0; i < ns; ++i) {
+        BatchSample& q = b.smp[i];
+        const BatchAnatomy& a = *b.anat[q.anat];
+        const double* pp = params + 6 * i;
+        int64_t nsteps = 0;
+        int rc = gh_schedule(b.real_kind, pp[0], a.L, pp[2], BS * a.bpd, cfl, &q.Dw, &q.Dg, &q.dt, &nsteps);
+        if (rc) return rc;
+        q.nsteps = nsteps;
+        q.rho = (double)(Real)pp[1];
+        q.T = pp[2];
+        for (int d = 0; d < 3; ++d) q.ic[d] = pp[3 + d];
+    }
+    int rc = build_layout_t<Real>(b);
+    if (rc) return rc;
+    // the reference's tumour: a few hundred cells per sample
     std::vector<uint64_t> idx;
     std::vector<Real> val;
     std::vector<std::pair<size_t, Real>> cells;
@@ -299,40 +381,124 @@ int set_samples_t(Batch& b, const double* params, double cfl) {
             val.push_back(e.second);
         }
     }
-    if (!idx.empty()) {
-        DevBuf di, dv;
-        if ((rc = di.ensure(idx.size() * sizeof(uint64_t))) || (rc = dv.ensure(val.size() * sizeof(Real)))) return rc;
-        GH_HIP_CHECK(hipMemcpyAsync(di.p, idx.data(), idx.size() * sizeof(uint64_t), hipMemcpyHostToDevice, b.stream));
-        GH_HIP_CHECK(hipMemcpyAsync(dv.p, val.data(), val.size() * sizeof(Real), hipMemcpyHostToDevice, b.stream));
-        hipLaunchKernelGGL(scatter_kernel<Real>, dim3((unsigned)((idx.size() + 255) / 256)), dim3(256), 0, b.stream,
-                           idx.size(), di.as<uint64_t>(), dv.as<Real>(), b.phi[0].as<Real>());
-        GH_HIP_CHECK(hipGetLastError());
-        GH_HIP_CHECK(hipStreamSynchronize(b.stream));
-        di.release();
-        dv.release();
-    }
-    GH_HIP_CHECK(hipStreamSynchronize(b.stream));
+    if ((rc = scatter_ic<Real>(b, idx, val))) return rc;
     b.ran = false;
+    b.adjoint = false;
     return GH_OK;
 }
 
+// The adjoint problem of gh_rd_adjoint per sample: Dw and rho as given
+// (Dg = Dw/10), dt = T/N, Gaussian IC peak*exp(-|x-x0|^2/(2 sigma^2)) on
+// cells with p_w + p_g > 0 (gh_gaussian_ic), evaluated in double exactly as
+// there, only where it does not round to 0 in Real.
+template <typename Real>
+int set_adjoint_samples_t(Batch& b, const double* params, const int64_t* nsteps, double sigma, double peak) {
+    const int ns = (int)b.smp.size();
+    for (int i = 0; i < ns; ++i) {
+        BatchSample& q = b.smp[i];
+        const double* pp = params + 6 * i;
+        q.Dw = pp[0];
+        q.Dg = pp[0] / 10.0;
+        q.rho = pp[1];
+        q.T = pp[2];
+        q.nsteps = nsteps[i];
+        q.dt = pp[2] / (double)nsteps[i];
+        for (int d = 0; d < 3; ++d) q.ic[d] = pp[3 + d];
+    }
+    int rc = build_layout_t<Real>(b);
+    if (rc) return rc;
+    const double is2 = 1.0 / (2.0 * sigma * sigma);
+    // beyond exponent qmax, peak*exp(-q) is below half the smallest positive Real
+    // (in logs: half the smallest double denormal is not representable)
+    const double log_tiny = sizeof(Real) == 4 ? std::log(1.401298464324817e-45) : std::log(4.940656458412465e-324);
+    const double qmax = std::log(std::fabs(peak)) - log_tiny + std::log(2.0) + 2.0;
+    const double R = peak == 0 ? -1.0 : std::min(std::sqrt(std::max(qmax, 0.0) / is2) + 1.0, 1.0e6);
+    std::vector<uint64_t> idx;
+    std::vector<Real> val;
+    for (int p = 0; p < ns; ++p) {
+        const BatchSample& q = b.smp[b.order[p]];
+        const BatchAnatomy& a = *b.anat[q.anat];
+        const int n = BS * a.bpd;
+        if (R < 0) continue;
+        int lo[3], hi[3];
+        for (int d = 0; d < 3; ++d) {
+            lo[d] = (int)std::max(0.0, std::floor(q.ic[d] - R));
+            hi[d] = (int)std::min((double)(n - 1), std::ceil(q.ic[d] + R));
+        }
+        for (int k = lo[2]; k <= hi[2]; ++k)
+            for (int j = lo[1]; j <= hi[1]; ++j)
+                for (int i = lo[0]; i <= hi[0]; ++i) {
+                    const size_t c = (size_t)i + (size_t)n * ((size_t)j + (size_t)n * k);
+                    if (!a.gate[c]) continue;
+                    const double d2 = (i - q.ic[0]) * (i - q.ic[0]) + (j - q.ic[1]) * (j - q.ic[1]) +
+                                      (k - q.ic[2]) * (k - q.ic[2]);
+                    const Real v = (Real)(peak * std::exp(-d2 * is2));
+                    if (v == (Real)0) continue;
+                    idx.push_back(b.slot0[p] * BCELLS + block_index(a.bpd, c));
+                    val.push_back(v);
+                }
+    }
+    if ((rc = scatter_ic<Real>(b, idx, val))) return rc;
+    b.ran = false;
+    b.adjoint = true;
+    return GH_OK;
+}
+
+// Launches of the forward sweep: one or two Euler steps over the prefix of
+// running samples, reading phi[cur] and writing phi[1 - cur].  Two steps are
+// taken when no running sample ends after the first of them and (with
+// checkpoints every K) the state after the first is not a checkpoint.  A
+// sample's final state is in the buffer current when it ends.
+template <typename Real>
+struct ForwardSweep {
+    Batch& b;
+    std::vector<int64_t> npos;  // step count per position (non-increasing)
+    BatchStepParams<Real> prm;
+    int cur = 0, active = 0;
+    explicit ForwardSweep(Batch& bb)
+        : b(bb), prm{bb.slotPos.as<int32_t>(), bb.range.as<SlotRange<Real>>()} {
+        const int ns = (int)b.smp.size();
+        npos.resize(ns);
+        for (int p = 0; p < ns; ++p) npos[p] = b.smp[b.order[p]].nsteps;
+        active = ns;
+        b.final_buf.assign(ns, 0);
+    }
+    // retire the samples that have reached step k
+    void retire(int64_t k) {
+        while (active > 0 && npos[active - 1] <= k) b.final_buf[--active] = cur;
+    }
+    // one launch from step k; returns the steps taken
+    int launch(int64_t k, int K) {
+        const int nsl = (int)b.slot0[active];
+        const bool two = npos[active - 1] >= k + 2 && (K <= 0 || (k + 1) % K != 0);
+        const Real* in = b.phi[cur].as<Real>();
+        Real* out = b.phi[1 - cur].as<Real>();
+        if (two) {
+            const int per_xcd = (nsl + 7) / 8;
+            hipLaunchKernelGGL((rd_step2_kernel<Real, BatchStepParams<Real>>), dim3(8 * per_xcd), dim3(512), 0,
+                               b.stream, nsl, per_xcd, b.nbr.as<int32_t>(), in, out, b.statD.as<Stat4<Real>>(),
+                               b.statSig.as<Real>(), prm);
+        } else {
+            const int per_xcd = ((nsl + 1) / 2 + 7) / 8;  // two blocks per workgroup
+            hipLaunchKernelGGL((rd_step_kernel<Real, true, 2, BatchStepParams<Real>>), dim3(8 * per_xcd), dim3(512),
+                               0, b.stream, nsl, per_xcd, b.nbr.as<int32_t>(), in, out, b.statD.as<Stat4<Real>>(),
+                               b.statSig.as<Real>(), prm);
+        }
+        cur = 1 - cur;
+        return two ? 2 : 1;
+    }
+};
+
 template <typename Real>
 int run_t(Batch& b, double* elapsed_ms) {
-    const int ns = (int)b.smp.size();
-    const int64_t maxsteps = ns ? b.smp[b.order[0]].nsteps : 0;
-    const BatchStepParams<Real> prm{b.slotPos.as<int32_t>(), b.range.as<SlotRange<Real>>()};
+    ForwardSweep<Real> fw(b);
+    const int64_t maxsteps = fw.npos.empty() ? 0 : fw.npos[0];
     GH_HIP_CHECK(hipEventRecord(b.ev0, b.stream));
-    int active = ns;
-    for (int64_t k = 0; k < maxsteps; ++k) {
-        while (active > 0 && b.smp[b.order[active - 1]].nsteps <= k) --active;
-        const int nsl = (int)b.slot0[active];
-        const int groups = (nsl + 1) / 2;  // two blocks per workgroup
-        const int per_xcd = (groups + 7) / 8;
-        const Real* in = b.phi[k & 1].as<Real>();
-        Real* out = b.phi[(k + 1) & 1].as<Real>();
-        hipLaunchKernelGGL((rd_step_kernel<Real, true, 2, BatchStepParams<Real>>), dim3(8 * per_xcd), dim3(512), 0,
-                           b.stream, nsl, per_xcd, b.nbr.as<int32_t>(), in, out, b.statD.as<Stat4<Real>>(),
-                           b.statSig.as<Real>(), prm);
+    int64_t k = 0;
+    while (true) {
+        fw.retire(k);
+        if (k >= maxsteps) break;
+        k += fw.launch(k, 0);
     }
     GH_HIP_CHECK(hipGetLastError());
     GH_HIP_CHECK(hipEventRecord(b.ev1, b.stream));
@@ -347,8 +513,7 @@ int run_t(Batch& b, double* elapsed_ms) {
 // the final state of the sample at position p
 template <typename Real>
 const Real* final_phi(const Batch& b, int p) {
-    const BatchSample& q = b.smp[b.order[p]];
-    return b.phi[q.nsteps & 1].as<Real>() + b.slot0[p] * BCELLS;
+    return b.phi[b.final_buf[p]].as<Real>() + b.slot0[p] * BCELLS;
 }
 
 template <typename Real>
@@ -387,6 +552,145 @@ int likelihood_t(Batch& b, const double* lik, int stepPET, int stepTi, int use_r
                                 b.stream));
     GH_HIP_CHECK(hipStreamSynchronize(b.stream));
     for (int p = 0; p < ns; ++p) finish(&sums[4 * (size_t)p], prm[p], out + 4 * (size_t)b.order[p], nullptr);
+    return GH_OK;
+}
+
+
+// Forward sweep with checkpoints every K steps, loss, seed, reverse sweep
+// with per-segment recompute, initial-condition gradient -- gh_rd_adjoint for
+// every sample, the samples still running at a step forming a slot prefix in
+// both directions (the reverse of sample s starts at its own step N_s - 1,
+// with its seed placed in the adjoint buffer of parity N_s).
+template <typename Real>
+int adjoint_t(Batch& b, int K, const double* lik, int stepPET, int stepTi, int use_roi, double* loss, double* grad,
+              double* elapsed_ms) {
+    const int ns = (int)b.smp.size();
+    const int64_t Nmax = b.smp[b.order[0]].nsteps;
+    std::vector<int64_t> npos(ns);
+    for (int p = 0; p < ns; ++p) npos[p] = b.smp[b.order[p]].nsteps;
+    auto active = [&](int64_t k) {  // positions with N > k (a prefix: npos is non-increasing)
+        return (int)(std::upper_bound(npos.begin(), npos.end(), k, [](int64_t v, int64_t e) { return v >= e; }) -
+                     npos.begin());
+    };
+    const size_t slot_bytes = BCELLS * sizeof(Real);
+    const int64_t J = (Nmax + K - 1) / K;
+    std::vector<size_t> ckoff(J + 1, 0);
+    for (int64_t j = 0; j < J; ++j) ckoff[j + 1] = ckoff[j] + b.slot0[active(j * K)];
+    const size_t nsl = b.nslots(), nc = nsl * BCELLS;
+    constexpr int IC_GRID = 512;
+    int rc;
+    if ((rc = b.ck.ensure(ckoff[J] * slot_bytes)) || (rc = b.tape.ensure((size_t)(K + 1) * nsl * slot_bytes)) ||
+        (rc = b.lam[0].ensure(nc * 8)) || (rc = b.lam[1].ensure(nc * 8)) || (rc = b.apart.ensure(3 * nsl * 8)) ||
+        (rc = b.icpart.ensure((size_t)ns * 3 * IC_GRID * 8)) || (rc = b.adjJobs.ensure(ns * sizeof(AdjJob<Real>))))
+        return rc;
+    const BatchStepParams<Real> fprm{b.slotPos.as<int32_t>(), b.range.as<SlotRange<Real>>()};
+    const BatchAdjParams<Real> aprm{b.slotPos.as<int32_t>(), b.range.as<SlotRange<Real>>()};
+    auto step = [&](const void* in, void* out, int nslots) {  // one Euler step (tape recompute)
+        const int per_xcd = ((nslots + 1) / 2 + 7) / 8;
+        hipLaunchKernelGGL((rd_step_kernel<Real, true, 2, BatchStepParams<Real>>), dim3(8 * per_xcd), dim3(512), 0,
+                           b.stream, nslots, per_xcd, b.nbr.as<int32_t>(), (const Real*)in, (Real*)out,
+                           b.statD.as<Stat4<Real>>(), b.statSig.as<Real>(), fprm);
+    };
+    char* ck = (char*)b.ck.p;
+    char* tape = (char*)b.tape.p;
+    GH_HIP_CHECK(hipEventRecord(b.ev0, b.stream));
+    // forward, with a checkpoint of the running samples every K steps
+    ForwardSweep<Real> fw(b);
+    for (int64_t k = 0;;) {
+        fw.retire(k);
+        if (k >= Nmax) break;
+        if (k % K == 0)
+            GH_HIP_CHECK(hipMemcpyAsync(ck + ckoff[k / K] * slot_bytes, b.phi[fw.cur].p, b.slot0[fw.active] * slot_bytes,
+                                        hipMemcpyDeviceToDevice, b.stream));
+        k += fw.launch(k, K);
+    }
+    GH_HIP_CHECK(hipGetLastError());
+    b.ran = true;
+    // loss at every sample's final state
+    std::vector<double> out((size_t)ns * 4);
+    if ((rc = likelihood_t<Real>(b, lik, stepPET, stepTi, use_roi, out.data()))) return rc;
+    // seeds dJ/du at u_N into the adjoint buffer of parity N
+    std::vector<AdjJob<Real>> jobs(ns);
+    size_t maxc = 0;
+    for (int p = 0; p < ns; ++p) {
+        const int i = b.order[p];
+        const BatchAnatomy& a = *b.anat[b.smp[i].anat];
+        LikParams lp;
+        if ((rc = check_params(lik + 5 * i, stepPET, stepTi, lp))) return rc;
+        jobs[p] = AdjJob<Real>{final_phi<Real>(b, p), b.lam[b.smp[i].nsteps & 1].as<double>() + b.slot0[p] * BCELLS,
+                               a.t1.as<unsigned char>(), a.fl.as<unsigned char>(), a.pet.as<double>(),
+                               a.nslots * BCELLS, a.bpd, lp, 0, 0, 0, 0};
+        maxc = std::max(maxc, a.nslots * BCELLS);
+    }
+    GH_HIP_CHECK(hipMemcpyAsync(b.adjJobs.p, jobs.data(), ns * sizeof(AdjJob<Real>), hipMemcpyHostToDevice, b.stream));
+    for (int p0 = 0; p0 < ns; p0 += 65535)
+        hipLaunchKernelGGL(batch_seed_kernel<Real>, dim3((unsigned)((maxc + 255) / 256), std::min(65535, ns - p0)),
+                           dim3(256), 0, b.stream, b.adjJobs.as<AdjJob<Real>>() + p0, use_roi ? 0 : 1);
+    GH_HIP_CHECK(hipGetLastError());
+    if (use_roi) {
+        for (int p = 0; p < ns; ++p) {
+            const int i = b.order[p];
+            const BatchAnatomy& a = *b.anat[b.smp[i].anat];
+            if (a.nroi == 0) continue;
+            hipLaunchKernelGGL(lik_seed_roi_kernel<Real>, dim3((unsigned)((a.nroi + 255) / 256)), dim3(256), 0,
+                               b.stream, a.nroi, a.roi.as<int4>(), jobs[p].u, a.t1.as<unsigned char>(),
+                               a.fl.as<unsigned char>(), jobs[p].p, jobs[p].lam);
+        }
+        GH_HIP_CHECK(hipGetLastError());
+    }
+    // reverse
+    GH_HIP_CHECK(hipMemsetAsync(b.apart.p, 0, 3 * nsl * 8, b.stream));
+    for (int64_t j = J - 1; j >= 0; --j) {
+        const int64_t k0 = j * K, k1 = std::min<int64_t>(Nmax, k0 + K);
+        GH_HIP_CHECK(hipMemcpyAsync(tape, ck + ckoff[j] * slot_bytes, b.slot0[active(k0)] * slot_bytes,
+                                    hipMemcpyDeviceToDevice, b.stream));
+        for (int64_t k = k0; k < k1; ++k)  // recompute u_{k0+1} .. u_{k1} of the running samples
+            step(tape + (size_t)(k - k0) * nsl * slot_bytes, tape + (size_t)(k - k0 + 1) * nsl * slot_bytes,
+                 (int)b.slot0[active(k)]);
+        for (int64_t k = k1 - 1; k >= k0; --k) {
+            const int nslots = (int)b.slot0[active(k)];
+            const int per_xcd = (nslots + 7) / 8;
+            hipLaunchKernelGGL((adj_step_kernel<Real, BatchAdjParams<Real>>), dim3(8 * per_xcd), dim3(512), 0,
+                               b.stream, nslots, per_xcd, b.nbr.as<int32_t>(),
+                               (const Real*)(tape + (size_t)(k - k0) * nsl * slot_bytes),
+                               (const Real*)(tape + (size_t)(k - k0 + 1) * nsl * slot_bytes),
+                               b.lam[(k + 1) & 1].as<double>(), b.lam[k & 1].as<double>(), b.statD.as<Stat4<Real>>(),
+                               b.statSig.as<Real>(), aprm, b.apart.as<double>());
+        }
+        GH_HIP_CHECK(hipGetLastError());
+    }
+    // initial-condition gradient: lambda_0 (buffer 0) against u_0 (checkpoint 0)
+    for (int p = 0; p < ns; ++p) {
+        const BatchSample& q = b.smp[b.order[p]];
+        jobs[p].u = (const Real*)(ck) + b.slot0[p] * BCELLS;
+        jobs[p].lam = b.lam[0].as<double>() + b.slot0[p] * BCELLS;
+        jobs[p].x0 = q.ic[0];
+        jobs[p].y0 = q.ic[1];
+        jobs[p].z0 = q.ic[2];
+        jobs[p].is2 = b.ic_is2;
+    }
+    GH_HIP_CHECK(hipMemcpyAsync(b.adjJobs.p, jobs.data(), ns * sizeof(AdjJob<Real>), hipMemcpyHostToDevice, b.stream));
+    for (int p0 = 0; p0 < ns; p0 += 65535)
+        hipLaunchKernelGGL(batch_ic_partials<Real>, dim3(IC_GRID, std::min(65535, ns - p0)), dim3(256), 0, b.stream,
+                           b.adjJobs.as<AdjJob<Real>>() + p0, b.icpart.as<double>() + (size_t)p0 * 3 * IC_GRID);
+    GH_HIP_CHECK(hipGetLastError());
+    GH_HIP_CHECK(hipEventRecord(b.ev1, b.stream));
+    std::vector<double> ap(3 * nsl), ip((size_t)ns * 3 * IC_GRID);
+    GH_HIP_CHECK(hipMemcpyAsync(ap.data(), b.apart.p, ap.size() * 8, hipMemcpyDeviceToHost, b.stream));
+    GH_HIP_CHECK(hipMemcpyAsync(ip.data(), b.icpart.p, ip.size() * 8, hipMemcpyDeviceToHost, b.stream));
+    GH_HIP_CHECK(hipStreamSynchronize(b.stream));
+    float ms = 0;
+    GH_HIP_CHECK(hipEventElapsedTime(&ms, b.ev0, b.ev1));
+    if (elapsed_ms) *elapsed_ms = ms;
+    for (int p = 0; p < ns; ++p) {
+        const int i = b.order[p];
+        double* gi = grad + 6 * (size_t)i;
+        sum_slot_partials(ap.data() + 3 * b.slot0[p], (int)(b.slot0[p + 1] - b.slot0[p]), gi);
+        gi[3] = gi[4] = gi[5] = 0;
+        for (int w = 0; w < IC_GRID; ++w)
+            for (int qd = 0; qd < 3; ++qd) gi[3 + qd] += ip[((size_t)p * IC_GRID + w) * 3 + qd];
+        for (int qd = 0; qd < 4; ++qd) loss[4 * (size_t)i + qd] = out[4 * (size_t)i + qd];
+    }
     return GH_OK;
 }
 
@@ -429,8 +733,9 @@ int gh_batch_destroy(gh_batch* bh) {
         a->pet.release();
         a->roi.release();
     }
-    DevBuf* bufs[] = {&b->phi[0], &b->phi[1], &b->statD, &b->statSig, &b->nbr, &b->slotPos, &b->range,
-                      &b->apw,    &b->apg,    &b->likPart, &b->likOut, &b->likJobs};
+    DevBuf* bufs[] = {&b->phi[0], &b->phi[1], &b->statD,  &b->statSig, &b->nbr,    &b->slotPos, &b->range,
+                      &b->apw,    &b->apg,    &b->likPart, &b->likOut, &b->likJobs, &b->ck,     &b->tape,
+                      &b->lam[0], &b->lam[1], &b->apart,   &b->icpart, &b->adjJobs};
     for (DevBuf* d : bufs) d->release();
     if (b->ev0) (void)hipEventDestroy(b->ev0);
     if (b->ev1) (void)hipEventDestroy(b->ev1);
@@ -510,8 +815,8 @@ int gh_batch_set_samples(gh_batch* bh, int nsamples, const int* anatomy_id, cons
             return set_error(GH_EINVAL, "gh_batch_set_samples: no such anatomy");
         const double* p = params + 6 * i;
         if (!(p[0] > 0) || !(p[1] >= 0) || !(p[2] >= 0) || !std::isfinite(p[0]) || !std::isfinite(p[1]) ||
-            !std::isfinite(p[2]))
-            return set_error(GH_EINVAL, "gh_batch_set_samples: need Dw > 0, rho >= 0, Tend >= 0 (finite)");
+            !std::isfinite(p[2]) || !std::isfinite(p[3]) || !std::isfinite(p[4]) || !std::isfinite(p[5]))
+            return set_error(GH_EINVAL, "gh_batch_set_samples: need Dw > 0, rho >= 0, Tend >= 0 (all finite)");
         const double steps = p[2] * 6.0 * p[0] / (b->anat[anatomy_id[i]]->L * b->anat[anatomy_id[i]]->L) /
                              (cfl / ((double)BS * b->anat[anatomy_id[i]]->bpd * BS * b->anat[anatomy_id[i]]->bpd));
         if (steps > 1e8) return set_error(GH_EINVAL, "gh_batch_set_samples: more than 1e8 steps for a sample");
@@ -541,6 +846,43 @@ int gh_batch_likelihood(gh_batch* bh, const double* lik_params, int stepPET, int
                                        : likelihood_t<double>(*b, lik_params, stepPET, stepTi, use_roi, out);
 }
 
+int gh_batch_set_adjoint_samples(gh_batch* bh, int nsamples, const int* anatomy_id, const double* params,
+                                 const int64_t* nsteps, double ic_sigma, double ic_peak) {
+    Batch* b = as_batch(bh);
+    if (!b || nsamples <= 0 || !anatomy_id || !params || !nsteps)
+        return set_error(GH_EINVAL, "gh_batch_set_adjoint_samples: bad argument");
+    if (!(ic_sigma > 0) || !std::isfinite(ic_peak))
+        return set_error(GH_EINVAL, "gh_batch_set_adjoint_samples: ic_sigma > 0 and a finite peak");
+    for (int i = 0; i < nsamples; ++i) {
+        if (anatomy_id[i] < 0 || anatomy_id[i] >= (int)b->anat.size())
+            return set_error(GH_EINVAL, "gh_batch_set_adjoint_samples: no such anatomy");
+        const double* p = params + 6 * i;
+        if (!(p[0] > 0) || !(p[1] >= 0) || !(p[2] > 0) || !std::isfinite(p[0]) || !std::isfinite(p[1]) ||
+            !std::isfinite(p[2]) || !std::isfinite(p[3]) || !std::isfinite(p[4]) || !std::isfinite(p[5]) ||
+            nsteps[i] < 1 || nsteps[i] > 100000000)
+            return set_error(GH_EINVAL, "gh_batch_set_adjoint_samples: need Dw > 0, rho >= 0, T > 0, 1 <= N <= 1e8");
+    }
+    GH_HIP_CHECK(hipSetDevice(b->device));
+    b->smp.assign(nsamples, BatchSample{});
+    for (int i = 0; i < nsamples; ++i) b->smp[i].anat = anatomy_id[i];
+    b->ic_is2 = 1.0 / (ic_sigma * ic_sigma);
+    return b->real_kind == GH_REAL_F32 ? set_adjoint_samples_t<float>(*b, params, nsteps, ic_sigma, ic_peak)
+                                       : set_adjoint_samples_t<double>(*b, params, nsteps, ic_sigma, ic_peak);
+}
+
+int gh_batch_adjoint(gh_batch* bh, int ckpt_every, const double* lik_params, int stepPET, int stepTi, int use_roi,
+                     double* loss, double* grad, double* elapsed_ms) {
+    Batch* b = as_batch(bh);
+    if (!b || !lik_params || !loss || !grad) return set_error(GH_EINVAL, "gh_batch_adjoint: NULL argument");
+    if (!b->adjoint) return set_error(GH_ESTATE, "gh_batch_adjoint: set the samples with gh_batch_set_adjoint_samples");
+    if (b->ran) return set_error(GH_ESTATE, "gh_batch_adjoint: already run; set the samples again");
+    if (ckpt_every < 1 || ckpt_every > 4096) return set_error(GH_EINVAL, "gh_batch_adjoint: 1 <= ckpt_every <= 4096");
+    GH_HIP_CHECK(hipSetDevice(b->device));
+    return b->real_kind == GH_REAL_F32
+               ? adjoint_t<float>(*b, ckpt_every, lik_params, stepPET, stepTi, use_roi, loss, grad, elapsed_ms)
+               : adjoint_t<double>(*b, ckpt_every, lik_params, stepPET, stepTi, use_roi, loss, grad, elapsed_ms);
+}
+
 int gh_batch_sample_info(gh_batch* bh, int sample, double* Dw, double* Dg, double* dt, int64_t* nsteps) {
     Batch* b = as_batch(bh);
     if (!b || sample < 0 || sample >= (int)b->smp.size()) return set_error(GH_EINVAL, "gh_batch_sample_info: no such sample");
@@ -560,8 +902,8 @@ int gh_batch_get_phi(gh_batch* bh, int sample, void* dense) {
     const int p = b->pos_of[sample];
     const BatchAnatomy& a = *b->anat[b->smp[sample].anat];
     const size_t nc = a.nslots * BCELLS;
-    const int64_t k = b->ran ? b->smp[sample].nsteps : 0;
-    const unsigned char* src = (const unsigned char*)b->phi[k & 1].p + b->slot0[p] * BCELLS * b->real_size;
+    const int buf = b->ran ? b->final_buf[p] : 0;  // before the run: the initial state
+    const unsigned char* src = (const unsigned char*)b->phi[buf].p + b->slot0[p] * BCELLS * b->real_size;
     std::vector<unsigned char> bm(nc * b->real_size);
     GH_HIP_CHECK(hipMemcpyAsync(bm.data(), src, bm.size(), hipMemcpyDeviceToHost, b->stream));
     GH_HIP_CHECK(hipStreamSynchronize(b->stream));

@@ -314,6 +314,8 @@ int gh_ic_from_anatomy(gh_ctx* ctx, int nx, int ny, int nz, const double* gm, co
     if (c->topo.uniform_bpd <= 0) return set_error(GH_ESTATE, "gh_ic_from_anatomy: needs a uniform grid");
     if (nx <= 0 || ny <= 0 || nz <= 0 || !gm || !wm || !csf)
         return set_error(GH_EINVAL, "gh_ic_from_anatomy: bad argument");
+    if (tumor_ic && !(std::isfinite(tumor_ic[0]) && std::isfinite(tumor_ic[1]) && std::isfinite(tumor_ic[2])))
+        return set_error(GH_EINVAL, "gh_ic_from_anatomy: the tumour centre must be finite");
     GH_HIP_CHECK(hipSetDevice(c->device));
     return c->real_kind == GH_REAL_F32 ? ic_from_anatomy_t<float>(c, nx, ny, nz, gm, wm, csf, tumor_ic)
                                        : ic_from_anatomy_t<double>(c, nx, ny, nz, gm, wm, csf, tumor_ic);
@@ -418,8 +420,9 @@ int gh_rd_steps(gh_ctx* ctx, int64_t nsteps, double dt, double* elapsed_ms) {
 int gh_rd_set_tile(gh_ctx* ctx, int blocks_per_workgroup) {
     Ctx* c = as_ctx(ctx);
     if (!c) return set_error(GH_EINVAL, "null context");
-    if (blocks_per_workgroup != 1 && blocks_per_workgroup != 2 && blocks_per_workgroup != 4)
-        return set_error(GH_EINVAL, "gh_rd_set_tile: blocks_per_workgroup must be 1, 2 or 4");
+    if (blocks_per_workgroup != 0 && blocks_per_workgroup != 1 && blocks_per_workgroup != 2 &&
+        blocks_per_workgroup != 4)
+        return set_error(GH_EINVAL, "gh_rd_set_tile: 1, 2 or 4 blocks per workgroup, or 0 (two steps per launch)");
     c->rd_bpw = blocks_per_workgroup;
     return GH_OK;
 }

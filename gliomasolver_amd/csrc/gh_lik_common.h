@@ -124,6 +124,59 @@ __device__ inline void lik_final_body(int nparts, const double* __restrict__ par
     if (threadIdx.x < 4) out[threadIdx.x] = red[threadIdx.x][0];
 }
 
+// d/du of log(alpha) (y == 1) or log(1 - alpha): 0 for u >= uc (alpha is
+// constant above uc and flat at uc), alpha = e/2 with e = exp(-(u-uc)^2/slope) below
+__device__ inline double dlog_bernoulli(double u, bool y, double uc, double is2) {
+    const double diff = u - uc;
+    if (!(diff < 0.)) return 0.;
+    const double a = 0.5 * exp(-diff * diff * is2);
+    const double da = a * (-2. * diff * is2);
+    return y ? da / a : -da / (1. - a);
+}
+
+// g = dJ/du of J = -logL at cell c: PET lattice term and (without ROI) the Bernoulli terms
+template <typename T>
+__device__ inline void lik_seed_cell(size_t c, const Layout& lay, const T* __restrict__ u,
+                                     const unsigned char* __restrict__ t1, const unsigned char* __restrict__ fl,
+                                     const double* __restrict__ pet, const LikParams& p, int full_bern,
+                                     double* __restrict__ g) {
+    int ix, iy, iz;
+    lay.coords(c, ix, iy, iz);
+    const double uc = (double)u[c];
+    double v = 0;
+    if ((ix % p.stepPET == 0) && (iy % p.stepPET == 0) && (iz % p.stepPET == 0) && pet[c] > 0.)
+        v += (uc - p.b * pet[c]) / p.s2;
+    if (full_bern) v -= dlog_bernoulli(uc, t1[c] != 0, p.uc1, p.is2) + dlog_bernoulli(uc, fl[c] != 0, p.uc2, p.is2);
+    g[c] = v;
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void lik_seed_kernel(size_t ncells, Layout lay, const T* __restrict__ u,
+                                                       const unsigned char* __restrict__ t1,
+                                                       const unsigned char* __restrict__ fl,
+                                                       const double* __restrict__ pet, LikParams p, int full_bern,
+                                                       double* __restrict__ g) {
+    const size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ncells) return;
+    lik_seed_cell<T>(c, lay, u, t1, fl, pet, p, full_bern, g);
+}
+
+// Bernoulli terms at ROI points on the stepMRI lattice (a point listed twice counts twice)
+template <typename T>
+__global__ __launch_bounds__(256) void lik_seed_roi_kernel(int nroi, const int4* __restrict__ roi,
+                                                           const T* __restrict__ u,
+                                                           const unsigned char* __restrict__ t1,
+                                                           const unsigned char* __restrict__ fl, LikParams p,
+                                                           double* __restrict__ g) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nroi) return;
+    const int4 q = roi[i];
+    if ((q.y % p.stepTi != 0) || (q.z % p.stepTi != 0) || (q.w % p.stepTi != 0)) return;
+    const double uc = (double)u[q.x];
+    atomicAdd(&g[q.x], -(dlog_bernoulli(uc, t1[q.x] != 0, p.uc1, p.is2) +
+                         dlog_bernoulli(uc, fl[q.x] != 0, p.uc2, p.is2)));
+}
+
 inline int check_params(const double* params, int stepPET, int stepTi, LikParams& p) {
     if (!params) return set_error(GH_EINVAL, "likelihood: params is NULL");
     if (stepPET < 1 || stepTi < 1) return set_error(GH_EINVAL, "likelihood: steps must be >= 1");

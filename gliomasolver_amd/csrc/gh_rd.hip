@@ -102,6 +102,21 @@ void launch_step(Ctx& c, const Real* in, Real* out, double dt) {
                        c.statSig.as<Real>(), prm);
 }
 
+// two fused steps per launch (rd_step2_kernel), one block per workgroup
+template <typename Real>
+int launch_step2(Ctx& c, const Real* in, Real* out, double dt) {
+    const int ns = c.topo.nblocks();
+    const int per_xcd = (ns + 7) / 8;
+    const double h = block_h(c);
+    const double ih2 = 1. / (h * h);
+    const UniformStepParams<Real> prm{(Real)c.rho, ih2, dt};
+    hipLaunchKernelGGL((rd_step2_kernel<Real, UniformStepParams<Real>>), dim3(8 * per_xcd), dim3(512), 0, c.stream,
+                       ns, per_xcd, c.nbr.as<int32_t>(), in, out, c.statD.as<Stat4<Real>>(), c.statSig.as<Real>(),
+                       prm);
+    GH_HIP_CHECK(hipGetLastError());
+    return GH_OK;
+}
+
 template <typename Real, bool FUSED>
 int step_t(Ctx& c, const Real* in, Real* out, double dt) {
     switch (c.rd_bpw) {
@@ -156,7 +171,7 @@ int rd_update(Ctx& c, double dt) {
 // (saved in the dphidt scratch and restored after every trial) and keeps the
 // fastest.
 int rd_autotune(Ctx& c, double dt, int trial_steps, int* best_out) {
-    static const int cand[3] = {1, 2, 4};
+    static const int cand[4] = {1, 2, 4, 0};
     if (c.topo.uniform_bpd <= 0) {  // the multi-level kernel has a single tile
         if (best_out) *best_out = c.rd_bpw;
         return GH_OK;
@@ -166,7 +181,7 @@ int rd_autotune(Ctx& c, double dt, int trial_steps, int* best_out) {
     const int cur0 = c.cur;
     float best_ms = 1e30f;
     int best = c.rd_bpw;
-    for (int k = 0; k < 3; ++k) {
+    for (int k = 0; k < 4; ++k) {
         c.rd_bpw = cand[k];
         int rc = rd_steps(c, 2, dt);  // warm
         if (rc) return rc;
@@ -196,7 +211,16 @@ int rd_step_buffers(Ctx& c, const void* in, void* out, double dt) {
 int rd_steps(Ctx& c, int64_t nsteps, double dt) {
     c.dt_last = dt;
     const bool ghosted = c.topo.uniform_bpd <= 0;
-    for (int64_t k = 0; k < nsteps; ++k) {
+    int64_t k = 0;
+    if (!ghosted && c.rd_bpw == 0)
+        for (; k + 1 < nsteps; k += 2) {
+            int rc = c.real_kind == GH_REAL_F32
+                         ? launch_step2<float>(c, c.phi[c.cur].as<float>(), c.phi[1 - c.cur].as<float>(), dt)
+                         : launch_step2<double>(c, c.phi[c.cur].as<double>(), c.phi[1 - c.cur].as<double>(), dt);
+            if (rc) return rc;
+            c.cur = 1 - c.cur;
+        }
+    for (; k < nsteps; ++k) {
         int rc = ghosted ? rd_step_ghosted(c, c.phi[c.cur].p, c.phi[1 - c.cur].p, dt, true)
                  : c.real_kind == GH_REAL_F32 ? step_t<float, true>(c, c.phi[1 - c.cur].as<float>())
                                               : step_t<double, true>(c, c.phi[1 - c.cur].as<double>());

@@ -244,4 +244,127 @@ __global__ __launch_bounds__(512) void rd_step_kernel(int nslots, int per_xcd, c
     }
 }
 
+
+// Two fused Euler steps per launch (temporal blocking): one workgroup per 8^3
+// block.  Step 1 is evaluated on the block and its six depth-1 face halos
+// (the cells step 2 reads), from the state on the block, the depth-1 and
+// depth-2 face halos and the twelve edge rows next to the depth-1 halos; step
+// 2 then runs on the block from the step-1 values in LDS.  A halo cell's
+// step-1 value is computed with exactly the inputs and arithmetic its own
+// block uses, so two launches of rd_step_kernel and one of this kernel give
+// the same bits.  Per two steps a block reads 1,376 states and constant
+// records (512 + 864 halo) instead of 2 x 896.
+__device__ inline int T12(int x, int y, int z) { return (z + 2) * 144 + (y + 2) * 12 + (x + 2); }  // [-2,10)^3
+
+template <typename Real, typename Params>
+__global__ __launch_bounds__(512) void rd_step2_kernel(int nslots, int per_xcd, const int32_t* __restrict__ nbr,
+                                                       const Real* __restrict__ phi, Real* __restrict__ out,
+                                                       const Stat4<Real>* __restrict__ st,
+                                                       const Real* __restrict__ sig, Params prm) {
+    const int s = (int)(blockIdx.x % 8) * per_xcd + (int)(blockIdx.x / 8);
+    if (s >= nslots) return;
+    __shared__ Real s_u0[1728], s_tau[1728], s_dx[1728], s_dy[1728], s_dz[1728];
+    __shared__ Real s_u1[1000];
+
+    const int t = threadIdx.x;
+    const int x = t & 7, y = (t >> 3) & 7, z = t >> 6;
+    const size_t g = (size_t)s * BCELLS + t;
+    const Real pc0 = phi[g];
+    const Stat4<Real> sv = st[g];
+    const Real sc = sig[g];
+    // halo cells: [0,384) depth-1 faces, [384,768) depth-2 faces, [768,864) edge rows
+    Real hsig = 0;         // sigma of this lane's depth-1 face cell (lanes < 384)
+    int fx = 0, fy = 0, fz = 0;
+#pragma unroll
+    for (int it = 0; it < 2; ++it) {
+        const int h = t + 512 * it;
+        if (h >= 864) break;
+        int src_slot, lx, ly, lz, tx, ty, tz;
+        if (h < 768) {
+            const int d = h < 384 ? 1 : 2;
+            const int r = h % 384, f = r >> 6, a = r & 7, b = (r >> 3) & 7;
+            src_slot = nbr[6 * s + f];
+            switch (f) {
+                case FXM: tx = -d; ty = a; tz = b; lx = 8 - d; ly = a; lz = b; break;
+                case FXP: tx = 7 + d; ty = a; tz = b; lx = d - 1; ly = a; lz = b; break;
+                case FYM: tx = a; ty = -d; tz = b; lx = a; ly = 8 - d; lz = b; break;
+                case FYP: tx = a; ty = 7 + d; tz = b; lx = a; ly = d - 1; lz = b; break;
+                case FZM: tx = a; ty = b; tz = -d; lx = a; ly = b; lz = 8 - d; break;
+                default:  tx = a; ty = b; tz = 7 + d; lx = a; ly = b; lz = d - 1; break;
+            }
+        } else {
+            const int e = (h - 768) >> 3, i = (h - 768) & 7;
+            const int pair = e >> 2, s1 = e & 1, s2 = (e >> 1) & 1;
+            const int c1 = s1 ? 8 : -1, c2 = s2 ? 8 : -1;  // tile coordinate on the two crossed axes
+            const int l1 = s1 ? 0 : 7, l2 = s2 ? 0 : 7;    // local coordinate in the diagonal block
+            int f1, f2;
+            if (pair == 0) {         // x and y crossed, z free
+                f1 = s1 ? FXP : FXM; f2 = s2 ? FYP : FYM;
+                tx = c1; ty = c2; tz = i; lx = l1; ly = l2; lz = i;
+            } else if (pair == 1) {  // x and z crossed, y free
+                f1 = s1 ? FXP : FXM; f2 = s2 ? FZP : FZM;
+                tx = c1; ty = i; tz = c2; lx = l1; ly = i; lz = l2;
+            } else {                 // y and z crossed, x free
+                f1 = s1 ? FYP : FYM; f2 = s2 ? FZP : FZM;
+                tx = i; ty = c1; tz = c2; lx = i; ly = l1; lz = l2;
+            }
+            src_slot = nbr[6 * nbr[6 * s + f1] + f2];
+        }
+        const size_t gs = (size_t)src_slot * BCELLS + lx + 8 * ly + 64 * lz;
+        const int q = T12(tx, ty, tz);
+        const Stat4<Real> hs = st[gs];
+        s_u0[q] = phi[gs];
+        s_tau[q] = hs.tau;
+        s_dx[q] = hs.dx;
+        s_dy[q] = hs.dy;
+        s_dz[q] = hs.dz;
+        if (h < 384) {
+            hsig = sig[gs];
+            fx = tx; fy = ty; fz = tz;
+        }
+    }
+    {
+        const int q = T12(x, y, z);
+        s_u0[q] = pc0;
+        s_tau[q] = sv.tau;
+        s_dx[q] = sv.dx;
+        s_dy[q] = sv.dy;
+        s_dz[q] = sv.dz;
+    }
+    Real rho;
+    double ih2, dt;
+    prm.get(s, rho, ih2, dt);
+    __syncthreads();
+
+    // step 1 on the block cell of this lane and (lanes < 384) one depth-1 face cell
+    auto step1 = [&](int cx, int cy, int cz, Real sc1) {
+        const int q = T12(cx, cy, cz);
+        const int n0 = T12(cx - 1, cy, cz), n1 = T12(cx + 1, cy, cz), n2 = T12(cx, cy - 1, cz),
+                  n3 = T12(cx, cy + 1, cz), n4 = T12(cx, cy, cz - 1), n5 = T12(cx, cy, cz + 1);
+        const Real p0 = s_u0[n0], p1 = s_u0[n1], p2 = s_u0[n2], p3 = s_u0[n3], p4 = s_u0[n4], p5 = s_u0[n5];
+        const Real pc = s_u0[q];
+        const Real d = rd_cell_rhs<Real>(s_dx[n0], s_dx[q], s_dy[n2], s_dy[q], s_dz[n4], s_dz[q], p0, p1, p2, p3, p4,
+                                         p5, p0 < s_tau[n0], p1 < s_tau[n1], p2 < s_tau[n2], p3 < s_tau[n3],
+                                         p4 < s_tau[n4], p5 < s_tau[n5], pc, sc1, rho, ih2);
+        s_u1[P(cx, cy, cz)] = rd_update(pc, d, dt);
+    };
+    step1(x, y, z, sc);
+    if (t < 384) step1(fx, fy, fz, hsig);
+    __syncthreads();
+
+    // step 2 on the block
+    {
+        const int q = T12(x, y, z);
+        const int n0 = T12(x - 1, y, z), n1 = T12(x + 1, y, z), n2 = T12(x, y - 1, z), n3 = T12(x, y + 1, z),
+                  n4 = T12(x, y, z - 1), n5 = T12(x, y, z + 1);
+        const Real p0 = s_u1[P(x - 1, y, z)], p1 = s_u1[P(x + 1, y, z)], p2 = s_u1[P(x, y - 1, z)],
+                   p3 = s_u1[P(x, y + 1, z)], p4 = s_u1[P(x, y, z - 1)], p5 = s_u1[P(x, y, z + 1)];
+        const Real pc = s_u1[P(x, y, z)];
+        const Real d = rd_cell_rhs<Real>(s_dx[n0], s_dx[q], s_dy[n2], s_dy[q], s_dz[n4], s_dz[q], p0, p1, p2, p3, p4,
+                                         p5, p0 < s_tau[n0], p1 < s_tau[n1], p2 < s_tau[n2], p3 < s_tau[n3],
+                                         p4 < s_tau[n4], p5 < s_tau[n5], pc, sc, rho, ih2);
+        out[g] = rd_update(pc, d, dt);
+    }
+}
+
 }  // namespace gh

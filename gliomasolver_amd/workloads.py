@@ -83,3 +83,53 @@ def load_context(case: Case, device: int = 0) -> Context:
     ctx.set_field(FIELD_PHI, gaussian_ic(case.real, pw, pg, case.seed_voxel, 1.0, 0.1))
     ctx.rd_prepare(case.Dw, case.Dg, case.rho)
     return ctx
+
+
+@dataclass
+class Stage:
+    """One TMCMC stage of samples over several grid sizes (see tmcmc_stage)."""
+    sizes: list           # grid size per anatomy
+    anatomies: list       # (gm, wm, csf) per anatomy, data at the grid size
+    wm_voxels: list       # candidate seed voxels per anatomy (WM > 0.9)
+    anatomy: np.ndarray   # per sample
+    params: np.ndarray    # per sample {Dw / L^2, rho, T, x0, y0, z0}
+    nsteps: np.ndarray    # per sample, dt = T / N below the CFL limit 0.9 h^2 / (6 Dw)
+    Dw_phys: np.ndarray
+    seed: int
+
+    @property
+    def cells(self) -> np.ndarray:
+        return np.array([self.sizes[a] ** 3 for a in self.anatomy], np.int64)
+
+    def cell_steps(self) -> int:
+        return int((self.cells * self.nsteps).sum())
+
+
+def tmcmc_stage(nsamples: int = 1024, sizes=(64, 96, 128), weights=(1, 2, 1), seed: int = 5) -> Stage:
+    """Samples of (Dw, rho, T, x0, y0, z0): Dw ~ logU[0.002, 0.015] cm^2/day,
+    rho ~ logU[0.002, 0.2]/day, T ~ U[100, 700] days, the Gaussian seed at a
+    uniformly drawn WM voxel of a phantom of the sample's size; sizes drawn
+    in the given ratio.  dt = T/N with N the smallest count that keeps
+    dt <= 0.9 h^2 / (6 Dw / L^2)."""
+    rng = np.random.default_rng(5000 + seed)
+    anats, wmv = [], []
+    for n in sizes:
+        gm, wm, csf, _ = phantom(n, seed)
+        anats.append((gm, wm, csf))
+        wmv.append(np.argwhere(wm > 0.9)[:, ::-1])  # (x, y, z)
+    w = np.asarray(weights, np.float64)
+    anatomy = rng.choice(len(sizes), size=nsamples, p=w / w.sum()).astype(np.int32)
+    Dw_phys = np.exp(rng.uniform(math.log(0.002), math.log(0.015), nsamples))
+    rho = np.exp(rng.uniform(math.log(0.002), math.log(0.2), nsamples))
+    T = rng.uniform(100.0, 700.0, nsamples)
+    params = np.zeros((nsamples, 6))
+    nsteps = np.zeros(nsamples, np.int64)
+    for i in range(nsamples):
+        a = anatomy[i]
+        n = sizes[a]
+        Dw = Dw_phys[i] / (L_CUBE * L_CUBE)
+        dt_max = 0.9 * (1.0 / n) ** 2 / (6.0 * Dw)
+        nsteps[i] = int(math.ceil(T[i] / dt_max))
+        v = wmv[a][rng.integers(len(wmv[a]))]
+        params[i] = (Dw, rho[i], T[i], float(v[0]), float(v[1]), float(v[2]))
+    return Stage(list(sizes), anats, wmv, anatomy, params, nsteps, Dw_phys, seed)

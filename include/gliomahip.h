@@ -142,10 +142,12 @@ int gh_rd_prepare(gh_ctx* ctx, double Dw, double Dg, double rho);
 int gh_rd_rhs(gh_ctx* ctx);
 int gh_rd_update(gh_ctx* ctx, double dt);
 int gh_rd_steps(gh_ctx* ctx, int64_t nsteps, double dt, double* elapsed_ms);
-/* gh_rd_set_tile / gh_rd_autotune (no reference counterpart): the step
- * kernel's tile is 1, 2 or 4 blocks of 8^3 per workgroup; the autotuner times
- * trial_steps of each on the current state (restored afterwards) and keeps the
- * fastest, returned in *best.  Results do not depend on the tile. */
+/* gh_rd_set_tile / gh_rd_autotune (no reference counterpart): the uniform
+ * step kernel's schedule -- 1, 2 or 4 blocks of 8^3 per workgroup and one
+ * Euler step per launch, or 0: two Euler steps per launch (temporal blocking,
+ * one block per workgroup); the autotuner times trial_steps of each on the
+ * current state (restored afterwards) and keeps the fastest, returned in
+ * *best.  Results do not depend on the schedule. */
 int gh_rd_set_tile(gh_ctx* ctx, int blocks_per_workgroup);
 int gh_rd_autotune(gh_ctx* ctx, double dt, int trial_steps, int* best);
 /* gh_rd_thresholds (host-side, no device): the per-cell predicates that
@@ -234,6 +236,21 @@ int gh_batch_set_samples(gh_batch* b, int nsamples, const int* anatomy_id, const
 int gh_batch_run(gh_batch* b, double* elapsed_ms);
 int gh_batch_likelihood(gh_batch* b, const double* lik_params, int stepPET, int stepTi, int use_roi,
                         double* out);
+/* gh_batch_set_adjoint_samples: the problem of gh_rd_adjoint per sample --
+ *   params = 6 doubles {Dw (already / L^2, Dg = Dw/10), rho, T, x0, y0, z0
+ *   (Gaussian centre in cells)}, nsteps[i] Euler steps of dt = T/N, initial
+ *   state ic_peak*exp(-|x-x0|^2/(2 ic_sigma^2)) where p_w + p_g > 0
+ *   (gh_gaussian_ic).
+ * gh_batch_adjoint: forward with checkpoints every ckpt_every steps, loss,
+ *   reverse sweep with recompute, for all samples at once; loss = 4 doubles
+ *   and grad = 6 doubles {dJ/dDw, rho, T, x0, y0, z0} per sample, each equal
+ *   to gh_rd_adjoint on that sample alone.  Memory: the checkpoints
+ *   (sum over samples of ceil(N/K) states), a (K+1)-state tape and two FP64
+ *   adjoint states per cell. */
+int gh_batch_set_adjoint_samples(gh_batch* b, int nsamples, const int* anatomy_id, const double* params,
+                                 const int64_t* nsteps, double ic_sigma, double ic_peak);
+int gh_batch_adjoint(gh_batch* b, int ckpt_every, const double* lik_params, int stepPET, int stepTi, int use_roi,
+                     double* loss, double* grad, double* elapsed_ms);
 int gh_batch_sample_info(gh_batch* b, int sample, double* Dw, double* Dg, double* dt, int64_t* nsteps);
 int gh_batch_get_phi(gh_batch* b, int sample, void* dense);
 

@@ -142,12 +142,14 @@ def test_gpu_checkpoint_restart_is_exact():
     assert bits_equal(a, b) and bits_equal(a, c["phi_ref"])
 
 
-@pytest.mark.parametrize("bpw", [1, 2, 4])
-def test_gpu_tiles_are_bit_identical(bpw):
-    c = load_case("rd_f32_random16")
-    say("tile", bpw)
+@pytest.mark.parametrize("name", ["rd_f32_random16", "rd_f64_phantom32"])
+@pytest.mark.parametrize("bpw", [0, 1, 2, 4])
+def test_gpu_tiles_are_bit_identical(bpw, name):
+    """Every schedule (0: two Euler steps per launch) equals the reference's golden result."""
+    c = load_case(name)
+    say("tile", bpw, name)
     n = c["phi0"].shape[0]
-    ctx = gh.Context(0, "f32")
+    ctx = gh.Context(0, "f32" if "f32" in name else "f64")
     ctx.grid_init_uniform(n // 8)
     ctx.set_field(FIELD_PW, c["p_w"])
     ctx.set_field(FIELD_PG, c["p_g"])
@@ -171,9 +173,37 @@ def test_gpu_autotune_restores_state():
     ctx.set_field(FIELD_PHI, c["phi0"])
     ctx.rd_prepare(float(c["Dw"]), float(c["Dg"]), float(c["rho"]))
     best = ctx.rd_autotune(float(c["dt"]), 5)
-    assert best in (1, 2, 4)
+    assert best in (0, 1, 2, 4)
     assert bits_equal(ctx.get_field(FIELD_PHI), c["phi0"])
     ctx.rd_steps(int(c["nsteps"]), float(c["dt"]))
     out = ctx.get_field(FIELD_PHI)
     ctx.close()
     assert bits_equal(out, c["phi_ref"])
+
+
+@pytest.mark.parametrize("n,nsteps", [(8, 7), (16, 9), (32, 12), (64, 1)])
+def test_gpu_two_step_schedule_matches_oracle(oracle, n, nsteps):
+    """Two steps per launch, with an odd remainder step, against the oracle on
+    grids of 1, 2, 4 and 8 blocks per side; bpd 1 makes a block its own
+    neighbour in every direction."""
+    rng = np.random.default_rng(n + nsteps)
+    pw = rng.uniform(0, 1, (n, n, n)).astype(np.float32)
+    pg = (rng.uniform(0, 1, (n, n, n)) * (1 - pw)).astype(np.float32)
+    pw[rng.uniform(size=pw.shape) < 0.15] = 0
+    pg[rng.uniform(size=pg.shape) < 0.15] = 0
+    phi = rng.uniform(0, 1, (n, n, n)).astype(np.float32)
+    say("two-step schedule n=%d steps=%d" % (n, nsteps))
+    ctx = gh.Context(0, "f32")
+    ctx.grid_init_uniform(n // 8)
+    ctx.set_field(FIELD_PW, pw)
+    ctx.set_field(FIELD_PG, pg)
+    ctx.set_field(FIELD_PHI, phi)
+    Dw, Dg, rho, dt = 0.0021, 0.00021, 0.31, 0.23 / (n * n) / 0.0021
+    ctx.rd_prepare(Dw, Dg, rho)
+    ctx.rd_set_tile(0)
+    ctx.rd_steps(nsteps, dt)
+    out = ctx.get_field(FIELD_PHI)
+    ctx.close()
+    ref = oracle.rd_steps(phi, pw, pg, np.float32(Dw), np.float32(Dg), np.float32(rho), np.float32(1.0 / n), dt,
+                          nsteps)
+    assert bits_equal(out, ref)
