@@ -95,13 +95,13 @@ BATCH_LIK = (0.0025, 1.0, 0.7, 0.25, 2.0)  # PET noise sigma 0.05, PET scale 1; 
 
 
 def _stage_memory(cells, nsteps, K, real_size=4):
-    """Device bytes of one sample in gh_batch_adjoint: two states, statics (4+1 Real),
+    """Device bytes of one sample in gh_batch_adjoint: two states, statics (4+1+4 Real),
     two FP64 adjoint states, the (K+1)-state tape and ceil(N/K) checkpoints."""
-    per_cell = real_size * (2 + 5 + (K + 1) + -(-nsteps // K)) + 16
+    per_cell = real_size * (2 + 9 + (K + 1) + -(-nsteps // K)) + 16
     return cells * per_cell
 
 
-def batched_leg(nsamples, K, budget_gb, seq_samples, acc_samples):
+def batched_leg(nsamples, K, budget_gb, seq_samples, acc_samples, chunk_cells=0):
     """One TMCMC stage of forward + likelihood + adjoint over a population of
     64^3 / 96^3 / 128^3 samples (1:2:1): batched on the device (gh_batch), in
     chunks that fit the memory budget, against the same samples solved one at a
@@ -142,15 +142,17 @@ def batched_leg(nsamples, K, budget_gb, seq_samples, acc_samples):
         pet = np.where(t1 > 0, u + rng.normal(0, 0.05, u.shape), 0.0)
         obs.append((t1, fl, pet))
         b.set_observations(aids[a], *obs[a])
-    # chunks under the budget
+    # chunks under the memory budget and, when given, a cell budget (a chunk whose
+    # per-step working set fits the 256 MB Infinity Cache streams from it, not HBM)
     mem = np.array([_stage_memory(int(c), int(n), K) for c, n in zip(cells, st.nsteps)], np.float64)
-    chunks, cur, acc = [], [], 0.0
+    chunks, cur, acc, acc_c = [], [], 0.0, 0
     for i in range(nsamples):
-        if cur and acc + mem[i] > budget_gb * 1e9:
+        if cur and (acc + mem[i] > budget_gb * 1e9 or (chunk_cells and acc_c + cells[i] > chunk_cells)):
             chunks.append(cur)
-            cur, acc = [], 0.0
+            cur, acc, acc_c = [], 0.0, 0
         cur.append(i)
         acc += mem[i]
+        acc_c += int(cells[i])
     chunks.append(cur)
     log("batched: %d chunks, largest %.1f GB" % (len(chunks), max(mem[c].sum() for c in chunks) / 1e9))
 
@@ -256,6 +258,9 @@ def adaptive_leg(tend, epoch_steps, max_epochs):
         epoch_s += t3 - t2
         leaf_updates += nb * 512 * k
         steps_done += k
+        if (e + 1) % 25 == 0 or e + 1 == epochs:
+            log("adaptive: epoch %d/%d, %d blocks, steps %.2f s, epochs %.2f s" % (e + 1, epochs, ctx.nblocks(),
+                                                                                   step_s, epoch_s))
     wall = time.perf_counter() - t0
     ctx.close()
     return {"metric": "adaptive RD leaf cell-updates/s", "value": leaf_updates / wall, "unit": "cell-updates/s",
@@ -278,6 +283,7 @@ def main():
     ap.add_argument("--batch-samples", type=int, default=1024)
     ap.add_argument("--batch-ckpt", type=int, default=32, help="checkpoint interval of the batched adjoint")
     ap.add_argument("--batch-budget-gb", type=float, default=120.0, help="device memory per batched chunk")
+    ap.add_argument("--batch-cells", type=float, default=0, help="cells per batched chunk (0: memory budget only)")
     ap.add_argument("--batch-seq", type=int, default=64, help="samples timed one at a time")
     ap.add_argument("--batch-acc", type=int, default=24, help="samples re-solved in FP64")
     ap.add_argument("--workload", default="uniform128", choices=["uniform128", "adaptive"],
@@ -356,7 +362,8 @@ def main():
         line["gpu_over_cpu"] = round(value / cpu["value"], 1)
     if not a.no_batch:
         log("batched TMCMC stage ...")
-        line["batched"] = batched_leg(a.batch_samples, a.batch_ckpt, a.batch_budget_gb, a.batch_seq, a.batch_acc)
+        line["batched"] = batched_leg(a.batch_samples, a.batch_ckpt, a.batch_budget_gb, a.batch_seq, a.batch_acc,
+                                      int(a.batch_cells))
         log("batched", json.dumps(line["batched"]))
     print(json.dumps(line), flush=True)
     return 0

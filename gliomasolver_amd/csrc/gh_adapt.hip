@@ -21,6 +21,8 @@
 #include <algorithm>
 #include <array>
 #include <cmath>
+#include <exception>
+#include <thread>
 #include <unordered_map>
 #include <unordered_set>
 #include <vector>
@@ -414,6 +416,52 @@ struct HostTable {
     }
 };
 
+// A table over items [0, n) built on up to 16 host threads: each thread fills
+// the table of a contiguous item range with fill(i0, i1, table), and the parts
+// are concatenated in item order, so the result equals a sequential build.
+template <typename F>
+HostTable parallel_table(int n, F fill) {
+    const int hw = (int)std::max(1u, std::thread::hardware_concurrency());
+    const int nt = std::max(1, std::min({16, hw, n / 16}));
+    HostTable h;
+    if (nt == 1) {
+        fill(0, n, h);
+        return h;
+    }
+    std::vector<HostTable> part(nt);
+    std::vector<std::exception_ptr> err(nt);
+    std::vector<std::thread> th;
+    for (int k = 0; k < nt; ++k)
+        th.emplace_back([&, k] {
+            try {
+                fill((int)((int64_t)n * k / nt), (int)((int64_t)n * (k + 1) / nt), part[k]);
+            } catch (...) {
+                err[k] = std::current_exception();
+            }
+        });
+    for (auto& t : th) t.join();
+    for (auto& e : err)
+        if (e) std::rethrow_exception(e);
+    size_t ne = 0, nterm = 0;
+    for (auto& p : part) {
+        ne += p.ent.size();
+        nterm += p.src.size();
+    }
+    h.ent.reserve(ne);
+    h.src.reserve(nterm);
+    h.w.reserve(nterm);
+    for (auto& p : part) {
+        const int32_t off = (int32_t)h.src.size();
+        for (GhostEntry e : p.ent) {
+            e.start += off;
+            h.ent.push_back(e);
+        }
+        h.src.insert(h.src.end(), p.src.begin(), p.src.end());
+        h.w.insert(h.w.end(), p.w.begin(), p.w.end());
+    }
+    return h;
+}
+
 template <typename Real>
 int upload_table(Ctx& c, const HostTable& h, DevBuf& ent, DevBuf& src, DevBuf& w) {
     int rc;
@@ -435,54 +483,55 @@ int upload_table(Ctx& c, const HostTable& h, DevBuf& ent, DevBuf& src, DevBuf& w
 
 // face ghosts of the RD stencil for faces without a same-level neighbour
 HostTable build_face_table(const Octree& t, const std::vector<int32_t>& nbr) {
-    HostTable h;
-    std::vector<GhostTerm> terms;
-    for (int s = 0; s < (int)t.leaves.size(); ++s) {
-        const BlockRec& b = t.leaves[s];
-        for (int f = 0; f < 6; ++f) {
-            if (nbr[6 * s + f] >= 0) continue;
-            for (int v = 0; v < 8; ++v)
-                for (int u = 0; u < 8; ++u) {
-                    int i[3];
-                    switch (f) {
-                        case FXM: i[0] = -1; i[1] = u; i[2] = v; break;
-                        case FXP: i[0] = 8; i[1] = u; i[2] = v; break;
-                        case FYM: i[0] = u; i[1] = -1; i[2] = v; break;
-                        case FYP: i[0] = u; i[1] = 8; i[2] = v; break;
-                        case FZM: i[0] = u; i[1] = v; i[2] = -1; break;
-                        default:  i[0] = u; i[1] = v; i[2] = 8; break;
+    return parallel_table((int)t.leaves.size(), [&](int s0, int s1, HostTable& h) {
+        std::vector<GhostTerm> terms;
+        for (int s = s0; s < s1; ++s) {
+            const BlockRec& b = t.leaves[s];
+            for (int f = 0; f < 6; ++f) {
+                if (nbr[6 * s + f] >= 0) continue;
+                for (int v = 0; v < 8; ++v)
+                    for (int u = 0; u < 8; ++u) {
+                        int i[3];
+                        switch (f) {
+                            case FXM: i[0] = -1; i[1] = u; i[2] = v; break;
+                            case FXP: i[0] = 8; i[1] = u; i[2] = v; break;
+                            case FYM: i[0] = u; i[1] = -1; i[2] = v; break;
+                            case FYP: i[0] = u; i[1] = 8; i[2] = v; break;
+                            case FZM: i[0] = u; i[1] = v; i[2] = -1; break;
+                            default:  i[0] = u; i[1] = v; i[2] = 8; break;
+                        }
+                        const int64_t q[3] = {8LL * b.index[0] + i[0], 8LL * b.index[1] + i[1],
+                                              8LL * b.index[2] + i[2]};
+                        terms.clear();
+                        t.value_terms(b.level, q, 1.0, terms);
+                        h.add(s * 384 + f * 64 + v * 8 + u, terms);
                     }
-                    const int64_t q[3] = {8LL * b.index[0] + i[0], 8LL * b.index[1] + i[1], 8LL * b.index[2] + i[2]};
-                    terms.clear();
-                    t.value_terms(b.level, q, 1.0, terms);
-                    h.add(s * 384 + f * 64 + v * 8 + u, terms);
-                }
+            }
         }
-    }
-    return h;
+    });
 }
 
 // interpolated ghosts of the full labs of the listed slots (compact index j)
 HostTable build_lab_table(const Octree& t, const std::vector<int>& slots, const std::vector<int32_t>& nbr27,
                           int order) {
-    HostTable h;
     const int LS = lab_start(order), LAB = lab_size(order), lab3 = LAB * LAB * LAB;
-    std::vector<GhostTerm> terms;
-    for (int j = 0; j < (int)slots.size(); ++j) {
-        const int s = slots[j];
-        const BlockRec& b = t.leaves[s];
-        for (int p = 0; p < lab3; ++p) {
-            const int i[3] = {p % LAB + LS, (p / LAB) % LAB + LS, p / (LAB * LAB) + LS};
-            const int code = (i[0] < 0 ? 0 : (i[0] < 8 ? 1 : 2)) + 3 * (i[1] < 0 ? 0 : (i[1] < 8 ? 1 : 2)) +
-                             9 * (i[2] < 0 ? 0 : (i[2] < 8 ? 1 : 2));
-            if (code == 13 || nbr27[27 * s + code] >= 0) continue;
-            const int64_t q[3] = {8LL * b.index[0] + i[0], 8LL * b.index[1] + i[1], 8LL * b.index[2] + i[2]};
-            terms.clear();
-            t.value_terms(b.level, q, 1.0, terms);
-            h.add(j * lab3 + p, terms);
+    return parallel_table((int)slots.size(), [&](int j0, int j1, HostTable& h) {
+        std::vector<GhostTerm> terms;
+        for (int j = j0; j < j1; ++j) {
+            const int s = slots[j];
+            const BlockRec& b = t.leaves[s];
+            for (int p = 0; p < lab3; ++p) {
+                const int i[3] = {p % LAB + LS, (p / LAB) % LAB + LS, p / (LAB * LAB) + LS};
+                const int code = (i[0] < 0 ? 0 : (i[0] < 8 ? 1 : 2)) + 3 * (i[1] < 0 ? 0 : (i[1] < 8 ? 1 : 2)) +
+                                 9 * (i[2] < 0 ? 0 : (i[2] < 8 ? 1 : 2));
+                if (code == 13 || nbr27[27 * s + code] >= 0) continue;
+                const int64_t q[3] = {8LL * b.index[0] + i[0], 8LL * b.index[1] + i[1], 8LL * b.index[2] + i[2]};
+                terms.clear();
+                t.value_terms(b.level, q, 1.0, terms);
+                h.add(j * lab3 + p, terms);
+            }
         }
-    }
-    return h;
+    });
 }
 
 }  // namespace
@@ -491,6 +540,7 @@ HostTable build_lab_table(const Octree& t, const std::vector<int>& slots, const 
 // face-ghost table; uploads everything and sizes the per-slot ghost buffers
 int apply_topology(Ctx& c) {
     Octree& t = c.tree;
+    ++c.tree_version;  // invalidates the cached lab tables
     Topology topo;
     topo.blocks = t.leaves;
     const int nb = (int)t.leaves.size();
@@ -568,47 +618,75 @@ int step_ghosted_t(Ctx& c, const Real* in, Real* out, double dt) {
 }
 
 // full labs of `slots` for `field` into `out` (compact, one lab per listed
-// slot); the interpolated ghosts come from the uploaded lab table `tab`
+// slot, slot list already on the device in dslots); the interpolated ghosts
+// come from the uploaded lab table (nent entries in ent/src/w)
 template <typename Real>
-int build_labs(Ctx& c, const std::vector<int>& slots, const Real* field, const HostTable& tab, DevBuf& out) {
+int build_labs(Ctx& c, int nslots, const DevBuf& dslots, const Real* field, int nent, const DevBuf& ent,
+               const DevBuf& src, const DevBuf& w, DevBuf& out) {
     const int order = c.tree.order;
     const int LS = lab_start(order), LAB = lab_size(order), lab3 = LAB * LAB * LAB;
     int rc;
-    if ((rc = out.ensure(std::max<size_t>(1, slots.size()) * lab3 * sizeof(Real))) ||
-        (rc = c.labSlots.ensure(std::max<size_t>(1, slots.size()) * 4)))
-        return rc;
-    if (slots.empty()) return GH_OK;
-    GH_HIP_CHECK(hipMemcpyAsync(c.labSlots.p, slots.data(), slots.size() * 4, hipMemcpyHostToDevice, c.stream));
-    hipLaunchKernelGGL(lab_fill_kernel<Real>, dim3((unsigned)slots.size()), dim3(256), 0, c.stream, (int)slots.size(),
-                       c.labSlots.as<int32_t>(), LS, LAB, c.dNbr27.as<int32_t>(), field, out.as<Real>());
+    if ((rc = out.ensure(std::max<size_t>(1, (size_t)nslots) * lab3 * sizeof(Real)))) return rc;
+    if (nslots == 0) return GH_OK;
+    hipLaunchKernelGGL(lab_fill_kernel<Real>, dim3((unsigned)nslots), dim3(256), 0, c.stream, nslots,
+                       dslots.as<int32_t>(), LS, LAB, c.dNbr27.as<int32_t>(), field, out.as<Real>());
     GH_HIP_CHECK(hipGetLastError());
-    if (!tab.ent.empty()) {
-        hipLaunchKernelGGL(ghost_entry_kernel<Real>, dim3((unsigned)((tab.ent.size() + 255) / 256)), dim3(256), 0,
-                           c.stream, (int)tab.ent.size(), c.labEnt.as<GhostEntry>(), c.labSrc.as<int32_t>(),
-                           c.labW.as<Real>(), field, out.as<Real>());
+    if (nent > 0) {
+        hipLaunchKernelGGL(ghost_entry_kernel<Real>, dim3((unsigned)((nent + 255) / 256)), dim3(256), 0, c.stream, nent,
+                           ent.as<GhostEntry>(), src.as<int32_t>(), w.as<Real>(), field, out.as<Real>());
         GH_HIP_CHECK(hipGetLastError());
     }
     return GH_OK;
 }
 
+// labs of the listed slots with their own freshly built ghost table
+template <typename Real>
+int build_labs_for(Ctx& c, const std::vector<int>& slots, const Real* field, const HostTable& tab, DevBuf& out) {
+    int rc;
+    if ((rc = c.labSlots.ensure(std::max<size_t>(1, slots.size()) * 4))) return rc;
+    if (!slots.empty())
+        GH_HIP_CHECK(hipMemcpyAsync(c.labSlots.p, slots.data(), slots.size() * 4, hipMemcpyHostToDevice, c.stream));
+    return build_labs<Real>(c, (int)slots.size(), c.labSlots, field, (int)tab.ent.size(), c.labEnt, c.labSrc, c.labW,
+                            out);
+}
+
+// Details of the listed slots.  The labs of every leaf are built from a ghost
+// table cached per tree version (an adaptation epoch that changes nothing
+// rebuilds nothing on the host); a lab does not depend on which other labs
+// are built, so the details equal those of labs built for the list alone.
 template <typename Real>
 int fwt_details_t(Ctx& c, const std::vector<int>& slots, std::vector<double>& det) {
     det.assign(slots.size(), 0.0);
     if (slots.empty()) return GH_OK;
     const int order = c.tree.order;
-    HostTable tab = build_lab_table(c.tree, slots, c.topo.nbr27, order);
+    const int n = c.topo.nblocks();
     int rc;
-    if ((rc = upload_table<Real>(c, tab, c.labEnt, c.labSrc, c.labW))) return rc;
-    if ((rc = build_labs<Real>(c, slots, c.phi[c.cur].as<Real>(), tab, c.labBuf))) return rc;
+    std::vector<int> all;  // source of an async upload: lives until the sync below
+    if (c.labAllVersion != c.tree_version) {
+        all.resize(n);
+        for (int i = 0; i < n; ++i) all[i] = i;
+        HostTable tab = build_lab_table(c.tree, all, c.topo.nbr27, order);
+        if ((rc = upload_table<Real>(c, tab, c.labAllEnt, c.labAllSrc, c.labAllW)) ||
+            (rc = c.labAllSlots.ensure(std::max<size_t>(1, all.size()) * 4)))
+            return rc;
+        GH_HIP_CHECK(hipMemcpyAsync(c.labAllSlots.p, all.data(), all.size() * 4, hipMemcpyHostToDevice, c.stream));
+        c.nLabAllEnt = (int)tab.ent.size();
+        c.labAllVersion = c.tree_version;
+    }
+    if ((rc = build_labs<Real>(c, n, c.labAllSlots, c.phi[c.cur].as<Real>(), c.nLabAllEnt, c.labAllEnt, c.labAllSrc,
+                               c.labAllW, c.labBuf)))
+        return rc;
     DevBuf d;
-    if ((rc = d.ensure(slots.size() * sizeof(double)))) return rc;
-    hipLaunchKernelGGL(fwt_kernel<Real>, dim3((unsigned)slots.size()), dim3(512), 0, c.stream, (int)slots.size(),
-                       lab_start(order), lab_size(order), taps<Real>(order), c.labBuf.as<Real>(), d.as<double>());
+    if ((rc = d.ensure((size_t)n * sizeof(double)))) return rc;
+    hipLaunchKernelGGL(fwt_kernel<Real>, dim3((unsigned)n), dim3(512), 0, c.stream, n, lab_start(order),
+                       lab_size(order), taps<Real>(order), c.labBuf.as<Real>(), d.as<double>());
+    std::vector<double> all_det(n);
     hipError_t e = hipGetLastError();
-    if (e == hipSuccess) e = hipMemcpyAsync(det.data(), d.p, det.size() * 8, hipMemcpyDeviceToHost, c.stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(all_det.data(), d.p, all_det.size() * 8, hipMemcpyDeviceToHost, c.stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c.stream);
     d.release();
     if (e != hipSuccess) return set_error(GH_EHIP, std::string("fwt: ") + hipGetErrorString(e));
+    for (size_t i = 0; i < slots.size(); ++i) det[i] = all_det[slots[i]];
     return GH_OK;
 }
 
@@ -626,7 +704,7 @@ int apply_new_tree(Ctx& c, const Octree& nt, const std::vector<Remap>& rm, const
         HostTable tab = build_lab_table(c.tree, split_parents, c.topo.nbr27, order);
         if ((rc = upload_table<Real>(c, tab, c.labEnt, c.labSrc, c.labW))) return rc;
         for (int f = 0; f < 4; ++f)
-            if ((rc = build_labs<Real>(c, split_parents, fields[f]->as<Real>(), tab, labs[f]))) return rc;
+            if ((rc = build_labs_for<Real>(c, split_parents, fields[f]->as<Real>(), tab, labs[f]))) return rc;
     }
     const int nnew = (int)nt.leaves.size();
     DevBuf drm;
@@ -787,6 +865,7 @@ int gh_grid_configure(gh_ctx* ctx, int wavelet_order, int max_level, int min_lev
     if ((wavelet_order != 2 && wavelet_order != 4) || max_level < 0 || max_level > 9 || min_level < 0 ||
         min_level > max_level)
         return set_error(GH_EINVAL, "gh_grid_configure: order 2|4, 0 <= min_level <= max_level <= 9");
+    if (c->tree.order != wavelet_order) c->labAllVersion = ~(uint64_t)0;  // labs are sized by the order
     c->tree.order = wavelet_order;
     c->tree.max_level = max_level;
     c->tree.min_level = min_level;

@@ -65,7 +65,8 @@ int adjoint_t(Ctx& c, int64_t N, double T, int K, const double* lik, int stepPET
     GH_HIP_CHECK(hipMemsetAsync(c.adjPart.p, 0, sizeof(double) * 3 * ns, c.stream));
     const double h = block_h_of(c);
     const double ih2 = 1. / (h * h);
-    const UniformAdjParams prm{c.rho, ih2, dt, 1.0 / (double)N, 1.0 / c.Dw};
+    const UniformAdjParams prm{c.rho, ih2, dt, 1.0 / (double)N, 1.0 / c.Dw, c.topo.uniform_bpd,
+                                bpd_magic(c.topo.uniform_bpd)};
     for (int64_t sgi = nck - 1; sgi >= 0; --sgi) {
         const int64_t k0 = sgi * K, k1 = std::min<int64_t>(N, k0 + K);
         GH_HIP_CHECK(hipMemcpyAsync(seg, ck + (size_t)sgi * nb, nb, hipMemcpyDeviceToDevice, c.stream));
@@ -73,7 +74,7 @@ int adjoint_t(Ctx& c, int64_t N, double T, int K, const double* lik, int stepPET
             if ((rc = rd_step_buffers(c, seg + (size_t)j * nb, seg + (size_t)(j + 1) * nb, dt))) return rc;
         for (int64_t k = k1 - 1; k >= k0; --k) {
             hipLaunchKernelGGL((adj_step_kernel<Real, UniformAdjParams>), dim3(grid), dim3(512), 0, c.stream, ns,
-                               per_xcd, c.nbr.as<int32_t>(), (const Real*)(seg + (size_t)(k - k0) * nb),
+                               per_xcd, (const Real*)(seg + (size_t)(k - k0) * nb),
                                (const Real*)(seg + (size_t)(k - k0 + 1) * nb), c.adjLam[lc].as<double>(),
                                c.adjLam[1 - lc].as<double>(), c.statD.as<Stat4<Real>>(), c.statSig.as<Real>(),
                                prm, c.adjPart.as<double>());

@@ -26,6 +26,8 @@ __device__ inline double block_sum(double v, double* red) {
 // Per-slot parameters of the reverse step: one solve, or packed solves.
 struct UniformAdjParams {
     double rho, ih2, dt, invN, invDw;
+    int bpd;
+    uint32_t magic;  // bpd_magic(bpd)
     __device__ inline void get(int, double& r, double& i, double& d, double& n, double& w) const {
         r = rho;
         i = ih2;
@@ -33,12 +35,13 @@ struct UniformAdjParams {
         n = invN;
         w = invDw;
     }
+    __device__ inline BlockPos pos(int s) const { return block_pos(s, 0, bpd, magic); }
 };
 
 // One reverse step; one workgroup per block (XCD-aware order).  partial[3 s + q]
 // accumulates the block's parameter-gradient terms, step after step.
 template <typename Real, typename Params>
-__global__ __launch_bounds__(512) void adj_step_kernel(int nslots, int per_xcd, const int32_t* __restrict__ nbr,
+__global__ __launch_bounds__(512) void adj_step_kernel(int nslots, int per_xcd,
                                                        const Real* __restrict__ uk, const Real* __restrict__ uk1,
                                                        const double* __restrict__ lam1, double* __restrict__ lam0,
                                                        const Stat4<Real>* __restrict__ st,
@@ -84,7 +87,7 @@ __global__ __launch_bounds__(512) void adj_step_kernel(int nslots, int per_xcd, 
             case FZM: src = a + 8 * b + 64 * in1; qx = a; qy = b; qz = lo; break;
             default:  src = a + 8 * b + 64 * in0; qx = a; qy = b; qz = hi; break;
         }
-        const size_t nb = (size_t)nbr[6 * s + f] * BCELLS + src;
+        const size_t nb = (size_t)face_nb(prm.pos(s), f) * BCELLS + src;
         const Real uh = uk[nb];
         const Stat4<Real> hs = st[nb];
         s_u[Q(qx, qy, qz)] = uh;

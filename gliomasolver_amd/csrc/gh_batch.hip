@@ -55,6 +55,8 @@ struct SlotRange {
     Real rho, Dw, Dg;
     double ih2, dt;
     double rho_d, invN, invDw;  // adjoint: rho and 1/Dw as given, 1/N
+    int bpd;                    // blocks per side of the sample's grid
+    uint32_t magic;             // bpd_magic(bpd)
 };
 
 struct BatchSample {
@@ -76,6 +78,10 @@ struct BatchAdjParams {
         d = q.dt;
         n = q.invN;
         w = q.invDw;
+    }
+    __device__ inline BlockPos pos(int s) const {
+        const SlotRange<Real>& q = range[slot_pos[s]];
+        return block_pos(s, (int)q.slot0, q.bpd, q.magic);
     }
 };
 
@@ -119,6 +125,10 @@ struct BatchStepParams {
         r = q.rho;
         i = q.ih2;
         d = q.dt;
+    }
+    __device__ inline BlockPos pos(int s) const {
+        const SlotRange<Real>& q = range[slot_pos[s]];
+        return block_pos(s, (int)q.slot0, q.bpd, q.magic);
     }
 };
 
@@ -233,6 +243,8 @@ struct Batch {
     std::vector<int> pos_of;        // sample index -> position
     std::vector<size_t> slot0;      // per position, plus the total
     DevBuf phi[2], statD, statSig, nbr, slotPos, range, apw, apg, likPart, likOut, likJobs;
+    DevBuf statF, flag;  // rd_step2f_kernel records (valid iff fast_ok), count scratch
+    bool fast_ok = false;
     DevBuf ck, tape, lam[2], apart, icpart, adjJobs;  // adjoint
     double ic_is2 = 1.0;                                 // adjoint: 1 / sigma^2 of the Gaussian IC
     std::vector<int> final_buf;                          // per position: phi buffer of the final state
@@ -314,6 +326,8 @@ int build_layout_t(Batch& b) {
         rng[p].rho_d = q.rho;
         rng[p].invN = 1.0 / (double)q.nsteps;
         rng[p].invDw = 1.0 / q.Dw;
+        rng[p].bpd = a.bpd;
+        rng[p].magic = bpd_magic(a.bpd);
         apw[p] = a.pw.p;
         apg[p] = a.pg.p;
     }
@@ -329,6 +343,9 @@ int build_layout_t(Batch& b) {
                        b.slotPos.as<int32_t>(), b.range.as<SlotRange<Real>>(), b.apw.as<const Real* const>(),
                        b.apg.as<const Real* const>(), b.statD.as<Stat4<Real>>(), b.statSig.as<Real>());
     GH_HIP_CHECK(hipGetLastError());
+    if ((rc = build_stat_fast(b.stream, sizeof(Real) == 4 ? GH_REAL_F32 : GH_REAL_F64, nc, b.statD.p, b.statSig.p,
+                              b.statF, b.flag, &b.fast_ok)))
+        return rc;
     GH_HIP_CHECK(hipMemsetAsync(b.phi[0].p, 0, nc * sizeof(Real), b.stream));
     GH_HIP_CHECK(hipStreamSynchronize(b.stream));
     return GH_OK;
@@ -467,22 +484,28 @@ struct ForwardSweep {
     void retire(int64_t k) {
         while (active > 0 && npos[active - 1] <= k) b.final_buf[--active] = cur;
     }
-    // one launch from step k; returns the steps taken
+    // one launch from step k; returns the steps taken.  From step 1 on every
+    // running state was written by the fused update (clamped), so pairs take
+    // rd_step2f_kernel when the anatomies allow it.
     int launch(int64_t k, int K) {
         const int nsl = (int)b.slot0[active];
-        const bool two = npos[active - 1] >= k + 2 && (K <= 0 || (k + 1) % K != 0);
+        const bool two = npos[active - 1] >= k + 2 && (K <= 0 || (k + 1) % K != 0) && (k >= 1 || !b.fast_ok);
         const Real* in = b.phi[cur].as<Real>();
         Real* out = b.phi[1 - cur].as<Real>();
         if (two) {
             const int per_xcd = (nsl + 7) / 8;
-            hipLaunchKernelGGL((rd_step2_kernel<Real, BatchStepParams<Real>>), dim3(8 * per_xcd), dim3(512), 0,
-                               b.stream, nsl, per_xcd, b.nbr.as<int32_t>(), in, out, b.statD.as<Stat4<Real>>(),
-                               b.statSig.as<Real>(), prm);
+            if (b.fast_ok)
+                hipLaunchKernelGGL((rd_step2f_kernel<Real, BatchStepParams<Real>>), dim3(8 * per_xcd), dim3(512), 0,
+                                   b.stream, nsl, per_xcd, in, out, b.statF.as<Stat4<Real>>(), prm);
+            else
+                hipLaunchKernelGGL((rd_step2_kernel<Real, BatchStepParams<Real>>), dim3(8 * per_xcd), dim3(512), 0,
+                                   b.stream, nsl, per_xcd, in, out, b.statD.as<Stat4<Real>>(), b.statSig.as<Real>(),
+                                   prm);
         } else {
             const int per_xcd = ((nsl + 1) / 2 + 7) / 8;  // two blocks per workgroup
             hipLaunchKernelGGL((rd_step_kernel<Real, true, 2, BatchStepParams<Real>>), dim3(8 * per_xcd), dim3(512),
-                               0, b.stream, nsl, per_xcd, b.nbr.as<int32_t>(), in, out, b.statD.as<Stat4<Real>>(),
-                               b.statSig.as<Real>(), prm);
+                               0, b.stream, nsl, per_xcd, in, out, b.statD.as<Stat4<Real>>(), b.statSig.as<Real>(),
+                               prm);
         }
         cur = 1 - cur;
         return two ? 2 : 1;
@@ -588,8 +611,8 @@ int adjoint_t(Batch& b, int K, const double* lik, int stepPET, int stepTi, int u
     auto step = [&](const void* in, void* out, int nslots) {  // one Euler step (tape recompute)
         const int per_xcd = ((nslots + 1) / 2 + 7) / 8;
         hipLaunchKernelGGL((rd_step_kernel<Real, true, 2, BatchStepParams<Real>>), dim3(8 * per_xcd), dim3(512), 0,
-                           b.stream, nslots, per_xcd, b.nbr.as<int32_t>(), (const Real*)in, (Real*)out,
-                           b.statD.as<Stat4<Real>>(), b.statSig.as<Real>(), fprm);
+                           b.stream, nslots, per_xcd, (const Real*)in, (Real*)out, b.statD.as<Stat4<Real>>(),
+                           b.statSig.as<Real>(), fprm);
     };
     char* ck = (char*)b.ck.p;
     char* tape = (char*)b.tape.p;
@@ -651,7 +674,7 @@ int adjoint_t(Batch& b, int K, const double* lik, int stepPET, int stepTi, int u
             const int nslots = (int)b.slot0[active(k)];
             const int per_xcd = (nslots + 7) / 8;
             hipLaunchKernelGGL((adj_step_kernel<Real, BatchAdjParams<Real>>), dim3(8 * per_xcd), dim3(512), 0,
-                               b.stream, nslots, per_xcd, b.nbr.as<int32_t>(),
+                               b.stream, nslots, per_xcd,
                                (const Real*)(tape + (size_t)(k - k0) * nsl * slot_bytes),
                                (const Real*)(tape + (size_t)(k - k0 + 1) * nsl * slot_bytes),
                                b.lam[(k + 1) & 1].as<double>(), b.lam[k & 1].as<double>(), b.statD.as<Stat4<Real>>(),
@@ -735,7 +758,7 @@ int gh_batch_destroy(gh_batch* bh) {
     }
     DevBuf* bufs[] = {&b->phi[0], &b->phi[1], &b->statD,  &b->statSig, &b->nbr,    &b->slotPos, &b->range,
                       &b->apw,    &b->apg,    &b->likPart, &b->likOut, &b->likJobs, &b->ck,     &b->tape,
-                      &b->lam[0], &b->lam[1], &b->apart,   &b->icpart, &b->adjJobs};
+                      &b->lam[0], &b->lam[1], &b->apart,   &b->icpart, &b->adjJobs, &b->statF,  &b->flag};
     for (DevBuf* d : bufs) d->release();
     if (b->ev0) (void)hipEventDestroy(b->ev0);
     if (b->ev1) (void)hipEventDestroy(b->ev1);

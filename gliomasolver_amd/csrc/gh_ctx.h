@@ -43,6 +43,10 @@ struct Ctx {
     DevBuf fgPw, fgPg;               // per slot: 6*64 ghost p_w, p_g (statics)
     // FWT / split labs (compact, one per listed slot)
     DevBuf labBuf, labSlots, labEnt, labSrc, labW;
+    // ghost table of the labs of every leaf, valid while tree_version == labAllVersion
+    DevBuf labAllEnt, labAllSrc, labAllW, labAllSlots;
+    int nLabAllEnt = 0;
+    uint64_t tree_version = 0, labAllVersion = ~(uint64_t)0;
     double last_fwt_ms = 0;
 
     DevBuf phi[2];      // ping-pong tumor density
@@ -51,6 +55,9 @@ struct Ctx {
     DevBuf pw, pg, pcsf;
     DevBuf statD;       // 4 * ncells: per cell {D of the +x,+y,+z face, tau}: outside iff phi < tau
     DevBuf statSig;     // ncells: cell is in the domain iff phi > sigma
+    DevBuf statF;       // 4 * ncells: {D+x, D+y, D+z, tau_enc} of rd_step2f_kernel (valid iff fast_ok)
+    DevBuf flag;        // small scratch word for device-side counts
+    bool fast_ok = false;  // every cell has p_w + p_g >= 0: clamped states may take rd_step2f_kernel
     DevBuf nbr;         // 6 * nblocks int32
     std::vector<DevBuf> ckpt;  // device-side copies of phi (gh_checkpoint_*)
     // likelihood data (block-major): y == 1 flags of T1c/FLAIR, PET values, ROI {cell, ix, iy, iz}
@@ -67,6 +74,7 @@ struct Ctx {
     size_t ncells() const { return (size_t)topo.nblocks() * BCELLS; }
     int alloc_fields();
     int upload_topology();
+    void release_all();  // every device buffer of the context
 };
 
 Ctx* as_ctx(gh_ctx* c);
@@ -79,6 +87,10 @@ int rd_steps(Ctx& c, int64_t nsteps, double dt);
 // one fused step from an explicit state buffer into another (same layout as phi)
 int rd_step_buffers(Ctx& c, const void* in, void* out, double dt);
 double block_h_of(const Ctx& c);
+// the rd_step2f_kernel records of n cells from {D+, tau} and sigma into out;
+// *ok = every cell has sigma <= 0 (p_w + p_g >= 0).  Synchronizes the stream.
+int build_stat_fast(hipStream_t stream, int real_kind, size_t n, const void* st, const void* sig, DevBuf& out,
+                    DevBuf& flag, bool* ok);
 int rd_autotune(Ctx& c, double dt, int trial_steps, int* best);
 
 // adaptive grid (gh_adapt.hip)

@@ -55,6 +55,16 @@ int Ctx::alloc_fields() {
     return GH_OK;
 }
 
+void Ctx::release_all() {
+    DevBuf* bufs[] = {&dLevel,   &dNbr27,  &faceEnt,    &faceSrc,   &faceW,     &fgPhi,       &fgTau,   &fgDm,
+                      &fgPw,     &fgPg,    &labBuf,     &labSlots,  &labEnt,    &labSrc,      &labW,    &labAllEnt,
+                      &labAllSrc, &labAllW, &labAllSlots, &phi[0],  &phi[1],    &dphidt,      &pw,      &pg,
+                      &pcsf,     &statD,   &statSig,    &statF,     &flag,      &nbr,         &obsT1,   &obsFl,
+                      &obsPet,   &obsRoi,  &likPart,    &adjCk,     &adjSeg,    &adjLam[0],   &adjLam[1], &adjPart};
+    for (DevBuf* b : bufs) b->release();
+    for (DevBuf& b : ckpt) b.release();
+}
+
 int Ctx::upload_topology() {
     int rc = nbr.ensure(topo.nbr.size() * sizeof(int32_t));
     if (rc) return rc;
@@ -252,12 +262,7 @@ int gh_ctx_destroy(gh_ctx* ctx) {
     if (!c) return GH_OK;
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
-    DevBuf* bufs[] = {&c->phi[0], &c->phi[1], &c->dphidt, &c->pw, &c->pg, &c->pcsf,
-                      &c->statD, &c->statSig, &c->nbr,
-                      &c->obsT1, &c->obsFl, &c->obsPet, &c->obsRoi, &c->likPart,
-                      &c->adjCk, &c->adjSeg, &c->adjLam[0], &c->adjLam[1], &c->adjPart};
-    for (DevBuf* b : bufs) b->release();
-    for (DevBuf& b : c->ckpt) b.release();
+    c->release_all();
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
     if (c->stream) (void)hipStreamDestroy(c->stream);

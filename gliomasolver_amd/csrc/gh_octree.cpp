@@ -17,6 +17,15 @@ Octree::Kind Octree::kind(int L, int x, int y, int z, int* slot) const {
     x = wrapi(x, n);
     y = wrapi(y, n);
     z = wrapi(z, n);
+    if (dense_levels > 0) {  // every level with a leaf or a subdivided node is dense
+        if (L >= dense_levels) return COARSER;
+        const int32_t v = dense[L][(size_t)x + ((size_t)y << L) + ((size_t)z << (2 * L))];
+        if (v >= 0) {
+            if (slot) *slot = v;
+            return LEAF;
+        }
+        return v == -2 ? FINER : COARSER;
+    }
     const uint64_t k = key(L, x, y, z);
     auto it = leaf_slot.find(k);
     if (it != leaf_slot.end()) {
@@ -51,6 +60,23 @@ void Octree::rebuild_index() {
     });
     leaf_slot.clear();
     for (size_t s = 0; s < leaves.size(); ++s) leaf_slot[key(leaves[s])] = (int)s;
+    for (int l = 0; l < MAXL; ++l) dense[l].clear();
+    dense_levels = 0;
+    const int top = levels_max();
+    if (leaves.empty() || top > DENSE_MAX) return;
+    for (int l = 0; l <= top; ++l) dense[l].assign((size_t)1 << (3 * l), -1);
+    for (uint64_t k : interior) {
+        const int l = (int)(k >> 48);
+        if (l > top) continue;
+        const size_t x = (k >> 32) & 0xffff, y = (k >> 16) & 0xffff, z = k & 0xffff;
+        dense[l][x + (y << l) + (z << (2 * l))] = -2;
+    }
+    for (size_t s = 0; s < leaves.size(); ++s) {
+        const BlockRec& b = leaves[s];
+        dense[b.level][(size_t)b.index[0] + ((size_t)b.index[1] << b.level) + ((size_t)b.index[2] << (2 * b.level))] =
+            (int32_t)s;
+    }
+    dense_levels = top + 1;
 }
 
 int Octree::levels_min() const {

@@ -40,6 +40,12 @@ struct Octree {
     std::vector<BlockRec> leaves;                 // slot order
     std::unordered_map<uint64_t, int> leaf_slot;  // key -> slot
     std::unordered_set<uint64_t> interior;        // keys of nodes that have children
+    // dense per-level form of the two maps for levels < dense_levels (built by
+    // rebuild_index when the finest leaf level is at most DENSE_MAX): slot of a
+    // leaf, -2 for a node with children, -1 for a point inside a coarser leaf
+    static constexpr int DENSE_MAX = 7;
+    int dense_levels = 0;
+    std::vector<int32_t> dense[MAXL];
 
     static uint64_t key(int L, int x, int y, int z) {
         return ((uint64_t)L << 48) | ((uint64_t)(uint32_t)x << 32) | ((uint64_t)(uint32_t)y << 16) | (uint64_t)(uint32_t)z;
@@ -52,7 +58,7 @@ struct Octree {
     Kind kind(int L, int x, int y, int z, int* slot = nullptr) const;
 
     void init_uniform(int level);
-    void rebuild_index();  // canonical slot order (level, z, y, x) + maps
+    void rebuild_index();  // canonical slot order (level, z, y, x) + maps; call after any change of `interior`
     int levels_min() const;
     int levels_max() const;
     bool is_uniform() const { return levels_min() == levels_max(); }

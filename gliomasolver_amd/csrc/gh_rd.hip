@@ -82,6 +82,9 @@ int prepare_t(Ctx& c, double Dw, double Dg, double rho) {
                        c.pw.as<Real>(), c.pg.as<Real>(), (Real)Dw, (Real)Dg, c.statD.as<Stat4<Real>>(),
                        c.statSig.as<Real>());
     GH_HIP_CHECK(hipGetLastError());
+    int rc;
+    if ((rc = build_stat_fast(c.stream, c.real_kind, nc, c.statD.p, c.statSig.p, c.statF, c.flag, &c.fast_ok)))
+        return rc;
     c.Dw = Dw;
     c.Dg = Dg;
     c.rho = rho;
@@ -96,10 +99,9 @@ void launch_step(Ctx& c, const Real* in, Real* out, double dt) {
     const int per_xcd = (groups + 7) / 8;
     const double h = block_h(c);
     const double ih2 = 1. / (h * h);
-    const UniformStepParams<Real> prm{(Real)c.rho, ih2, dt};
+    const UniformStepParams<Real> prm{(Real)c.rho, ih2, dt, c.topo.uniform_bpd, bpd_magic(c.topo.uniform_bpd)};
     hipLaunchKernelGGL((rd_step_kernel<Real, FUSED, BPW, UniformStepParams<Real>>), dim3(8 * per_xcd), dim3(512), 0,
-                       c.stream, ns, per_xcd, c.nbr.as<int32_t>(), in, out, c.statD.as<Stat4<Real>>(),
-                       c.statSig.as<Real>(), prm);
+                       c.stream, ns, per_xcd, in, out, c.statD.as<Stat4<Real>>(), c.statSig.as<Real>(), prm);
 }
 
 // two fused steps per launch (rd_step2_kernel), one block per workgroup
@@ -109,10 +111,23 @@ int launch_step2(Ctx& c, const Real* in, Real* out, double dt) {
     const int per_xcd = (ns + 7) / 8;
     const double h = block_h(c);
     const double ih2 = 1. / (h * h);
-    const UniformStepParams<Real> prm{(Real)c.rho, ih2, dt};
+    const UniformStepParams<Real> prm{(Real)c.rho, ih2, dt, c.topo.uniform_bpd, bpd_magic(c.topo.uniform_bpd)};
     hipLaunchKernelGGL((rd_step2_kernel<Real, UniformStepParams<Real>>), dim3(8 * per_xcd), dim3(512), 0, c.stream,
-                       ns, per_xcd, c.nbr.as<int32_t>(), in, out, c.statD.as<Stat4<Real>>(), c.statSig.as<Real>(),
-                       prm);
+                       ns, per_xcd, in, out, c.statD.as<Stat4<Real>>(), c.statSig.as<Real>(), prm);
+    GH_HIP_CHECK(hipGetLastError());
+    return GH_OK;
+}
+
+// two fused steps per launch on a clamped state (rd_step2f_kernel)
+template <typename Real>
+int launch_step2f(Ctx& c, const Real* in, Real* out, double dt) {
+    const int ns = c.topo.nblocks();
+    const int per_xcd = (ns + 7) / 8;
+    const double h = block_h(c);
+    const double ih2 = 1. / (h * h);
+    const UniformStepParams<Real> prm{(Real)c.rho, ih2, dt, c.topo.uniform_bpd, bpd_magic(c.topo.uniform_bpd)};
+    hipLaunchKernelGGL((rd_step2f_kernel<Real, UniformStepParams<Real>>), dim3(8 * per_xcd), dim3(512), 0, c.stream,
+                       ns, per_xcd, in, out, c.statF.as<Stat4<Real>>(), prm);
     GH_HIP_CHECK(hipGetLastError());
     return GH_OK;
 }
@@ -203,23 +218,61 @@ int rd_autotune(Ctx& c, double dt, int trial_steps, int* best_out) {
 
 double block_h_of(const Ctx& c) { return block_h(c); }
 
+int build_stat_fast(hipStream_t stream, int real_kind, size_t n, const void* st, const void* sig, DevBuf& out,
+                    DevBuf& flag, bool* ok) {
+    const size_t rs = real_kind == GH_REAL_F32 ? 4 : 8;
+    int rc;
+    if ((rc = out.ensure(std::max<size_t>(1, n) * 4 * rs)) || (rc = flag.ensure(sizeof(unsigned int)))) return rc;
+    GH_HIP_CHECK(hipMemsetAsync(flag.p, 0, sizeof(unsigned int), stream));
+    const unsigned grid = (unsigned)((n + 255) / 256);
+    if (n > 0) {
+        if (real_kind == GH_REAL_F32)
+            hipLaunchKernelGGL(stat_fast_kernel<float>, dim3(grid), dim3(256), 0, stream, n, (const Stat4<float>*)st,
+                               (const float*)sig, out.as<Stat4<float>>(), flag.as<unsigned int>());
+        else
+            hipLaunchKernelGGL(stat_fast_kernel<double>, dim3(grid), dim3(256), 0, stream, n, (const Stat4<double>*)st,
+                               (const double*)sig, out.as<Stat4<double>>(), flag.as<unsigned int>());
+        GH_HIP_CHECK(hipGetLastError());
+    }
+    unsigned int bad = 0;
+    GH_HIP_CHECK(hipMemcpyAsync(&bad, flag.p, sizeof(bad), hipMemcpyDeviceToHost, stream));
+    GH_HIP_CHECK(hipStreamSynchronize(stream));
+    *ok = bad == 0;
+    return GH_OK;
+}
+
 int rd_step_buffers(Ctx& c, const void* in, void* out, double dt) {
     return c.real_kind == GH_REAL_F32 ? step_t<float, true>(c, (const float*)in, (float*)out, dt)
                                       : step_t<double, true>(c, (const double*)in, (double*)out, dt);
 }
 
+// Tile 0 takes two steps per launch.  A state written by the fused update is
+// clamped, so after one single step from whatever state the call starts with,
+// the pairs run rd_step2f_kernel when the anatomy allows it (fast_ok).
 int rd_steps(Ctx& c, int64_t nsteps, double dt) {
     c.dt_last = dt;
     const bool ghosted = c.topo.uniform_bpd <= 0;
+    const bool f32 = c.real_kind == GH_REAL_F32;
     int64_t k = 0;
-    if (!ghosted && c.rd_bpw == 0)
+    if (!ghosted && c.rd_bpw == 0) {
+        const bool fast = c.fast_ok && nsteps >= 3;
+        if (fast) {
+            int rc = f32 ? step_t<float, true>(c, c.phi[1 - c.cur].as<float>())
+                         : step_t<double, true>(c, c.phi[1 - c.cur].as<double>());
+            if (rc) return rc;
+            c.cur = 1 - c.cur;
+            k = 1;
+        }
         for (; k + 1 < nsteps; k += 2) {
-            int rc = c.real_kind == GH_REAL_F32
-                         ? launch_step2<float>(c, c.phi[c.cur].as<float>(), c.phi[1 - c.cur].as<float>(), dt)
-                         : launch_step2<double>(c, c.phi[c.cur].as<double>(), c.phi[1 - c.cur].as<double>(), dt);
+            const int cur = c.cur;
+            int rc = fast ? (f32 ? launch_step2f<float>(c, c.phi[cur].as<float>(), c.phi[1 - cur].as<float>(), dt)
+                                 : launch_step2f<double>(c, c.phi[cur].as<double>(), c.phi[1 - cur].as<double>(), dt))
+                          : (f32 ? launch_step2<float>(c, c.phi[cur].as<float>(), c.phi[1 - cur].as<float>(), dt)
+                                 : launch_step2<double>(c, c.phi[cur].as<double>(), c.phi[1 - cur].as<double>(), dt));
             if (rc) return rc;
             c.cur = 1 - c.cur;
         }
+    }
     for (; k < nsteps; ++k) {
         int rc = ghosted ? rd_step_ghosted(c, c.phi[c.cur].p, c.phi[1 - c.cur].p, dt, true)
                  : c.real_kind == GH_REAL_F32 ? step_t<float, true>(c, c.phi[1 - c.cur].as<float>())

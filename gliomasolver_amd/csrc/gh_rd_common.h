@@ -102,21 +102,13 @@ __host__ __device__ inline Real domain_threshold(Real pw, Real pg) {
 // LDS index of cell (x, y, z) in a 10^3 block tile with a one-cell halo
 __device__ inline int P(int x, int y, int z) { return (z + 1) * 100 + (y + 1) * 10 + (x + 1); }
 
-// dphidt of one cell (ReactionDiffusionOperator.h:103-138) from its face
-// coefficients d0..d5 (-x,+x,-y,+y,-z,+z, before the no-flux rule), the
-// neighbour values p0..p5 with their outside flags o0..o5, its own value pc
-// and domain threshold sc.
+// dphidt of one cell (ReactionDiffusionOperator.h:103-138) from its six face
+// coefficients d0..d5 (-x,+x,-y,+y,-z,+z) after the no-flux rule, the
+// neighbour values p0..p5, its own value pc and whether it lies in the brain
+// domain.
 template <typename Real>
-__device__ inline Real rd_cell_rhs(Real d0, Real d1, Real d2, Real d3, Real d4, Real d5, Real p0, Real p1, Real p2,
-                                   Real p3, Real p4, Real p5, bool o0, bool o1, bool o2, bool o3, bool o4, bool o5,
-                                   Real pc, Real sc, Real rho, double ih2) {
-    // _applyNoFluxBC: an outside neighbour doubles the opposite face (exact)
-    if (o0) d1 = (Real)((double)d1 * 2.0);
-    if (o1) d0 = (Real)((double)d0 * 2.0);
-    if (o2) d3 = (Real)((double)d3 * 2.0);
-    if (o3) d2 = (Real)((double)d2 * 2.0);
-    if (o4) d5 = (Real)((double)d5 * 2.0);
-    if (o5) d4 = (Real)((double)d4 * 2.0);
+__device__ inline Real rd_rhs_core(Real d0, Real d1, Real d2, Real d3, Real d4, Real d5, Real p0, Real p1, Real p2,
+                                   Real p3, Real p4, Real p5, Real pc, bool dom, Real rho, double ih2) {
     // diffusionFluxIn = ih2 * (Real sum of df_i*phi_i, left to right)
     Real sin_ = d0 * p0;
     sin_ = sin_ + d1 * p1;
@@ -137,7 +129,24 @@ __device__ inline Real rd_cell_rhs(Real d0, Real d1, Real d2, Real d3, Real d4, 
     const Real rp = rho * pc;
     const double react = (double)rp * (1. - (double)pc);
     const Real dphidt = (Real)((fout + fin) + react);
-    return (pc > sc) ? dphidt : (Real)0.;  // outside the brain domain (:103, :137-138)
+    return dom ? dphidt : (Real)0.;  // outside the brain domain (:103, :137-138)
+}
+
+// dphidt of one cell from its face coefficients d0..d5 before the no-flux
+// rule, the neighbour values p0..p5 with their outside flags o0..o5, its own
+// value pc and domain threshold sc.
+template <typename Real>
+__device__ inline Real rd_cell_rhs(Real d0, Real d1, Real d2, Real d3, Real d4, Real d5, Real p0, Real p1, Real p2,
+                                   Real p3, Real p4, Real p5, bool o0, bool o1, bool o2, bool o3, bool o4, bool o5,
+                                   Real pc, Real sc, Real rho, double ih2) {
+    // _applyNoFluxBC: an outside neighbour doubles the opposite face (exact)
+    if (o0) d1 = (Real)((double)d1 * 2.0);
+    if (o1) d0 = (Real)((double)d0 * 2.0);
+    if (o2) d3 = (Real)((double)d3 * 2.0);
+    if (o3) d2 = (Real)((double)d2 * 2.0);
+    if (o4) d5 = (Real)((double)d5 * 2.0);
+    if (o5) d4 = (Real)((double)d4 * 2.0);
+    return rd_rhs_core<Real>(d0, d1, d2, d3, d4, d5, p0, p1, p2, p3, p4, p5, pc, pc > sc, rho, ih2);
 }
 
 // UpdateTumor (ReactionDiffusionOperator.h:219-221): phi += dt*dphidt in
@@ -149,17 +158,56 @@ __device__ inline Real rd_update(Real pc, Real dphidt, double dt) {
     return (p < (Real)1.) ? p : (Real)1.;
 }
 
+// Position of a block in a uniform periodic grid of bpd^3 blocks whose slots
+// start at slot0 (slot = slot0 + bx + bpd*(by + bpd*bz)): the step kernels
+// find neighbour blocks by arithmetic on the workgroup-uniform block index,
+// so no halo load waits on a table load.
+struct BlockPos {
+    int slot0, bpd, bx, by, bz;
+    __device__ inline int nb(int dx, int dy, int dz) const {
+        int x = bx + dx, y = by + dy, z = bz + dz;
+        x += x < 0 ? bpd : 0;
+        x -= x >= bpd ? bpd : 0;
+        y += y < 0 ? bpd : 0;
+        y -= y >= bpd ? bpd : 0;
+        z += z < 0 ? bpd : 0;
+        z -= z >= bpd ? bpd : 0;
+        return slot0 + x + bpd * (y + bpd * z);
+    }
+};
+// magic multiplier of the division by bpd in block_pos: m = ceil(2^32 / bpd),
+// m * bpd = 2^32 + r with 0 <= r < bpd.  floor(l * m / 2^32) = floor(l / bpd)
+// whenever l * r < 2^32: always for a power of two (r = 0), and for l < bpd^3
+// with bpd <= 64 otherwise (bpd = 1 has l = 0 only).
+__host__ __device__ inline uint32_t bpd_magic(int bpd) {
+    return bpd <= 1 ? 0xffffffffu : (uint32_t)((0x100000000ull + (uint64_t)bpd - 1) / (uint64_t)bpd);
+}
+__device__ inline BlockPos block_pos(int s, int slot0, int bpd, uint32_t magic) {
+    const uint32_t l = (uint32_t)(s - slot0);
+    const uint32_t q = __umulhi(l, magic);  // l / bpd
+    const uint32_t z = __umulhi(q, magic);  // l / bpd^2
+    return BlockPos{slot0, bpd, (int)(l - q * (uint32_t)bpd), (int)(q - z * (uint32_t)bpd), (int)z};
+}
+// unit offset of face f (FXM, FXP, FYM, FYP, FZM, FZP)
+__device__ inline int face_dx(int f) { return f == FXM ? -1 : (f == FXP ? 1 : 0); }
+__device__ inline int face_dy(int f) { return f == FYM ? -1 : (f == FYP ? 1 : 0); }
+__device__ inline int face_dz(int f) { return f == FZM ? -1 : (f == FZP ? 1 : 0); }
+__device__ inline int face_nb(const BlockPos& p, int f) { return p.nb(face_dx(f), face_dy(f), face_dz(f)); }
+
 // Step parameters of a slot: one solve (the same rho, 1/h^2, dt for every
 // slot) or many independent solves packed slot range after slot range.
 template <typename Real>
 struct UniformStepParams {
     Real rho;
     double ih2, dt;
+    int bpd;
+    uint32_t magic;  // bpd_magic(bpd)
     __device__ inline void get(int, Real& r, double& i, double& d) const {
         r = rho;
         i = ih2;
         d = dt;
     }
+    __device__ inline BlockPos pos(int s) const { return block_pos(s, 0, bpd, magic); }
 };
 
 // BPW blocks per workgroup of 512 lanes; lane = cell (x fastest) of each
@@ -169,7 +217,7 @@ struct UniformStepParams {
 // (w % 8) * per_xcd + w / 8 and each XCD sweeps a contiguous range of blocks
 // whose halos stay in its own L2.
 template <typename Real, bool FUSED, int BPW, typename Params>
-__global__ __launch_bounds__(512) void rd_step_kernel(int nslots, int per_xcd, const int32_t* __restrict__ nbr,
+__global__ __launch_bounds__(512) void rd_step_kernel(int nslots, int per_xcd,
                                                       const Real* __restrict__ phi, Real* __restrict__ out,
                                                       const Stat4<Real>* __restrict__ st,
                                                       const Real* __restrict__ sig, Params prm) {
@@ -207,7 +255,7 @@ __global__ __launch_bounds__(512) void rd_step_kernel(int nslots, int per_xcd, c
         for (int j = 0; j < BPW; ++j) {
             const int s = grp * BPW + j;
             if (s >= nslots) break;
-            const size_t nb = (size_t)nbr[6 * s + f] * BCELLS + src;
+            const size_t nb = (size_t)face_nb(prm.pos(s), f) * BCELLS + src;
             const Real ph = phi[nb];
             const Stat4<Real> hs = st[nb];
             s_phi[j][dst] = ph;
@@ -257,7 +305,7 @@ __global__ __launch_bounds__(512) void rd_step_kernel(int nslots, int per_xcd, c
 __device__ inline int T12(int x, int y, int z) { return (z + 2) * 144 + (y + 2) * 12 + (x + 2); }  // [-2,10)^3
 
 template <typename Real, typename Params>
-__global__ __launch_bounds__(512) void rd_step2_kernel(int nslots, int per_xcd, const int32_t* __restrict__ nbr,
+__global__ __launch_bounds__(512) void rd_step2_kernel(int nslots, int per_xcd,
                                                        const Real* __restrict__ phi, Real* __restrict__ out,
                                                        const Stat4<Real>* __restrict__ st,
                                                        const Real* __restrict__ sig, Params prm) {
@@ -272,6 +320,7 @@ __global__ __launch_bounds__(512) void rd_step2_kernel(int nslots, int per_xcd, 
     const Real pc0 = phi[g];
     const Stat4<Real> sv = st[g];
     const Real sc = sig[g];
+    const BlockPos bp = prm.pos(s);
     // halo cells: [0,384) depth-1 faces, [384,768) depth-2 faces, [768,864) edge rows
     Real hsig = 0;         // sigma of this lane's depth-1 face cell (lanes < 384)
     int fx = 0, fy = 0, fz = 0;
@@ -283,7 +332,7 @@ __global__ __launch_bounds__(512) void rd_step2_kernel(int nslots, int per_xcd, 
         if (h < 768) {
             const int d = h < 384 ? 1 : 2;
             const int r = h % 384, f = r >> 6, a = r & 7, b = (r >> 3) & 7;
-            src_slot = nbr[6 * s + f];
+            src_slot = face_nb(bp, f);
             switch (f) {
                 case FXM: tx = -d; ty = a; tz = b; lx = 8 - d; ly = a; lz = b; break;
                 case FXP: tx = 7 + d; ty = a; tz = b; lx = d - 1; ly = a; lz = b; break;
@@ -308,7 +357,7 @@ __global__ __launch_bounds__(512) void rd_step2_kernel(int nslots, int per_xcd, 
                 f1 = s1 ? FYP : FYM; f2 = s2 ? FZP : FZM;
                 tx = i; ty = c1; tz = c2; lx = i; ly = l1; lz = l2;
             }
-            src_slot = nbr[6 * nbr[6 * s + f1] + f2];
+            src_slot = bp.nb(face_dx(f1) + face_dx(f2), face_dy(f1) + face_dy(f2), face_dz(f1) + face_dz(f2));
         }
         const size_t gs = (size_t)src_slot * BCELLS + lx + 8 * ly + 64 * lz;
         const int q = T12(tx, ty, tz);
@@ -364,6 +413,224 @@ __global__ __launch_bounds__(512) void rd_step2_kernel(int nslots, int per_xcd, 
                                          p5, p0 < s_tau[n0], p1 < s_tau[n1], p2 < s_tau[n2], p3 < s_tau[n3],
                                          p4 < s_tau[n4], p5 < s_tau[n5], pc, sc, rho, ih2);
         out[g] = rd_update(pc, d, dt);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Two fused steps per launch on clamped states (rd_step2f_kernel).
+//
+// Preconditions, checked by the callers: every state value has a clear sign
+// bit (true of any state written by the fused update, whose clamp returns +0
+// for p <= 0 and for NaN), and every cell has p_w + p_g >= 0.  Then
+//   * "phi < tau" equals "phi < max(tau, +0)" and "phi > sigma" equals
+//     "sigma < 0 or phi > 0", so one record {D+x, D+y, D+z, tau_enc} per cell
+//     (tau_enc = max(tau, +0), negated for cells with sigma < 0) carries all
+//     per-solve constants (StatFast);
+//   * a neighbour's outside flag travels in the sign bit of its value in LDS
+//     (-phi when phi < tau), so a stencil reads six values instead of six
+//     values and six thresholds, and the flag doubles a face by ldexp.
+// The arithmetic after the no-flux doubling is rd_rhs_core, and doubling is
+// exact, so the results are those of rd_step2_kernel bit for bit.
+//
+// LDS: s_uu holds the encoded step-0 state on [-2,10)^3 and, in the second
+// half of each 24-value row, the encoded step-1 state on [-1,9)^3; rows of 24
+// and planes of 293 values make the 8x8 lane patterns of a wave conflict-free
+// for the block interior.  s_dmx/y/z[c] = D+x(c-x), D+y(c-y), D+z(c-z): the
+// minus face coefficients of every cell evaluated in step 1.
+// ---------------------------------------------------------------------------
+__device__ inline int UU(int x, int y, int z) { return (x + 2) + 24 * (y + 2) + 293 * (z + 2); }  // [-2,10)^3
+constexpr int UU1 = 12;  // offset of the step-1 state in a row
+constexpr int UU_SIZE = 3516;
+__device__ inline int DMI(int x, int y, int z) { return (x + 1) + 10 * (y + 1) + 100 * (z + 1); }  // [-1,9)^3
+
+__device__ inline uint32_t sign_of(float v) { return __builtin_bit_cast(uint32_t, v) >> 31; }
+__device__ inline uint32_t sign_of(double v) { return (uint32_t)(__builtin_bit_cast(uint64_t, v) >> 63); }
+__device__ inline float twice_if(float d, uint32_t o) { return __builtin_amdgcn_ldexpf(d, (int)o); }
+__device__ inline double twice_if(double d, uint32_t o) { return __builtin_amdgcn_ldexp(d, (int)o); }
+// value with the outside flag in its sign bit (value >= +0)
+template <typename Real>
+__device__ inline Real flag_in_sign(Real u, Real tau) { return u < tau ? -u : u; }
+
+// dphidt from encoded neighbours e0..e5 (-x,+x,-y,+y,-z,+z) and the face
+// coefficients d0..d5 before the no-flux rule
+template <typename Real>
+__device__ inline Real rd_rhs_enc(Real d0, Real d1, Real d2, Real d3, Real d4, Real d5, Real e0, Real e1, Real e2,
+                                  Real e3, Real e4, Real e5, Real pc, bool inside, Real rho, double ih2) {
+    d1 = twice_if(d1, sign_of(e0));
+    d0 = twice_if(d0, sign_of(e1));
+    d3 = twice_if(d3, sign_of(e2));
+    d2 = twice_if(d2, sign_of(e3));
+    d5 = twice_if(d5, sign_of(e4));
+    d4 = twice_if(d4, sign_of(e5));
+    return rd_rhs_core<Real>(d0, d1, d2, d3, d4, d5, std::fabs(e0), std::fabs(e1), std::fabs(e2), std::fabs(e3),
+                             std::fabs(e4), std::fabs(e5), pc, inside || pc > (Real)0., rho, ih2);
+}
+
+// {dx, dy, dz, tau} + sigma -> the record of rd_step2f_kernel; counts the
+// cells that break the precondition p_w + p_g >= 0 (sigma <= 0)
+template <typename Real>
+__global__ __launch_bounds__(256) void stat_fast_kernel(size_t n, const Stat4<Real>* __restrict__ st,
+                                                        const Real* __restrict__ sig, Stat4<Real>* __restrict__ out,
+                                                        unsigned int* __restrict__ bad) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    Stat4<Real> r = st[i];
+    const Real sg = sig[i];
+    const Real t = r.tau > (Real)0. ? r.tau : (Real)0.;
+    r.tau = sg < (Real)0. ? -t : t;
+    out[i] = r;
+    if (!(sg <= (Real)0.)) atomicAdd(bad, 1u);
+}
+
+// Where cell (a, b) of face f at depth d (1 or 2) outside the block sits:
+// LDS offsets uu (state tile) and dm (coefficient tile) and the cell index in
+// the neighbour block across the face, each base + a * step_a + b * step_b
+// with wave-uniform bases and steps.
+struct FaceMap {
+    int uu, uua, uub;
+    int dm, dma, dmb;
+    int lo, loa, lob;
+};
+__device__ inline FaceMap face_map(int f, int d) {
+    const int u0 = UU(0, 0, 0), m0 = DMI(0, 0, 0);
+    switch (f) {
+        case FXM: return {u0 - d, 24, 293, m0 - d, 10, 100, 8 - d, 8, 64};
+        case FXP: return {u0 + 7 + d, 24, 293, m0 + 7 + d, 10, 100, d - 1, 8, 64};
+        case FYM: return {u0 - 24 * d, 1, 293, m0 - 10 * d, 1, 100, 8 * (8 - d), 1, 64};
+        case FYP: return {u0 + 24 * (7 + d), 1, 293, m0 + 10 * (7 + d), 1, 100, 8 * (d - 1), 1, 64};
+        case FZM: return {u0 - 293 * d, 1, 24, m0 - 100 * d, 1, 10, 64 * (8 - d), 1, 8};
+        default:  return {u0 + 293 * (7 + d), 1, 24, m0 + 100 * (7 + d), 1, 10, 64 * (d - 1), 1, 8};
+    }
+}
+__device__ inline int mad24(int a, int b, int c) { return (int)__umul24((unsigned)a, (unsigned)b) + c; }
+
+template <typename Real, typename Params>
+__global__ __launch_bounds__(512) void rd_step2f_kernel(int nslots, int per_xcd, const Real* __restrict__ phi,
+                                                        Real* __restrict__ out, const Stat4<Real>* __restrict__ stf,
+                                                        Params prm) {
+    const int s = (int)(blockIdx.x % 8) * per_xcd + (int)(blockIdx.x / 8);
+    if (s >= nslots) return;
+    __shared__ Real s_uu[UU_SIZE];
+    __shared__ Real s_dmx[1000], s_dmy[1000], s_dmz[1000];
+
+    const int t = threadIdx.x;
+    const int w = __builtin_amdgcn_readfirstlane(t >> 6);  // wave = interior plane z; halo role
+    const int a = t & 7, b = (t >> 3) & 7;
+    const BlockPos bp = prm.pos(s);
+    const size_t base = (size_t)s * BCELLS;
+    const unsigned ut = (unsigned)t;
+
+    // ---- loads: own cell, then (waves 0-5) the depth-1 and depth-2 cells of
+    // face w, (waves 6-7) the 96 edge cells next to the depth-1 faces; block
+    // base pointers are wave-uniform, lane offsets 32-bit
+    const Real u_c = (phi + base)[ut];
+    const Stat4<Real> st_c = (stf + base)[ut];
+    Real u_1 = 0, u_2 = 0, u_e = 0;  // depth-1, depth-2 and edge cell
+    Stat4<Real> st_1, st_2, st_e;
+    int uu1 = 0, dm1 = 0, uu2 = 0, dm2 = 0;
+    int ex = 0, ey = 0, ez = 0;  // edge cell: tile coordinates
+    bool edge = false;
+    if (w < 6) {
+        const FaceMap m1 = face_map(w, 1), m2 = face_map(w, 2);
+        const size_t nb = (size_t)face_nb(bp, w) * BCELLS;
+        const unsigned l1 = (unsigned)mad24(b, m1.lob, mad24(a, m1.loa, m1.lo));
+        const unsigned l2 = (unsigned)mad24(b, m2.lob, mad24(a, m2.loa, m2.lo));
+        __builtin_assume(l1 < BCELLS && l2 < BCELLS);
+        const Real* pn = phi + nb;
+        const Stat4<Real>* sn = stf + nb;
+        u_1 = pn[l1];
+        st_1 = sn[l1];
+        u_2 = pn[l2];
+        st_2 = sn[l2];
+        uu1 = mad24(b, m1.uub, mad24(a, m1.uua, m1.uu));
+        dm1 = mad24(b, m1.dmb, mad24(a, m1.dma, m1.dm));
+        uu2 = mad24(b, m2.uub, mad24(a, m2.uua, m2.uu));
+        dm2 = mad24(b, m2.dmb, mad24(a, m2.dma, m2.dm));
+    } else {
+        const int e = (w - 6) * 64 + (t & 63);
+        edge = e < 96;
+        if (edge) {
+            const int k = e >> 3, i = e & 7;
+            const int pair = k >> 2, s1 = k & 1, s2 = (k >> 1) & 1;
+            const int c1 = s1 ? 8 : -1, c2 = s2 ? 8 : -1;  // tile coordinate on the two crossed axes
+            const int m1 = s1 ? 0 : 7, m2 = s2 ? 0 : 7;    // coordinate in the diagonal block
+            const int d1 = s1 ? 1 : -1, d2 = s2 ? 1 : -1;
+            int sl, lc;
+            if (pair == 0) {
+                ex = c1; ey = c2; ez = i; lc = m1 + 8 * m2 + 64 * i; sl = bp.nb(d1, d2, 0);
+            } else if (pair == 1) {
+                ex = c1; ey = i; ez = c2; lc = m1 + 8 * i + 64 * m2; sl = bp.nb(d1, 0, d2);
+            } else {
+                ex = i; ey = c1; ez = c2; lc = i + 8 * m1 + 64 * m2; sl = bp.nb(0, d1, d2);
+            }
+            const size_t gs = (size_t)sl * BCELLS + lc;
+            u_e = phi[gs];
+            st_e = stf[gs];
+        }
+    }
+    Real rho;
+    double ih2, dt;
+    prm.get(s, rho, ih2, dt);
+
+    // ---- LDS: encoded step-0 state, minus-face coefficients of the step-1 cells
+    const int uuc = mad24(w, 293, mad24(b, 24, a + UU(0, 0, 0)));
+    const int dmc = mad24(w, 100, mad24(b, 10, a + DMI(0, 0, 0)));
+    const Real tau_c = std::fabs(st_c.tau);
+    s_uu[uuc] = flag_in_sign(u_c, tau_c);
+    s_dmx[dmc + 1] = st_c.dx;
+    s_dmy[dmc + 10] = st_c.dy;
+    s_dmz[dmc + 100] = st_c.dz;
+    Real tau_1 = 0;
+    if (w < 6) {
+        tau_1 = std::fabs(st_1.tau);
+        s_uu[uu1] = flag_in_sign(u_1, tau_1);
+        s_uu[uu2] = flag_in_sign(u_2, std::fabs(st_2.tau));
+        // depth-1 cell: every component whose target stays in [-1,9)^3;
+        // depth-2 cell: only the normal component of a minus face
+        if (w != FXP) s_dmx[dm1 + 1] = st_1.dx;
+        if (w != FYP) s_dmy[dm1 + 10] = st_1.dy;
+        if (w != FZP) s_dmz[dm1 + 100] = st_1.dz;
+        if (w == FXM) s_dmx[dm2 + 1] = st_2.dx;
+        if (w == FYM) s_dmy[dm2 + 10] = st_2.dy;
+        if (w == FZM) s_dmz[dm2 + 100] = st_2.dz;
+    } else if (edge) {
+        s_uu[UU(ex, ey, ez)] = flag_in_sign(u_e, std::fabs(st_e.tau));
+        // an edge cell feeds the minus face of a depth-1 cell across each of
+        // its crossed axes at -1
+        if (ex == -1) s_dmx[DMI(0, ey, ez)] = st_e.dx;
+        if (ey == -1) s_dmy[DMI(ex, 0, ez)] = st_e.dy;
+        if (ez == -1) s_dmz[DMI(ex, ey, 0)] = st_e.dz;
+    }
+    __syncthreads();
+
+    // ---- step 1 on the interior cell and (waves 0-5) the depth-1 face cell;
+    // the step-1 half of a row (offsets 13..22) is disjoint from the step-0
+    // half (offsets 0..11), so the writes race with no read
+    const bool in_c = sign_of(st_c.tau) != 0;
+    const Real dmx_c = s_dmx[dmc], dmy_c = s_dmy[dmc], dmz_c = s_dmz[dmc];
+    Real u1c;
+    {
+        const int q = uuc;
+        const Real d = rd_rhs_enc<Real>(dmx_c, st_c.dx, dmy_c, st_c.dy, dmz_c, st_c.dz, s_uu[q - 1], s_uu[q + 1],
+                                        s_uu[q - 24], s_uu[q + 24], s_uu[q - 293], s_uu[q + 293], u_c, in_c, rho, ih2);
+        u1c = rd_update(u_c, d, dt);
+        s_uu[q + UU1] = flag_in_sign(u1c, tau_c);
+    }
+    if (w < 6) {
+        const int q = uu1;
+        const Real d = rd_rhs_enc<Real>(s_dmx[dm1], st_1.dx, s_dmy[dm1], st_1.dy, s_dmz[dm1], st_1.dz, s_uu[q - 1], s_uu[q + 1],
+                                        s_uu[q - 24], s_uu[q + 24], s_uu[q - 293], s_uu[q + 293], u_1,
+                                        sign_of(st_1.tau) != 0, rho, ih2);
+        s_uu[q + UU1] = flag_in_sign(rd_update(u_1, d, dt), tau_1);
+    }
+    __syncthreads();
+
+    // ---- step 2 on the interior cell
+    {
+        const int q = uuc + UU1;
+        const Real d = rd_rhs_enc<Real>(dmx_c, st_c.dx, dmy_c, st_c.dy, dmz_c, st_c.dz, s_uu[q - 1], s_uu[q + 1],
+                                        s_uu[q - 24], s_uu[q + 24], s_uu[q - 293], s_uu[q + 293], u1c, in_c, rho, ih2);
+        (out + base)[ut] = rd_update(u1c, d, dt);
     }
 }
 

@@ -207,3 +207,40 @@ def test_gpu_two_step_schedule_matches_oracle(oracle, n, nsteps):
     ref = oracle.rd_steps(phi, pw, pg, np.float32(Dw), np.float32(Dg), np.float32(rho), np.float32(1.0 / n), dt,
                           nsteps)
     assert bits_equal(out, ref)
+
+
+@pytest.mark.parametrize("real", ["f32", "f64"])
+def test_gpu_clamped_pairs_match_oracle_on_edge_states(oracle, real):
+    """Two steps per launch on clamped states (sign-encoded outside flags and
+    one constant record per cell) against the oracle: an initial state with
+    negative values, -0 and values above 1, cells with p_w + p_g = 0 (domain
+    boundary), = 0.1 - phi (outside threshold) and >= 0.1 (never outside)."""
+    n, nsteps = 32, 11
+    dt_ = np.float32 if real == "f32" else np.float64
+    rng = np.random.default_rng(77)
+    pw = rng.uniform(0, 1, (n, n, n))
+    pg = rng.uniform(0, 1, (n, n, n)) * (1 - pw)
+    z = rng.uniform(size=pw.shape)
+    pw[z < 0.2] = 0
+    pg[z < 0.2] = 0
+    pw[(z >= 0.2) & (z < 0.3)] = 0.05
+    pg[(z >= 0.2) & (z < 0.3)] = 0.0
+    pw, pg = pw.astype(dt_), pg.astype(dt_)
+    phi = rng.uniform(-0.2, 1.2, (n, n, n)).astype(dt_)
+    phi[rng.uniform(size=phi.shape) < 0.05] = dt_(-0.0)
+    phi[(z >= 0.2) & (z < 0.3)] = dt_(0.05)
+    say("clamped pairs", real, "n=%d steps=%d" % (n, nsteps))
+    ctx = gh.Context(0, real)
+    ctx.grid_init_uniform(n // 8)
+    ctx.set_field(FIELD_PW, pw)
+    ctx.set_field(FIELD_PG, pg)
+    ctx.set_field(FIELD_PHI, phi)
+    Dw, Dg, rho = 0.0021, 0.00021, 0.31
+    dt = 0.2 / (n * n) / Dw
+    ctx.rd_prepare(Dw, Dg, rho)
+    ctx.rd_set_tile(0)
+    ctx.rd_steps(nsteps, dt)
+    out = ctx.get_field(FIELD_PHI)
+    ctx.close()
+    ref = oracle.rd_steps(phi, pw, pg, dt_(Dw), dt_(Dg), dt_(rho), dt_(1.0 / n), dt, nsteps)
+    assert bits_equal(out, ref)
