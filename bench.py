@@ -44,7 +44,7 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 BYTES_PER_CELL_UPDATE_F32 = 16
 METRIC = "RD cell-updates/s (% HBM roofline); batched fwd+adjoint evals/s vs plain"
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "rd_step_uniform128_f32_traffic.json")
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "rd_step2f_uniform128_f32_traffic.json")
 
 
 def log(*a):
@@ -309,6 +309,7 @@ def main():
     pw, pg, phi0 = ctx.get_field(FIELD_PW), ctx.get_field(FIELD_PG), ctx.get_field(FIELD_PHI)
     ctx.checkpoint_save(0)
     tile = ctx.rd_autotune(case.dt, 100)
+    clamped = ctx.rd_schedule_info()[1]
     log("autotuned schedule: %s" % ("two Euler steps per launch" if tile == 0 else
                                     "%d blocks of 8^3 per workgroup" % tile))
     for i in range(a.warmup):
@@ -335,12 +336,14 @@ def main():
     avg_launch_s = kernel_ms * 1e-3 / launches
     # algorithmic bytes of all cell-updates over the device time of all launches
     achieved = BYTES_PER_CELL_UPDATE_F32 * updates / (kernel_ms * 1e-3) / 1e9
+    kbase = ("rd_step2f_kernel" if clamped else "rd_step2_kernel") if tile == 0 else "rd_step_kernel"
     traffic = None
     if os.path.exists(TRAFFIC_FILE):
         tf = json.load(open(TRAFFIC_FILE))
-        if tf.get("kernel", "rd_step_kernel") == ("rd_step2_kernel" if tile == 0 else "rd_step_kernel"):
+        if tf.get("kernel") == kbase:
             traffic = tf.get("bytes_per_launch")
-    kname = "rd_step2_kernel<float> (2 Euler steps per launch)" if tile == 0 else "rd_step_kernel<float,true,%d>" % tile
+    kname = ("%s<float> (2 Euler steps per launch)" % kbase if tile == 0 else
+             "rd_step_kernel<float,true,%d>" % tile)
     roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
             "kernel": kname, "avg_launch_us": round(avg_launch_s * 1e6, 3), "steps_per_launch": steps_per_launch,

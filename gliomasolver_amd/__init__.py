@@ -241,6 +241,13 @@ class Context:
         check(self.lib.gh_rd_autotune(self.h, dt, trial_steps, C.byref(best)))
         return best.value
 
+    def rd_schedule_info(self):
+        """(tile, clamped_pairs): the step schedule and whether pairs of steps
+        run the clamped-state kernel."""
+        tile, cp = C.c_int(), C.c_int()
+        check(self.lib.gh_rd_schedule_info(self.h, C.byref(tile), C.byref(cp)))
+        return tile.value, bool(cp.value)
+
     # ---- likelihood (HGG_Likelihood) ----
     LIK_DEFAULT = (0.000361, 0.85, 0.7, 0.25, 2.0)  # LikelihoodInput.txt
 

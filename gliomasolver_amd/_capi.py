@@ -58,6 +58,7 @@ SIGNATURES = {
     "gh_rd_steps": (_i, [_vp, _i64, _d, _pd]),
     "gh_rd_set_tile": (_i, [_vp, _i]),
     "gh_rd_autotune": (_i, [_vp, _d, _i, _pi]),
+    "gh_rd_schedule_info": (_i, [_vp, _pi, _pi]),
     "gh_rd_thresholds": (_i, [_i, _i64, _vp, _vp, _vp, _vp]),
     "gh_lik_set_data": (_i, [_vp, _i, _vp, _vp, _vp, _vp, _i]),
     "gh_likelihood": (_i, [_vp, _pd, _i, _i, _i, _pd, _pi64]),

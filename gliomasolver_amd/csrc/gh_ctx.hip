@@ -432,6 +432,14 @@ int gh_rd_set_tile(gh_ctx* ctx, int blocks_per_workgroup) {
     return GH_OK;
 }
 
+int gh_rd_schedule_info(gh_ctx* ctx, int* tile, int* clamped_pairs) {
+    Ctx* c = as_ctx(ctx);
+    if (!c || !tile || !clamped_pairs) return set_error(GH_EINVAL, "gh_rd_schedule_info: NULL argument");
+    *tile = c->rd_bpw;
+    *clamped_pairs = (c->prepared && c->fast_ok && c->topo.uniform_bpd > 0) ? 1 : 0;
+    return GH_OK;
+}
+
 int gh_rd_autotune(gh_ctx* ctx, double dt, int trial_steps, int* best) {
     Ctx* c = as_ctx(ctx);
     if (!c) return set_error(GH_EINVAL, "null context");

@@ -150,6 +150,12 @@ int gh_rd_steps(gh_ctx* ctx, int64_t nsteps, double dt, double* elapsed_ms);
  * *best.  Results do not depend on the schedule. */
 int gh_rd_set_tile(gh_ctx* ctx, int blocks_per_workgroup);
 int gh_rd_autotune(gh_ctx* ctx, double dt, int trial_steps, int* best);
+/* gh_rd_schedule_info (no reference counterpart): the current schedule
+ * (*tile as in gh_rd_set_tile) and whether two-steps-per-launch runs the
+ * clamped-state kernel (*clamped_pairs = 1: p_w + p_g >= 0 in every cell, so
+ * after one single step from the caller's state every pair of steps takes
+ * it; 0: the general two-step kernel). */
+int gh_rd_schedule_info(gh_ctx* ctx, int* tile, int* clamped_pairs);
 /* gh_rd_thresholds (host-side, no device): the per-cell predicates that
  * gh_rd_prepare stores, for inspection and testing.  tau: the neighbour test
  * "phi + p_w + p_g < 0.1" of ReactionDiffusionOperator.h:115-120,166-181 holds

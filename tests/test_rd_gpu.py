@@ -244,3 +244,30 @@ def test_gpu_clamped_pairs_match_oracle_on_edge_states(oracle, real):
     ctx.close()
     ref = oracle.rd_steps(phi, pw, pg, dt_(Dw), dt_(Dg), dt_(rho), dt_(1.0 / n), dt, nsteps)
     assert bits_equal(out, ref)
+
+
+def test_gpu_schedule_info_reports_the_clamped_pair_kernel():
+    """p_w + p_g >= 0 everywhere enables the clamped-state pair kernel; one
+    cell with p_w + p_g < 0 disables it (and results stay golden-exact)."""
+    c = load_case("rd_f32_random16")
+    say("schedule info f32 random16")
+    n = c["phi0"].shape[0]
+    pw = c["p_w"].copy()
+    for negative, expect in ((False, True), (True, False)):
+        p = pw.copy()
+        if negative:
+            p[3, 4, 5] = np.float32(-0.25)
+            p_g = c["p_g"].copy()
+            p_g[3, 4, 5] = np.float32(0.0)
+        else:
+            p = np.maximum(p, np.float32(0.0))
+            p_g = c["p_g"]
+        ctx = gh.Context(0, "f32")
+        ctx.grid_init_uniform(n // 8)
+        ctx.set_field(FIELD_PW, p)
+        ctx.set_field(FIELD_PG, p_g)
+        ctx.set_field(FIELD_PHI, c["phi0"])
+        ctx.rd_prepare(float(c["Dw"]), float(c["Dg"]), float(c["rho"]))
+        ctx.rd_set_tile(0)
+        assert ctx.rd_schedule_info() == (0, expect)
+        ctx.close()
