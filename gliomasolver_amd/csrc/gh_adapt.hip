@@ -14,10 +14,13 @@
 //
 // Ghost values: a ghost lattice point at the block's level is a same-level
 // copy when a same-level leaf covers it, else a linear combination of leaf
-// cells built on the host (Octree::value_terms): injection from finer regions
-// (Ha = {1}), Hs synthesis from coarser ones.  Terms are accumulated in Real
-// in ascending (source cell, weight) order; the weights are dyadic, exact in
-// float and double.
+// cells: injection from finer regions (Ha = {1}), Hs synthesis from coarser
+// ones, recursively (Octree::value_terms on the host, ghost_value on the
+// device).  Terms are accumulated in Real, product by product, in recursion
+// order (synthesis taps z-major, then y, then x); the weights are dyadic,
+// exact in float and double.  The labs of the FWT and of the split are
+// evaluated on the device from per-level block maps; the RD face ghosts of a
+// multi-level grid use a host-built table (applied every step).
 #include <algorithm>
 #include <array>
 #include <cmath>
@@ -105,6 +108,99 @@ __global__ __launch_bounds__(256) void ghost_entry_kernel(int n, const GhostEntr
         acc = acc + prod;
     }
     out[e.dst] = acc;
+}
+
+// Per-level block maps (Octree::dense on the device): map[off[l] + x +
+// (y << l) + (z << 2l)] = slot of a leaf at (l, x, y, z), -2 for a node with
+// children, -1 for a point inside a coarser leaf; levels >= nlev hold no node.
+struct LevelMaps {
+    const int32_t* __restrict__ map;
+    int off[Octree::MAXL + 1];
+    int nlev;
+};
+
+// Hs synthesis taps of fine point q (MRAGWavelets_StaticData.cpp:13-21,
+// 31-39): even q -> coarse q/2 with weight 1; odd q -> (q-1)/2 + k, k < 2,
+// weight 1/2 (order 2) or (q-3)/2 + k, k < 4, weights -1/16, 9/16, 9/16, -1/16
+__device__ inline int synth_n(int order, int64_t q) { return (q & 1) == 0 ? 1 : (order == 2 ? 2 : 4); }
+__device__ inline int64_t synth_src(int order, int64_t q, int k) {
+    return (q & 1) == 0 ? q / 2 : (order == 2 ? (q - 1) / 2 + k : (q - 3) / 2 + k);
+}
+__device__ inline double synth_w(int order, int64_t q, int k) {
+    return (q & 1) == 0 ? 1.0 : (order == 2 ? 0.5 : ((k == 0 || k == 3) ? -1.0 / 16 : 9.0 / 16));
+}
+
+// acc += value of lattice point q at level L times w (Octree::value_terms,
+// term by term in the same order).  D more synthesis levels are allowed
+// below this one; deeper recursion sets bad.
+template <int D, typename Real>
+__device__ inline void ghost_acc(const LevelMaps& m, int order, int L, int64_t qx, int64_t qy, int64_t qz, double w,
+                                 const Real* __restrict__ f, Real& acc, int& bad) {
+    for (;;) {
+        const int64_t mask = ((int64_t)BS << L) - 1;  // periodic wrap, power-of-two lattice
+        qx &= mask;
+        qy &= mask;
+        qz &= mask;
+        if (L >= m.nlev) break;  // inside a coarser leaf
+        const int bx = (int)(qx >> 3), by = (int)(qy >> 3), bz = (int)(qz >> 3);
+        const int32_t v = m.map[m.off[L] + bx + ((int64_t)by << L) + ((int64_t)bz << (2 * L))];
+        if (v >= 0) {
+            const Real prod = f[(size_t)v * BCELLS + (qx & 7) + 8 * (qy & 7) + 64 * (qz & 7)] * (Real)w;
+            acc = acc + prod;
+            return;
+        }
+        if (v != -2) break;  // coarser
+        ++L;                 // finer region: the coincident fine vertex (Ha = {1})
+        qx *= 2;
+        qy *= 2;
+        qz *= 2;
+    }
+    if constexpr (D == 0) {
+        bad = 1;
+    } else {
+        const int nz = synth_n(order, qz), ny = synth_n(order, qy), nx = synth_n(order, qx);
+#pragma unroll 1
+        for (int kz = 0; kz < nz; ++kz)
+#pragma unroll 1
+            for (int ky = 0; ky < ny; ++ky)
+#pragma unroll 1
+                for (int kx = 0; kx < nx; ++kx)
+                    ghost_acc<D - 1, Real>(
+                        m, order, L - 1, synth_src(order, qx, kx), synth_src(order, qy, ky), synth_src(order, qz, kz),
+                        w * (synth_w(order, qz, kz) * synth_w(order, qy, ky) * synth_w(order, qx, kx)), f, acc, bad);
+    }
+}
+
+// Full labs of the listed slots (slots == nullptr: every slot j), ghosts
+// evaluated on the device; *bad is set when a ghost needs more than two
+// synthesis levels (the caller then uses host tables).
+template <typename Real>
+__global__ __launch_bounds__(256) void lab_kernel(int nlist, const int32_t* __restrict__ slots, int LS, int LAB,
+                                                  int order, const int32_t* __restrict__ nbr27,
+                                                  const int4* __restrict__ blk, LevelMaps m,
+                                                  const Real* __restrict__ f, Real* __restrict__ lab,
+                                                  int* __restrict__ bad) {
+    const int j = blockIdx.x;
+    if (j >= nlist) return;
+    const int s = slots ? slots[j] : j;
+    const int4 b = blk[s];
+    const int lab3 = LAB * LAB * LAB;
+    int my_bad = 0;
+    for (int p = threadIdx.x; p < lab3; p += blockDim.x) {
+        const int ix = p % LAB + LS, iy = (p / LAB) % LAB + LS, iz = p / (LAB * LAB) + LS;
+        const int code = dir_code(ix, iy, iz);
+        const int src = code == 13 ? s : nbr27[27 * s + code];
+        Real v;
+        if (src >= 0) {
+            const int cx = (ix + 8) & 7, cy = (iy + 8) & 7, cz = (iz + 8) & 7;
+            v = f[(size_t)src * BCELLS + cx + 8 * cy + 64 * cz];
+        } else {
+            v = (Real)0;
+            ghost_acc<2, Real>(m, order, b.x, 8LL * b.y + ix, 8LL * b.z + iy, 8LL * b.w + iz, 1.0, f, v, my_bad);
+        }
+        lab[(size_t)j * lab3 + p] = v;
+    }
+    if (my_bad) atomicOr(bad, 1);
 }
 
 // BlockFWT::_fwt / _filter (MRAGBlockFWT.h:219-298): 64 coarse points x 8
@@ -394,20 +490,12 @@ __global__ __launch_bounds__(512) void uq_resample_kernel(int nslots, const Bloc
 // host side
 // ---------------------------------------------------------------------------
 
-// sort terms into the canonical accumulation order: ascending source cell,
-// then weight
-void canonical(std::vector<GhostTerm>& v) {
-    std::sort(v.begin(), v.end(), [](const GhostTerm& a, const GhostTerm& b) {
-        return a.src != b.src ? a.src < b.src : a.w < b.w;
-    });
-}
-
+// a table of ghost entries, each with its terms in recursion order
 struct HostTable {
     std::vector<GhostEntry> ent;
     std::vector<int32_t> src;
     std::vector<double> w;
-    void add(int32_t dst, std::vector<GhostTerm>& terms) {
-        canonical(terms);
+    void add(int32_t dst, const std::vector<GhostTerm>& terms) {
         ent.push_back(GhostEntry{dst, (int32_t)src.size(), (int32_t)terms.size()});
         for (auto& t : terms) {
             src.push_back(t.src);
@@ -560,6 +648,23 @@ int apply_topology(Ctx& c) {
     int rc;
     if ((rc = c.upload_topology())) return rc;
     if ((rc = c.dNbr27.ensure(n27.size() * 4)) || (rc = c.dLevel.ensure(lev.size() * 4))) return rc;
+    // level maps and block coordinates for the device-side lab ghosts
+    std::vector<int32_t> dense;
+    c.nDense = t.dense_levels;
+    for (int l = 0; l < c.nDense; ++l) {
+        c.denseOff[l] = (int)dense.size();
+        dense.insert(dense.end(), t.dense[l].begin(), t.dense[l].end());
+    }
+    std::vector<int4> blk(nb);
+    for (int s = 0; s < nb; ++s)
+        blk[s] = make_int4(t.leaves[s].level, t.leaves[s].index[0], t.leaves[s].index[1], t.leaves[s].index[2]);
+    if ((rc = c.dDense.ensure(std::max<size_t>(1, dense.size()) * 4)) ||
+        (rc = c.dBlk.ensure(std::max<size_t>(1, blk.size()) * sizeof(int4))) || (rc = c.dBad.ensure(sizeof(int))))
+        return rc;
+    if (!dense.empty())
+        GH_HIP_CHECK(hipMemcpyAsync(c.dDense.p, dense.data(), dense.size() * 4, hipMemcpyHostToDevice, c.stream));
+    if (nb > 0)
+        GH_HIP_CHECK(hipMemcpyAsync(c.dBlk.p, blk.data(), blk.size() * sizeof(int4), hipMemcpyHostToDevice, c.stream));
     GH_HIP_CHECK(hipMemcpyAsync(c.dNbr27.p, n27.data(), n27.size() * 4, hipMemcpyHostToDevice, c.stream));
     GH_HIP_CHECK(hipMemcpyAsync(c.dLevel.p, lev.data(), lev.size() * 4, hipMemcpyHostToDevice, c.stream));
     HostTable ft = build_face_table(t, c.topo.nbr);
@@ -650,10 +755,45 @@ int build_labs_for(Ctx& c, const std::vector<int>& slots, const Real* field, con
                             out);
 }
 
-// Details of the listed slots.  The labs of every leaf are built from a ghost
-// table cached per tree version (an adaptation epoch that changes nothing
-// rebuilds nothing on the host); a lab does not depend on which other labs
-// are built, so the details equal those of labs built for the list alone.
+// Labs of nlist slots (dslots on the device, or nullptr: slots 0..nlist-1)
+// with the ghosts evaluated on the device; clears and then may set c.dBad
+// (a ghost deeper than the kernel's recursion: use host tables instead).
+template <typename Real>
+int device_labs(Ctx& c, int nlist, const int32_t* dslots, const Real* field, DevBuf& out) {
+    const int order = c.tree.order;
+    const int LS = lab_start(order), LAB = lab_size(order), lab3 = LAB * LAB * LAB;
+    int rc;
+    if ((rc = out.ensure(std::max<size_t>(1, (size_t)nlist) * lab3 * sizeof(Real)))) return rc;
+    if (nlist == 0) return GH_OK;
+    LevelMaps m{};
+    m.map = c.dDense.as<int32_t>();
+    for (int l = 0; l <= Octree::MAXL; ++l) m.off[l] = c.denseOff[l];
+    m.nlev = c.nDense;
+    hipLaunchKernelGGL(lab_kernel<Real>, dim3((unsigned)nlist), dim3(256), 0, c.stream, nlist, dslots, LS, LAB, order,
+                       c.dNbr27.as<int32_t>(), c.dBlk.as<int4>(), m, field, out.as<Real>(), c.dBad.as<int>());
+    GH_HIP_CHECK(hipGetLastError());
+    return GH_OK;
+}
+
+// synchronizes; true if no lab since the last clear needed host tables
+bool device_labs_ok(Ctx& c) {
+    int bad = 1;
+    if (hipMemcpyAsync(&bad, c.dBad.p, sizeof(int), hipMemcpyDeviceToHost, c.stream) != hipSuccess ||
+        hipStreamSynchronize(c.stream) != hipSuccess)
+        return false;
+    return bad == 0;
+}
+
+int clear_bad(Ctx& c) {
+    GH_HIP_CHECK(hipMemsetAsync(c.dBad.p, 0, sizeof(int), c.stream));
+    return GH_OK;
+}
+
+// Details of the listed slots, from the labs of every leaf: evaluated on the
+// device when per-level maps exist, else built from a ghost table cached per
+// tree version (an adaptation epoch that changes nothing rebuilds nothing on
+// the host); a lab does not depend on which other labs are built, so the
+// details equal those of labs built for the list alone.
 template <typename Real>
 int fwt_details_t(Ctx& c, const std::vector<int>& slots, std::vector<double>& det) {
     det.assign(slots.size(), 0.0);
@@ -661,8 +801,14 @@ int fwt_details_t(Ctx& c, const std::vector<int>& slots, std::vector<double>& de
     const int order = c.tree.order;
     const int n = c.topo.nblocks();
     int rc;
+    bool on_device = false;
+    if (c.nDense > 0) {
+        if ((rc = clear_bad(c)) || (rc = device_labs<Real>(c, n, nullptr, c.phi[c.cur].as<Real>(), c.labBuf)))
+            return rc;
+        on_device = device_labs_ok(c);
+    }
     std::vector<int> all;  // source of an async upload: lives until the sync below
-    if (c.labAllVersion != c.tree_version) {
+    if (!on_device && c.labAllVersion != c.tree_version) {
         all.resize(n);
         for (int i = 0; i < n; ++i) all[i] = i;
         HostTable tab = build_lab_table(c.tree, all, c.topo.nbr27, order);
@@ -673,7 +819,8 @@ int fwt_details_t(Ctx& c, const std::vector<int>& slots, std::vector<double>& de
         c.nLabAllEnt = (int)tab.ent.size();
         c.labAllVersion = c.tree_version;
     }
-    if ((rc = build_labs<Real>(c, n, c.labAllSlots, c.phi[c.cur].as<Real>(), c.nLabAllEnt, c.labAllEnt, c.labAllSrc,
+    if (!on_device &&
+        (rc = build_labs<Real>(c, n, c.labAllSlots, c.phi[c.cur].as<Real>(), c.nLabAllEnt, c.labAllEnt, c.labAllSrc,
                                c.labAllW, c.labBuf)))
         return rc;
     DevBuf d;
@@ -701,10 +848,23 @@ int apply_new_tree(Ctx& c, const Octree& nt, const std::vector<Remap>& rm, const
     DevBuf* fields[4] = {&c.phi[c.cur], &c.pw, &c.pg, &c.pcsf};
     DevBuf labs[4];
     if (!split_parents.empty()) {
-        HostTable tab = build_lab_table(c.tree, split_parents, c.topo.nbr27, order);
-        if ((rc = upload_table<Real>(c, tab, c.labEnt, c.labSrc, c.labW))) return rc;
-        for (int f = 0; f < 4; ++f)
-            if ((rc = build_labs_for<Real>(c, split_parents, fields[f]->as<Real>(), tab, labs[f]))) return rc;
+        bool on_device = false;
+        if (c.nDense > 0) {
+            if ((rc = c.labSlots.ensure(split_parents.size() * 4)) || (rc = clear_bad(c))) return rc;
+            GH_HIP_CHECK(hipMemcpyAsync(c.labSlots.p, split_parents.data(), split_parents.size() * 4,
+                                        hipMemcpyHostToDevice, c.stream));
+            for (int f = 0; f < 4; ++f)
+                if ((rc = device_labs<Real>(c, (int)split_parents.size(), c.labSlots.as<int32_t>(),
+                                            fields[f]->as<Real>(), labs[f])))
+                    return rc;
+            on_device = device_labs_ok(c);
+        }
+        if (!on_device) {
+            HostTable tab = build_lab_table(c.tree, split_parents, c.topo.nbr27, order);
+            if ((rc = upload_table<Real>(c, tab, c.labEnt, c.labSrc, c.labW))) return rc;
+            for (int f = 0; f < 4; ++f)
+                if ((rc = build_labs_for<Real>(c, split_parents, fields[f]->as<Real>(), tab, labs[f]))) return rc;
+        }
     }
     const int nnew = (int)nt.leaves.size();
     DevBuf drm;

@@ -44,9 +44,15 @@ struct Ctx {
     // FWT / split labs (compact, one per listed slot)
     DevBuf labBuf, labSlots, labEnt, labSrc, labW;
     // ghost table of the labs of every leaf, valid while tree_version == labAllVersion
+    // (host-table path, used when the level maps below are not available)
     DevBuf labAllEnt, labAllSrc, labAllW, labAllSlots;
     int nLabAllEnt = 0;
     uint64_t tree_version = 0, labAllVersion = ~(uint64_t)0;
+    // per-level block maps for device-side ghost evaluation (Octree::dense,
+    // concatenated; nDense = 0: not available), block (level, x, y, z) per slot
+    DevBuf dDense, dBlk, dBad;
+    int denseOff[Octree::MAXL + 1] = {0};
+    int nDense = 0;
     double last_fwt_ms = 0;
 
     DevBuf phi[2];      // ping-pong tumor density

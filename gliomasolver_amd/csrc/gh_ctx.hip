@@ -58,7 +58,7 @@ int Ctx::alloc_fields() {
 void Ctx::release_all() {
     DevBuf* bufs[] = {&dLevel,   &dNbr27,  &faceEnt,    &faceSrc,   &faceW,     &fgPhi,       &fgTau,   &fgDm,
                       &fgPw,     &fgPg,    &labBuf,     &labSlots,  &labEnt,    &labSrc,      &labW,    &labAllEnt,
-                      &labAllSrc, &labAllW, &labAllSlots, &phi[0],  &phi[1],    &dphidt,      &pw,      &pg,
+                      &labAllSrc, &labAllW, &labAllSlots, &dDense, &dBlk, &dBad, &phi[0],  &phi[1],    &dphidt,      &pw,      &pg,
                       &pcsf,     &statD,   &statSig,    &statF,     &flag,      &nbr,         &obsT1,   &obsFl,
                       &obsPet,   &obsRoi,  &likPart,    &adjCk,     &adjSeg,    &adjLam[0],   &adjLam[1], &adjPart};
     for (DevBuf* b : bufs) b->release();

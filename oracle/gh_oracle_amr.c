@@ -11,7 +11,8 @@
  * a lattice point at level L covered by a leaf of level L is that cell; a
  * point inside a finer region is the coincident fine vertex (Ha = {1}); a
  * point inside a coarser region is Hs-synthesised from level L-1.  The value
- * is accumulated in Real over its terms sorted by (source cell, weight).
+ * is accumulated in Real, product by product, over its terms in recursion
+ * order (synthesis taps z-major, then y, then x).
  */
 #include "gh_oracle.h"
 #include <math.h>
@@ -96,12 +97,6 @@ static void push(terms_t* T, int64_t src, double w) {
     T->n++;
 }
 
-static int cmp_term(const void* a, const void* b) {
-    const term_t *x = (const term_t*)a, *y = (const term_t*)b;
-    if (x->src != y->src) return x->src < y->src ? -1 : 1;
-    return x->w < y->w ? -1 : (x->w > y->w);
-}
-
 /* value of lattice point q at level L as terms (source = rank*512 + cell) */
 static void collect(const tree_t* t, int order, int L, int64_t qx, int64_t qy, int64_t qz, double w, terms_t* T) {
     const int64_t n = (int64_t)8 << L;
@@ -164,7 +159,6 @@ static int lab_n(int order) { return order == 4 ? 13 : 10; }
             }                                                                                        \
             T->n = 0;                                                                                \
             collect(t, order, L, 8LL * B[0] + i[0], 8LL * B[1] + i[1], 8LL * B[2] + i[2], 1.0, T);   \
-            qsort(T->v, T->n, sizeof(term_t), cmp_term);                                             \
             REAL acc = (REAL)0;                                                                      \
             for (int k = 0; k < T->n; ++k) {                                                         \
                 REAL prod = field[T->v[k].src] * (REAL)T->v[k].w;                                    \

@@ -28,6 +28,26 @@ def bits_equal(a, b):
     return a.dtype == b.dtype and a.shape == b.shape and np.array_equal(a.view(np.uint8), b.view(np.uint8))
 
 
+def deep_grid(levels=8):
+    """Refinements around one point down to level 8, 2:1-balanced: deeper than
+    the device-side per-level maps (levels <= 7), so labs come from host-built
+    ghost tables."""
+    from amr_util import apply_refine, neighbours
+    rows = canonical([[1, x, y, z] for z in range(2) for y in range(2) for x in range(2)])
+    p = (0.3, 0.45, 0.6)
+    for L in range(1, levels):
+        i = [k for k in range(len(rows)) if rows[k, 0] == L and
+             all(int(p[a] * (1 << L)) == rows[k, 1 + a] for a in range(3))]
+        rows = apply_refine(rows, i)
+        while True:
+            nb = neighbours(rows)
+            bad = sorted({j for s in range(len(rows)) for j, _ in nb[s] if rows[s, 0] - rows[j, 0] > 1})
+            if not bad:
+                break
+            rows = apply_refine(rows, bad)
+    return rows
+
+
 def grids():
     return {"central": central_two_level_grid(),
             "random": balanced_random_grid(np.random.default_rng(11), base_level=1, max_level=3, nrefine=4)}
@@ -71,6 +91,18 @@ def test_gpu_fwt_details_bit_exact(oracle, real, order, grid):
     fields = random_fields(rows, t, 1)
     say("fwt", grid, real, order, len(rows), "blocks")
     ctx = make_ctx(real, rows, order, fields)
+    det = ctx.fwt_details()
+    ctx.close()
+    ref = np.array([oracle.amr_fwt(order, oracle.amr_lab(rows, order, fields[FIELD_PHI], b)) for b in range(len(rows))])
+    assert np.array_equal(det, ref), "max |diff| %g" % np.abs(det - ref).max()
+
+
+@pytest.mark.parametrize("order", [4, 2])
+def test_gpu_fwt_details_host_tables_on_deep_grid(oracle, order):
+    rows = deep_grid()
+    fields = random_fields(rows, np.float32, 3)
+    say("fwt deep", order, len(rows), "blocks, levels", rows[:, 0].min(), "-", rows[:, 0].max())
+    ctx = make_ctx("f32", rows, order, fields, max_level=9)
     det = ctx.fwt_details()
     ctx.close()
     ref = np.array([oracle.amr_fwt(order, oracle.amr_lab(rows, order, fields[FIELD_PHI], b)) for b in range(len(rows))])
