@@ -119,6 +119,14 @@ struct LevelMaps {
     int nlev;
 };
 
+inline LevelMaps level_maps(const Ctx& c) {
+    LevelMaps m{};
+    m.map = c.dDense.as<int32_t>();
+    for (int l = 0; l <= Octree::MAXL; ++l) m.off[l] = c.denseOff[l];
+    m.nlev = c.nDense;
+    return m;
+}
+
 // Hs synthesis taps of fine point q (MRAGWavelets_StaticData.cpp:13-21,
 // 31-39): even q -> coarse q/2 with weight 1; odd q -> (q-1)/2 + k, k < 2,
 // weight 1/2 (order 2) or (q-3)/2 + k, k < 4, weights -1/16, 9/16, 9/16, -1/16
@@ -130,12 +138,13 @@ __device__ inline double synth_w(int order, int64_t q, int k) {
     return (q & 1) == 0 ? 1.0 : (order == 2 ? 0.5 : ((k == 0 || k == 3) ? -1.0 / 16 : 9.0 / 16));
 }
 
-// acc += value of lattice point q at level L times w (Octree::value_terms,
-// term by term in the same order).  D more synthesis levels are allowed
-// below this one; deeper recursion sets bad.
-template <int D, typename Real>
-__device__ inline void ghost_acc(const LevelMaps& m, int order, int L, int64_t qx, int64_t qy, int64_t qz, double w,
-                                 const Real* __restrict__ f, Real& acc, int& bad) {
+// Calls visit(src, w) for every term of the value of lattice point q at
+// level L times w, in the order of Octree::value_terms (src = slot * 512 +
+// cell).  D more synthesis levels are allowed below this one; deeper
+// recursion sets bad.
+template <int D, typename Visit>
+__device__ inline void ghost_visit(const LevelMaps& m, int order, int L, int64_t qx, int64_t qy, int64_t qz, double w,
+                                   Visit& visit, int& bad) {
     for (;;) {
         const int64_t mask = ((int64_t)BS << L) - 1;  // periodic wrap, power-of-two lattice
         qx &= mask;
@@ -145,8 +154,7 @@ __device__ inline void ghost_acc(const LevelMaps& m, int order, int L, int64_t q
         const int bx = (int)(qx >> 3), by = (int)(qy >> 3), bz = (int)(qz >> 3);
         const int32_t v = m.map[m.off[L] + bx + ((int64_t)by << L) + ((int64_t)bz << (2 * L))];
         if (v >= 0) {
-            const Real prod = f[(size_t)v * BCELLS + (qx & 7) + 8 * (qy & 7) + 64 * (qz & 7)] * (Real)w;
-            acc = acc + prod;
+            visit((int32_t)(v * BCELLS + (qx & 7) + 8 * (qy & 7) + 64 * (qz & 7)), w);
             return;
         }
         if (v != -2) break;  // coarser
@@ -165,10 +173,64 @@ __device__ inline void ghost_acc(const LevelMaps& m, int order, int L, int64_t q
             for (int ky = 0; ky < ny; ++ky)
 #pragma unroll 1
                 for (int kx = 0; kx < nx; ++kx)
-                    ghost_acc<D - 1, Real>(
-                        m, order, L - 1, synth_src(order, qx, kx), synth_src(order, qy, ky), synth_src(order, qz, kz),
-                        w * (synth_w(order, qz, kz) * synth_w(order, qy, ky) * synth_w(order, qx, kx)), f, acc, bad);
+                    ghost_visit<D - 1>(m, order, L - 1, synth_src(order, qx, kx), synth_src(order, qy, ky),
+                                       synth_src(order, qz, kz),
+                                       w * (synth_w(order, qz, kz) * synth_w(order, qy, ky) * synth_w(order, qx, kx)),
+                                       visit, bad);
     }
+}
+
+// the ghost of face point (u, v) of face f of block b (level, x, y, z)
+__device__ inline void face_point(int f, int u, int v, int i[3]) {
+    switch (f) {
+        case FXM: i[0] = -1; i[1] = u; i[2] = v; break;
+        case FXP: i[0] = 8; i[1] = u; i[2] = v; break;
+        case FYM: i[0] = u; i[1] = -1; i[2] = v; break;
+        case FYP: i[0] = u; i[1] = 8; i[2] = v; break;
+        case FZM: i[0] = u; i[1] = v; i[2] = -1; break;
+        default:  i[0] = u; i[1] = v; i[2] = 8; break;
+    }
+}
+
+// Face-ghost table on the device, pass 1: term count of ghost g = 64 * k + p
+// of jump face k (faces[k] = 6 * slot + f), p = v * 8 + u.
+__global__ __launch_bounds__(256) void face_count_kernel(int n, const int32_t* __restrict__ faces,
+                                                         const int4* __restrict__ blk, LevelMaps m, int order,
+                                                         int32_t* __restrict__ count, int* __restrict__ bad) {
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= n) return;
+    const int sf = faces[g >> 6], s = sf / 6, f = sf % 6, p = g & 63;
+    const int4 b = blk[s];
+    int i[3];
+    face_point(f, p & 7, p >> 3, i);
+    int cnt = 0, my_bad = 0;
+    auto visit = [&](int32_t, double) { ++cnt; };
+    ghost_visit<2>(m, order, b.x, 8LL * b.y + i[0], 8LL * b.z + i[1], 8LL * b.w + i[2], 1.0, visit, my_bad);
+    count[g] = cnt;
+    if (my_bad) atomicOr(bad, 1);
+}
+
+// pass 2: entries {dst = slot * 384 + f * 64 + p, start, count} and terms
+template <typename Real>
+__global__ __launch_bounds__(256) void face_fill_kernel(int n, const int32_t* __restrict__ faces,
+                                                        const int4* __restrict__ blk, LevelMaps m, int order,
+                                                        const int32_t* __restrict__ start, GhostEntry* __restrict__ ent,
+                                                        int32_t* __restrict__ tsrc, Real* __restrict__ tw) {
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= n) return;
+    const int sf = faces[g >> 6], s = sf / 6, f = sf % 6, p = g & 63;
+    const int4 b = blk[s];
+    int i[3];
+    face_point(f, p & 7, p >> 3, i);
+    const int k0 = start[g];
+    int k = k0, my_bad = 0;
+    auto visit = [&](int32_t src, double w) {
+        tsrc[k] = src;
+        tw[k] = (Real)w;
+        ++k;
+    };
+    ghost_visit<2>(m, order, b.x, 8LL * b.y + i[0], 8LL * b.z + i[1], 8LL * b.w + i[2], 1.0, visit, my_bad);
+    ent[g] = GhostEntry{s * 384 + f * 64 + p, k0, k - k0};
 }
 
 // Full labs of the listed slots (slots == nullptr: every slot j), ghosts
@@ -195,8 +257,13 @@ __global__ __launch_bounds__(256) void lab_kernel(int nlist, const int32_t* __re
             const int cx = (ix + 8) & 7, cy = (iy + 8) & 7, cz = (iz + 8) & 7;
             v = f[(size_t)src * BCELLS + cx + 8 * cy + 64 * cz];
         } else {
-            v = (Real)0;
-            ghost_acc<2, Real>(m, order, b.x, 8LL * b.y + ix, 8LL * b.z + iy, 8LL * b.w + iz, 1.0, f, v, my_bad);
+            Real acc = (Real)0;
+            auto visit = [&](int32_t src, double w) {
+                const Real prod = f[src] * (Real)w;
+                acc = acc + prod;
+            };
+            ghost_visit<2>(m, order, b.x, 8LL * b.y + ix, 8LL * b.z + iy, 8LL * b.w + iz, 1.0, visit, my_bad);
+            v = acc;
         }
         lab[(size_t)j * lab3 + p] = v;
     }
@@ -622,6 +689,73 @@ HostTable build_lab_table(const Octree& t, const std::vector<int>& slots, const 
     });
 }
 
+// The face-ghost table of the RD step built on the device (same entries and
+// term order as build_face_table): a counting pass, a host prefix sum, a
+// filling pass.  *done = false when a ghost is deeper than the device
+// recursion (the caller then builds it on the host).
+int device_face_table(Ctx& c, bool& done) {
+    done = false;
+    std::vector<int32_t> faces;
+    const int nb = c.topo.nblocks();
+    for (int s = 0; s < nb; ++s)
+        for (int f = 0; f < 6; ++f)
+            if (c.topo.nbr[6 * (size_t)s + f] < 0) faces.push_back(6 * s + f);
+    const int n = (int)faces.size() * 64;
+    c.nFaceEnt = n;
+    if (n == 0) {
+        done = true;
+        return GH_OK;
+    }
+    const LevelMaps m = level_maps(c);
+    const int order = c.tree.order;
+    DevBuf dfaces, dcount;
+    int rc;
+    if ((rc = dfaces.ensure(faces.size() * 4)) || (rc = dcount.ensure((size_t)n * 4))) return rc;
+    GH_HIP_CHECK(hipMemcpyAsync(dfaces.p, faces.data(), faces.size() * 4, hipMemcpyHostToDevice, c.stream));
+    GH_HIP_CHECK(hipMemsetAsync(c.dBad.p, 0, sizeof(int), c.stream));
+    const unsigned grid = (unsigned)((n + 255) / 256);
+    hipLaunchKernelGGL(face_count_kernel, dim3(grid), dim3(256), 0, c.stream, n, dfaces.as<int32_t>(),
+                       c.dBlk.as<int4>(), m, order, dcount.as<int32_t>(), c.dBad.as<int>());
+    GH_HIP_CHECK(hipGetLastError());
+    std::vector<int32_t> cnt(n);
+    int bad = 1;
+    GH_HIP_CHECK(hipMemcpyAsync(cnt.data(), dcount.p, (size_t)n * 4, hipMemcpyDeviceToHost, c.stream));
+    GH_HIP_CHECK(hipMemcpyAsync(&bad, c.dBad.p, sizeof(int), hipMemcpyDeviceToHost, c.stream));
+    GH_HIP_CHECK(hipStreamSynchronize(c.stream));
+    if (bad) {
+        dfaces.release();
+        dcount.release();
+        return GH_OK;
+    }
+    int64_t total = 0;
+    for (int g = 0; g < n; ++g) {  // exclusive prefix sum, in place
+        const int32_t k = cnt[g];
+        cnt[g] = (int32_t)total;
+        total += k;
+    }
+    if (total > INT32_MAX) return set_error(GH_EINVAL, "face ghost table: too many terms");
+    const size_t rs = c.real_size;
+    if ((rc = c.faceEnt.ensure((size_t)n * sizeof(GhostEntry))) ||
+        (rc = c.faceSrc.ensure(std::max<int64_t>(1, total) * 4)) ||
+        (rc = c.faceW.ensure(std::max<int64_t>(1, total) * rs)))
+        return rc;
+    GH_HIP_CHECK(hipMemcpyAsync(dcount.p, cnt.data(), (size_t)n * 4, hipMemcpyHostToDevice, c.stream));
+    if (c.real_kind == GH_REAL_F32)
+        hipLaunchKernelGGL(face_fill_kernel<float>, dim3(grid), dim3(256), 0, c.stream, n, dfaces.as<int32_t>(),
+                           c.dBlk.as<int4>(), m, order, dcount.as<int32_t>(), c.faceEnt.as<GhostEntry>(),
+                           c.faceSrc.as<int32_t>(), c.faceW.as<float>());
+    else
+        hipLaunchKernelGGL(face_fill_kernel<double>, dim3(grid), dim3(256), 0, c.stream, n, dfaces.as<int32_t>(),
+                           c.dBlk.as<int4>(), m, order, dcount.as<int32_t>(), c.faceEnt.as<GhostEntry>(),
+                           c.faceSrc.as<int32_t>(), c.faceW.as<double>());
+    GH_HIP_CHECK(hipGetLastError());
+    GH_HIP_CHECK(hipStreamSynchronize(c.stream));  // cnt and faces are read by the copies above
+    dfaces.release();
+    dcount.release();
+    done = true;
+    return GH_OK;
+}
+
 }  // namespace
 
 // topology derived from the octree: slots, same-level neighbours, levels,
@@ -667,11 +801,15 @@ int apply_topology(Ctx& c) {
         GH_HIP_CHECK(hipMemcpyAsync(c.dBlk.p, blk.data(), blk.size() * sizeof(int4), hipMemcpyHostToDevice, c.stream));
     GH_HIP_CHECK(hipMemcpyAsync(c.dNbr27.p, n27.data(), n27.size() * 4, hipMemcpyHostToDevice, c.stream));
     GH_HIP_CHECK(hipMemcpyAsync(c.dLevel.p, lev.data(), lev.size() * 4, hipMemcpyHostToDevice, c.stream));
-    HostTable ft = build_face_table(t, c.topo.nbr);
-    c.nFaceEnt = (int)ft.ent.size();
-    rc = c.real_kind == GH_REAL_F32 ? upload_table<float>(c, ft, c.faceEnt, c.faceSrc, c.faceW)
-                                    : upload_table<double>(c, ft, c.faceEnt, c.faceSrc, c.faceW);
-    if (rc) return rc;
+    bool on_device = false;
+    if (c.nDense > 0 && (rc = device_face_table(c, on_device))) return rc;
+    if (!on_device) {
+        HostTable ft = build_face_table(t, c.topo.nbr);
+        c.nFaceEnt = (int)ft.ent.size();
+        rc = c.real_kind == GH_REAL_F32 ? upload_table<float>(c, ft, c.faceEnt, c.faceSrc, c.faceW)
+                                        : upload_table<double>(c, ft, c.faceEnt, c.faceSrc, c.faceW);
+        if (rc) return rc;
+    }
     const size_t rs = c.real_size;
     if ((rc = c.fgPhi.ensure(std::max(1, nb) * 384 * rs)) || (rc = c.fgTau.ensure(std::max(1, nb) * 384 * rs)) ||
         (rc = c.fgDm.ensure(std::max(1, nb) * 192 * rs)) || (rc = c.fgPw.ensure(std::max(1, nb) * 384 * rs)) ||
@@ -765,10 +903,7 @@ int device_labs(Ctx& c, int nlist, const int32_t* dslots, const Real* field, Dev
     int rc;
     if ((rc = out.ensure(std::max<size_t>(1, (size_t)nlist) * lab3 * sizeof(Real)))) return rc;
     if (nlist == 0) return GH_OK;
-    LevelMaps m{};
-    m.map = c.dDense.as<int32_t>();
-    for (int l = 0; l <= Octree::MAXL; ++l) m.off[l] = c.denseOff[l];
-    m.nlev = c.nDense;
+    const LevelMaps m = level_maps(c);
     hipLaunchKernelGGL(lab_kernel<Real>, dim3((unsigned)nlist), dim3(256), 0, c.stream, nlist, dslots, LS, LAB, order,
                        c.dNbr27.as<int32_t>(), c.dBlk.as<int4>(), m, field, out.as<Real>(), c.dBad.as<int>());
     GH_HIP_CHECK(hipGetLastError());
