@@ -239,6 +239,7 @@ def adaptive_leg(tend, epoch_steps, max_epochs):
     ctx.sync()
     t_init = time.perf_counter() - t0
     log("adaptive: %d blocks after the initial compression (%.2f s)" % (ctx.nblocks(), t_init))
+    ctx.adapt_profile(reset=True)
     leaf_updates, steps_done, step_s, epoch_s, blocks = 0, 0, 0.0, 0.0, []
     epochs = min(max_epochs, -(-nsteps // epoch_steps))
     t0 = time.perf_counter()
@@ -262,6 +263,8 @@ def adaptive_leg(tend, epoch_steps, max_epochs):
             log("adaptive: epoch %d/%d, %d blocks, steps %.2f s, epochs %.2f s" % (e + 1, epochs, ctx.nblocks(),
                                                                                    step_s, epoch_s))
     wall = time.perf_counter() - t0
+    prof = {k: round(v, 1) for k, v in ctx.adapt_profile().items()}
+    log("adaptive: epoch profile (ms)", json.dumps(prof))
     ctx.close()
     return {"metric": "adaptive RD leaf cell-updates/s", "value": leaf_updates / wall, "unit": "cell-updates/s",
             "config": {"workload": "wavelet-adaptive levels 3-5 (256^3 finest), order-4 interpolating wavelets, "
@@ -269,7 +272,8 @@ def adaptive_leg(tend, epoch_steps, max_epochs):
                        "steps": steps_done, "of_total_steps": nsteps, "dt_days": dt, "epochs": epochs,
                        "blocks_min_mean_max": [min(blocks), float(np.mean(blocks)), max(blocks)]},
             "step_only_cell_updates_per_s": leaf_updates / step_s, "step_s": step_s, "epoch_s": epoch_s,
-            "epoch_ms_mean": 1e3 * epoch_s / max(1, epochs), "initial_compress_s": t_init, "wall_s": wall}
+            "epoch_ms_mean": 1e3 * epoch_s / max(1, epochs), "initial_compress_s": t_init, "wall_s": wall,
+            "epoch_profile_ms": prof}
 
 
 def main():

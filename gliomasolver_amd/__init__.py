@@ -182,6 +182,15 @@ class Context:
         check(self.lib.gh_compress(self.h, tol, max_loops, C.byref(n)))
         return n.value
 
+    ADAPT_PHASES = ("ref_fwt", "ref", "comp_fwt", "comp", "plan", "move", "topology", "statics")
+
+    def adapt_profile(self, reset: bool = False) -> dict:
+        """Host wall time (ms) of gh_refine / gh_compress per phase since the
+        last reset (MRAG Profiler agents, then plan/move/topology/statics)."""
+        ms = np.zeros(8)
+        check(self.lib.gh_adapt_profile(self.h, _ptr(ms), int(reset)))
+        return dict(zip(self.ADAPT_PHASES, ms.tolist()))
+
     def set_field_blocks(self, field: int, values: np.ndarray):
         v = np.ascontiguousarray(values, self.dtype).reshape(self.nblocks(), 8, 8, 8)
         check(self.lib.gh_set_field_blocks(self.h, field, _ptr(v)))

@@ -41,6 +41,7 @@ SIGNATURES = {
     "gh_fwt_details": (_i, [_vp, _vp]),
     "gh_refine": (_i, [_vp, _d, _i, _pi]),
     "gh_compress": (_i, [_vp, _d, _i, _pi]),
+    "gh_adapt_profile": (_i, [_vp, _vp, _i]),
     "gh_set_field_blocks": (_i, [_vp, _i, _vp]),
     "gh_get_field_blocks": (_i, [_vp, _i, _vp]),
     "gh_uq_resample": (_i, [_vp, _i, _vp]),

@@ -23,6 +23,7 @@
 // multi-level grid use a host-built table (applied every step).
 #include <algorithm>
 #include <array>
+#include <chrono>
 #include <cmath>
 #include <exception>
 #include <thread>
@@ -37,6 +38,15 @@
 
 namespace gh {
 namespace {
+
+// adds the wall time of its scope to one profile slot
+struct ProfScope {
+    double& slot;
+    std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
+    explicit ProfScope(double& s) : slot(s) {}
+    ~ProfScope() { slot += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count(); }
+};
+enum ProfSlot { P_REF_FWT = 0, P_REF = 1, P_COMP_FWT = 2, P_COMP = 3, P_PLAN = 4, P_MOVE = 5, P_TOPO = 6, P_STAT = 7 };
 
 // interpolating wavelet taps in Real (MRAGWavelets_StaticData.cpp:13-21, 31-39)
 template <typename Real>
@@ -901,7 +911,7 @@ int device_labs(Ctx& c, int nlist, const int32_t* dslots, const Real* field, Dev
     const int order = c.tree.order;
     const int LS = lab_start(order), LAB = lab_size(order), lab3 = LAB * LAB * LAB;
     int rc;
-    if ((rc = out.ensure(std::max<size_t>(1, (size_t)nlist) * lab3 * sizeof(Real)))) return rc;
+    if ((rc = out.ensure_grow(std::max<size_t>(1, (size_t)nlist) * lab3 * sizeof(Real)))) return rc;
     if (nlist == 0) return GH_OK;
     const LevelMaps m = level_maps(c);
     hipLaunchKernelGGL(lab_kernel<Real>, dim3((unsigned)nlist), dim3(256), 0, c.stream, nlist, dslots, LS, LAB, order,
@@ -958,15 +968,14 @@ int fwt_details_t(Ctx& c, const std::vector<int>& slots, std::vector<double>& de
         (rc = build_labs<Real>(c, n, c.labAllSlots, c.phi[c.cur].as<Real>(), c.nLabAllEnt, c.labAllEnt, c.labAllSrc,
                                c.labAllW, c.labBuf)))
         return rc;
-    DevBuf d;
-    if ((rc = d.ensure((size_t)n * sizeof(double)))) return rc;
+    DevBuf& d = c.detBuf;
+    if ((rc = d.ensure_grow((size_t)n * sizeof(double)))) return rc;
     hipLaunchKernelGGL(fwt_kernel<Real>, dim3((unsigned)n), dim3(512), 0, c.stream, n, lab_start(order),
                        lab_size(order), taps<Real>(order), c.labBuf.as<Real>(), d.as<double>());
     std::vector<double> all_det(n);
     hipError_t e = hipGetLastError();
     if (e == hipSuccess) e = hipMemcpyAsync(all_det.data(), d.p, all_det.size() * 8, hipMemcpyDeviceToHost, c.stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c.stream);
-    d.release();
     if (e != hipSuccess) return set_error(GH_EHIP, std::string("fwt: ") + hipGetErrorString(e));
     for (size_t i = 0; i < slots.size(); ++i) det[i] = all_det[slots[i]];
     return GH_OK;
@@ -981,11 +990,12 @@ int apply_new_tree(Ctx& c, const Octree& nt, const std::vector<Remap>& rm, const
     const int LS = lab_start(order), LAB = lab_size(order);
     int rc;
     DevBuf* fields[4] = {&c.phi[c.cur], &c.pw, &c.pg, &c.pcsf};
-    DevBuf labs[4];
+    DevBuf* labs = c.labSplit;
+    const auto t_move = std::chrono::steady_clock::now();
     if (!split_parents.empty()) {
         bool on_device = false;
         if (c.nDense > 0) {
-            if ((rc = c.labSlots.ensure(split_parents.size() * 4)) || (rc = clear_bad(c))) return rc;
+            if ((rc = c.labSlots.ensure_grow(split_parents.size() * 4)) || (rc = clear_bad(c))) return rc;
             GH_HIP_CHECK(hipMemcpyAsync(c.labSlots.p, split_parents.data(), split_parents.size() * 4,
                                         hipMemcpyHostToDevice, c.stream));
             for (int f = 0; f < 4; ++f)
@@ -1002,32 +1012,30 @@ int apply_new_tree(Ctx& c, const Octree& nt, const std::vector<Remap>& rm, const
         }
     }
     const int nnew = (int)nt.leaves.size();
-    DevBuf drm;
-    if ((rc = drm.ensure(rm.size() * sizeof(Remap)))) return rc;
+    DevBuf& drm = c.remapBuf;
+    if ((rc = drm.ensure_grow(rm.size() * sizeof(Remap)))) return rc;
     GH_HIP_CHECK(hipMemcpyAsync(drm.p, rm.data(), rm.size() * sizeof(Remap), hipMemcpyHostToDevice, c.stream));
-    DevBuf nf[4];
+    DevBuf* nf = c.spare;  // the new arrays go to the spares, which then swap with the live ones
     for (int f = 0; f < 4; ++f) {
-        if ((rc = nf[f].ensure((size_t)nnew * BCELLS * sizeof(Real)))) return rc;
+        if ((rc = nf[f].ensure_grow((size_t)nnew * BCELLS * sizeof(Real)))) return rc;
         hipLaunchKernelGGL(remap_kernel<Real>, dim3(nnew), dim3(512), 0, c.stream, nnew, drm.as<Remap>(),
                            fields[f]->as<Real>(), labs[f].as<Real>(), LS, LAB, taps<Real>(order), nf[f].as<Real>());
         GH_HIP_CHECK(hipGetLastError());
     }
-    GH_HIP_CHECK(hipStreamSynchronize(c.stream));
-    for (int f = 0; f < 4; ++f) {
-        fields[f]->release();
-        *fields[f] = nf[f];
-        nf[f].p = nullptr;
-        nf[f].bytes = 0;
-        labs[f].release();
+    GH_HIP_CHECK(hipStreamSynchronize(c.stream));  // rm is read by the copy above
+    for (int f = 0; f < 4; ++f) std::swap(*fields[f], nf[f]);
+    c.prof[P_MOVE] += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_move).count();
+    {
+        ProfScope topo(c.prof[P_TOPO]);
+        c.tree = nt;
+        if ((rc = apply_topology(c))) return rc;
     }
-    drm.release();
-    c.tree = nt;
-    if ((rc = apply_topology(c))) return rc;
     // the other state buffers follow the new size
     const size_t n = c.ncells() * c.real_size;
-    if ((rc = c.phi[1 - c.cur].ensure(n)) || (rc = c.dphidt.ensure(n)) || (rc = c.statD.ensure(4 * n)) ||
-        (rc = c.statSig.ensure(n)))
+    if ((rc = c.phi[1 - c.cur].ensure_grow(n)) || (rc = c.dphidt.ensure_grow(n)) || (rc = c.statD.ensure_grow(4 * n)) ||
+        (rc = c.statSig.ensure_grow(n)))
         return rc;
+    ProfScope stat(c.prof[P_STAT]);
     if (c.prepared) return rd_prepare(c, c.Dw, c.Dg, c.rho);
     return GH_OK;
 }
@@ -1052,6 +1060,7 @@ int fwt_details(Ctx& c, const std::vector<int>& slots, std::vector<double>& det)
 
 // Grid::refine with Refiner_SpaceExtension (MRAGrid.inl:247-355)
 int grid_refine(Ctx& c, const std::vector<char>& flagged, int* nrefined) {
+    const auto t_plan = std::chrono::steady_clock::now();
     const Octree& t = c.tree;
     std::vector<int> victims = t.refinement_victims(flagged);
     if (nrefined) *nrefined = (int)victims.size();
@@ -1096,12 +1105,14 @@ int grid_refine(Ctx& c, const std::vector<char>& flagged, int* nrefined) {
         }
         rm[s] = r;
     }
+    c.prof[P_PLAN] += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_plan).count();
     return c.real_kind == GH_REAL_F32 ? apply_new_tree<float>(c, nt, rm, victims)
                                       : apply_new_tree<double>(c, nt, rm, victims);
 }
 
 // Grid::compress with Compressor (MRAGrid.inl:383-486)
 int grid_compress(Ctx& c, const std::vector<char>& flagged, int* ncollapsed) {
+    const auto t_plan = std::chrono::steady_clock::now();
     const Octree& t = c.tree;
     std::vector<uint64_t> parents = t.collapse_parents(flagged);
     if (ncollapsed) *ncollapsed = (int)parents.size();
@@ -1144,6 +1155,7 @@ int grid_compress(Ctx& c, const std::vector<char>& flagged, int* ncollapsed) {
         }
         rm[s] = r;
     }
+    c.prof[P_PLAN] += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_plan).count();
     return c.real_kind == GH_REAL_F32 ? apply_new_tree<float>(c, nt, rm, {})
                                       : apply_new_tree<double>(c, nt, rm, {});
 }
@@ -1228,8 +1240,11 @@ int gh_refine(gh_ctx* ctx, double tol, int max_loops, int* nrefined) {
         }
         if (cand.empty()) break;
         std::vector<double> det;
-        int rc = fwt_details(*c, cand, det);
-        if (rc) return rc;
+        int rc;
+        {
+            ProfScope fwt(c->prof[P_REF_FWT]);
+            if ((rc = fwt_details(*c, cand, det))) return rc;
+        }
         std::vector<char> flagged(c->topo.nblocks(), 0);
         int nflag = 0;
         for (size_t i = 0; i < cand.size(); ++i) {
@@ -1242,7 +1257,10 @@ int gh_refine(gh_ctx* ctx, double tol, int max_loops, int* nrefined) {
         }
         if (nflag == 0) break;
         int nr = 0;
-        if ((rc = grid_refine(*c, flagged, &nr))) return rc;
+        {
+            ProfScope ref(c->prof[P_REF]);
+            if ((rc = grid_refine(*c, flagged, &nr))) return rc;
+        }
         total += nr;
         ++loops;
         if (max_loops > 0 && loops >= max_loops) break;
@@ -1263,8 +1281,11 @@ int gh_compress(gh_ctx* ctx, double tol, int max_loops, int* ncollapsed) {
         for (int s = 0; s < c->topo.nblocks(); ++s)
             if (!nice.count(Octree::key(c->tree.leaves[s]))) cand.push_back(s);
         std::vector<double> det;
-        int rc = fwt_details(*c, cand, det);
-        if (rc) return rc;
+        int rc;
+        {
+            ProfScope fwt(c->prof[P_COMP_FWT]);
+            if ((rc = fwt_details(*c, cand, det))) return rc;
+        }
         std::vector<char> flagged(c->topo.nblocks(), 0);
         int nflag = 0;
         for (size_t i = 0; i < cand.size(); ++i) {
@@ -1277,13 +1298,26 @@ int gh_compress(gh_ctx* ctx, double tol, int max_loops, int* ncollapsed) {
         }
         if (nflag == 0) break;
         int nc = 0;
-        if ((rc = grid_compress(*c, flagged, &nc))) return rc;
+        {
+            ProfScope comp(c->prof[P_COMP]);
+            if ((rc = grid_compress(*c, flagged, &nc))) return rc;
+        }
         if (nc == 0) break;
         total += nc;
         ++loops;
         if (max_loops > 0 && loops >= max_loops) break;
     }
     if (ncollapsed) *ncollapsed = total;
+    return GH_OK;
+}
+
+int gh_adapt_profile(gh_ctx* ctx, double* ms, int reset) {
+    Ctx* c = as_ctx(ctx);
+    if (!c) return set_error(GH_EINVAL, "null context");
+    if (ms)
+        for (int i = 0; i < 8; ++i) ms[i] = c->prof[i];
+    if (reset)
+        for (double& v : c->prof) v = 0;
     return GH_OK;
 }
 

@@ -11,6 +11,33 @@ namespace gh {
 
 __device__ inline int Q(int x, int y, int z) { return (z + 2) * 144 + (y + 2) * 12 + (x + 2); }  // [-2,10)^3
 
+// three workgroup sums at once (one LDS round trip, two barriers); red holds
+// 3 * (blockDim / 64) doubles; the sums are valid in thread 0, each in the
+// same fixed order as block_sum
+__device__ inline void block_sum3(double& a, double& b, double& c, double* red) {
+    for (int o = 32; o > 0; o >>= 1) {
+        a += __shfl_xor(a, o, 64);
+        b += __shfl_xor(b, o, 64);
+        c += __shfl_xor(c, o, 64);
+    }
+    const int w = threadIdx.x / 64, lane = threadIdx.x % 64, nw = (int)(blockDim.x / 64);
+    __syncthreads();
+    if (lane == 0) {
+        red[w] = a;
+        red[nw + w] = b;
+        red[2 * nw + w] = c;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        a = b = c = 0;
+        for (int k = 0; k < nw; ++k) {
+            a += red[k];
+            b += red[nw + k];
+            c += red[2 * nw + k];
+        }
+    }
+}
+
 __device__ inline double block_sum(double v, double* red) {
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
     const int w = threadIdx.x / 64, lane = threadIdx.x % 64;
@@ -55,7 +82,7 @@ __global__ __launch_bounds__(512) void adj_step_kernel(int nslots, int per_xcd,
     __shared__ unsigned char s_o[1728];
     __shared__ double s_w[1000];
     __shared__ Real s_Dx[576], s_Dy[576], s_Dz[576];
-    __shared__ double s_red[8];
+    __shared__ double s_red[24];
 
     const int t = threadIdx.x;
     const int x = t & 7, y = (t >> 3) & 7, z = t >> 6;
@@ -72,10 +99,16 @@ __global__ __launch_bounds__(512) void adj_step_kernel(int nslots, int per_xcd,
     s_Dx[z * 72 + y * 9 + x + 1] = sv.dx;
     s_Dy[z * 72 + (y + 1) * 8 + x] = sv.dy;
     s_Dz[(z + 1) * 64 + y * 8 + x] = sv.dz;
-    // face halos: depth 1 (state, flags, w, minus-face D) and depth 2 (state, flags)
-    for (int h = t; h < 768; h += 512) {
-        const int depth = h / 384, r = h % 384;
-        const int f = r >> 6, a = r & 7, b = (r >> 3) & 7;
+    // face halos: depth 1 (state, flags, w, minus-face D) and depth 2 (state,
+    // flags).  Halo wave hw = wave (first pass) or wave + 8 (second pass,
+    // waves 0-3) covers depth hw / 6, face hw % 6: wave-uniform (scalar) face
+    // logic, and every load of both passes is issued before the first store.
+    struct HaloCell {
+        size_t src;  // global cell
+        int q, p;    // tile indices (state/flags, w)
+    };
+    auto halo_cell = [&](int hw) {
+        const int depth = hw / 6, f = hw % 6, a = t & 7, b = (t >> 3) & 7;
         const int in1 = depth == 0 ? 7 : 6, in0 = depth == 0 ? 0 : 1;  // source plane in the neighbour
         const int lo = depth == 0 ? -1 : -2, hi = depth == 0 ? 8 : 9;  // tile plane
         int src, qx, qy, qz;
@@ -87,19 +120,39 @@ __global__ __launch_bounds__(512) void adj_step_kernel(int nslots, int per_xcd,
             case FZM: src = a + 8 * b + 64 * in1; qx = a; qy = b; qz = lo; break;
             default:  src = a + 8 * b + 64 * in0; qx = a; qy = b; qz = hi; break;
         }
-        const size_t nb = (size_t)face_nb(prm.pos(s), f) * BCELLS + src;
-        const Real uh = uk[nb];
-        const Stat4<Real> hs = st[nb];
-        s_u[Q(qx, qy, qz)] = uh;
-        s_o[Q(qx, qy, qz)] = uh < hs.tau;
-        if (depth == 0) {
-            const bool dm = uh > sig[nb];
-            const bool mm = uk1[nb] < (Real)1;
-            s_w[P(qx, qy, qz)] = (dm && mm) ? lam1[nb] : 0.0;
-            if (f == FXM) s_Dx[b * 72 + a * 9] = hs.dx;
-            else if (f == FYM) s_Dy[b * 72 + a] = hs.dy;
-            else if (f == FZM) s_Dz[b * 8 + a] = hs.dz;
-        }
+        return HaloCell{(size_t)face_nb(prm.pos(s), f) * BCELLS + src, Q(qx, qy, qz), depth == 0 ? P(qx, qy, qz) : 0};
+    };
+    const int wv = __builtin_amdgcn_readfirstlane(t >> 6);
+    const HaloCell h0 = halo_cell(wv);
+    const Real u_h0 = uk[h0.src];
+    const Stat4<Real> s_h0 = st[h0.src];
+    Real sg_h0 = 0, u1_h0 = 0;
+    double l_h0 = 0;
+    if (wv < 6) {  // depth 1
+        sg_h0 = sig[h0.src];
+        u1_h0 = uk1[h0.src];
+        l_h0 = lam1[h0.src];
+    }
+    HaloCell h1{0, 0, 0};
+    Real u_h1 = 0;
+    Stat4<Real> s_h1{};
+    if (wv < 4) {  // second pass: depth 2, faces 2-5
+        h1 = halo_cell(wv + 8);
+        u_h1 = uk[h1.src];
+        s_h1 = st[h1.src];
+    }
+    s_u[h0.q] = u_h0;
+    s_o[h0.q] = u_h0 < s_h0.tau;
+    if (wv < 6) {
+        const int a = t & 7, b = (t >> 3) & 7;
+        s_w[h0.p] = (u_h0 > sg_h0 && u1_h0 < (Real)1) ? l_h0 : 0.0;
+        if (wv == FXM) s_Dx[b * 72 + a * 9] = s_h0.dx;
+        else if (wv == FYM) s_Dy[b * 72 + a] = s_h0.dy;
+        else if (wv == FZM) s_Dz[b * 8 + a] = s_h0.dz;
+    }
+    if (wv < 4) {
+        s_u[h1.q] = u_h1;
+        s_o[h1.q] = u_h1 < s_h1.tau;
     }
     __syncthreads();
 
@@ -124,9 +177,8 @@ __global__ __launch_bounds__(512) void adj_step_kernel(int nslots, int per_xcd,
     lam0[g] = self + dt * ih2 * acc;
 
     const double wc = s_w[P(x, y, z)];
-    const double gd = block_sum(wc * dt * rdiff * invDw, s_red);
-    const double gr = block_sum(wc * dt * u0 * (1. - u0), s_red);
-    const double gt = block_sum(wc * R * invN, s_red);
+    double gd = wc * dt * rdiff * invDw, gr = wc * dt * u0 * (1. - u0), gt = wc * R * invN;
+    block_sum3(gd, gr, gt, s_red);
     if (t == 0) {
         partial[3 * (size_t)s + 0] += gd;
         partial[3 * (size_t)s + 1] += gr;

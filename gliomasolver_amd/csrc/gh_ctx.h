@@ -19,6 +19,9 @@ struct DevBuf {
     void* p = nullptr;
     size_t bytes = 0;
     int ensure(size_t nbytes);
+    // as ensure, but a reallocation takes 25% headroom (buffers of a growing
+    // adaptive grid are not reallocated at every adaptation)
+    int ensure_grow(size_t nbytes);
     void release();
     template <typename T> T* as() const { return static_cast<T*>(p); }
 };
@@ -54,6 +57,11 @@ struct Ctx {
     int denseOff[Octree::MAXL + 1] = {0};
     int nDense = 0;
     double last_fwt_ms = 0;
+    // adaptation scratch kept across epochs: spare field arrays swapped with
+    // the live ones by apply_new_tree, split labs, remap list, FWT details
+    DevBuf spare[4], labSplit[4], remapBuf, detBuf;
+    // adaptation profile (gh_adapt_profile), host wall time in ms per phase
+    double prof[8] = {0};
 
     DevBuf phi[2];      // ping-pong tumor density
     int cur = 0;        // phi[cur] holds the current state

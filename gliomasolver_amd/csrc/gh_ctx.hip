@@ -29,6 +29,10 @@ int DevBuf::ensure(size_t nbytes) {
     bytes = nbytes;
     return GH_OK;
 }
+int DevBuf::ensure_grow(size_t nbytes) {
+    if (nbytes <= bytes && p) return GH_OK;
+    return ensure(nbytes + nbytes / 4);
+}
 void DevBuf::release() {
     if (p) (void)hipFree(p);
     p = nullptr;
@@ -58,7 +62,8 @@ int Ctx::alloc_fields() {
 void Ctx::release_all() {
     DevBuf* bufs[] = {&dLevel,   &dNbr27,  &faceEnt,    &faceSrc,   &faceW,     &fgPhi,       &fgTau,   &fgDm,
                       &fgPw,     &fgPg,    &labBuf,     &labSlots,  &labEnt,    &labSrc,      &labW,    &labAllEnt,
-                      &labAllSrc, &labAllW, &labAllSlots, &dDense, &dBlk, &dBad, &phi[0],  &phi[1],    &dphidt,      &pw,      &pg,
+                      &labAllSrc, &labAllW, &labAllSlots, &dDense, &dBlk, &dBad, &spare[0], &spare[1], &spare[2], &spare[3],
+                      &labSplit[0], &labSplit[1], &labSplit[2], &labSplit[3], &remapBuf, &detBuf, &phi[0],  &phi[1],    &dphidt,      &pw,      &pg,
                       &pcsf,     &statD,   &statSig,    &statF,     &flag,      &nbr,         &obsT1,   &obsFl,
                       &obsPet,   &obsRoi,  &likPart,    &adjCk,     &adjSeg,    &adjLam[0],   &adjLam[1], &adjPart};
     for (DevBuf* b : bufs) b->release();

@@ -77,6 +77,16 @@ int gh_grid_init_leaves(gh_ctx* ctx, int nleaves, const int* level, const int* i
 int gh_fwt_details(gh_ctx* ctx, double* detail);
 int gh_refine(gh_ctx* ctx, double tol, int max_loops, int* nrefined);
 int gh_compress(gh_ctx* ctx, double tol, int max_loops, int* ncollapsed);
+/* gh_adapt_profile: host wall time (ms) spent by gh_refine / gh_compress
+ * since the last reset, per phase -- the agents of MRAG's Profiler in
+ * Science::AutomaticRefinement/Compression (MRAG/MRAGscience/
+ * MRAGScienceCore.h:69-93,136-152: "AutoRef::FWT", "AutoRef::refine",
+ * "AutoCompr::FWT", "AutoCompr::compress"), then the refine/compress time
+ * split into plan, data movement (labs + remap), topology (neighbours, ghost
+ * tables, uploads) and per-solve constants:
+ * ms[8] = {ref_fwt, ref, comp_fwt, comp, plan, move, topology, statics}.
+ * reset != 0 zeroes the counters after reading.  ms may be NULL. */
+int gh_adapt_profile(gh_ctx* ctx, double* ms, int reset);
 int gh_set_field_blocks(gh_ctx* ctx, int field, const void* values);
 int gh_get_field_blocks(gh_ctx* ctx, int field, void* values);
 int gh_uq_resample(gh_ctx* ctx, int gpd, double* out);

@@ -300,3 +300,17 @@ def test_gpu_adaptive_run_matches_oracle_driven_simulation(oracle):
         levels_seen |= set(rows[:, 0].tolist())
     ctx.close()
     assert len(levels_seen) >= 2
+
+
+def test_gpu_adapt_profile_counts_and_resets():
+    rows = grids()["random"]
+    fields = random_fields(rows, np.float32, 5)
+    say("adapt profile")
+    ctx = make_ctx("f32", rows, 4, fields)
+    ctx.refine(1e-3, 1)
+    ctx.compress(1e-4, -1)
+    prof = ctx.adapt_profile(reset=True)
+    assert set(prof) == set(gh.Context.ADAPT_PHASES)
+    assert all(v >= 0 for v in prof.values()) and prof["ref_fwt"] > 0 and prof["comp_fwt"] > 0
+    assert all(v == 0 for v in ctx.adapt_profile().values())
+    ctx.close()
