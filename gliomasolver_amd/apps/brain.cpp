@@ -18,12 +18,19 @@
 //   -bpd 16 (blocks per dimension), -maxLevel (4 for bpd 16, 3 for bpd 8,
 //   else log2 bpd), -wavelets 2|4 (default 2), -precision float|double
 //   (default float, the reference's Real), -device 0.
+//
+// -model UQprep runs the reference's UQ data preprocessing
+// (Glioma/Glioma_UQ_DataPreprocessing.cpp, gh_uqprep.h) on the host:
+// -PatFileName P, -bSynt 0|1, -bpd; writes tumPET/tumT1c/tumFLAIR.dat,
+// ROI.dat, PriorRange.txt and LogPriorRange.txt (GenericPrior.txt in the
+// working directory).
 #include <chrono>
 #include <cstdio>
 #include <fstream>
 #include <string>
 
 #include "gh_args.h"
+#include "gh_uqprep.h"
 #include "gliomahip.hpp"
 
 using namespace gliomahip;
@@ -137,17 +144,43 @@ int run(const Args& a) {
     return 0;
 }
 
+// Glioma_UQ_DataPreprocessing::run (:539-548) with its constructor's _ic choice (:17-52)
+template <typename Real>
+int run_uqprep(const Args& a) {
+    const bool verbose = a.flag("-verbose", true);
+    UQPreprocessing<Real> uq(a.integer("-bpd", 16), verbose);
+    const std::string pat = a.str("-PatFileName");
+    if (verbose) std::printf("Reading data from file: %s \n", pat.c_str());
+    if (a.integer("-bSynt", 0) == 1) uq.ic_synthetic(pat);
+    else uq.ic(pat);
+    uq.normalise_pet();
+    uq.prior_range();
+    uq.dump_uq_data();
+    const int np = uq.dump_roi();
+    if (verbose) std::printf("ROI points %d\n\n\n Run Finished \n\n", np);
+    return 0;
+}
+
 }  // namespace
 
 int main(int argc, const char** argv) {
     Args a(argc, argv);
     const std::string model = a.str("-model");
+    const bool dbl = a.str("-precision", "float") == "double";
+    if (model == "UQprep") {
+        try {
+            return dbl ? run_uqprep<double>(a) : run_uqprep<float>(a);
+        } catch (const Error& e) {
+            std::fprintf(stderr, "brain: %s\n", e.what());
+            return 1;
+        }
+    }
     if (model != "RD") {
-        std::fprintf(stderr, "brain: -model %s is not provided by this build (RD only)\n", model.c_str());
+        std::fprintf(stderr, "brain: -model %s is not provided by this build (RD, UQprep)\n", model.c_str());
         return 2;
     }
     try {
-        return a.str("-precision", "float") == "double" ? run<double>(a) : run<float>(a);
+        return dbl ? run<double>(a) : run<float>(a);
     } catch (const Error& e) {
         std::fprintf(stderr, "brain: %s\n", e.what());
         return 1;

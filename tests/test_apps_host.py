@@ -61,7 +61,7 @@ def test_hpp_compiles_standalone(tmp_path):
 
 def test_brain_rejects_other_models(tmp_path):
     r = subprocess.run([os.path.join(BIN, "brain"), "-model", "PFFCH"], cwd=tmp_path, capture_output=True, text=True)
-    assert r.returncode == 2 and "RD only" in r.stderr
+    assert r.returncode == 2 and "not provided" in r.stderr
 
 
 def test_likelihood_requires_input_file(tmp_path):
