@@ -520,11 +520,10 @@ __global__ __launch_bounds__(512) void rd_step2f_kernel(int nslots, int per_xcd,
     const size_t base = (size_t)s * BCELLS;
     const unsigned ut = (unsigned)t;
 
-    // ---- loads: own cell, then (waves 0-5) the depth-1 and depth-2 cells of
-    // face w, (waves 6-7) the 96 edge cells next to the depth-1 faces; block
-    // base pointers are wave-uniform, lane offsets 32-bit
-    const Real u_c = (phi + base)[ut];
-    const Stat4<Real> st_c = (stf + base)[ut];
+    // ---- loads: (waves 0-5) the depth-1 and depth-2 cells of face w, (waves
+    // 6-7) the 96 edge cells next to the depth-1 faces, then the own cell; all
+    // offsets are computed first so the loads issue back to back; block base
+    // pointers are wave-uniform, lane offsets 32-bit
     Real u_1 = 0, u_2 = 0, u_e = 0;  // depth-1, depth-2 and edge cell
     Stat4<Real> st_1, st_2, st_e;
     int uu1 = 0, dm1 = 0, uu2 = 0, dm2 = 0;
@@ -533,19 +532,18 @@ __global__ __launch_bounds__(512) void rd_step2f_kernel(int nslots, int per_xcd,
     if (w < 6) {
         const FaceMap m1 = face_map(w, 1), m2 = face_map(w, 2);
         const size_t nb = (size_t)face_nb(bp, w) * BCELLS;
-        const unsigned l1 = (unsigned)mad24(b, m1.lob, mad24(a, m1.loa, m1.lo));
-        const unsigned l2 = (unsigned)mad24(b, m2.lob, mad24(a, m2.loa, m2.lo));
-        __builtin_assume(l1 < BCELLS && l2 < BCELLS);
+        const unsigned l1 = (unsigned)mad24(b, m1.lob, mad24(a, m1.loa, m1.lo)) & (BCELLS - 1);
+        const unsigned l2 = (l1 + (unsigned)(m2.lo - m1.lo)) & (BCELLS - 1);
+        uu1 = mad24(b, m1.uub, mad24(a, m1.uua, m1.uu));
+        dm1 = mad24(b, m1.dmb, mad24(a, m1.dma, m1.dm));
+        uu2 = uu1 + (m2.uu - m1.uu);
+        dm2 = dm1 + (m2.dm - m1.dm);
         const Real* pn = phi + nb;
         const Stat4<Real>* sn = stf + nb;
         u_1 = pn[l1];
         st_1 = sn[l1];
         u_2 = pn[l2];
         st_2 = sn[l2];
-        uu1 = mad24(b, m1.uub, mad24(a, m1.uua, m1.uu));
-        dm1 = mad24(b, m1.dmb, mad24(a, m1.dma, m1.dm));
-        uu2 = mad24(b, m2.uub, mad24(a, m2.uua, m2.uu));
-        dm2 = mad24(b, m2.dmb, mad24(a, m2.dma, m2.dm));
     } else {
         const int e = (w - 6) * 64 + (t & 63);
         edge = e < 96;
@@ -568,6 +566,8 @@ __global__ __launch_bounds__(512) void rd_step2f_kernel(int nslots, int per_xcd,
             st_e = stf[gs];
         }
     }
+    const Real u_c = (phi + base)[ut];
+    const Stat4<Real> st_c = (stf + base)[ut];
     Real rho;
     double ih2, dt;
     prm.get(s, rho, ih2, dt);
