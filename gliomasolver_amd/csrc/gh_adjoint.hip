@@ -43,12 +43,20 @@ int adjoint_t(Ctx& c, int64_t N, double T, int K, const double* lik, int stepPET
     const double dt = T / (double)N;
     char* ck = (char*)c.adjCk.p;
     char* seg = (char*)c.adjSeg.p;
-    // forward sweep with checkpoints every K steps
-    for (int64_t k = 0; k < N; ++k) {
+    // forward sweep with checkpoints every K steps; two steps per launch from
+    // a clamped state (k >= 1) when no checkpoint falls in between
+    const bool pairs = c.fast_ok && c.topo.uniform_bpd > 0;
+    for (int64_t k = 0; k < N;) {
         if (k % K == 0)
             GH_HIP_CHECK(hipMemcpyAsync(ck + (size_t)(k / K) * nb, c.phi[c.cur].p, nb, hipMemcpyDeviceToDevice,
                                         c.stream));
-        if ((rc = rd_step_buffers(c, c.phi[c.cur].p, c.phi[1 - c.cur].p, dt))) return rc;
+        if (pairs && k >= 1 && k + 2 <= N && (k + 1) % K != 0) {
+            if ((rc = rd_step2_buffers(c, c.phi[c.cur].p, c.phi[1 - c.cur].p, nullptr, dt))) return rc;
+            k += 2;
+        } else {
+            if ((rc = rd_step_buffers(c, c.phi[c.cur].p, c.phi[1 - c.cur].p, dt))) return rc;
+            k += 1;
+        }
         c.cur = 1 - c.cur;
     }
     // loss at u_N and its gradient as the terminal adjoint
@@ -70,8 +78,17 @@ int adjoint_t(Ctx& c, int64_t N, double T, int K, const double* lik, int stepPET
     for (int64_t sgi = nck - 1; sgi >= 0; --sgi) {
         const int64_t k0 = sgi * K, k1 = std::min<int64_t>(N, k0 + K);
         GH_HIP_CHECK(hipMemcpyAsync(seg, ck + (size_t)sgi * nb, nb, hipMemcpyDeviceToDevice, c.stream));
-        for (int64_t j = 0; j < k1 - k0; ++j)
-            if ((rc = rd_step_buffers(c, seg + (size_t)j * nb, seg + (size_t)(j + 1) * nb, dt))) return rc;
+        for (int64_t j = 0; j < k1 - k0;) {  // the segment's states, pairs from clamped states
+            if (pairs && k0 + j >= 1 && j + 2 <= k1 - k0) {
+                if ((rc = rd_step2_buffers(c, seg + (size_t)j * nb, seg + (size_t)(j + 2) * nb,
+                                           seg + (size_t)(j + 1) * nb, dt)))
+                    return rc;
+                j += 2;
+            } else {
+                if ((rc = rd_step_buffers(c, seg + (size_t)j * nb, seg + (size_t)(j + 1) * nb, dt))) return rc;
+                j += 1;
+            }
+        }
         for (int64_t k = k1 - 1; k >= k0; --k) {
             hipLaunchKernelGGL((adj_step_kernel<Real, UniformAdjParams>), dim3(grid), dim3(512), 0, c.stream, ns,
                                per_xcd, (const Real*)(seg + (size_t)(k - k0) * nb),

@@ -496,7 +496,7 @@ struct ForwardSweep {
             const int per_xcd = (nsl + 7) / 8;
             if (b.fast_ok)
                 hipLaunchKernelGGL((rd_step2f_kernel<Real, BatchStepParams<Real>>), dim3(8 * per_xcd), dim3(512), 0,
-                                   b.stream, nsl, per_xcd, in, out, b.statF.as<Stat4<Real>>(), prm);
+                                   b.stream, nsl, per_xcd, in, out, (Real*)nullptr, b.statF.as<Stat4<Real>>(), prm);
             else
                 hipLaunchKernelGGL((rd_step2_kernel<Real, BatchStepParams<Real>>), dim3(8 * per_xcd), dim3(512), 0,
                                    b.stream, nsl, per_xcd, in, out, b.statD.as<Stat4<Real>>(), b.statSig.as<Real>(),
@@ -667,9 +667,25 @@ int adjoint_t(Batch& b, int K, const double* lik, int stepPET, int stepTi, int u
         const int64_t k0 = j * K, k1 = std::min<int64_t>(Nmax, k0 + K);
         GH_HIP_CHECK(hipMemcpyAsync(tape, ck + ckoff[j] * slot_bytes, b.slot0[active(k0)] * slot_bytes,
                                     hipMemcpyDeviceToDevice, b.stream));
-        for (int64_t k = k0; k < k1; ++k)  // recompute u_{k0+1} .. u_{k1} of the running samples
-            step(tape + (size_t)(k - k0) * nsl * slot_bytes, tape + (size_t)(k - k0 + 1) * nsl * slot_bytes,
-                 (int)b.slot0[active(k)]);
+        // recompute u_{k0+1} .. u_{k1} of the running samples, two steps per
+        // launch (both states written) where the input is a clamped state
+        // (k >= 1) and both fit the segment; a sample ending after the first of
+        // the two gets a second state past its end that no reverse step reads
+        for (int64_t k = k0; k < k1;) {
+            char* in = tape + (size_t)(k - k0) * nsl * slot_bytes;
+            const int nslots = (int)b.slot0[active(k)];
+            if (b.fast_ok && k >= 1 && k + 2 <= k1) {
+                const int per_xcd = (nslots + 7) / 8;
+                hipLaunchKernelGGL((rd_step2f_kernel<Real, BatchStepParams<Real>>), dim3(8 * per_xcd), dim3(512), 0,
+                                   b.stream, nslots, per_xcd, (const Real*)in,
+                                   (Real*)(in + 2 * (size_t)nsl * slot_bytes), (Real*)(in + (size_t)nsl * slot_bytes),
+                                   b.statF.as<Stat4<Real>>(), fprm);
+                k += 2;
+            } else {
+                step(in, in + (size_t)nsl * slot_bytes, nslots);
+                k += 1;
+            }
+        }
         for (int64_t k = k1 - 1; k >= k0; --k) {
             const int nslots = (int)b.slot0[active(k)];
             const int per_xcd = (nslots + 7) / 8;

@@ -100,6 +100,9 @@ int rd_update(Ctx& c, double dt);
 int rd_steps(Ctx& c, int64_t nsteps, double dt);
 // one fused step from an explicit state buffer into another (same layout as phi)
 int rd_step_buffers(Ctx& c, const void* in, void* out, double dt);
+// two fused steps from a clamped state (every value >= +0, as written by the
+// fused update) when c.fast_ok; mid (may be null) gets the state in between
+int rd_step2_buffers(Ctx& c, const void* in, void* out, void* mid, double dt);
 double block_h_of(const Ctx& c);
 // the rd_step2f_kernel records of n cells from {D+, tau} and sigma into out;
 // *ok = every cell has sigma <= 0 (p_w + p_g >= 0).  Synchronizes the stream.

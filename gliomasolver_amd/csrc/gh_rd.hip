@@ -118,16 +118,17 @@ int launch_step2(Ctx& c, const Real* in, Real* out, double dt) {
     return GH_OK;
 }
 
-// two fused steps per launch on a clamped state (rd_step2f_kernel)
+// two fused steps per launch on a clamped state (rd_step2f_kernel); mid, when
+// not null, receives the state after the first step
 template <typename Real>
-int launch_step2f(Ctx& c, const Real* in, Real* out, double dt) {
+int launch_step2f(Ctx& c, const Real* in, Real* out, double dt, Real* mid = nullptr) {
     const int ns = c.topo.nblocks();
     const int per_xcd = (ns + 7) / 8;
     const double h = block_h(c);
     const double ih2 = 1. / (h * h);
     const UniformStepParams<Real> prm{(Real)c.rho, ih2, dt, c.topo.uniform_bpd, bpd_magic(c.topo.uniform_bpd)};
     hipLaunchKernelGGL((rd_step2f_kernel<Real, UniformStepParams<Real>>), dim3(8 * per_xcd), dim3(512), 0, c.stream,
-                       ns, per_xcd, in, out, c.statF.as<Stat4<Real>>(), prm);
+                       ns, per_xcd, in, out, mid, c.statF.as<Stat4<Real>>(), prm);
     GH_HIP_CHECK(hipGetLastError());
     return GH_OK;
 }
@@ -239,6 +240,13 @@ int build_stat_fast(hipStream_t stream, int real_kind, size_t n, const void* st,
     GH_HIP_CHECK(hipStreamSynchronize(stream));
     *ok = bad == 0;
     return GH_OK;
+}
+
+int rd_step2_buffers(Ctx& c, const void* in, void* out, void* mid, double dt) {
+    if (!c.fast_ok || c.topo.uniform_bpd <= 0)
+        return set_error(GH_ESTATE, "two-step launch: needs a uniform grid with p_w + p_g >= 0");
+    return c.real_kind == GH_REAL_F32 ? launch_step2f<float>(c, (const float*)in, (float*)out, dt, (float*)mid)
+                                      : launch_step2f<double>(c, (const double*)in, (double*)out, dt, (double*)mid);
 }
 
 int rd_step_buffers(Ctx& c, const void* in, void* out, double dt) {

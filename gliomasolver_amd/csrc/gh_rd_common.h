@@ -506,8 +506,8 @@ __device__ inline int mad24(int a, int b, int c) { return (int)__umul24((unsigne
 
 template <typename Real, typename Params>
 __global__ __launch_bounds__(512) void rd_step2f_kernel(int nslots, int per_xcd, const Real* __restrict__ phi,
-                                                        Real* __restrict__ out, const Stat4<Real>* __restrict__ stf,
-                                                        Params prm) {
+                                                        Real* __restrict__ out, Real* __restrict__ mid,
+                                                        const Stat4<Real>* __restrict__ stf, Params prm) {
     const int s = (int)(blockIdx.x % 8) * per_xcd + (int)(blockIdx.x / 8);
     if (s >= nslots) return;
     __shared__ Real s_uu[UU_SIZE];
@@ -615,6 +615,7 @@ __global__ __launch_bounds__(512) void rd_step2f_kernel(int nslots, int per_xcd,
                                         s_uu[q - 24], s_uu[q + 24], s_uu[q - 293], s_uu[q + 293], u_c, in_c, rho, ih2);
         u1c = rd_update(u_c, d, dt);
         s_uu[q + UU1] = flag_in_sign(u1c, tau_c);
+        if (mid) (mid + base)[ut] = u1c;  // the state after the first step, when asked for
     }
     if (w < 6) {
         const int q = uu1;
