@@ -69,8 +69,8 @@ int adjoint_t(Ctx& c, int64_t N, double T, int K, const double* lik, int stepPET
     const int ns = c.topo.nblocks();
     const int per_xcd = (ns + 7) / 8;
     const unsigned grid = 8 * per_xcd;
-    if ((rc = c.adjPart.ensure(sizeof(double) * 3 * ns))) return rc;
-    GH_HIP_CHECK(hipMemsetAsync(c.adjPart.p, 0, sizeof(double) * 3 * ns, c.stream));
+    if ((rc = c.adjPart.ensure(sizeof(double) * ADJ_PART * ns))) return rc;
+    GH_HIP_CHECK(hipMemsetAsync(c.adjPart.p, 0, sizeof(double) * ADJ_PART * ns, c.stream));
     const double h = block_h_of(c);
     const double ih2 = 1. / (h * h);
     const UniformAdjParams prm{c.rho, ih2, dt, 1.0 / (double)N, 1.0 / c.Dw, c.topo.uniform_bpd,
@@ -106,7 +106,7 @@ int adjoint_t(Ctx& c, int64_t N, double T, int K, const double* lik, int stepPET
                        c.adjLam[lc].as<double>(), (const Real*)ck, ic[0], ic[1], ic[2], 1.0 / (sigma * sigma),
                        c.likPart.as<double>());
     GH_HIP_CHECK(hipGetLastError());
-    std::vector<double> pp(3 * (size_t)ns), ip(3 * igrid);
+    std::vector<double> pp(ADJ_PART * (size_t)ns), ip(3 * igrid);
     GH_HIP_CHECK(hipMemcpyAsync(pp.data(), c.adjPart.p, pp.size() * 8, hipMemcpyDeviceToHost, c.stream));
     GH_HIP_CHECK(hipMemcpyAsync(ip.data(), c.likPart.p, ip.size() * 8, hipMemcpyDeviceToHost, c.stream));
     GH_HIP_CHECK(hipStreamSynchronize(c.stream));

@@ -11,31 +11,35 @@ namespace gh {
 
 __device__ inline int Q(int x, int y, int z) { return (z + 2) * 144 + (y + 2) * 12 + (x + 2); }  // [-2,10)^3
 
-// three workgroup sums at once (one LDS round trip, two barriers); red holds
-// 3 * (blockDim / 64) doubles; the sums are valid in thread 0, each in the
-// same fixed order as block_sum
-__device__ inline void block_sum3(double& a, double& b, double& c, double* red) {
-    for (int o = 32; o > 0; o >>= 1) {
-        a += __shfl_xor(a, o, 64);
-        b += __shfl_xor(b, o, 64);
-        c += __shfl_xor(c, o, 64);
-    }
-    const int w = threadIdx.x / 64, lane = threadIdx.x % 64, nw = (int)(blockDim.x / 64);
-    __syncthreads();
-    if (lane == 0) {
-        red[w] = a;
-        red[nw + w] = b;
-        red[2 * nw + w] = c;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        a = b = c = 0;
-        for (int k = 0; k < nw; ++k) {
-            a += red[k];
-            b += red[nw + k];
-            c += red[2 * nw + k];
-        }
-    }
+// Doubles of parameter-gradient partials per slot: one (dDw, drho, dT) triple
+// per wave of the 512-lane workgroup.
+constexpr int ADJ_PART = 3 * 8;
+
+// v moved across lanes by the DPP pattern CTRL (both 32-bit halves)
+template <int CTRL>
+__device__ inline double dpp_move(double v) {
+    const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+    const int lo = __builtin_amdgcn_mov_dpp((int)(unsigned)b, CTRL, 0xF, 0xF, true);
+    const int hi = __builtin_amdgcn_mov_dpp((int)(unsigned)(b >> 32), CTRL, 0xF, 0xF, true);
+    return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+}
+__device__ inline double read_lane(double v, int lane) {
+    const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)b, lane);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(b >> 32), lane);
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+// Sum over the 64 lanes of a wave (all lanes active), valid in lanes 0-15.
+// Each 16-lane row is reduced by butterflies (lane pairs, quad halves, row
+// half mirror, row mirror) that leave the same row sum in every lane of the
+// row (floating-point addition commutes); the rows are then added in order
+// 0, 1, 2, 3.  Fixed order, no LDS, no barrier.
+__device__ inline double wave_sum_dpp(double v) {
+    v += dpp_move<0xB1>(v);   // quad_perm [1,0,3,2]
+    v += dpp_move<0x4E>(v);   // quad_perm [2,3,0,1]
+    v += dpp_move<0x141>(v);  // row_half_mirror
+    v += dpp_move<0x140>(v);  // row_mirror
+    return ((v + read_lane(v, 16)) + read_lane(v, 32)) + read_lane(v, 48);
 }
 
 __device__ inline double block_sum(double v, double* red) {
@@ -65,8 +69,9 @@ struct UniformAdjParams {
     __device__ inline BlockPos pos(int s) const { return block_pos(s, 0, bpd, magic); }
 };
 
-// One reverse step; one workgroup per block (XCD-aware order).  partial[3 s + q]
-// accumulates the block's parameter-gradient terms, step after step.
+// One reverse step; one workgroup per block (XCD-aware order).
+// partial[ADJ_PART s + 3 w + q] accumulates wave w's parameter-gradient terms
+// of block s, step after step.
 template <typename Real, typename Params>
 __global__ __launch_bounds__(512) void adj_step_kernel(int nslots, int per_xcd,
                                                        const Real* __restrict__ uk, const Real* __restrict__ uk1,
@@ -82,7 +87,6 @@ __global__ __launch_bounds__(512) void adj_step_kernel(int nslots, int per_xcd,
     __shared__ unsigned char s_o[1728];
     __shared__ double s_w[1000];
     __shared__ Real s_Dx[576], s_Dy[576], s_Dz[576];
-    __shared__ double s_red[24];
 
     const int t = threadIdx.x;
     const int x = t & 7, y = (t >> 3) & 7, z = t >> 6;
@@ -157,18 +161,20 @@ __global__ __launch_bounds__(512) void adj_step_kernel(int nslots, int per_xcd,
     __syncthreads();
 
     const double u0 = (double)uc;
-    const double D[6] = {(double)s_Dx[z * 72 + y * 9 + x], (double)sv.dx, (double)s_Dy[z * 72 + y * 8 + x],
-                         (double)sv.dy, (double)s_Dz[z * 64 + y * 8 + x], (double)sv.dz};
+    const Real D[6] = {s_Dx[z * 72 + y * 9 + x], sv.dx, s_Dy[z * 72 + y * 8 + x],
+                       sv.dy, s_Dz[z * 64 + y * 8 + x], sv.dz};
     const int nq[6] = {Q(x - 1, y, z), Q(x + 1, y, z), Q(x, y - 1, z), Q(x, y + 1, z), Q(x, y, z - 1), Q(x, y, z + 1)};
     const int np[6] = {P(x - 1, y, z), P(x + 1, y, z), P(x, y - 1, z), P(x, y + 1, z), P(x, y, z - 1), P(x, y, z + 1)};
     const int n2[6] = {Q(x - 2, y, z), Q(x + 2, y, z), Q(x, y - 2, z), Q(x, y + 2, z), Q(x, y, z - 2), Q(x, y, z + 2)};
     double rdiff = 0, sumd = 0, acc = 0;
 #pragma unroll
     for (int i = 0; i < 6; ++i) {
-        const double dp = D[i] * (1.0 + (double)s_o[nq[i ^ 1]]);  // doubled if the opposite neighbour is outside
-        rdiff += dp * ((double)s_u[nq[i]] - u0);
+        // face coefficients doubled (exactly, in Real) where the opposite
+        // neighbour of c, or of the neighbour, is outside
+        const double dp = (double)(s_o[nq[i ^ 1]] ? D[i] + D[i] : D[i]);
+        rdiff = fma(dp, (double)s_u[nq[i]] - u0, rdiff);
         sumd += dp;
-        acc += s_w[np[i]] * D[i] * (1.0 + (double)s_o[n2[i]]);
+        acc = fma(s_w[np[i]], (double)(s_o[n2[i]] ? D[i] + D[i] : D[i]), acc);
     }
     rdiff *= ih2;
     const double R = rdiff + rho * u0 * (1. - u0);
@@ -177,12 +183,13 @@ __global__ __launch_bounds__(512) void adj_step_kernel(int nslots, int per_xcd,
     lam0[g] = self + dt * ih2 * acc;
 
     const double wc = s_w[P(x, y, z)];
-    double gd = wc * dt * rdiff * invDw, gr = wc * dt * u0 * (1. - u0), gt = wc * R * invN;
-    block_sum3(gd, gr, gt, s_red);
-    if (t == 0) {
-        partial[3 * (size_t)s + 0] += gd;
-        partial[3 * (size_t)s + 1] += gr;
-        partial[3 * (size_t)s + 2] += gt;
+    const double gd = wave_sum_dpp(wc * dt * rdiff * invDw), gr = wave_sum_dpp(wc * dt * u0 * (1. - u0)),
+                 gt = wave_sum_dpp(wc * R * invN);
+    if ((t & 63) == 0) {
+        double* pw = partial + ADJ_PART * (size_t)s + 3 * (t >> 6);
+        pw[0] += gd;
+        pw[1] += gr;
+        pw[2] += gt;
     }
 }
 
@@ -216,15 +223,16 @@ __global__ __launch_bounds__(256) void adj_ic_partials(size_t nc, int bpd, const
     adj_ic_body<Real>((int)blockIdx.x, (int)gridDim.x, nc, bpd, lam, u0, x0, y0, z0, is2, partial);
 }
 
-// The host's summation of the per-slot partials: workgroup order w = 0..8*per_xcd-1
-// of the step kernel's grid, slot = (w % 8) * per_xcd + w / 8.
+// The host's summation of the per-wave partials: workgroup order g = 0..8*per_xcd-1
+// of the step kernel's grid, slot = (g % 8) * per_xcd + g / 8, then wave order.
 inline void sum_slot_partials(const double* part, int nslots, double out[3]) {
     const int per_xcd = (nslots + 7) / 8;
     out[0] = out[1] = out[2] = 0;
-    for (int w = 0; w < 8 * per_xcd; ++w) {
-        const int s = (w % 8) * per_xcd + w / 8;
+    for (int g = 0; g < 8 * per_xcd; ++g) {
+        const int s = (g % 8) * per_xcd + g / 8;
         if (s >= nslots) continue;
-        for (int q = 0; q < 3; ++q) out[q] += part[3 * (size_t)s + q];
+        for (int w = 0; w < 8; ++w)
+            for (int q = 0; q < 3; ++q) out[q] += part[ADJ_PART * (size_t)s + 3 * w + q];
     }
 }
 

@@ -603,7 +603,7 @@ int adjoint_t(Batch& b, int K, const double* lik, int stepPET, int stepTi, int u
     constexpr int IC_GRID = 512;
     int rc;
     if ((rc = b.ck.ensure(ckoff[J] * slot_bytes)) || (rc = b.tape.ensure((size_t)(K + 1) * nsl * slot_bytes)) ||
-        (rc = b.lam[0].ensure(nc * 8)) || (rc = b.lam[1].ensure(nc * 8)) || (rc = b.apart.ensure(3 * nsl * 8)) ||
+        (rc = b.lam[0].ensure(nc * 8)) || (rc = b.lam[1].ensure(nc * 8)) || (rc = b.apart.ensure(ADJ_PART * nsl * 8)) ||
         (rc = b.icpart.ensure((size_t)ns * 3 * IC_GRID * 8)) || (rc = b.adjJobs.ensure(ns * sizeof(AdjJob<Real>))))
         return rc;
     const BatchStepParams<Real> fprm{b.slotPos.as<int32_t>(), b.range.as<SlotRange<Real>>()};
@@ -662,7 +662,7 @@ int adjoint_t(Batch& b, int K, const double* lik, int stepPET, int stepTi, int u
         GH_HIP_CHECK(hipGetLastError());
     }
     // reverse
-    GH_HIP_CHECK(hipMemsetAsync(b.apart.p, 0, 3 * nsl * 8, b.stream));
+    GH_HIP_CHECK(hipMemsetAsync(b.apart.p, 0, ADJ_PART * nsl * 8, b.stream));
     for (int64_t j = J - 1; j >= 0; --j) {
         const int64_t k0 = j * K, k1 = std::min<int64_t>(Nmax, k0 + K);
         GH_HIP_CHECK(hipMemcpyAsync(tape, ck + ckoff[j] * slot_bytes, b.slot0[active(k0)] * slot_bytes,
@@ -714,7 +714,7 @@ int adjoint_t(Batch& b, int K, const double* lik, int stepPET, int stepTi, int u
                            b.adjJobs.as<AdjJob<Real>>() + p0, b.icpart.as<double>() + (size_t)p0 * 3 * IC_GRID);
     GH_HIP_CHECK(hipGetLastError());
     GH_HIP_CHECK(hipEventRecord(b.ev1, b.stream));
-    std::vector<double> ap(3 * nsl), ip((size_t)ns * 3 * IC_GRID);
+    std::vector<double> ap(ADJ_PART * nsl), ip((size_t)ns * 3 * IC_GRID);
     GH_HIP_CHECK(hipMemcpyAsync(ap.data(), b.apart.p, ap.size() * 8, hipMemcpyDeviceToHost, b.stream));
     GH_HIP_CHECK(hipMemcpyAsync(ip.data(), b.icpart.p, ip.size() * 8, hipMemcpyDeviceToHost, b.stream));
     GH_HIP_CHECK(hipStreamSynchronize(b.stream));
@@ -724,7 +724,7 @@ int adjoint_t(Batch& b, int K, const double* lik, int stepPET, int stepTi, int u
     for (int p = 0; p < ns; ++p) {
         const int i = b.order[p];
         double* gi = grad + 6 * (size_t)i;
-        sum_slot_partials(ap.data() + 3 * b.slot0[p], (int)(b.slot0[p + 1] - b.slot0[p]), gi);
+        sum_slot_partials(ap.data() + ADJ_PART * b.slot0[p], (int)(b.slot0[p + 1] - b.slot0[p]), gi);
         gi[3] = gi[4] = gi[5] = 0;
         for (int w = 0; w < IC_GRID; ++w)
             for (int qd = 0; qd < 3; ++qd) gi[3 + qd] += ip[((size_t)p * IC_GRID + w) * 3 + qd];
