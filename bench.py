@@ -13,6 +13,10 @@ roofline   = 16 B per cell-update (read u, write u, read p_w, read p_g) x cell-
              updates / device time of the step launches (HIP events on the
              context stream), against 8 TB/s; per launch: 16 B x cells x the
              launch's Euler steps (two for the temporally blocked kernel)
+adjoint    = forward + likelihood + discrete adjoint (six gradients) of the
+             same 128^3 solve, checkpoints every 64 steps: reverse cell-steps/s
+             over whole solves, against 52 B per cell-step (forward 16 +
+             reverse 20 + tape recompute 16)
 batched    = one TMCMC stage (forward + likelihood + adjoint of every sample,
              1,024 samples on 64^3/96^3/128^3 grids) solved by gh_batch in
              memory-sized chunks; evals/s against the same samples one call
@@ -99,6 +103,49 @@ def _stage_memory(cells, nsteps, K, real_size=4):
     two FP64 adjoint states, the (K+1)-state tape and ceil(N/K) checkpoints."""
     per_cell = real_size * (2 + 9 + (K + 1) + -(-nsteps // K)) + 16
     return cells * per_cell
+
+
+ADJ_BYTES_PER_CELL_STEP_F32 = 52  # forward 16 + reverse 20 + tape recompute 16
+
+
+def adjoint_leg(ctx, case, final, solves, K):
+    """Forward + likelihood + discrete adjoint (dJ/d(Dw, rho, T, x0, y0, z0)) of
+    the uniform case from its IC, against observations thresholded from its own
+    final state (T1Gd u >= 0.7, FLAIR u >= 0.25, 5 % flips, PET = u + N(0, 0.05^2)
+    inside T1Gd).  Rate = reverse cell-steps over the wall clock of whole solves."""
+    rng = np.random.default_rng(41)
+    u = final.astype(np.float64)
+    t1 = (u >= 0.7).astype(np.float64)
+    fl = (u >= 0.25).astype(np.float64)
+    for m in (t1, fl):
+        f = rng.uniform(size=m.shape) < 0.05
+        m[f] = 1 - m[f]
+    pet = np.where(t1 > 0, u + rng.normal(0, 0.05, u.shape), 0.0)
+    ctx.lik_set_data(t1, fl, pet)
+    T = case.nsteps * case.dt
+    ic = [float(v) for v in case.seed_voxel]
+    ctx.checkpoint_load(0)
+    ctx.rd_adjoint(case.nsteps, T, ic, 1.0, K)  # warm-up (buffers)
+    ctx.sync()
+    t0 = time.perf_counter()
+    for _ in range(solves):
+        ctx.checkpoint_load(0)
+        loss, grad = ctx.rd_adjoint(case.nsteps, T, ic, 1.0, K)
+    ctx.sync()
+    wall = (time.perf_counter() - t0) / solves
+    cell_steps = case.ncells * case.nsteps
+    gbs = ADJ_BYTES_PER_CELL_STEP_F32 * cell_steps / wall / 1e9
+    out = {"workload": "128^3 FP32 forward + likelihood + discrete adjoint of the same solve (gradient w.r.t. Dw, "
+                       "rho, T, x0, y0, z0), FP64 adjoint and gradients, checkpoints every %d steps" % K,
+           "euler_steps": case.nsteps, "ms_per_solve": round(wall * 1e3, 3),
+           "cell_steps_per_s": cell_steps / wall,
+           "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": round(gbs / HBM_PEAK_GBS, 4),
+                        "bytes_per_cell_step": ADJ_BYTES_PER_CELL_STEP_F32,
+                        "what": "forward 16 + reverse 20 + recompute 16 B per cell-step, whole-solve wall clock"},
+           "loss": float(loss[0]), "grad": [float(g) for g in grad]}
+    log("adjoint", json.dumps(out))
+    return out
 
 
 def batched_leg(nsamples, K, budget_gb, seq_samples, acc_samples, chunk_cells=0):
@@ -284,6 +331,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--seed", type=int, default=2)
     ap.add_argument("--no-batch", action="store_true", help="skip the batched TMCMC-stage leg")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the single-solve adjoint leg")
+    ap.add_argument("--adjoint-ckpt", type=int, default=64, help="checkpoint interval of the single-solve adjoint")
+    ap.add_argument("--adjoint-solves", type=int, default=3, help="timed adjoint solves")
     ap.add_argument("--batch-samples", type=int, default=1024)
     ap.add_argument("--batch-ckpt", type=int, default=32, help="checkpoint interval of the batched adjoint")
     ap.add_argument("--batch-budget-gb", type=float, default=120.0, help="device memory per batched chunk")
@@ -331,6 +381,7 @@ def main():
     wall = time.perf_counter() - t0
     log("timed %d solves x %d Euler steps in %.3f s" % (a.steps, case.nsteps, wall))
     final = ctx.get_field(FIELD_PHI)
+    adj = None if a.no_adjoint else adjoint_leg(ctx, case, final, a.adjoint_solves, a.adjoint_ckpt)
     ctx.close()
 
     updates = case.ncells * case.nsteps * a.steps
@@ -367,6 +418,8 @@ def main():
             "roofline": roof, "cpu_baseline": cpu}
     if cpu:
         line["gpu_over_cpu"] = round(value / cpu["value"], 1)
+    if adj:
+        line["adjoint"] = adj
     if not a.no_batch:
         log("batched TMCMC stage ...")
         line["batched"] = batched_leg(a.batch_samples, a.batch_ckpt, a.batch_budget_gb, a.batch_seq, a.batch_acc,

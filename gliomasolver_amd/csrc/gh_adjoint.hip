@@ -18,8 +18,8 @@
 // (the face of neighbour e towards c is doubled when e's opposite neighbour,
 // c + 2 dir_i, is outside).  Parameter gradients accumulate per cell
 //   dJ/dDw += w_c dt Rdiff_c / Dw, dJ/drho += w_c dt u_c(1-u_c), dJ/dT += w_c R_c / N
-// in FP64 per workgroup; every workgroup owns the same blocks in every step,
-// so the accumulation order is fixed.  States u_k are recomputed per segment
+// in FP64 per wave; every wave owns the same cells in every step, so the
+// accumulation order is fixed.  States u_k are recomputed per segment
 // of K steps from checkpoints taken in the forward sweep.
 #include <cmath>
 #include <vector>
@@ -90,11 +90,17 @@ int adjoint_t(Ctx& c, int64_t N, double T, int K, const double* lik, int stepPET
             }
         }
         for (int64_t k = k1 - 1; k >= k0; --k) {
-            hipLaunchKernelGGL((adj_step_kernel<Real, UniformAdjParams>), dim3(grid), dim3(512), 0, c.stream, ns,
-                               per_xcd, (const Real*)(seg + (size_t)(k - k0) * nb),
-                               (const Real*)(seg + (size_t)(k - k0 + 1) * nb), c.adjLam[lc].as<double>(),
-                               c.adjLam[1 - lc].as<double>(), c.statD.as<Stat4<Real>>(), c.statSig.as<Real>(),
-                               prm, c.adjPart.as<double>());
+            const Real* uk = (const Real*)(seg + (size_t)(k - k0) * nb);
+            const Real* uk1 = (const Real*)(seg + (size_t)(k - k0 + 1) * nb);
+            if (c.fast_ok && k >= 1)  // u_k is a clamped state: the clamped-state records
+                hipLaunchKernelGGL((adj_step_kernel<Real, UniformAdjParams, true>), dim3(grid), dim3(512), 0,
+                                   c.stream, ns, per_xcd, uk, uk1, c.adjLam[lc].as<double>(),
+                                   c.adjLam[1 - lc].as<double>(), c.statF.as<Stat4<Real>>(), (const Real*)nullptr,
+                                   prm, c.adjPart.as<double>());
+            else
+                hipLaunchKernelGGL((adj_step_kernel<Real, UniformAdjParams>), dim3(grid), dim3(512), 0, c.stream,
+                                   ns, per_xcd, uk, uk1, c.adjLam[lc].as<double>(), c.adjLam[1 - lc].as<double>(),
+                                   c.statD.as<Stat4<Real>>(), c.statSig.as<Real>(), prm, c.adjPart.as<double>());
             GH_HIP_CHECK(hipGetLastError());
             lc = 1 - lc;
         }

@@ -69,10 +69,28 @@ struct UniformAdjParams {
     __device__ inline BlockPos pos(int s) const { return block_pos(s, 0, bpd, magic); }
 };
 
+// Domain test and outside flag of a cell with state u and record r.  Generic
+// records (CLAMPED false) carry tau, with the domain threshold in sig.  Clamped
+// records (stat_fast_kernel: every threshold <= 0, state u >= +0, i.e. any step
+// k >= 1) carry +-max(tau, 0) with the sign set where the threshold is < 0:
+// then u > sig holds there, and elsewhere it is u > 0; u < tau is
+// u < max(tau, 0).
+template <bool CLAMPED, typename Real>
+__device__ inline bool adj_in_domain(Real u, const Stat4<Real>& r, const Real* sig, size_t g) {
+    if (CLAMPED) return sign_of(r.tau) || u > (Real)0;
+    return u > sig[g];
+}
+template <bool CLAMPED, typename Real>
+__device__ inline bool adj_outside(Real u, const Stat4<Real>& r) {
+    if (CLAMPED) return u < fabs(r.tau);
+    return u < r.tau;
+}
+
 // One reverse step; one workgroup per block (XCD-aware order).
 // partial[ADJ_PART s + 3 w + q] accumulates wave w's parameter-gradient terms
-// of block s, step after step.
-template <typename Real, typename Params>
+// of block s, step after step.  CLAMPED: st holds the clamped-state records and
+// sig is not read (steps k >= 1 when every threshold is <= 0).
+template <typename Real, typename Params, bool CLAMPED = false>
 __global__ __launch_bounds__(512) void adj_step_kernel(int nslots, int per_xcd,
                                                        const Real* __restrict__ uk, const Real* __restrict__ uk1,
                                                        const double* __restrict__ lam1, double* __restrict__ lam0,
@@ -93,12 +111,11 @@ __global__ __launch_bounds__(512) void adj_step_kernel(int nslots, int per_xcd,
     const size_t g = (size_t)s * BCELLS + t;
     const Real uc = uk[g];
     const Stat4<Real> sv = st[g];
-    const Real sc = sig[g];
-    const bool domc = uc > sc;
+    const bool domc = adj_in_domain<CLAMPED>(uc, sv, sig, g);
     const bool mc = uk1[g] < (Real)1;
     const double l1c = lam1[g];
     s_u[Q(x, y, z)] = uc;
-    s_o[Q(x, y, z)] = uc < sv.tau;
+    s_o[Q(x, y, z)] = adj_outside<CLAMPED>(uc, sv);
     s_w[P(x, y, z)] = (mc && domc) ? l1c : 0.0;
     s_Dx[z * 72 + y * 9 + x + 1] = sv.dx;
     s_Dy[z * 72 + (y + 1) * 8 + x] = sv.dy;
@@ -130,10 +147,11 @@ __global__ __launch_bounds__(512) void adj_step_kernel(int nslots, int per_xcd,
     const HaloCell h0 = halo_cell(wv);
     const Real u_h0 = uk[h0.src];
     const Stat4<Real> s_h0 = st[h0.src];
-    Real sg_h0 = 0, u1_h0 = 0;
+    Real u1_h0 = 0;
     double l_h0 = 0;
+    bool dom_h0 = false;
     if (wv < 6) {  // depth 1
-        sg_h0 = sig[h0.src];
+        dom_h0 = adj_in_domain<CLAMPED>(u_h0, s_h0, sig, h0.src);
         u1_h0 = uk1[h0.src];
         l_h0 = lam1[h0.src];
     }
@@ -146,17 +164,17 @@ __global__ __launch_bounds__(512) void adj_step_kernel(int nslots, int per_xcd,
         s_h1 = st[h1.src];
     }
     s_u[h0.q] = u_h0;
-    s_o[h0.q] = u_h0 < s_h0.tau;
+    s_o[h0.q] = adj_outside<CLAMPED>(u_h0, s_h0);
     if (wv < 6) {
         const int a = t & 7, b = (t >> 3) & 7;
-        s_w[h0.p] = (u_h0 > sg_h0 && u1_h0 < (Real)1) ? l_h0 : 0.0;
+        s_w[h0.p] = (dom_h0 && u1_h0 < (Real)1) ? l_h0 : 0.0;
         if (wv == FXM) s_Dx[b * 72 + a * 9] = s_h0.dx;
         else if (wv == FYM) s_Dy[b * 72 + a] = s_h0.dy;
         else if (wv == FZM) s_Dz[b * 8 + a] = s_h0.dz;
     }
     if (wv < 4) {
         s_u[h1.q] = u_h1;
-        s_o[h1.q] = u_h1 < s_h1.tau;
+        s_o[h1.q] = adj_outside<CLAMPED>(u_h1, s_h1);
     }
     __syncthreads();
 

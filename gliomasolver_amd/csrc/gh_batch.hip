@@ -689,12 +689,18 @@ int adjoint_t(Batch& b, int K, const double* lik, int stepPET, int stepTi, int u
         for (int64_t k = k1 - 1; k >= k0; --k) {
             const int nslots = (int)b.slot0[active(k)];
             const int per_xcd = (nslots + 7) / 8;
-            hipLaunchKernelGGL((adj_step_kernel<Real, BatchAdjParams<Real>>), dim3(8 * per_xcd), dim3(512), 0,
-                               b.stream, nslots, per_xcd,
-                               (const Real*)(tape + (size_t)(k - k0) * nsl * slot_bytes),
-                               (const Real*)(tape + (size_t)(k - k0 + 1) * nsl * slot_bytes),
-                               b.lam[(k + 1) & 1].as<double>(), b.lam[k & 1].as<double>(), b.statD.as<Stat4<Real>>(),
-                               b.statSig.as<Real>(), aprm, b.apart.as<double>());
+            const Real* uk = (const Real*)(tape + (size_t)(k - k0) * nsl * slot_bytes);
+            const Real* uk1 = (const Real*)(tape + (size_t)(k - k0 + 1) * nsl * slot_bytes);
+            if (b.fast_ok && k >= 1)  // clamped states: the clamped-state records
+                hipLaunchKernelGGL((adj_step_kernel<Real, BatchAdjParams<Real>, true>), dim3(8 * per_xcd), dim3(512),
+                                   0, b.stream, nslots, per_xcd, uk, uk1, b.lam[(k + 1) & 1].as<double>(),
+                                   b.lam[k & 1].as<double>(), b.statF.as<Stat4<Real>>(), (const Real*)nullptr, aprm,
+                                   b.apart.as<double>());
+            else
+                hipLaunchKernelGGL((adj_step_kernel<Real, BatchAdjParams<Real>>), dim3(8 * per_xcd), dim3(512), 0,
+                                   b.stream, nslots, per_xcd, uk, uk1, b.lam[(k + 1) & 1].as<double>(),
+                                   b.lam[k & 1].as<double>(), b.statD.as<Stat4<Real>>(), b.statSig.as<Real>(), aprm,
+                                   b.apart.as<double>());
         }
         GH_HIP_CHECK(hipGetLastError());
     }
