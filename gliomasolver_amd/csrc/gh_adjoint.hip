@@ -18,9 +18,12 @@
 // (the face of neighbour e towards c is doubled when e's opposite neighbour,
 // c + 2 dir_i, is outside).  Parameter gradients accumulate per cell
 //   dJ/dDw += w_c dt Rdiff_c / Dw, dJ/drho += w_c dt u_c(1-u_c), dJ/dT += w_c R_c / N
-// in FP64 per wave; every wave owns the same cells in every step, so the
-// accumulation order is fixed.  States u_k are recomputed per segment
-// of K steps from checkpoints taken in the forward sweep.
+// with R_c = Rdiff_c + rho u_c (1 - u_c): all three are linear in the sums
+// A = sum w_c Rdiff_c and B = sum w_c u_c (1 - u_c), which are accumulated in
+// FP64 per wave (dt, rho, N and Dw are constant over the solve and applied
+// once at the end); every wave owns the same cells in every step, so the
+// accumulation order is fixed.  States u_k are recomputed per segment of K
+// steps from checkpoints taken in the forward sweep.
 #include <cmath>
 #include <vector>
 
@@ -117,7 +120,9 @@ int adjoint_t(Ctx& c, int64_t N, double T, int K, const double* lik, int stepPET
     GH_HIP_CHECK(hipMemcpyAsync(ip.data(), c.likPart.p, ip.size() * 8, hipMemcpyDeviceToHost, c.stream));
     GH_HIP_CHECK(hipStreamSynchronize(c.stream));
     for (int q = 0; q < 6; ++q) grad[q] = 0;
-    sum_slot_partials(pp.data(), ns, grad);
+    double ab[2];
+    sum_slot_partials(pp.data(), ns, ab);
+    adj_param_grads(ab, prm.dt, prm.invDw, prm.invN, prm.rho, grad);
     for (unsigned w = 0; w < igrid; ++w)
         for (int q = 0; q < 3; ++q) grad[3 + q] += ip[3 * w + q];
     return GH_OK;

@@ -11,9 +11,18 @@ namespace gh {
 
 __device__ inline int Q(int x, int y, int z) { return (z + 2) * 144 + (y + 2) * 12 + (x + 2); }  // [-2,10)^3
 
-// Doubles of parameter-gradient partials per slot: one (dDw, drho, dT) triple
-// per wave of the 512-lane workgroup.
-constexpr int ADJ_PART = 3 * 8;
+// Doubles of parameter-gradient partials per slot: per wave of the 512-lane
+// workgroup the two sums A = sum w_c Rdiff_c and B = sum w_c u_c (1 - u_c)
+// (Rdiff including ih2); the parameter gradients are linear in them
+// (adj_param_grads).
+constexpr int ADJ_PART = 2 * 8;
+
+// dJ/dDw = dt A / Dw, dJ/drho = dt B, dJ/dT = sum w_c R_c / N = (A + rho B) / N
+inline void adj_param_grads(const double ab[2], double dt, double invDw, double invN, double rho, double out[3]) {
+    out[0] = dt * ab[0] * invDw;
+    out[1] = dt * ab[1];
+    out[2] = (ab[0] + rho * ab[1]) * invN;
+}
 
 // v moved across lanes by the DPP pattern CTRL (both 32-bit halves)
 template <int CTRL>
@@ -86,9 +95,28 @@ __device__ inline bool adj_outside(Real u, const Stat4<Real>& r) {
     return u < r.tau;
 }
 
+// Face halo cell (a, b) of face f at depth d (1 or 2): the cell index in the
+// neighbour block across the face and the tile indices Q (state, flags) and,
+// at depth 1, P (w), each base + a * step_a + b * step_b with wave-uniform
+// bases and steps.
+struct AdjHalo {
+    int src, sa, sb, q, qa, qb, p, pa, pb;
+};
+__device__ inline AdjHalo adj_halo(int f, int d) {
+    const int q0 = Q(0, 0, 0), p0 = P(0, 0, 0);
+    switch (f) {
+        case FXM: return {8 - d, 8, 64, q0 - d, 12, 144, p0 - d, 10, 100};
+        case FXP: return {d - 1, 8, 64, q0 + 7 + d, 12, 144, p0 + 7 + d, 10, 100};
+        case FYM: return {8 * (8 - d), 1, 64, q0 - 12 * d, 1, 144, p0 - 10 * d, 1, 100};
+        case FYP: return {8 * (d - 1), 1, 64, q0 + 12 * (7 + d), 1, 144, p0 + 10 * (7 + d), 1, 100};
+        case FZM: return {64 * (8 - d), 1, 8, q0 - 144 * d, 1, 12, p0 - 100 * d, 1, 10};
+        default:  return {64 * (d - 1), 1, 8, q0 + 144 * (7 + d), 1, 12, p0 + 100 * (7 + d), 1, 10};
+    }
+}
+
 // One reverse step; one workgroup per block (XCD-aware order).
-// partial[ADJ_PART s + 3 w + q] accumulates wave w's parameter-gradient terms
-// of block s, step after step.  CLAMPED: st holds the clamped-state records and
+// partial[ADJ_PART s + 2 w + q] accumulates wave w's sums A (q = 0) and B
+// (q = 1) of block s, step after step.  CLAMPED: st holds the clamped-state records and
 // sig is not read (steps k >= 1 when every threshold is <= 0).
 template <typename Real, typename Params, bool CLAMPED = false>
 __global__ __launch_bounds__(512) void adj_step_kernel(int nslots, int per_xcd,
@@ -129,19 +157,10 @@ __global__ __launch_bounds__(512) void adj_step_kernel(int nslots, int per_xcd,
         int q, p;    // tile indices (state/flags, w)
     };
     auto halo_cell = [&](int hw) {
-        const int depth = hw / 6, f = hw % 6, a = t & 7, b = (t >> 3) & 7;
-        const int in1 = depth == 0 ? 7 : 6, in0 = depth == 0 ? 0 : 1;  // source plane in the neighbour
-        const int lo = depth == 0 ? -1 : -2, hi = depth == 0 ? 8 : 9;  // tile plane
-        int src, qx, qy, qz;
-        switch (f) {
-            case FXM: src = in1 + 8 * a + 64 * b; qx = lo; qy = a; qz = b; break;
-            case FXP: src = in0 + 8 * a + 64 * b; qx = hi; qy = a; qz = b; break;
-            case FYM: src = a + 8 * in1 + 64 * b; qx = a; qy = lo; qz = b; break;
-            case FYP: src = a + 8 * in0 + 64 * b; qx = a; qy = hi; qz = b; break;
-            case FZM: src = a + 8 * b + 64 * in1; qx = a; qy = b; qz = lo; break;
-            default:  src = a + 8 * b + 64 * in0; qx = a; qy = b; qz = hi; break;
-        }
-        return HaloCell{(size_t)face_nb(prm.pos(s), f) * BCELLS + src, Q(qx, qy, qz), depth == 0 ? P(qx, qy, qz) : 0};
+        const int f = hw % 6, d = hw / 6 + 1, a = t & 7, b = (t >> 3) & 7;
+        const AdjHalo m = adj_halo(f, d);
+        return HaloCell{(size_t)face_nb(prm.pos(s), f) * BCELLS + (size_t)mad24(b, m.sb, mad24(a, m.sa, m.src)),
+                        mad24(b, m.qb, mad24(a, m.qa, m.q)), mad24(b, m.pb, mad24(a, m.pa, m.p))};
     };
     const int wv = __builtin_amdgcn_readfirstlane(t >> 6);
     const HaloCell h0 = halo_cell(wv);
@@ -179,35 +198,32 @@ __global__ __launch_bounds__(512) void adj_step_kernel(int nslots, int per_xcd,
     __syncthreads();
 
     const double u0 = (double)uc;
-    const Real D[6] = {s_Dx[z * 72 + y * 9 + x], sv.dx, s_Dy[z * 72 + y * 8 + x],
-                       sv.dy, s_Dz[z * 64 + y * 8 + x], sv.dz};
+    const double D[6] = {(double)s_Dx[z * 72 + y * 9 + x], (double)sv.dx, (double)s_Dy[z * 72 + y * 8 + x],
+                         (double)sv.dy, (double)s_Dz[z * 64 + y * 8 + x], (double)sv.dz};
     const int nq[6] = {Q(x - 1, y, z), Q(x + 1, y, z), Q(x, y - 1, z), Q(x, y + 1, z), Q(x, y, z - 1), Q(x, y, z + 1)};
     const int np[6] = {P(x - 1, y, z), P(x + 1, y, z), P(x, y - 1, z), P(x, y + 1, z), P(x, y, z - 1), P(x, y, z + 1)};
     const int n2[6] = {Q(x - 2, y, z), Q(x + 2, y, z), Q(x, y - 2, z), Q(x, y + 2, z), Q(x, y, z - 2), Q(x, y, z + 2)};
-    double rdiff = 0, sumd = 0, acc = 0;
+    double sdu = 0, sumd = 0, acc = 0;
 #pragma unroll
     for (int i = 0; i < 6; ++i) {
-        // face coefficients doubled (exactly, in Real) where the opposite
-        // neighbour of c, or of the neighbour, is outside
-        const double dp = (double)(s_o[nq[i ^ 1]] ? D[i] + D[i] : D[i]);
-        rdiff = fma(dp, (double)s_u[nq[i]] - u0, rdiff);
+        // face coefficients doubled (exactly: ldexp by the 0/1 flag) where the
+        // opposite neighbour of c, or of the neighbour, is outside
+        const double dp = ldexp(D[i], (int)s_o[nq[i ^ 1]]);
+        sdu = fma(dp, (double)s_u[nq[i]], sdu);
         sumd += dp;
-        acc = fma(s_w[np[i]], (double)(s_o[n2[i]] ? D[i] + D[i] : D[i]), acc);
+        acc = fma(s_w[np[i]], ldexp(D[i], (int)s_o[n2[i]]), acc);
     }
-    rdiff *= ih2;
-    const double R = rdiff + rho * u0 * (1. - u0);
+    const double rdiff = fma(-sumd, u0, sdu) * ih2;  // ih2 sum_i d'_i (u_i - u_c)
     double self = 0.0;
     if (mc) self = domc ? l1c * (1. + dt * (-ih2 * sumd + rho * (1. - 2. * u0))) : l1c;
     lam0[g] = self + dt * ih2 * acc;
 
     const double wc = s_w[P(x, y, z)];
-    const double gd = wave_sum_dpp(wc * dt * rdiff * invDw), gr = wave_sum_dpp(wc * dt * u0 * (1. - u0)),
-                 gt = wave_sum_dpp(wc * R * invN);
+    const double A = wave_sum_dpp(wc * rdiff), B = wave_sum_dpp(wc * (u0 * (1. - u0)));
     if ((t & 63) == 0) {
-        double* pw = partial + ADJ_PART * (size_t)s + 3 * (t >> 6);
-        pw[0] += gd;
-        pw[1] += gr;
-        pw[2] += gt;
+        double* pw = partial + ADJ_PART * (size_t)s + 2 * (t >> 6);
+        pw[0] += A;
+        pw[1] += B;
     }
 }
 
@@ -243,14 +259,14 @@ __global__ __launch_bounds__(256) void adj_ic_partials(size_t nc, int bpd, const
 
 // The host's summation of the per-wave partials: workgroup order g = 0..8*per_xcd-1
 // of the step kernel's grid, slot = (g % 8) * per_xcd + g / 8, then wave order.
-inline void sum_slot_partials(const double* part, int nslots, double out[3]) {
+inline void sum_slot_partials(const double* part, int nslots, double ab[2]) {
     const int per_xcd = (nslots + 7) / 8;
-    out[0] = out[1] = out[2] = 0;
+    ab[0] = ab[1] = 0;
     for (int g = 0; g < 8 * per_xcd; ++g) {
         const int s = (g % 8) * per_xcd + g / 8;
         if (s >= nslots) continue;
         for (int w = 0; w < 8; ++w)
-            for (int q = 0; q < 3; ++q) out[q] += part[ADJ_PART * (size_t)s + 3 * w + q];
+            for (int q = 0; q < 2; ++q) ab[q] += part[ADJ_PART * (size_t)s + 2 * w + q];
     }
 }
 

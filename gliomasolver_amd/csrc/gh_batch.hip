@@ -730,7 +730,10 @@ int adjoint_t(Batch& b, int K, const double* lik, int stepPET, int stepTi, int u
     for (int p = 0; p < ns; ++p) {
         const int i = b.order[p];
         double* gi = grad + 6 * (size_t)i;
-        sum_slot_partials(ap.data() + ADJ_PART * b.slot0[p], (int)(b.slot0[p + 1] - b.slot0[p]), gi);
+        const BatchSample& q = b.smp[i];
+        double ab[2];
+        sum_slot_partials(ap.data() + ADJ_PART * b.slot0[p], (int)(b.slot0[p + 1] - b.slot0[p]), ab);
+        adj_param_grads(ab, q.dt, 1.0 / q.Dw, 1.0 / (double)q.nsteps, q.rho, gi);
         gi[3] = gi[4] = gi[5] = 0;
         for (int w = 0; w < IC_GRID; ++w)
             for (int qd = 0; qd < 3; ++qd) gi[3 + qd] += ip[((size_t)p * IC_GRID + w) * 3 + qd];
