@@ -68,6 +68,9 @@ int adjoint_t(Ctx& c, int64_t N, double T, int K, const double* lik, int stepPET
     int lc = 0;
     if ((rc = lik_seed_gradient(c, c.phi[c.cur].p, c.adjLam[lc].as<double>(), lik, stepPET, stepTi, use_roi)))
         return rc;
+    hipLaunchKernelGGL(adj_mask_kernel<Real>, dim3((unsigned)((nc + 255) / 256)), dim3(256), 0, c.stream, nc,
+                       c.adjLam[lc].as<double>(), (const Real*)c.phi[c.cur].p);
+    GH_HIP_CHECK(hipGetLastError());
     // reverse sweep
     const int ns = c.topo.nblocks();
     const int per_xcd = (ns + 7) / 8;
@@ -94,15 +97,15 @@ int adjoint_t(Ctx& c, int64_t N, double T, int K, const double* lik, int stepPET
         }
         for (int64_t k = k1 - 1; k >= k0; --k) {
             const Real* uk = (const Real*)(seg + (size_t)(k - k0) * nb);
-            const Real* uk1 = (const Real*)(seg + (size_t)(k - k0 + 1) * nb);
+            const int mask_out = k >= 1;
             if (c.fast_ok && k >= 1)  // u_k is a clamped state: the clamped-state records
                 hipLaunchKernelGGL((adj_step_kernel<Real, UniformAdjParams, true>), dim3(grid), dim3(512), 0,
-                                   c.stream, ns, per_xcd, uk, uk1, c.adjLam[lc].as<double>(),
+                                   c.stream, ns, per_xcd, uk, mask_out, c.adjLam[lc].as<double>(),
                                    c.adjLam[1 - lc].as<double>(), c.statF.as<Stat4<Real>>(), (const Real*)nullptr,
                                    prm, c.adjPart.as<double>());
             else
                 hipLaunchKernelGGL((adj_step_kernel<Real, UniformAdjParams>), dim3(grid), dim3(512), 0, c.stream,
-                                   ns, per_xcd, uk, uk1, c.adjLam[lc].as<double>(), c.adjLam[1 - lc].as<double>(),
+                                   ns, per_xcd, uk, mask_out, c.adjLam[lc].as<double>(), c.adjLam[1 - lc].as<double>(),
                                    c.statD.as<Stat4<Real>>(), c.statSig.as<Real>(), prm, c.adjPart.as<double>());
             GH_HIP_CHECK(hipGetLastError());
             lc = 1 - lc;

@@ -114,13 +114,30 @@ __device__ inline AdjHalo adj_halo(int f, int d) {
     }
 }
 
-// One reverse step; one workgroup per block (XCD-aware order).
+// lambda[c] = 0 where u[c] >= 1 (the upper clamp of the step that produced u
+// is active): the masked adjoint the reverse step reads.
+template <typename Real>
+__device__ inline void adj_mask_cell(size_t c, double* __restrict__ lam, const Real* __restrict__ u) {
+    if (!(u[c] < (Real)1)) lam[c] = 0.0;
+}
+template <typename Real>
+__global__ __launch_bounds__(256) void adj_mask_kernel(size_t n, double* __restrict__ lam,
+                                                       const Real* __restrict__ u) {
+    const size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c < n) adj_mask_cell(c, lam, u);
+}
+
+// One reverse step; one workgroup per block (XCD-aware order).  lam1 holds
+// m lambda_{k+1} with m = [u_{k+1} < 1] (the seed masked by adj_mask_kernel,
+// later steps by the previous launch); lam0 receives lambda_k, masked by
+// [u_k < 1] when mask_out (k >= 1: the next reverse step reads it), plain
+// at k = 0 (the initial-condition gradient reads it).
 // partial[ADJ_PART s + 2 w + q] accumulates wave w's sums A (q = 0) and B
 // (q = 1) of block s, step after step.  CLAMPED: st holds the clamped-state records and
 // sig is not read (steps k >= 1 when every threshold is <= 0).
 template <typename Real, typename Params, bool CLAMPED = false>
 __global__ __launch_bounds__(512) void adj_step_kernel(int nslots, int per_xcd,
-                                                       const Real* __restrict__ uk, const Real* __restrict__ uk1,
+                                                       const Real* __restrict__ uk, int mask_out,
                                                        const double* __restrict__ lam1, double* __restrict__ lam0,
                                                        const Stat4<Real>* __restrict__ st,
                                                        const Real* __restrict__ sig, Params prm,
@@ -140,11 +157,10 @@ __global__ __launch_bounds__(512) void adj_step_kernel(int nslots, int per_xcd,
     const Real uc = uk[g];
     const Stat4<Real> sv = st[g];
     const bool domc = adj_in_domain<CLAMPED>(uc, sv, sig, g);
-    const bool mc = uk1[g] < (Real)1;
     const double l1c = lam1[g];
     s_u[Q(x, y, z)] = uc;
     s_o[Q(x, y, z)] = adj_outside<CLAMPED>(uc, sv);
-    s_w[P(x, y, z)] = (mc && domc) ? l1c : 0.0;
+    s_w[P(x, y, z)] = domc ? l1c : 0.0;
     s_Dx[z * 72 + y * 9 + x + 1] = sv.dx;
     s_Dy[z * 72 + (y + 1) * 8 + x] = sv.dy;
     s_Dz[(z + 1) * 64 + y * 8 + x] = sv.dz;
@@ -166,12 +182,10 @@ __global__ __launch_bounds__(512) void adj_step_kernel(int nslots, int per_xcd,
     const HaloCell h0 = halo_cell(wv);
     const Real u_h0 = uk[h0.src];
     const Stat4<Real> s_h0 = st[h0.src];
-    Real u1_h0 = 0;
     double l_h0 = 0;
     bool dom_h0 = false;
     if (wv < 6) {  // depth 1
         dom_h0 = adj_in_domain<CLAMPED>(u_h0, s_h0, sig, h0.src);
-        u1_h0 = uk1[h0.src];
         l_h0 = lam1[h0.src];
     }
     HaloCell h1{0, 0, 0};
@@ -186,7 +200,7 @@ __global__ __launch_bounds__(512) void adj_step_kernel(int nslots, int per_xcd,
     s_o[h0.q] = adj_outside<CLAMPED>(u_h0, s_h0);
     if (wv < 6) {
         const int a = t & 7, b = (t >> 3) & 7;
-        s_w[h0.p] = (dom_h0 && u1_h0 < (Real)1) ? l_h0 : 0.0;
+        s_w[h0.p] = dom_h0 ? l_h0 : 0.0;
         if (wv == FXM) s_Dx[b * 72 + a * 9] = s_h0.dx;
         else if (wv == FYM) s_Dy[b * 72 + a] = s_h0.dy;
         else if (wv == FZM) s_Dz[b * 8 + a] = s_h0.dz;
@@ -214,9 +228,9 @@ __global__ __launch_bounds__(512) void adj_step_kernel(int nslots, int per_xcd,
         acc = fma(s_w[np[i]], ldexp(D[i], (int)s_o[n2[i]]), acc);
     }
     const double rdiff = fma(-sumd, u0, sdu) * ih2;  // ih2 sum_i d'_i (u_i - u_c)
-    double self = 0.0;
-    if (mc) self = domc ? l1c * (1. + dt * (-ih2 * sumd + rho * (1. - 2. * u0))) : l1c;
-    lam0[g] = self + dt * ih2 * acc;
+    const double self = domc ? l1c * (1. + dt * (-ih2 * sumd + rho * (1. - 2. * u0))) : l1c;
+    const double lk = self + dt * ih2 * acc;
+    lam0[g] = (mask_out && !(uc < (Real)1)) ? 0.0 : lk;
 
     const double wc = s_w[P(x, y, z)];
     const double A = wave_sum_dpp(wc * rdiff), B = wave_sum_dpp(wc * (u0 * (1. - u0)));

@@ -108,6 +108,14 @@ __global__ __launch_bounds__(256) void batch_seed_kernel(const AdjJob<Real>* __r
     lik_seed_cell<Real>(c, lay, j.u, j.t1, j.fl, j.pet, j.p, full_bern, j.lam);
 }
 
+// each sample's seed masked by [u_N < 1] (adj_mask_kernel per sample)
+template <typename Real>
+__global__ __launch_bounds__(256) void batch_mask_kernel(const AdjJob<Real>* __restrict__ jobs) {
+    const AdjJob<Real>& j = jobs[blockIdx.y];
+    const size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c < j.ncells) adj_mask_cell(c, j.lam, j.u);
+}
+
 template <typename Real>
 __global__ __launch_bounds__(256) void batch_ic_partials(const AdjJob<Real>* __restrict__ jobs,
                                                          double* __restrict__ partial) {
@@ -661,6 +669,10 @@ int adjoint_t(Batch& b, int K, const double* lik, int stepPET, int stepTi, int u
         }
         GH_HIP_CHECK(hipGetLastError());
     }
+    for (int p0 = 0; p0 < ns; p0 += 65535)
+        hipLaunchKernelGGL(batch_mask_kernel<Real>, dim3((unsigned)((maxc + 255) / 256), std::min(65535, ns - p0)),
+                           dim3(256), 0, b.stream, b.adjJobs.as<AdjJob<Real>>() + p0);
+    GH_HIP_CHECK(hipGetLastError());
     // reverse
     GH_HIP_CHECK(hipMemsetAsync(b.apart.p, 0, ADJ_PART * nsl * 8, b.stream));
     for (int64_t j = J - 1; j >= 0; --j) {
@@ -690,15 +702,15 @@ int adjoint_t(Batch& b, int K, const double* lik, int stepPET, int stepTi, int u
             const int nslots = (int)b.slot0[active(k)];
             const int per_xcd = (nslots + 7) / 8;
             const Real* uk = (const Real*)(tape + (size_t)(k - k0) * nsl * slot_bytes);
-            const Real* uk1 = (const Real*)(tape + (size_t)(k - k0 + 1) * nsl * slot_bytes);
+            const int mask_out = k >= 1;
             if (b.fast_ok && k >= 1)  // clamped states: the clamped-state records
                 hipLaunchKernelGGL((adj_step_kernel<Real, BatchAdjParams<Real>, true>), dim3(8 * per_xcd), dim3(512),
-                                   0, b.stream, nslots, per_xcd, uk, uk1, b.lam[(k + 1) & 1].as<double>(),
+                                   0, b.stream, nslots, per_xcd, uk, mask_out, b.lam[(k + 1) & 1].as<double>(),
                                    b.lam[k & 1].as<double>(), b.statF.as<Stat4<Real>>(), (const Real*)nullptr, aprm,
                                    b.apart.as<double>());
             else
                 hipLaunchKernelGGL((adj_step_kernel<Real, BatchAdjParams<Real>>), dim3(8 * per_xcd), dim3(512), 0,
-                                   b.stream, nslots, per_xcd, uk, uk1, b.lam[(k + 1) & 1].as<double>(),
+                                   b.stream, nslots, per_xcd, uk, mask_out, b.lam[(k + 1) & 1].as<double>(),
                                    b.lam[k & 1].as<double>(), b.statD.as<Stat4<Real>>(), b.statSig.as<Real>(), aprm,
                                    b.apart.as<double>());
         }
