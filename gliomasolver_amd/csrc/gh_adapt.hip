@@ -140,11 +140,11 @@ inline LevelMaps level_maps(const Ctx& c) {
 // Hs synthesis taps of fine point q (MRAGWavelets_StaticData.cpp:13-21,
 // 31-39): even q -> coarse q/2 with weight 1; odd q -> (q-1)/2 + k, k < 2,
 // weight 1/2 (order 2) or (q-3)/2 + k, k < 4, weights -1/16, 9/16, 9/16, -1/16
-__device__ inline int synth_n(int order, int64_t q) { return (q & 1) == 0 ? 1 : (order == 2 ? 2 : 4); }
-__device__ inline int64_t synth_src(int order, int64_t q, int k) {
+__device__ inline int synth_n(int order, int q) { return (q & 1) == 0 ? 1 : (order == 2 ? 2 : 4); }
+__device__ inline int synth_src(int order, int q, int k) {
     return (q & 1) == 0 ? q / 2 : (order == 2 ? (q - 1) / 2 + k : (q - 3) / 2 + k);
 }
-__device__ inline double synth_w(int order, int64_t q, int k) {
+__device__ inline double synth_w(int order, int q, int k) {
     return (q & 1) == 0 ? 1.0 : (order == 2 ? 0.5 : ((k == 0 || k == 3) ? -1.0 / 16 : 9.0 / 16));
 }
 
@@ -153,16 +153,16 @@ __device__ inline double synth_w(int order, int64_t q, int k) {
 // cell).  D more synthesis levels are allowed below this one; deeper
 // recursion sets bad.
 template <int D, typename Visit>
-__device__ inline void ghost_visit(const LevelMaps& m, int order, int L, int64_t qx, int64_t qy, int64_t qz, double w,
+__device__ inline void ghost_visit(const LevelMaps& m, int order, int L, int qx, int qy, int qz, double w,
                                    Visit& visit, int& bad) {
     for (;;) {
-        const int64_t mask = ((int64_t)BS << L) - 1;  // periodic wrap, power-of-two lattice
+        const int mask = (BS << L) - 1;  // periodic wrap, power-of-two lattice
         qx &= mask;
         qy &= mask;
         qz &= mask;
         if (L >= m.nlev) break;  // inside a coarser leaf
         const int bx = (int)(qx >> 3), by = (int)(qy >> 3), bz = (int)(qz >> 3);
-        const int32_t v = m.map[m.off[L] + bx + ((int64_t)by << L) + ((int64_t)bz << (2 * L))];
+        const int32_t v = m.map[m.off[L] + bx + (by << L) + (bz << (2 * L))];
         if (v >= 0) {
             visit((int32_t)(v * BCELLS + (qx & 7) + 8 * (qy & 7) + 64 * (qz & 7)), w);
             return;
@@ -215,7 +215,7 @@ __global__ __launch_bounds__(256) void face_count_kernel(int n, const int32_t* _
     face_point(f, p & 7, p >> 3, i);
     int cnt = 0, my_bad = 0;
     auto visit = [&](int32_t, double) { ++cnt; };
-    ghost_visit<2>(m, order, b.x, 8LL * b.y + i[0], 8LL * b.z + i[1], 8LL * b.w + i[2], 1.0, visit, my_bad);
+    ghost_visit<2>(m, order, b.x, 8 * b.y + i[0], 8 * b.z + i[1], 8 * b.w + i[2], 1.0, visit, my_bad);
     count[g] = cnt;
     if (my_bad) atomicOr(bad, 1);
 }
@@ -239,7 +239,7 @@ __global__ __launch_bounds__(256) void face_fill_kernel(int n, const int32_t* __
         tw[k] = (Real)w;
         ++k;
     };
-    ghost_visit<2>(m, order, b.x, 8LL * b.y + i[0], 8LL * b.z + i[1], 8LL * b.w + i[2], 1.0, visit, my_bad);
+    ghost_visit<2>(m, order, b.x, 8 * b.y + i[0], 8 * b.z + i[1], 8 * b.w + i[2], 1.0, visit, my_bad);
     ent[g] = GhostEntry{s * 384 + f * 64 + p, k0, k - k0};
 }
 
@@ -272,7 +272,7 @@ __global__ __launch_bounds__(256) void lab_kernel(int nlist, const int32_t* __re
                 const Real prod = f[src] * (Real)w;
                 acc = acc + prod;
             };
-            ghost_visit<2>(m, order, b.x, 8LL * b.y + ix, 8LL * b.z + iy, 8LL * b.w + iz, 1.0, visit, my_bad);
+            ghost_visit<2>(m, order, b.x, 8 * b.y + ix, 8 * b.z + iy, 8 * b.w + iz, 1.0, visit, my_bad);
             v = acc;
         }
         lab[(size_t)j * lab3 + p] = v;
@@ -350,21 +350,26 @@ __global__ __launch_bounds__(512) void remap_kernel(int nnew, const Remap* __res
     } else {
         // destination in the parent's doubled lattice
         const int d[3] = {8 * (r.octant & 1) + x, 8 * ((r.octant >> 1) & 1) + y, 8 * ((r.octant >> 2) & 1) + z};
-        // source range [S0, E0) (MRAGBlockSplitter.h:49-59)
+        // source range [S0, E0) (MRAGBlockSplitter.h:49-59); of it, source s
+        // contributes to destination d (0 <= d < 16) iff
+        // 2 s + 1 - he0 <= d < 2 s + 1 - hs0, i.e. s in [lo(d), hi(d)] below:
+        // the loops visit exactly the contributing sources, in increasing order
         const int S0 = (int)ceil(tp.hs0 * 0.5), E0 = 1 + (int)floor((tp.he0 - 1) * 0.5 + 8);
+        int lo[3], hi[3];
+        for (int a = 0; a < 3; ++a) {
+            lo[a] = max(S0, ((d[a] - 1 + tp.hs0) >> 1) + 1);  // arithmetic shift: floor division by 2
+            hi[a] = min(E0 - 1, (d[a] - 1 + tp.he0) >> 1);
+        }
         const Real* L = lab + (size_t)r.src * LAB * LAB * LAB;
         Real acc = (Real)0;
-        for (int sz = S0; sz < E0; ++sz) {
+        for (int sz = lo[2]; sz <= hi[2]; ++sz) {
             const int kz = 2 * sz - d[2];
-            if (!(d[2] >= max(0, 2 * sz + 1 - tp.he0) && d[2] < min(16, 2 * sz + 1 - tp.hs0))) continue;
             const double wZ = (double)tp.hs[kz - tp.hs0];
-            for (int sy = S0; sy < E0; ++sy) {
+            for (int sy = lo[1]; sy <= hi[1]; ++sy) {
                 const int ky = 2 * sy - d[1];
-                if (!(d[1] >= max(0, 2 * sy + 1 - tp.he0) && d[1] < min(16, 2 * sy + 1 - tp.hs0))) continue;
                 const double wZwY = wZ * (double)tp.hs[ky - tp.hs0];
-                for (int sx = S0; sx < E0; ++sx) {
+                for (int sx = lo[0]; sx <= hi[0]; ++sx) {
                     const int kx = 2 * sx - d[0];
-                    if (!(d[0] >= max(0, 2 * sx + 1 - tp.he0) && d[0] < min(16, 2 * sx + 1 - tp.hs0))) continue;
                     const Real w = (Real)(wZwY * (double)tp.hs[kx - tp.hs0]);
                     const Real prod = L[(sz - LS) * LAB * LAB + (sy - LS) * LAB + (sx - LS)] * w;
                     acc = acc + prod;
@@ -630,9 +635,9 @@ HostTable parallel_table(int n, F fill) {
 template <typename Real>
 int upload_table(Ctx& c, const HostTable& h, DevBuf& ent, DevBuf& src, DevBuf& w) {
     int rc;
-    if ((rc = ent.ensure(std::max<size_t>(1, h.ent.size()) * sizeof(GhostEntry))) ||
-        (rc = src.ensure(std::max<size_t>(1, h.src.size()) * sizeof(int32_t))) ||
-        (rc = w.ensure(std::max<size_t>(1, h.w.size()) * sizeof(Real))))
+    if ((rc = ent.ensure_grow(std::max<size_t>(1, h.ent.size()) * sizeof(GhostEntry))) ||
+        (rc = src.ensure_grow(std::max<size_t>(1, h.src.size()) * sizeof(int32_t))) ||
+        (rc = w.ensure_grow(std::max<size_t>(1, h.w.size()) * sizeof(Real))))
         return rc;
     std::vector<Real> wr(h.w.size());
     for (size_t i = 0; i < wr.size(); ++i) wr[i] = (Real)h.w[i];
@@ -720,7 +725,7 @@ int device_face_table(Ctx& c, bool& done) {
     const int order = c.tree.order;
     DevBuf dfaces, dcount;
     int rc;
-    if ((rc = dfaces.ensure(faces.size() * 4)) || (rc = dcount.ensure((size_t)n * 4))) return rc;
+    if ((rc = dfaces.ensure_grow(faces.size() * 4)) || (rc = dcount.ensure_grow((size_t)n * 4))) return rc;
     GH_HIP_CHECK(hipMemcpyAsync(dfaces.p, faces.data(), faces.size() * 4, hipMemcpyHostToDevice, c.stream));
     GH_HIP_CHECK(hipMemsetAsync(c.dBad.p, 0, sizeof(int), c.stream));
     const unsigned grid = (unsigned)((n + 255) / 256);
@@ -745,9 +750,9 @@ int device_face_table(Ctx& c, bool& done) {
     }
     if (total > INT32_MAX) return set_error(GH_EINVAL, "face ghost table: too many terms");
     const size_t rs = c.real_size;
-    if ((rc = c.faceEnt.ensure((size_t)n * sizeof(GhostEntry))) ||
-        (rc = c.faceSrc.ensure(std::max<int64_t>(1, total) * 4)) ||
-        (rc = c.faceW.ensure(std::max<int64_t>(1, total) * rs)))
+    if ((rc = c.faceEnt.ensure_grow((size_t)n * sizeof(GhostEntry))) ||
+        (rc = c.faceSrc.ensure_grow(std::max<int64_t>(1, total) * 4)) ||
+        (rc = c.faceW.ensure_grow(std::max<int64_t>(1, total) * rs)))
         return rc;
     GH_HIP_CHECK(hipMemcpyAsync(dcount.p, cnt.data(), (size_t)n * 4, hipMemcpyHostToDevice, c.stream));
     if (c.real_kind == GH_REAL_F32)
@@ -791,7 +796,7 @@ int apply_topology(Ctx& c) {
     c.topo = std::move(topo);
     int rc;
     if ((rc = c.upload_topology())) return rc;
-    if ((rc = c.dNbr27.ensure(n27.size() * 4)) || (rc = c.dLevel.ensure(lev.size() * 4))) return rc;
+    if ((rc = c.dNbr27.ensure_grow(n27.size() * 4)) || (rc = c.dLevel.ensure_grow(lev.size() * 4))) return rc;
     // level maps and block coordinates for the device-side lab ghosts
     std::vector<int32_t> dense;
     c.nDense = t.dense_levels;
@@ -802,8 +807,8 @@ int apply_topology(Ctx& c) {
     std::vector<int4> blk(nb);
     for (int s = 0; s < nb; ++s)
         blk[s] = make_int4(t.leaves[s].level, t.leaves[s].index[0], t.leaves[s].index[1], t.leaves[s].index[2]);
-    if ((rc = c.dDense.ensure(std::max<size_t>(1, dense.size()) * 4)) ||
-        (rc = c.dBlk.ensure(std::max<size_t>(1, blk.size()) * sizeof(int4))) || (rc = c.dBad.ensure(sizeof(int))))
+    if ((rc = c.dDense.ensure_grow(std::max<size_t>(1, dense.size()) * 4)) ||
+        (rc = c.dBlk.ensure_grow(std::max<size_t>(1, blk.size()) * sizeof(int4))) || (rc = c.dBad.ensure_grow(sizeof(int))))
         return rc;
     if (!dense.empty())
         GH_HIP_CHECK(hipMemcpyAsync(c.dDense.p, dense.data(), dense.size() * 4, hipMemcpyHostToDevice, c.stream));
@@ -821,9 +826,9 @@ int apply_topology(Ctx& c) {
         if (rc) return rc;
     }
     const size_t rs = c.real_size;
-    if ((rc = c.fgPhi.ensure(std::max(1, nb) * 384 * rs)) || (rc = c.fgTau.ensure(std::max(1, nb) * 384 * rs)) ||
-        (rc = c.fgDm.ensure(std::max(1, nb) * 192 * rs)) || (rc = c.fgPw.ensure(std::max(1, nb) * 384 * rs)) ||
-        (rc = c.fgPg.ensure(std::max(1, nb) * 384 * rs)))
+    if ((rc = c.fgPhi.ensure_grow(std::max(1, nb) * 384 * rs)) || (rc = c.fgTau.ensure_grow(std::max(1, nb) * 384 * rs)) ||
+        (rc = c.fgDm.ensure_grow(std::max(1, nb) * 192 * rs)) || (rc = c.fgPw.ensure_grow(std::max(1, nb) * 384 * rs)) ||
+        (rc = c.fgPg.ensure_grow(std::max(1, nb) * 384 * rs)))
         return rc;
     GH_HIP_CHECK(hipStreamSynchronize(c.stream));
     return GH_OK;
@@ -879,7 +884,7 @@ int build_labs(Ctx& c, int nslots, const DevBuf& dslots, const Real* field, int 
     const int order = c.tree.order;
     const int LS = lab_start(order), LAB = lab_size(order), lab3 = LAB * LAB * LAB;
     int rc;
-    if ((rc = out.ensure(std::max<size_t>(1, (size_t)nslots) * lab3 * sizeof(Real)))) return rc;
+    if ((rc = out.ensure_grow(std::max<size_t>(1, (size_t)nslots) * lab3 * sizeof(Real)))) return rc;
     if (nslots == 0) return GH_OK;
     hipLaunchKernelGGL(lab_fill_kernel<Real>, dim3((unsigned)nslots), dim3(256), 0, c.stream, nslots,
                        dslots.as<int32_t>(), LS, LAB, c.dNbr27.as<int32_t>(), field, out.as<Real>());
@@ -896,7 +901,7 @@ int build_labs(Ctx& c, int nslots, const DevBuf& dslots, const Real* field, int 
 template <typename Real>
 int build_labs_for(Ctx& c, const std::vector<int>& slots, const Real* field, const HostTable& tab, DevBuf& out) {
     int rc;
-    if ((rc = c.labSlots.ensure(std::max<size_t>(1, slots.size()) * 4))) return rc;
+    if ((rc = c.labSlots.ensure_grow(std::max<size_t>(1, slots.size()) * 4))) return rc;
     if (!slots.empty())
         GH_HIP_CHECK(hipMemcpyAsync(c.labSlots.p, slots.data(), slots.size() * 4, hipMemcpyHostToDevice, c.stream));
     return build_labs<Real>(c, (int)slots.size(), c.labSlots, field, (int)tab.ent.size(), c.labEnt, c.labSrc, c.labW,
@@ -934,11 +939,11 @@ int clear_bad(Ctx& c) {
     return GH_OK;
 }
 
-// Details of the listed slots, from the labs of every leaf: evaluated on the
-// device when per-level maps exist, else built from a ghost table cached per
-// tree version (an adaptation epoch that changes nothing rebuilds nothing on
-// the host); a lab does not depend on which other labs are built, so the
-// details equal those of labs built for the list alone.
+// Details of the listed slots: labs of those slots evaluated on the device
+// when per-level maps exist, else labs of every leaf from a ghost table cached
+// per tree version (an adaptation epoch that changes nothing rebuilds nothing
+// on the host); a lab does not depend on which other labs are built, so both
+// give the details of labs built for the list alone.
 template <typename Real>
 int fwt_details_t(Ctx& c, const std::vector<int>& slots, std::vector<double>& det) {
     det.assign(slots.size(), 0.0);
@@ -946,30 +951,41 @@ int fwt_details_t(Ctx& c, const std::vector<int>& slots, std::vector<double>& de
     const int order = c.tree.order;
     const int n = c.topo.nblocks();
     int rc;
-    bool on_device = false;
+    DevBuf& d = c.detBuf;
+    if ((rc = d.ensure_grow((size_t)n * sizeof(double)))) return rc;
     if (c.nDense > 0) {
-        if ((rc = clear_bad(c)) || (rc = device_labs<Real>(c, n, nullptr, c.phi[c.cur].as<Real>(), c.labBuf)))
-            return rc;
-        on_device = device_labs_ok(c);
+        // labs and details of the listed slots only; the recursion check is
+        // read back together with the details (one synchronization)
+        const int m = (int)slots.size();
+        if ((rc = c.labSlots.ensure_grow((size_t)m * 4)) || (rc = clear_bad(c))) return rc;
+        GH_HIP_CHECK(hipMemcpyAsync(c.labSlots.p, slots.data(), (size_t)m * 4, hipMemcpyHostToDevice, c.stream));
+        if ((rc = device_labs<Real>(c, m, c.labSlots.as<int32_t>(), c.phi[c.cur].as<Real>(), c.labBuf))) return rc;
+        hipLaunchKernelGGL(fwt_kernel<Real>, dim3((unsigned)m), dim3(512), 0, c.stream, m, lab_start(order),
+                           lab_size(order), taps<Real>(order), c.labBuf.as<Real>(), d.as<double>());
+        int bad = 1;
+        hipError_t e = hipGetLastError();
+        if (e == hipSuccess) e = hipMemcpyAsync(det.data(), d.p, (size_t)m * 8, hipMemcpyDeviceToHost, c.stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(&bad, c.dBad.p, sizeof(int), hipMemcpyDeviceToHost, c.stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(c.stream);
+        if (e != hipSuccess) return set_error(GH_EHIP, std::string("fwt: ") + hipGetErrorString(e));
+        if (bad == 0) return GH_OK;
     }
+    // host-built lab table of every slot (cached per tree version)
     std::vector<int> all;  // source of an async upload: lives until the sync below
-    if (!on_device && c.labAllVersion != c.tree_version) {
+    if (c.labAllVersion != c.tree_version) {
         all.resize(n);
         for (int i = 0; i < n; ++i) all[i] = i;
         HostTable tab = build_lab_table(c.tree, all, c.topo.nbr27, order);
         if ((rc = upload_table<Real>(c, tab, c.labAllEnt, c.labAllSrc, c.labAllW)) ||
-            (rc = c.labAllSlots.ensure(std::max<size_t>(1, all.size()) * 4)))
+            (rc = c.labAllSlots.ensure_grow(std::max<size_t>(1, all.size()) * 4)))
             return rc;
         GH_HIP_CHECK(hipMemcpyAsync(c.labAllSlots.p, all.data(), all.size() * 4, hipMemcpyHostToDevice, c.stream));
         c.nLabAllEnt = (int)tab.ent.size();
         c.labAllVersion = c.tree_version;
     }
-    if (!on_device &&
-        (rc = build_labs<Real>(c, n, c.labAllSlots, c.phi[c.cur].as<Real>(), c.nLabAllEnt, c.labAllEnt, c.labAllSrc,
+    if ((rc = build_labs<Real>(c, n, c.labAllSlots, c.phi[c.cur].as<Real>(), c.nLabAllEnt, c.labAllEnt, c.labAllSrc,
                                c.labAllW, c.labBuf)))
         return rc;
-    DevBuf& d = c.detBuf;
-    if ((rc = d.ensure_grow((size_t)n * sizeof(double)))) return rc;
     hipLaunchKernelGGL(fwt_kernel<Real>, dim3((unsigned)n), dim3(512), 0, c.stream, n, lab_start(order),
                        lab_size(order), taps<Real>(order), c.labBuf.as<Real>(), d.as<double>());
     std::vector<double> all_det(n);

@@ -7,10 +7,8 @@
 
 namespace gh {
 
-static inline int wrapi(int64_t v, int64_t n) {
-    int64_t r = v % n;
-    return (int)(r < 0 ? r + n : r);
-}
+// periodic wrap into [0, n) for a power-of-two n (every lattice here is one)
+static inline int wrapi(int64_t v, int64_t n) { return (int)(v & (n - 1)); }
 
 Octree::Kind Octree::kind(int L, int x, int y, int z, int* slot) const {
     const int n = 1 << L;
@@ -213,20 +211,85 @@ void Octree::value_terms(int L, const int64_t q_in[3], double w, std::vector<Gho
             }
 }
 
+// Region of the 26-neighbourhood of leaf s in direction code: its index P at
+// the leaf's level (wrapped periodically), whether the wrap was needed, and
+// its kind (a leaf of the same level, subdivided, or inside a coarser leaf).
+Octree::Kind Octree::neighbour_region(int s, int code, int P[3], bool& wrapped, int* slot) const {
+    const BlockRec& b = leaves[s];
+    const int L = b.level, n = 1 << L;
+    const int d[3] = {code % 3 - 1, (code / 3) % 3 - 1, code / 9 - 1};
+    wrapped = false;
+    for (int a = 0; a < 3; ++a) {
+        P[a] = b.index[a] + d[a];
+        if (P[a] < 0 || P[a] >= n) wrapped = true;
+        P[a] = wrapi(P[a], n);
+    }
+    return kind(L, P[0], P[1], P[2], slot);
+}
+
+// A subdivided neighbour region always holds finer leaves that touch the
+// block (the leaves tiling it reach its boundary), and only a subdivided
+// region holds leaves finer than the block: so s has a finer neighbour
+// reached without the periodic wrap iff some unwrapped direction is FINER.
+bool Octree::has_finer_direct_neighbour(int s) const {
+    for (int code = 0; code < 27; ++code) {
+        if (code == 13) continue;
+        int P[3];
+        bool wrapped;
+        if (neighbour_region(s, code, P, wrapped) == FINER && !wrapped) return true;
+    }
+    return false;
+}
+
+// Every neighbour of leaf s (periodic images included) is within one level of
+// L, for s at level L + 1: a subdivided region holds leaves of level >= L + 2;
+// a coarser region lies in the leaf found walking up, which must be at level
+// >= L - 1 (the walk of neighbours()).
+bool Octree::neighbours_within_one_of(int s, int L) const {
+    const int K = leaves[s].level;
+    for (int code = 0; code < 27; ++code) {
+        if (code == 13) continue;
+        int P[3];
+        bool wrapped;
+        const Kind k = neighbour_region(s, code, P, wrapped);
+        if (k == LEAF) {
+            if (std::abs(K - L) > 1) return false;
+        } else if (k == FINER) {
+            return false;
+        } else {
+            for (int l = K - 1; l >= 0; --l) {
+                const int sh = K - l;
+                if (kind(l, P[0] >> sh, P[1] >> sh, P[2] >> sh) == LEAF) {
+                    if (std::abs(l - L) > 1) return false;
+                    break;
+                }
+            }
+        }
+    }
+    return true;
+}
+
 std::vector<int> Octree::refinement_victims(const std::vector<char>& flagged) const {
     const int nl = (int)leaves.size();
     std::vector<char> flag(flagged.begin(), flagged.end());
     flag.resize(nl, 0);
-    std::vector<std::vector<Nbr>> nb(nl);
-    for (int s = 0; s < nl; ++s) nb[s] = neighbours(s);
     // space extension: a leaf with a finer non-periodic-image neighbour is refined
-    // (MRAGRefiner_SpaceExtension.h:35-51)
-    for (int s = 0; s < nl; ++s)
-        for (const Nbr& n : nb[s])
-            if (!n.wrapped && leaves[n.slot].level > leaves[s].level) {
-                flag[s] = 1;
-                break;
-            }
+    // (MRAGRefiner_SpaceExtension.h:35-51).  Such a neighbour lies in a
+    // subdivided region of the leaf's level next to it without the wrap
+    // (has_finer_direct_neighbour), so the leaves to flag are the same-level
+    // leaves next to each interior node, found from the interior nodes.
+    for (uint64_t k : interior) {
+        const int L = (int)(k >> 48), n = 1 << L;
+        const int I[3] = {(int)((k >> 32) & 0xffff), (int)((k >> 16) & 0xffff), (int)(k & 0xffff)};
+        for (int code = 0; code < 27; ++code) {
+            if (code == 13) continue;
+            const int d[3] = {code % 3 - 1, (code / 3) % 3 - 1, code / 9 - 1};
+            const int P[3] = {I[0] + d[0], I[1] + d[1], I[2] + d[2]};
+            if (P[0] < 0 || P[0] >= n || P[1] < 0 || P[1] >= n || P[2] < 0 || P[2] >= n) continue;  // wrapped
+            int slot;
+            if (kind(L, P[0], P[1], P[2], &slot) == LEAF) flag[slot] = 1;
+        }
+    }
     // nothing at the maximum level is refined (:54-63)
     for (int s = 0; s < nl; ++s)
         if (leaves[s].level >= max_level) flag[s] = 0;
@@ -240,7 +303,7 @@ std::vector<int> Octree::refinement_victims(const std::vector<char>& flagged) co
         std::set<int> next;
         for (int v : front) {
             const int new_level = leaves[v].level + 1;
-            for (const Nbr& n : nb[v])
+            for (const Nbr& n : neighbours(v))
                 if (std::abs(leaves[n.slot].level - new_level) > 1 && !victims.count(n.slot)) {
                     victims.insert(n.slot);
                     next.insert(n.slot);
@@ -271,11 +334,7 @@ std::vector<uint64_t> Octree::collapse_parents(const std::vector<char>& flagged)
         }
         // every neighbour of every child within one level of the parent (MRAGCompressor.h:95-107)
         for (int o = 0; o < 8 && ok; ++o)
-            for (const Nbr& n : neighbours(kids[o]))
-                if (std::abs(leaves[n.slot].level - L) > 1) {
-                    ok = false;
-                    break;
-                }
+            if (!neighbours_within_one_of(kids[o], L)) ok = false;
         if (ok) out.push_back(pk);
     }
     return out;
