@@ -250,15 +250,16 @@ template <typename Real>
 __global__ __launch_bounds__(256) void lab_kernel(int nlist, const int32_t* __restrict__ slots, int LS, int LAB,
                                                   int order, const int32_t* __restrict__ nbr27,
                                                   const int4* __restrict__ blk, LevelMaps m,
-                                                  const Real* __restrict__ f, Real* __restrict__ lab,
-                                                  int* __restrict__ bad) {
+                                                  const int16_t* __restrict__ perm, const Real* __restrict__ f,
+                                                  Real* __restrict__ lab, int* __restrict__ bad) {
     const int j = blockIdx.x;
     if (j >= nlist) return;
     const int s = slots ? slots[j] : j;
     const int4 b = blk[s];
     const int lab3 = LAB * LAB * LAB;
     int my_bad = 0;
-    for (int p = threadIdx.x; p < lab3; p += blockDim.x) {
+    for (int i = threadIdx.x; i < lab3; i += blockDim.x) {
+        const int p = perm[i];  // lab cells in lab_order(): waves see uniform work
         const int ix = p % LAB + LS, iy = (p / LAB) % LAB + LS, iz = p / (LAB * LAB) + LS;
         const int code = dir_code(ix, iy, iz);
         const int src = code == 13 ? s : nbr27[27 * s + code];
@@ -293,7 +294,9 @@ __global__ __launch_bounds__(512) void fwt_kernel(int nlist, int LS, int LAB, WT
     const int lab3 = LAB * LAB * LAB;
     for (int p = threadIdx.x; p < lab3; p += blockDim.x) s_lab[p] = lab[(size_t)j * lab3 + p];
     __syncthreads();
-    const int code = threadIdx.x & 7, pt = threadIdx.x >> 3;
+    // one wavelet code per wave (wave-uniform trip counts: 1 to 343 terms),
+    // one coarse point per lane
+    const int code = threadIdx.x >> 6, pt = threadIdx.x & 63;
     const int dx = pt & 3, dy = (pt >> 2) & 3, dz = pt >> 4;
     const int bw[3] = {code & 1, (code >> 1) & 1, (code >> 2) & 1};
     const int o[3] = {2 * dx, 2 * dy, 2 * dz};
@@ -908,6 +911,34 @@ int build_labs_for(Ctx& c, const std::vector<int>& slots, const Real* field, con
                             out);
 }
 
+// The order in which lab_kernel visits the lab cells: by region of the
+// 27-neighbourhood (same-level copies, finer and coarser neighbours differ
+// per region), then by the parity of the cell's coordinates (a ghost inside a
+// coarser leaf sums 1, 2, 4, ... 64 terms by parity), then by position; so
+// the lanes of a wave mostly take the same path with the same trip counts.
+int lab_order(Ctx& c, int order) {
+    if (c.labPermOrder == order) return GH_OK;
+    const int LS = lab_start(order), LAB = lab_size(order), lab3 = LAB * LAB * LAB;
+    std::vector<std::pair<int, int>> key(lab3);
+    for (int p = 0; p < lab3; ++p) {
+        const int i[3] = {p % LAB + LS, (p / LAB) % LAB + LS, p / (LAB * LAB) + LS};
+        int code = 0, par = 0;
+        for (int a = 2; a >= 0; --a) code = 3 * code + (i[a] < 0 ? 0 : (i[a] < 8 ? 1 : 2));
+        for (int a = 0; a < 3; ++a) par |= (i[a] & 1) << a;
+        key[p] = {code * 8 + par, p};
+    }
+    std::sort(key.begin(), key.end());
+    std::vector<int16_t> perm(lab3);
+    for (int q = 0; q < lab3; ++q) perm[q] = (int16_t)key[q].second;
+    int rc;
+    if ((rc = c.labPerm.ensure((size_t)lab3 * sizeof(int16_t)))) return rc;
+    GH_HIP_CHECK(hipMemcpyAsync(c.labPerm.p, perm.data(), perm.size() * sizeof(int16_t), hipMemcpyHostToDevice,
+                                c.stream));
+    GH_HIP_CHECK(hipStreamSynchronize(c.stream));  // perm is read by the copy
+    c.labPermOrder = order;
+    return GH_OK;
+}
+
 // Labs of nlist slots (dslots on the device, or nullptr: slots 0..nlist-1)
 // with the ghosts evaluated on the device; clears and then may set c.dBad
 // (a ghost deeper than the kernel's recursion: use host tables instead).
@@ -918,9 +949,11 @@ int device_labs(Ctx& c, int nlist, const int32_t* dslots, const Real* field, Dev
     int rc;
     if ((rc = out.ensure_grow(std::max<size_t>(1, (size_t)nlist) * lab3 * sizeof(Real)))) return rc;
     if (nlist == 0) return GH_OK;
+    if ((rc = lab_order(c, order))) return rc;
     const LevelMaps m = level_maps(c);
     hipLaunchKernelGGL(lab_kernel<Real>, dim3((unsigned)nlist), dim3(256), 0, c.stream, nlist, dslots, LS, LAB, order,
-                       c.dNbr27.as<int32_t>(), c.dBlk.as<int4>(), m, field, out.as<Real>(), c.dBad.as<int>());
+                       c.dNbr27.as<int32_t>(), c.dBlk.as<int4>(), m, c.labPerm.as<int16_t>(), field, out.as<Real>(),
+                       c.dBad.as<int>());
     GH_HIP_CHECK(hipGetLastError());
     return GH_OK;
 }

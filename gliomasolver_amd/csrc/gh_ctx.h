@@ -46,6 +46,8 @@ struct Ctx {
     DevBuf fgPw, fgPg;               // per slot: 6*64 ghost p_w, p_g (statics)
     // FWT / split labs (compact, one per listed slot)
     DevBuf labBuf, labSlots, labEnt, labSrc, labW;
+    DevBuf labPerm;        // lab_kernel's cell order (int16, lab_order)
+    int labPermOrder = 0;  // wavelet order labPerm was built for (0: none)
     // ghost table of the labs of every leaf, valid while tree_version == labAllVersion
     // (host-table path, used when the level maps below are not available)
     DevBuf labAllEnt, labAllSrc, labAllW, labAllSlots;

@@ -61,13 +61,14 @@ int Ctx::alloc_fields() {
 
 void Ctx::release_all() {
     DevBuf* bufs[] = {&dLevel,   &dNbr27,  &faceEnt,    &faceSrc,   &faceW,     &fgPhi,       &fgTau,   &fgDm,
-                      &fgPw,     &fgPg,    &labBuf,     &labSlots,  &labEnt,    &labSrc,      &labW,    &labAllEnt,
+                      &fgPw,     &fgPg,    &labBuf,     &labSlots,  &labEnt,    &labSrc,      &labW,    &labPerm,  &labAllEnt,
                       &labAllSrc, &labAllW, &labAllSlots, &dDense, &dBlk, &dBad, &spare[0], &spare[1], &spare[2], &spare[3],
                       &labSplit[0], &labSplit[1], &labSplit[2], &labSplit[3], &remapBuf, &detBuf, &phi[0],  &phi[1],    &dphidt,      &pw,      &pg,
                       &pcsf,     &statD,   &statSig,    &statF,     &flag,      &nbr,         &obsT1,   &obsFl,
                       &obsPet,   &obsRoi,  &likPart,    &adjCk,     &adjSeg,    &adjLam[0],   &adjLam[1], &adjPart};
     for (DevBuf* b : bufs) b->release();
     for (DevBuf& b : ckpt) b.release();
+    labPermOrder = 0;
 }
 
 int Ctx::upload_topology() {
