@@ -114,6 +114,33 @@ __device__ inline AdjHalo adj_halo(int f, int d) {
     }
 }
 
+// One cell of one reverse step: lambda_k (lk) and ih2 sum_i d'_i (u_i - u_c)
+// (rdiff) from the face coefficients D (-x, +x, -y, +y, -z, +z), per face i
+// the outside flags of the opposite neighbour (oo: face i ^ 1's neighbour) and
+// of the cell two steps out (of), the neighbour state un and weight wn, and
+// the cell's state u0, masked adjoint l1 and domain test.
+// nb(i, oo, of, un, wn) reads face i's neighbour data where it is used.
+template <typename Nb>
+__device__ inline void adj_cell(const double D[6], Nb nb, double u0, double l1, bool dom, double rho, double ih2,
+                                double dt, double& lk, double& rdiff) {
+    double sdu = 0, sumd = 0, acc = 0;
+#pragma unroll
+    for (int i = 0; i < 6; ++i) {
+        int oo, of;
+        double un, wn;
+        nb(i, oo, of, un, wn);
+        // face coefficients doubled (exactly: ldexp by the 0/1 flag) where the
+        // opposite neighbour of c, or of the neighbour, is outside
+        const double dp = ldexp(D[i], oo);
+        sdu = fma(dp, un, sdu);
+        sumd += dp;
+        acc = fma(wn, ldexp(D[i], of), acc);
+    }
+    rdiff = fma(-sumd, u0, sdu) * ih2;  // ih2 sum_i d'_i (u_i - u_c)
+    const double self = dom ? l1 * (1. + dt * (-ih2 * sumd + rho * (1. - 2. * u0))) : l1;
+    lk = self + dt * ih2 * acc;
+}
+
 // lambda[c] = 0 where u[c] >= 1 (the upper clamp of the step that produced u
 // is active): the masked adjoint the reverse step reads.
 template <typename Real>
@@ -217,19 +244,16 @@ __global__ __launch_bounds__(512) void adj_step_kernel(int nslots, int per_xcd,
     const int nq[6] = {Q(x - 1, y, z), Q(x + 1, y, z), Q(x, y - 1, z), Q(x, y + 1, z), Q(x, y, z - 1), Q(x, y, z + 1)};
     const int np[6] = {P(x - 1, y, z), P(x + 1, y, z), P(x, y - 1, z), P(x, y + 1, z), P(x, y, z - 1), P(x, y, z + 1)};
     const int n2[6] = {Q(x - 2, y, z), Q(x + 2, y, z), Q(x, y - 2, z), Q(x, y + 2, z), Q(x, y, z - 2), Q(x, y, z + 2)};
-    double sdu = 0, sumd = 0, acc = 0;
-#pragma unroll
-    for (int i = 0; i < 6; ++i) {
-        // face coefficients doubled (exactly: ldexp by the 0/1 flag) where the
-        // opposite neighbour of c, or of the neighbour, is outside
-        const double dp = ldexp(D[i], (int)s_o[nq[i ^ 1]]);
-        sdu = fma(dp, (double)s_u[nq[i]], sdu);
-        sumd += dp;
-        acc = fma(s_w[np[i]], ldexp(D[i], (int)s_o[n2[i]]), acc);
-    }
-    const double rdiff = fma(-sumd, u0, sdu) * ih2;  // ih2 sum_i d'_i (u_i - u_c)
-    const double self = domc ? l1c * (1. + dt * (-ih2 * sumd + rho * (1. - 2. * u0))) : l1c;
-    const double lk = self + dt * ih2 * acc;
+    double lk, rdiff;
+    adj_cell(
+        D,
+        [&](int i, int& oo, int& of, double& un, double& wn) {
+            oo = s_o[nq[i ^ 1]];
+            of = s_o[n2[i]];
+            un = (double)s_u[nq[i]];
+            wn = s_w[np[i]];
+        },
+        u0, l1c, domc, rho, ih2, dt, lk, rdiff);
     lam0[g] = (mask_out && !(uc < (Real)1)) ? 0.0 : lk;
 
     const double wc = s_w[P(x, y, z)];
