@@ -1114,10 +1114,9 @@ int grid_refine(Ctx& c, const std::vector<char>& flagged, int* nrefined) {
     std::vector<int> victims = t.refinement_victims(flagged);
     if (nrefined) *nrefined = (int)victims.size();
     if (victims.empty()) return GH_OK;
-    Octree nt = t;
+    Octree nt = t.shell();
     std::vector<char> is_victim(t.leaves.size(), 0);
     for (int v : victims) is_victim[v] = 1;
-    nt.leaves.clear();
     std::unordered_map<uint64_t, int> parent_of;  // child key -> compact parent index
     std::unordered_map<uint64_t, int> octant_of;
     for (int s = 0; s < (int)t.leaves.size(); ++s) {
@@ -1166,7 +1165,7 @@ int grid_compress(Ctx& c, const std::vector<char>& flagged, int* ncollapsed) {
     std::vector<uint64_t> parents = t.collapse_parents(flagged);
     if (ncollapsed) *ncollapsed = (int)parents.size();
     if (parents.empty()) return GH_OK;
-    Octree nt = t;
+    Octree nt = t.shell();
     std::unordered_map<uint64_t, std::array<int, 8>> kids;
     std::unordered_set<int> gone;
     for (uint64_t pk : parents) {
@@ -1182,7 +1181,6 @@ int grid_compress(Ctx& c, const std::vector<char>& flagged, int* ncollapsed) {
         kids[pk] = k;
         nt.interior.erase(pk);
     }
-    nt.leaves.clear();
     for (int s = 0; s < (int)t.leaves.size(); ++s)
         if (!gone.count(s)) nt.leaves.push_back(t.leaves[s]);
     for (uint64_t pk : parents)

@@ -57,6 +57,18 @@ struct Octree {
     // subdivided node, or nothing (a coarser leaf covers it)
     Kind kind(int L, int x, int y, int z, int* slot = nullptr) const;
 
+    // a copy of the configuration and the node sets without the index
+    // (leaf_slot, dense maps): for building a new tree that rebuild_index
+    // then indexes
+    Octree shell() const {
+        Octree o;
+        o.order = order;
+        o.max_level = max_level;
+        o.min_level = min_level;
+        o.interior = interior;
+        return o;
+    }
+
     void init_uniform(int level);
     void rebuild_index();  // canonical slot order (level, z, y, x) + maps; call after any change of `interior`
     int levels_min() const;
