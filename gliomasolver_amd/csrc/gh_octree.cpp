@@ -269,6 +269,57 @@ bool Octree::neighbours_within_one_of(int s, int L) const {
     return true;
 }
 
+// leaves of the subtree at (L, P) that touch the side facing -d and are at
+// level >= min_level (touching_children's walk, filtered)
+static void touching_deep(const Octree& t, int L, const int P[3], const int d[3], int min_level, std::vector<int>& out) {
+    int slot;
+    const Octree::Kind k = t.kind(L, P[0], P[1], P[2], &slot);
+    if (k == Octree::LEAF) {
+        if (L >= min_level) out.push_back(slot);
+        return;
+    }
+    if (k != Octree::FINER) return;
+    for (int o = 0; o < 8; ++o) {
+        const int off[3] = {o & 1, (o >> 1) & 1, (o >> 2) & 1};
+        bool ok = true;
+        for (int a = 0; a < 3; ++a) {
+            if (d[a] < 0 && off[a] != 1) ok = false;
+            if (d[a] > 0 && off[a] != 0) ok = false;
+        }
+        if (ok) {
+            const int C[3] = {2 * P[0] + off[0], 2 * P[1] + off[1], 2 * P[2] + off[2]};
+            touching_deep(t, L + 1, C, d, min_level, out);
+        }
+    }
+}
+
+// The neighbours of leaf v (periodic images included) whose level differs by
+// more than one from L + 1, L = v's level: the chain reaction's targets when v
+// is refined.  A same-level leaf differs by one; a coarser leaf (level <= L - 1)
+// always qualifies; inside a finer region only leaves of level >= L + 3 do.
+void Octree::chain_targets(int v, std::vector<int>& out) const {
+    const int L = leaves[v].level;
+    for (int code = 0; code < 27; ++code) {
+        if (code == 13) continue;
+        const int d[3] = {code % 3 - 1, (code / 3) % 3 - 1, code / 9 - 1};
+        int P[3], slot;
+        bool wrapped;
+        const Kind k = neighbour_region(v, code, P, wrapped, &slot);
+        if (k == LEAF) continue;
+        if (k == FINER) {
+            touching_deep(*this, L, P, d, L + 3, out);
+            continue;
+        }
+        for (int l = L - 1; l >= 0; --l) {
+            const int sh = L - l;
+            if (kind(l, P[0] >> sh, P[1] >> sh, P[2] >> sh, &slot) == LEAF) {
+                out.push_back(slot);
+                break;
+            }
+        }
+    }
+}
+
 std::vector<int> Octree::refinement_victims(const std::vector<char>& flagged) const {
     const int nl = (int)leaves.size();
     std::vector<char> flag(flagged.begin(), flagged.end());
@@ -295,23 +346,32 @@ std::vector<int> Octree::refinement_victims(const std::vector<char>& flagged) co
         if (leaves[s].level >= max_level) flag[s] = 0;
     // chain reaction: neighbours that would end up more than one level coarser
     // than a refined block are refined too (MRAGRefiner.h:65-152)
-    std::set<int> victims;
+    // (the targets of a block depend on the tree only, so the closure does not
+    // depend on the order the front is processed in)
+    std::vector<char> victim(nl, 0);
+    std::vector<int> front, next, tg;
     for (int s = 0; s < nl; ++s)
-        if (flag[s]) victims.insert(s);
-    std::set<int> front = victims;
+        if (flag[s]) {
+            victim[s] = 1;
+            front.push_back(s);
+        }
     while (!front.empty()) {
-        std::set<int> next;
+        next.clear();
         for (int v : front) {
-            const int new_level = leaves[v].level + 1;
-            for (const Nbr& n : neighbours(v))
-                if (std::abs(leaves[n.slot].level - new_level) > 1 && !victims.count(n.slot)) {
-                    victims.insert(n.slot);
-                    next.insert(n.slot);
+            tg.clear();
+            chain_targets(v, tg);
+            for (int n : tg)
+                if (!victim[n]) {
+                    victim[n] = 1;
+                    next.push_back(n);
                 }
         }
         front.swap(next);
     }
-    return std::vector<int>(victims.begin(), victims.end());
+    std::vector<int> out;
+    for (int s = 0; s < nl; ++s)
+        if (victim[s]) out.push_back(s);
+    return out;
 }
 
 std::vector<uint64_t> Octree::collapse_parents(const std::vector<char>& flagged) const {

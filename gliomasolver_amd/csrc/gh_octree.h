@@ -86,6 +86,7 @@ struct Octree {
     Kind neighbour_region(int slot, int code, int P[3], bool& wrapped, int* region_slot = nullptr) const;
     bool has_finer_direct_neighbour(int slot) const;
     bool neighbours_within_one_of(int slot, int L) const;
+    void chain_targets(int slot, std::vector<int>& out) const;
     // same-level neighbour slot in each of the 27 directions (-1 if none)
     void same_level_neighbours(int slot, int out[27]) const;
 
