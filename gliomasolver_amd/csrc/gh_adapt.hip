@@ -726,7 +726,8 @@ int device_face_table(Ctx& c, bool& done) {
     }
     const LevelMaps m = level_maps(c);
     const int order = c.tree.order;
-    DevBuf dfaces, dcount;
+    DevBuf& dfaces = c.faceList;  // kept across calls: no allocation per adaptation
+    DevBuf& dcount = c.faceCount;
     int rc;
     if ((rc = dfaces.ensure_grow(faces.size() * 4)) || (rc = dcount.ensure_grow((size_t)n * 4))) return rc;
     GH_HIP_CHECK(hipMemcpyAsync(dfaces.p, faces.data(), faces.size() * 4, hipMemcpyHostToDevice, c.stream));
@@ -740,11 +741,7 @@ int device_face_table(Ctx& c, bool& done) {
     GH_HIP_CHECK(hipMemcpyAsync(cnt.data(), dcount.p, (size_t)n * 4, hipMemcpyDeviceToHost, c.stream));
     GH_HIP_CHECK(hipMemcpyAsync(&bad, c.dBad.p, sizeof(int), hipMemcpyDeviceToHost, c.stream));
     GH_HIP_CHECK(hipStreamSynchronize(c.stream));
-    if (bad) {
-        dfaces.release();
-        dcount.release();
-        return GH_OK;
-    }
+    if (bad) return GH_OK;
     int64_t total = 0;
     for (int g = 0; g < n; ++g) {  // exclusive prefix sum, in place
         const int32_t k = cnt[g];
@@ -768,8 +765,6 @@ int device_face_table(Ctx& c, bool& done) {
                            c.faceSrc.as<int32_t>(), c.faceW.as<double>());
     GH_HIP_CHECK(hipGetLastError());
     GH_HIP_CHECK(hipStreamSynchronize(c.stream));  // cnt and faces are read by the copies above
-    dfaces.release();
-    dcount.release();
     done = true;
     return GH_OK;
 }

@@ -60,7 +60,7 @@ int Ctx::alloc_fields() {
 }
 
 void Ctx::release_all() {
-    DevBuf* bufs[] = {&dLevel,   &dNbr27,  &faceEnt,    &faceSrc,   &faceW,     &fgPhi,       &fgTau,   &fgDm,
+    DevBuf* bufs[] = {&dLevel,   &dNbr27,  &faceEnt,    &faceSrc,   &faceW,     &faceList,    &faceCount,     &fgPhi,       &fgTau,   &fgDm,
                       &fgPw,     &fgPg,    &labBuf,     &labSlots,  &labEnt,    &labSrc,      &labW,    &labPerm,  &labAllEnt,
                       &labAllSrc, &labAllW, &labAllSlots, &dDense, &dBlk, &dBad, &spare[0], &spare[1], &spare[2], &spare[3],
                       &labSplit[0], &labSplit[1], &labSplit[2], &labSplit[3], &remapBuf, &detBuf, &phi[0],  &phi[1],    &dphidt,      &pw,      &pg,
@@ -72,7 +72,7 @@ void Ctx::release_all() {
 }
 
 int Ctx::upload_topology() {
-    int rc = nbr.ensure(topo.nbr.size() * sizeof(int32_t));
+    int rc = nbr.ensure_grow(topo.nbr.size() * sizeof(int32_t));
     if (rc) return rc;
     GH_HIP_CHECK(hipMemcpyAsync(nbr.p, topo.nbr.data(), topo.nbr.size() * sizeof(int32_t), hipMemcpyHostToDevice,
                                 stream));

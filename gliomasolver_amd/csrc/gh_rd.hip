@@ -223,7 +223,7 @@ int build_stat_fast(hipStream_t stream, int real_kind, size_t n, const void* st,
                     DevBuf& flag, bool* ok) {
     const size_t rs = real_kind == GH_REAL_F32 ? 4 : 8;
     int rc;
-    if ((rc = out.ensure(std::max<size_t>(1, n) * 4 * rs)) || (rc = flag.ensure(sizeof(unsigned int)))) return rc;
+    if ((rc = out.ensure_grow(std::max<size_t>(1, n) * 4 * rs)) || (rc = flag.ensure(sizeof(unsigned int)))) return rc;
     GH_HIP_CHECK(hipMemsetAsync(flag.p, 0, sizeof(unsigned int), stream));
     const unsigned grid = (unsigned)((n + 255) / 256);
     if (n > 0) {
