@@ -1066,7 +1066,8 @@ int apply_new_tree(Ctx& c, const Octree& nt, const std::vector<Remap>& rm, const
                            fields[f]->as<Real>(), labs[f].as<Real>(), LS, LAB, taps<Real>(order), nf[f].as<Real>());
         GH_HIP_CHECK(hipGetLastError());
     }
-    GH_HIP_CHECK(hipStreamSynchronize(c.stream));  // rm is read by the copy above
+    // rm (and the split list) stay alive in the caller until after
+    // apply_topology, whose uploads synchronize the stream
     for (int f = 0; f < 4; ++f) std::swap(*fields[f], nf[f]);
     c.prof[P_MOVE] += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_move).count();
     {
