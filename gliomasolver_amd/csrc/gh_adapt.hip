@@ -713,7 +713,11 @@ HostTable build_lab_table(const Octree& t, const std::vector<int>& slots, const 
 // recursion (the caller then builds it on the host).
 int device_face_table(Ctx& c, bool& done) {
     done = false;
-    std::vector<int32_t> faces;
+    // host sides of asynchronous uploads: context members, so they outlive
+    // this call until apply_topology synchronizes
+    std::vector<int32_t>& faces = c.hFaces;
+    std::vector<int32_t>& cnt = c.hFaceCnt;
+    faces.clear();
     const int nb = c.topo.nblocks();
     for (int s = 0; s < nb; ++s)
         for (int f = 0; f < 6; ++f)
@@ -736,7 +740,7 @@ int device_face_table(Ctx& c, bool& done) {
     hipLaunchKernelGGL(face_count_kernel, dim3(grid), dim3(256), 0, c.stream, n, dfaces.as<int32_t>(),
                        c.dBlk.as<int4>(), m, order, dcount.as<int32_t>(), c.dBad.as<int>());
     GH_HIP_CHECK(hipGetLastError());
-    std::vector<int32_t> cnt(n);
+    cnt.assign(n, 0);
     int bad = 1;
     GH_HIP_CHECK(hipMemcpyAsync(cnt.data(), dcount.p, (size_t)n * 4, hipMemcpyDeviceToHost, c.stream));
     GH_HIP_CHECK(hipMemcpyAsync(&bad, c.dBad.p, sizeof(int), hipMemcpyDeviceToHost, c.stream));
@@ -764,8 +768,7 @@ int device_face_table(Ctx& c, bool& done) {
                            c.dBlk.as<int4>(), m, order, dcount.as<int32_t>(), c.faceEnt.as<GhostEntry>(),
                            c.faceSrc.as<int32_t>(), c.faceW.as<double>());
     GH_HIP_CHECK(hipGetLastError());
-    GH_HIP_CHECK(hipStreamSynchronize(c.stream));  // cnt and faces are read by the copies above
-    done = true;
+    done = true;  // the copies above read c.hFaces / c.hFaceCnt until apply_topology synchronizes
     return GH_OK;
 }
 
@@ -815,7 +818,10 @@ int apply_topology(Ctx& c) {
     GH_HIP_CHECK(hipMemcpyAsync(c.dNbr27.p, n27.data(), n27.size() * 4, hipMemcpyHostToDevice, c.stream));
     GH_HIP_CHECK(hipMemcpyAsync(c.dLevel.p, lev.data(), lev.size() * 4, hipMemcpyHostToDevice, c.stream));
     bool on_device = false;
-    if (c.nDense > 0 && (rc = device_face_table(c, on_device))) return rc;
+    if (c.nDense > 0 && (rc = device_face_table(c, on_device))) {
+        (void)hipStreamSynchronize(c.stream);  // host vectors above are being uploaded
+        return rc;
+    }
     if (!on_device) {
         HostTable ft = build_face_table(t, c.topo.nbr);
         c.nFaceEnt = (int)ft.ent.size();
@@ -824,12 +830,14 @@ int apply_topology(Ctx& c) {
         if (rc) return rc;
     }
     const size_t rs = c.real_size;
-    if ((rc = c.fgPhi.ensure_grow(std::max(1, nb) * 384 * rs)) || (rc = c.fgTau.ensure_grow(std::max(1, nb) * 384 * rs)) ||
-        (rc = c.fgDm.ensure_grow(std::max(1, nb) * 192 * rs)) || (rc = c.fgPw.ensure_grow(std::max(1, nb) * 384 * rs)) ||
-        (rc = c.fgPg.ensure_grow(std::max(1, nb) * 384 * rs)))
-        return rc;
+    rc = c.fgPhi.ensure_grow(std::max(1, nb) * 384 * rs);
+    if (!rc) rc = c.fgTau.ensure_grow(std::max(1, nb) * 384 * rs);
+    if (!rc) rc = c.fgDm.ensure_grow(std::max(1, nb) * 192 * rs);
+    if (!rc) rc = c.fgPw.ensure_grow(std::max(1, nb) * 384 * rs);
+    if (!rc) rc = c.fgPg.ensure_grow(std::max(1, nb) * 384 * rs);
+    // the uploads above read host vectors of this call and of the face-table build
     GH_HIP_CHECK(hipStreamSynchronize(c.stream));
-    return GH_OK;
+    return rc;
 }
 
 namespace {

@@ -74,9 +74,10 @@ void Ctx::release_all() {
 int Ctx::upload_topology() {
     int rc = nbr.ensure_grow(topo.nbr.size() * sizeof(int32_t));
     if (rc) return rc;
+    // topo.nbr lives until the next topology change; the caller synchronizes
+    // before that (apply_topology's face-table build)
     GH_HIP_CHECK(hipMemcpyAsync(nbr.p, topo.nbr.data(), topo.nbr.size() * sizeof(int32_t), hipMemcpyHostToDevice,
                                 stream));
-    GH_HIP_CHECK(hipStreamSynchronize(stream));
     return GH_OK;
 }
 
