@@ -202,6 +202,67 @@ __device__ inline void face_point(int f, int u, int v, int i[3]) {
     }
 }
 
+// In-place exclusive prefix sum of n int32 counts on the device, in chunks of
+// 1024: chunk totals, one workgroup scanning them (grand total to *total),
+// then each chunk's own scan plus its offset.  Exact integer arithmetic.
+template <typename T>
+__device__ inline T block_exclusive_scan(T v, T* s_w, T& block_total) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    T x = v;
+    for (int o = 1; o < 64; o <<= 1) {
+        const T y = __shfl_up(x, o, 64);
+        if (lane >= o) x += y;
+    }
+    if (lane == 63) s_w[w] = x;
+    __syncthreads();
+    if (w == 0) {
+        T t = lane < nw ? s_w[lane] : (T)0;
+        for (int o = 1; o < 64; o <<= 1) {
+            const T y = __shfl_up(t, o, 64);
+            if (lane >= o) t += y;
+        }
+        if (lane < nw) s_w[lane] = t;
+    }
+    __syncthreads();
+    block_total = s_w[nw - 1];
+    const T res = (w > 0 ? s_w[w - 1] : (T)0) + x - v;
+    __syncthreads();  // s_w is reused by the caller
+    return res;
+}
+
+__global__ __launch_bounds__(1024) void scan_chunk_sums(int n, const int32_t* __restrict__ cnt,
+                                                        long long* __restrict__ sums) {
+    __shared__ long long s_w[16];
+    const int g = blockIdx.x * 1024 + threadIdx.x;
+    long long tot;
+    (void)block_exclusive_scan<long long>(g < n ? cnt[g] : 0, s_w, tot);
+    if (threadIdx.x == 0) sums[blockIdx.x] = tot;
+}
+
+__global__ __launch_bounds__(1024) void scan_chunk_offsets(int nchunk, const long long* __restrict__ sums,
+                                                           long long* __restrict__ offs,
+                                                           long long* __restrict__ total) {
+    __shared__ long long s_w[16];
+    long long carry = 0;
+    for (int b0 = 0; b0 < nchunk; b0 += 1024) {
+        const int b = b0 + (int)threadIdx.x;
+        long long tot;
+        const long long e = block_exclusive_scan<long long>(b < nchunk ? sums[b] : 0, s_w, tot);
+        if (b < nchunk) offs[b] = carry + e;
+        carry += tot;
+    }
+    if (threadIdx.x == 0) *total = carry;
+}
+
+__global__ __launch_bounds__(1024) void scan_chunk_apply(int n, int32_t* __restrict__ cnt,
+                                                         const long long* __restrict__ offs) {
+    __shared__ long long s_w[16];
+    const int g = blockIdx.x * 1024 + threadIdx.x;
+    long long tot;
+    const long long e = block_exclusive_scan<long long>(g < n ? cnt[g] : 0, s_w, tot);
+    if (g < n) cnt[g] = (int32_t)(offs[blockIdx.x] + e);  // the caller rejects totals above INT32_MAX
+}
+
 // Face-ghost table on the device, pass 1: term count of ghost g = 64 * k + p
 // of jump face k (faces[k] = 6 * slot + f), p = v * 8 + u.
 __global__ __launch_bounds__(256) void face_count_kernel(int n, const int32_t* __restrict__ faces,
@@ -713,10 +774,9 @@ HostTable build_lab_table(const Octree& t, const std::vector<int>& slots, const 
 // recursion (the caller then builds it on the host).
 int device_face_table(Ctx& c, bool& done) {
     done = false;
-    // host sides of asynchronous uploads: context members, so they outlive
+    // host side of an asynchronous upload: a context member, so it outlives
     // this call until apply_topology synchronizes
     std::vector<int32_t>& faces = c.hFaces;
-    std::vector<int32_t>& cnt = c.hFaceCnt;
     faces.clear();
     const int nb = c.topo.nblocks();
     for (int s = 0; s < nb; ++s)
@@ -740,25 +800,28 @@ int device_face_table(Ctx& c, bool& done) {
     hipLaunchKernelGGL(face_count_kernel, dim3(grid), dim3(256), 0, c.stream, n, dfaces.as<int32_t>(),
                        c.dBlk.as<int4>(), m, order, dcount.as<int32_t>(), c.dBad.as<int>());
     GH_HIP_CHECK(hipGetLastError());
-    cnt.assign(n, 0);
+    // start of each entry's terms: exclusive prefix sum of the counts, in place
+    const int nchunk = (n + 1023) / 1024;
+    if ((rc = c.scanBuf.ensure_grow(((size_t)2 * nchunk + 1) * sizeof(long long)))) return rc;
+    long long* sums = c.scanBuf.as<long long>();
+    hipLaunchKernelGGL(scan_chunk_sums, dim3(nchunk), dim3(1024), 0, c.stream, n, dcount.as<int32_t>(), sums);
+    hipLaunchKernelGGL(scan_chunk_offsets, dim3(1), dim3(1024), 0, c.stream, nchunk, sums, sums + nchunk,
+                       sums + 2 * nchunk);
+    hipLaunchKernelGGL(scan_chunk_apply, dim3(nchunk), dim3(1024), 0, c.stream, n, dcount.as<int32_t>(),
+                       sums + nchunk);
+    GH_HIP_CHECK(hipGetLastError());
     int bad = 1;
-    GH_HIP_CHECK(hipMemcpyAsync(cnt.data(), dcount.p, (size_t)n * 4, hipMemcpyDeviceToHost, c.stream));
+    long long total = 0;
+    GH_HIP_CHECK(hipMemcpyAsync(&total, sums + 2 * nchunk, sizeof(long long), hipMemcpyDeviceToHost, c.stream));
     GH_HIP_CHECK(hipMemcpyAsync(&bad, c.dBad.p, sizeof(int), hipMemcpyDeviceToHost, c.stream));
     GH_HIP_CHECK(hipStreamSynchronize(c.stream));
     if (bad) return GH_OK;
-    int64_t total = 0;
-    for (int g = 0; g < n; ++g) {  // exclusive prefix sum, in place
-        const int32_t k = cnt[g];
-        cnt[g] = (int32_t)total;
-        total += k;
-    }
     if (total > INT32_MAX) return set_error(GH_EINVAL, "face ghost table: too many terms");
     const size_t rs = c.real_size;
     if ((rc = c.faceEnt.ensure_grow((size_t)n * sizeof(GhostEntry))) ||
         (rc = c.faceSrc.ensure_grow(std::max<int64_t>(1, total) * 4)) ||
         (rc = c.faceW.ensure_grow(std::max<int64_t>(1, total) * rs)))
         return rc;
-    GH_HIP_CHECK(hipMemcpyAsync(dcount.p, cnt.data(), (size_t)n * 4, hipMemcpyHostToDevice, c.stream));
     if (c.real_kind == GH_REAL_F32)
         hipLaunchKernelGGL(face_fill_kernel<float>, dim3(grid), dim3(256), 0, c.stream, n, dfaces.as<int32_t>(),
                            c.dBlk.as<int4>(), m, order, dcount.as<int32_t>(), c.faceEnt.as<GhostEntry>(),
@@ -768,7 +831,7 @@ int device_face_table(Ctx& c, bool& done) {
                            c.dBlk.as<int4>(), m, order, dcount.as<int32_t>(), c.faceEnt.as<GhostEntry>(),
                            c.faceSrc.as<int32_t>(), c.faceW.as<double>());
     GH_HIP_CHECK(hipGetLastError());
-    done = true;  // the copies above read c.hFaces / c.hFaceCnt until apply_topology synchronizes
+    done = true;  // the upload above reads c.hFaces until apply_topology synchronizes
     return GH_OK;
 }
 

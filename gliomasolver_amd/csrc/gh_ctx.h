@@ -42,7 +42,8 @@ struct Ctx {
     DevBuf dLevel, dNbr27;
     DevBuf faceEnt, faceSrc, faceW;  // GhostEntry / int32 / Real
     DevBuf faceList, faceCount;      // device face-table build: jump faces, per-ghost term counts
-    std::vector<int32_t> hFaces, hFaceCnt;  // their host sides
+    std::vector<int32_t> hFaces;     // the face list's host side
+    DevBuf scanBuf;                  // face-table prefix sum: chunk sums, offsets, total
     int nFaceEnt = 0;
     DevBuf fgPhi, fgTau, fgDm;       // per slot: 6*64 ghost phi, 6*64 ghost tau, 3*64 minus-face D
     DevBuf fgPw, fgPg;               // per slot: 6*64 ghost p_w, p_g (statics)
