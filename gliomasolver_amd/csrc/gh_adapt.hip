@@ -848,12 +848,24 @@ int apply_topology(Ctx& c) {
     topo.nbr.resize(6 * (size_t)nb);
     std::vector<int32_t> n27(27 * (size_t)nb), lev(nb);
     static const int face_code[6] = {12, 14, 10, 16, 4, 22};
-    for (int s = 0; s < nb; ++s) {
-        int o[27];
-        t.same_level_neighbours(s, o);
-        for (int k = 0; k < 27; ++k) n27[27 * (size_t)s + k] = o[k];
-        for (int f = 0; f < 6; ++f) topo.nbr[6 * (size_t)s + f] = o[face_code[f]];
-        lev[s] = t.leaves[s].level;
+    auto fill = [&](int s0, int s1) {  // disjoint output ranges per slot
+        for (int s = s0; s < s1; ++s) {
+            int o[27];
+            t.same_level_neighbours(s, o);
+            for (int k = 0; k < 27; ++k) n27[27 * (size_t)s + k] = o[k];
+            for (int f = 0; f < 6; ++f) topo.nbr[6 * (size_t)s + f] = o[face_code[f]];
+            lev[s] = t.leaves[s].level;
+        }
+    };
+    // large grids: the 27 neighbour lookups per block on several host threads
+    const int nth = nb >= 4096 ? (int)std::min(8u, std::max(1u, std::thread::hardware_concurrency())) : 1;
+    if (nth <= 1) {
+        fill(0, nb);
+    } else {
+        std::vector<std::thread> th;
+        for (int k = 0; k < nth; ++k)
+            th.emplace_back(fill, (int)((int64_t)nb * k / nth), (int)((int64_t)nb * (k + 1) / nth));
+        for (auto& x : th) x.join();
     }
     topo.uniform_bpd = t.is_uniform() ? (1 << t.levels_min()) : 0;
     topo.nbr27 = n27;
