@@ -1196,8 +1196,8 @@ int grid_refine(Ctx& c, const std::vector<char>& flagged, int* nrefined) {
     Octree nt = t.shell();
     std::vector<char> is_victim(t.leaves.size(), 0);
     for (int v : victims) is_victim[v] = 1;
-    std::unordered_map<uint64_t, int> parent_of;  // child key -> compact parent index
-    std::unordered_map<uint64_t, int> octant_of;
+    std::vector<int> victim_index(t.leaves.size(), -1);  // old slot -> compact parent index
+    for (int j = 0; j < (int)victims.size(); ++j) victim_index[victims[j]] = j;
     for (int s = 0; s < (int)t.leaves.size(); ++s) {
         const BlockRec& b = t.leaves[s];
         if (!is_victim[s]) {
@@ -1212,8 +1212,6 @@ int grid_refine(Ctx& c, const std::vector<char>& flagged, int* nrefined) {
             BlockRec ch{b.level + 1, {2 * b.index[0] + (o & 1), 2 * b.index[1] + ((o >> 1) & 1),
                                       2 * b.index[2] + ((o >> 2) & 1)}};
             nt.leaves.push_back(ch);
-            parent_of[Octree::key(ch)] = j;
-            octant_of[Octree::key(ch)] = o;
         }
     }
     nt.rebuild_index();
@@ -1225,10 +1223,15 @@ int grid_refine(Ctx& c, const std::vector<char>& flagged, int* nrefined) {
         if (it != t.leaf_slot.end()) {
             r.kind = 0;
             r.src = it->second;
-        } else {
+        } else {  // a child of a split leaf: the parent is the old leaf one level up
+            const BlockRec& ch = nt.leaves[s];
+            int ps = -1;
+            if (ch.level == 0 || t.kind(ch.level - 1, ch.index[0] >> 1, ch.index[1] >> 1, ch.index[2] >> 1, &ps) !=
+                                     Octree::LEAF || victim_index[ps] < 0)
+                return set_error(GH_EINVAL, "grid_refine: inconsistent plan");
             r.kind = 1;
-            r.src = parent_of.at(k);
-            r.octant = octant_of.at(k);
+            r.src = victim_index[ps];
+            r.octant = (ch.index[0] & 1) | (ch.index[1] & 1) << 1 | (ch.index[2] & 1) << 2;
         }
         rm[s] = r;
     }

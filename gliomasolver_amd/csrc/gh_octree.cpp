@@ -57,6 +57,7 @@ void Octree::rebuild_index() {
                std::make_tuple(b.level, b.index[2], b.index[1], b.index[0]);
     });
     leaf_slot.clear();
+    leaf_slot.reserve(leaves.size());
     for (size_t s = 0; s < leaves.size(); ++s) leaf_slot[key(leaves[s])] = (int)s;
     for (int l = 0; l < MAXL; ++l) dense[l].clear();
     dense_levels = 0;
@@ -377,9 +378,12 @@ std::vector<int> Octree::refinement_victims(const std::vector<char>& flagged) co
 std::vector<uint64_t> Octree::collapse_parents(const std::vector<char>& flagged) const {
     std::vector<uint64_t> out;
     // candidate parents: interior nodes whose 8 children are leaves
-    std::set<uint64_t> cand;
+    std::vector<uint64_t> cand;  // ascending, distinct
+    cand.reserve(leaves.size());
     for (const BlockRec& b : leaves)
-        if (b.level > 0) cand.insert(key(b.level - 1, b.index[0] >> 1, b.index[1] >> 1, b.index[2] >> 1));
+        if (b.level > 0) cand.push_back(key(b.level - 1, b.index[0] >> 1, b.index[1] >> 1, b.index[2] >> 1));
+    std::sort(cand.begin(), cand.end());
+    cand.erase(std::unique(cand.begin(), cand.end()), cand.end());
     for (uint64_t pk : cand) {
         const int L = (int)(pk >> 48);
         if (L < min_level) continue;
