@@ -144,20 +144,15 @@ int run(const Args& a) {
     return 0;
 }
 
-// Glioma_UQ_DataPreprocessing::run (:539-548) with its constructor's _ic choice (:17-52)
+// Glioma_UQ_DataPreprocessing::run (:539-548) with the constructor's choice
+// of scans or ground truth (-bSynt, :17-52)
 template <typename Real>
 int run_uqprep(const Args& a) {
     const bool verbose = a.flag("-verbose", true);
-    UQPreprocessing<Real> uq(a.integer("-bpd", 16), verbose);
     const std::string pat = a.str("-PatFileName");
-    if (verbose) std::printf("Reading data from file: %s \n", pat.c_str());
-    if (a.integer("-bSynt", 0) == 1) uq.ic_synthetic(pat);
-    else uq.ic(pat);
-    uq.normalise_pet();
-    uq.prior_range();
-    uq.dump_uq_data();
-    const int np = uq.dump_roi();
-    if (verbose) std::printf("ROI points %d\n\n\n Run Finished \n\n", np);
+    if (verbose) std::printf("[uqprep] patient files %s*\n", pat.c_str());
+    const int points = uqprep::prepare<Real>(a.integer("-bpd", 16), pat, a.integer("-bSynt", 0) == 1, verbose);
+    if (verbose) std::printf("[uqprep] done: %d ROI points\n", points);
     return 0;
 }
 
