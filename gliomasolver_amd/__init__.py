@@ -35,6 +35,24 @@ def _ptr(a: np.ndarray):
     return a.ctypes.data_as(C.c_void_p)
 
 
+def _cubes(arrs, what: str, n: int = None) -> int:
+    """Side n of arrays that must all be (n, n, n); the C side reads n^3 values
+    from each, so a wrong shape is refused here rather than read past."""
+    if n is None:
+        n = arrs[0].shape[0] if arrs[0].ndim == 3 else -1
+    for a in arrs:
+        if a.shape != (n, n, n):
+            raise ValueError("%s: every array must have shape (%d, %d, %d), got %s" % (what, n, n, n, a.shape))
+    return n
+
+
+def _roi_rows(roi, what: str):
+    r = np.asarray(roi, np.float64)
+    if r.ndim != 2 or r.shape[1] != 3:
+        raise ValueError("%s: roi must have shape (N, 3), got %s" % (what, r.shape))
+    return np.ascontiguousarray(r.T), r.shape[0]
+
+
 def phantom(n: int, seed: int):
     """Seeded synthetic brain: returns (gm, wm, csf, seed_voxel) as n^3 float64 arrays (x fastest)."""
     lib = load()
@@ -59,9 +77,9 @@ def schedule(real, Dw_phys: float, L: float, tend: float, gpd: int, cfl: float =
 def gaussian_ic(real, p_w: np.ndarray, p_g: np.ndarray, center, sigma: float = 1.0, peak: float = 0.1):
     k = real_kind(real)
     t = np_real(k)
-    n = p_w.shape[0]
     pw = np.ascontiguousarray(p_w, t)
     pg = np.ascontiguousarray(p_g, t)
+    n = _cubes([pw, pg], "gaussian_ic")
     out = np.zeros_like(pw)
     c = (C.c_double * 3)(*[float(v) for v in center])
     check(load().gh_gaussian_ic(k, n, _ptr(pw), _ptr(pg), c, sigma, peak, _ptr(out)))
@@ -207,20 +225,22 @@ class Context:
 
     # ---- fields ----
     def set_anatomy(self, gm, wm, csf):
-        n = gm.shape[0]
         arrs = [np.ascontiguousarray(a, np.float64) for a in (gm, wm, csf)]
+        n = _cubes(arrs, "set_anatomy", self.n)
         check(self.lib.gh_set_anatomy(self.h, n, *[_ptr(a) for a in arrs]))
 
     def ic_from_anatomy(self, gm, wm, csf, tumor_ic=None):
         """Glioma_ReactionDiffusion::_ic for anatomy arrays of shape (nz, ny, nx)."""
         arrs = [np.ascontiguousarray(a, np.float64) for a in (gm, wm, csf)]
+        if arrs[0].ndim != 3 or any(a.shape != arrs[0].shape for a in arrs):
+            raise ValueError("ic_from_anatomy: gm, wm, csf must be 3-D arrays of one shape")
         nz, ny, nx = arrs[0].shape
         ic = None if tumor_ic is None else (C.c_double * 3)(*[float(v) for v in tumor_ic])
         check(self.lib.gh_ic_from_anatomy(self.h, nx, ny, nz, *[_ptr(a) for a in arrs], ic))
 
     def set_field(self, field: int, values: np.ndarray):
         v = np.ascontiguousarray(values, self.dtype)
-        check(self.lib.gh_set_field_dense(self.h, field, v.shape[0], _ptr(v)))
+        check(self.lib.gh_set_field_dense(self.h, field, _cubes([v], "set_field", self.n), _ptr(v)))
 
     def get_field(self, field: int) -> np.ndarray:
         out = np.zeros((self.n, self.n, self.n), self.dtype)
@@ -262,9 +282,9 @@ class Context:
 
     def lik_set_data(self, t1c, flair, pet, roi=None):
         arrs = [np.ascontiguousarray(a, np.float64) for a in (t1c, flair, pet)]
-        r = None if roi is None else np.ascontiguousarray(np.asarray(roi, np.float64).T)
-        check(self.lib.gh_lik_set_data(self.h, arrs[0].shape[0], *[_ptr(a) for a in arrs],
-                                       None if r is None else _ptr(r), 0 if roi is None else len(roi)))
+        n = _cubes(arrs, "lik_set_data", self.n)
+        r, nroi = (None, 0) if roi is None else _roi_rows(roi, "lik_set_data")
+        check(self.lib.gh_lik_set_data(self.h, n, *[_ptr(a) for a in arrs], None if r is None else _ptr(r), nroi))
 
     def likelihood(self, params=LIK_DEFAULT, stepPET=1, stepTi=1, use_roi=False):
         """Returns (-logL, (L_PET, L_T1, L_T2), N_PET) for the current phi."""
@@ -276,13 +296,15 @@ class Context:
 
     def likelihood_dense(self, model, t1c, flair, pet, roi=None, params=LIK_DEFAULT, stepPET=1, stepTi=1):
         arrs = [np.ascontiguousarray(a, np.float64) for a in (model, t1c, flair, pet)]
+        if arrs[0].ndim != 3 or any(a.shape != arrs[0].shape for a in arrs):
+            raise ValueError("likelihood_dense: model, t1c, flair, pet must be 3-D arrays of one shape")
         nz, ny, nx = arrs[0].shape
-        r = None if roi is None else np.ascontiguousarray(np.asarray(roi, np.float64).T)
+        r, nroi = (None, 0) if roi is None else _roi_rows(roi, "likelihood_dense")
         p = (C.c_double * 5)(*params)
         out = (C.c_double * 4)()
         npet = C.c_int64()
         check(self.lib.gh_likelihood_dense(self.h, nx, ny, nz, *[_ptr(a) for a in arrs],
-                                           None if r is None else _ptr(r), 0 if roi is None else len(roi), p,
+                                           None if r is None else _ptr(r), nroi, p,
                                            stepPET, stepTi, out, C.byref(npet)))
         return out[0], (out[1], out[2], out[3]), npet.value
 
@@ -343,6 +365,8 @@ class Batch:
     def add_anatomy(self, bpd: int, gm, wm, csf) -> int:
         """GM/WM/CSF of shape (nz, ny, nx), any size; grid of bpd^3 blocks."""
         arrs = [np.ascontiguousarray(a, np.float64) for a in (gm, wm, csf)]
+        if arrs[0].ndim != 3 or any(a.shape != arrs[0].shape for a in arrs):
+            raise ValueError("add_anatomy: gm, wm, csf must be 3-D arrays of one shape")
         nz, ny, nx = arrs[0].shape
         aid = C.c_int()
         check(self.lib.gh_batch_add_anatomy(self.h, bpd, nx, ny, nz, *[_ptr(a) for a in arrs], C.byref(aid)))
@@ -350,10 +374,13 @@ class Batch:
         return aid.value
 
     def set_observations(self, anatomy: int, t1c, flair, pet, roi=None):
+        if not 0 <= anatomy < len(self.sizes):
+            raise ValueError("set_observations: unknown anatomy %d" % anatomy)
         arrs = [np.ascontiguousarray(a, np.float64) for a in (t1c, flair, pet)]
-        r = None if roi is None else np.ascontiguousarray(np.asarray(roi, np.float64).T)
+        _cubes(arrs, "set_observations", self.sizes[anatomy])
+        r, nroi = (None, 0) if roi is None else _roi_rows(roi, "set_observations")
         check(self.lib.gh_batch_set_observations(self.h, anatomy, *[_ptr(a) for a in arrs],
-                                                 None if r is None else _ptr(r), 0 if roi is None else len(roi)))
+                                                 None if r is None else _ptr(r), nroi))
 
     def set_samples(self, anatomy, params, cfl: float = 0.99):
         """params: (N, 6) rows {Dw, rho, Tend, icx, icy, icz}."""

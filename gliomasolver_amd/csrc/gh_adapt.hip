@@ -842,6 +842,7 @@ int device_face_table(Ctx& c, bool& done) {
 int apply_topology(Ctx& c) {
     Octree& t = c.tree;
     ++c.tree_version;  // invalidates the cached lab tables
+    c.lik_ready = false;  // observations were laid out for the old topology
     Topology topo;
     topo.blocks = t.leaves;
     const int nb = (int)t.leaves.size();

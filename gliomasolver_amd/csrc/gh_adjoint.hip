@@ -142,7 +142,8 @@ extern "C" int gh_rd_adjoint(gh_ctx* ctx, int64_t nsteps, double T, int ckpt_eve
     Ctx* c = as_ctx(ctx);
     if (!c || !lik_params || !ic_center || !loss || !grad) return set_error(GH_EINVAL, "null argument");
     if (!c->prepared) return set_error(GH_ESTATE, "gh_rd_adjoint: call gh_rd_prepare first");
-    if (!c->lik_ready) return set_error(GH_ESTATE, "gh_rd_adjoint: call gh_lik_set_data first");
+    if (!c->lik_valid())
+        return set_error(GH_ESTATE, "gh_rd_adjoint: no observations for the current grid (call gh_lik_set_data)");
     if (c->topo.uniform_bpd <= 0) return set_error(GH_ESTATE, "gh_rd_adjoint: needs a uniform grid");
     if (nsteps < 1 || ckpt_every < 1 || ckpt_every > 4096 || !(T > 0) || !(ic_sigma > 0) || !(c->Dw > 0))
         return set_error(GH_EINVAL, "gh_rd_adjoint: bad argument");

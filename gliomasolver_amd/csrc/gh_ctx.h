@@ -14,10 +14,29 @@ namespace gh {
             return ::gh::set_error(GH_EHIP, std::string(#expr) + ": " + hipGetErrorString(_e)); \
     } while (0)
 
-// Device buffer owned by the context (plain hipMalloc; freed in the dtor).
+// Device buffer (plain hipMalloc), move-only; the destructor frees it, so an
+// early error return releases every local buffer.
 struct DevBuf {
     void* p = nullptr;
     size_t bytes = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    DevBuf(DevBuf&& o) noexcept : p(o.p), bytes(o.bytes) {
+        o.p = nullptr;
+        o.bytes = 0;
+    }
+    DevBuf& operator=(DevBuf&& o) noexcept {
+        if (this != &o) {
+            release();
+            p = o.p;
+            bytes = o.bytes;
+            o.p = nullptr;
+            o.bytes = 0;
+        }
+        return *this;
+    }
+    ~DevBuf() { release(); }
     int ensure(size_t nbytes);
     // as ensure, but a reallocation takes 25% headroom (buffers of a growing
     // adaptive grid are not reallocated at every adaptation)
@@ -79,12 +98,17 @@ struct Ctx {
     bool fast_ok = false;  // every cell has p_w + p_g >= 0: clamped states may take rd_step2f_kernel
     DevBuf nbr;         // 6 * nblocks int32
     std::vector<DevBuf> ckpt;  // device-side copies of phi (gh_checkpoint_*)
+    std::vector<uint64_t> ckptVersion;  // tree_version each checkpoint was saved on
     // likelihood data (block-major): y == 1 flags of T1c/FLAIR, PET values, ROI {cell, ix, iy, iz}
     DevBuf obsT1, obsFl, obsPet, obsRoi, likPart;
     // adjoint: forward checkpoints, segment tape, adjoint ping-pong (FP64), parameter partials
     DevBuf adjCk, adjSeg, adjLam[2], adjPart;
     int nroi = 0;
     bool lik_ready = false;
+    uint64_t lik_version = 0;  // tree_version the likelihood data was laid out for
+    // likelihood data usable: set for this topology (apply_topology clears
+    // lik_ready) on a uniform grid of the same size
+    bool lik_valid() const { return lik_ready && lik_version == tree_version && topo.uniform_bpd > 0; }
     bool prepared = false;
     double Dw = 0, Dg = 0, rho = 0;
     double dt_last = 0;  // dt of the running gh_rd_steps call

@@ -462,11 +462,15 @@ int gh_checkpoint_save(gh_ctx* ctx, int slot) {
     if (slot < 0 || slot > 4096) return set_error(GH_EINVAL, "gh_checkpoint_save: slot out of range");
     if (c->topo.nblocks() == 0) return set_error(GH_ESTATE, "gh_checkpoint_save: no grid");
     GH_HIP_CHECK(hipSetDevice(c->device));
-    if ((int)c->ckpt.size() <= slot) c->ckpt.resize(slot + 1);
+    if ((int)c->ckpt.size() <= slot) {
+        c->ckpt.resize(slot + 1);
+        c->ckptVersion.resize(slot + 1, ~(uint64_t)0);
+    }
     const size_t nb = c->ncells() * c->real_size;
     int rc = c->ckpt[slot].ensure(nb);
     if (rc) return rc;
     GH_HIP_CHECK(hipMemcpyAsync(c->ckpt[slot].p, c->phi[c->cur].p, nb, hipMemcpyDeviceToDevice, c->stream));
+    c->ckptVersion[slot] = c->tree_version;
     return GH_OK;
 }
 
@@ -476,6 +480,8 @@ int gh_checkpoint_load(gh_ctx* ctx, int slot) {
     const size_t nb = c->ncells() * c->real_size;
     if (slot < 0 || slot >= (int)c->ckpt.size() || c->ckpt[slot].p == nullptr || c->ckpt[slot].bytes < nb)
         return set_error(GH_EINVAL, "gh_checkpoint_load: slot was never saved for this grid");
+    if (c->ckptVersion[slot] != c->tree_version)
+        return set_error(GH_ESTATE, "gh_checkpoint_load: the grid changed since the slot was saved");
     GH_HIP_CHECK(hipSetDevice(c->device));
     GH_HIP_CHECK(hipMemcpyAsync(c->phi[c->cur].p, c->ckpt[slot].p, nb, hipMemcpyDeviceToDevice, c->stream));
     return GH_OK;

@@ -15,6 +15,7 @@
 #include "gh_lik_common.h"
 
 namespace gh {
+
 namespace {
 
 // stage 1: per-workgroup partials {sse, npet, L_T1, L_T2}
@@ -34,12 +35,17 @@ __global__ __launch_bounds__(256) void lik_final_kernel(int nparts, const double
     lik_final_body(nparts, partials, out);
 }
 
+// parts: 4 * (LIK_GRID + 1) doubles of device memory (null: the context's own
+// buffer, allocated on first use); every slot is written before it is read
 template <typename T>
 int run_lik(Ctx& c, size_t ncells, Layout lay, const T* u, const unsigned char* t1, const unsigned char* fl,
-            const double* pet, const int4* roi, int nroi, const LikParams& p, double out[4], int64_t* npet) {
-    int rc = c.likPart.ensure(sizeof(double) * 4 * (LIK_GRID + 1));
-    if (rc) return rc;
-    double* parts = c.likPart.as<double>();
+            const double* pet, const int4* roi, int nroi, const LikParams& p, double out[4], int64_t* npet,
+            double* parts = nullptr) {
+    if (!parts) {
+        int rc = c.likPart.ensure(sizeof(double) * 4 * (LIK_GRID + 1));
+        if (rc) return rc;
+        parts = c.likPart.as<double>();
+    }
     hipLaunchKernelGGL(lik_partials_kernel<T>, dim3(LIK_GRID), dim3(LIK_THREADS), 0, c.stream, ncells, lay, u, t1,
                        fl, pet, roi, nroi, p, parts);
     GH_HIP_CHECK(hipGetLastError());
@@ -65,9 +71,12 @@ void to_blocks(int n, const T* dense, T* bm) {
 
 }  // namespace
 
-int lik_eval_current(Ctx& c, const double* params, int stepPET, int stepTi, int use_roi, double* out,
-                     int64_t* npet) {
-    if (!c.lik_ready) return set_error(GH_ESTATE, "likelihood: call gh_lik_set_data first");
+namespace {
+int lik_eval_into(Ctx& c, const double* params, int stepPET, int stepTi, int use_roi, double* out, int64_t* npet,
+                  double* parts) {
+    if (!c.lik_valid())
+        return set_error(GH_ESTATE, "likelihood: no observations for the current grid (call gh_lik_set_data after "
+                                    "the last grid change)");
     if (use_roi && c.nroi == 0) return set_error(GH_ESTATE, "likelihood: no ROI was given");
     LikParams p;
     int rc = check_params(params, stepPET, stepTi, p);
@@ -76,13 +85,22 @@ int lik_eval_current(Ctx& c, const double* params, int stepPET, int stepTi, int 
     const int4* roi = use_roi ? c.obsRoi.as<int4>() : nullptr;
     if (c.real_kind == GH_REAL_F32)
         return run_lik<float>(c, c.ncells(), lay, c.phi[c.cur].as<float>(), c.obsT1.as<unsigned char>(),
-                              c.obsFl.as<unsigned char>(), c.obsPet.as<double>(), roi, c.nroi, p, out, npet);
+                              c.obsFl.as<unsigned char>(), c.obsPet.as<double>(), roi, c.nroi, p, out, npet, parts);
     return run_lik<double>(c, c.ncells(), lay, c.phi[c.cur].as<double>(), c.obsT1.as<unsigned char>(),
-                           c.obsFl.as<unsigned char>(), c.obsPet.as<double>(), roi, c.nroi, p, out, npet);
+                           c.obsFl.as<unsigned char>(), c.obsPet.as<double>(), roi, c.nroi, p, out, npet, parts);
+}
+}  // namespace
+
+int lik_eval_current(Ctx& c, const double* params, int stepPET, int stepTi, int use_roi, double* out,
+                     int64_t* npet) {
+    return lik_eval_into(c, params, stepPET, stepTi, use_roi, out, npet, nullptr);
 }
 
 int lik_seed_gradient(Ctx& c, const void* u, double* g, const double* params, int stepPET, int stepTi,
                       int use_roi) {
+    if (!c.lik_valid())
+        return set_error(GH_ESTATE, "likelihood: no observations for the current grid (call gh_lik_set_data after "
+                                    "the last grid change)");
     LikParams p;
     int rc = check_params(params, stepPET, stepTi, p);
     if (rc) return rc;
@@ -163,6 +181,7 @@ int gh_lik_set_data(gh_ctx* ctx, int n, const double* t1c, const double* flair, 
     }
     GH_HIP_CHECK(hipStreamSynchronize(c->stream));
     c->lik_ready = true;
+    c->lik_version = c->tree_version;
     return GH_OK;
 }
 

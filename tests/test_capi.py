@@ -116,3 +116,22 @@ def test_rd_thresholds_equal_brute_force_predicates(real):
         assert np.array_equal(outside(phi), phi < tau)
         assert np.array_equal(domain(phi), phi > sig)
     assert np.array_equal(domain(np.zeros(m, t)), np.zeros(m, t) > sig)
+
+
+def test_wrapper_shape_checks_refuse_before_the_c_call():
+    """The C side reads n^3 values from every dense array: the wrappers refuse
+    arrays of another shape (or a mismatched set) with ValueError first."""
+    a = np.zeros((8, 8, 8))
+    assert gh._cubes([a, a, a], "t") == 8
+    assert gh._cubes([a], "t", 8) == 8
+    for bad in ([a, np.zeros((8, 8, 4))], [np.zeros((8, 8, 8, 1))], [np.zeros((8, 8))]):
+        with pytest.raises(ValueError):
+            gh._cubes(bad, "t")
+    with pytest.raises(ValueError):
+        gh._cubes([a], "t", 16)
+    r, n = gh._roi_rows([[1, 2, 3], [4, 5, 6]], "t")
+    assert n == 2 and r.shape == (3, 2) and r[1, 0] == 2 and r.flags["C_CONTIGUOUS"]
+    with pytest.raises(ValueError):
+        gh._roi_rows([[1, 2], [3, 4]], "t")
+    with pytest.raises(ValueError):
+        gh.gaussian_ic("f32", np.zeros((8, 8, 8)), np.zeros((8, 8, 4)), (1, 1, 1))
