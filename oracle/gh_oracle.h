@@ -91,6 +91,16 @@ void gho_amr_rd_steps_f64(int nleaves, const int* leaves, int order, double* phi
                           const double* p_g, double Dw, double Dg, double rho, double dt, int64_t nsteps);
 void gho_amr_resample(int nleaves, const int* leaves, const double* phi, int is_f32, int gpd, double* out);
 
+int gho_amr_block_terms(int nleaves, const int* leaves, int order, int leaf, int is_f32, int cap, int* counts,
+                        int* src, double* w);
+/* ---- MRAG's own ghost construction (gh_oracle_mrag.c): instruction lists of
+ * MRAG_BBInfoCreator evaluated as BlockLab::load does.  Return 0, or -1 when
+ * the grid is not 2:1 balanced around the block or a ghost is left unfilled. */
+int gho_mrag_lab_f32(int nleaves, const int* leaves, int order, const float* field, int leaf, float* lab);
+int gho_mrag_lab_f64(int nleaves, const int* leaves, int order, const double* field, int leaf, double* lab);
+int gho_mrag_block_terms(int nleaves, const int* leaves, int order, int leaf, int is_f32, int cap, int* counts,
+                         int* src, double* w);
+
 /* dJ/du of J = -logL (HGG_Likelihood.cpp:40-156 differentiated), full grid */
 void gho_dloss_du(int n, const double* u, const double* t1c, const double* flair, const double* pet,
                   const double* lik, int stepPET, int stepTi, double* g);

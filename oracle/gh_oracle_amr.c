@@ -187,6 +187,36 @@ void gho_amr_lab_f64(int nleaves, const int* leaves, int order, const double* fi
     tree_free(&t);
 }
 
+/* the terms of every lab position of `leaf` in recursion order (N^3
+ * positions, x fastest; interior positions have none): counts[p], then up to
+ * cap sources (leaf * 512 + cell) and weights rounded to FP32 (is_f32) or kept
+ * in FP64 as the lab applies them, list after list; returns the total */
+int gho_amr_block_terms(int nleaves, const int* leaves, int order, int leaf, int is_f32, int cap, int* counts,
+                        int* src, double* w) {
+    tree_t t;
+    terms_t T = {0, 0, 0};
+    tree_build(&t, nleaves, leaves);
+    const int L = leaves[4 * leaf], lo = lab_lo(order), N = lab_n(order);
+    int total = 0;
+    for (int p = 0; p < N * N * N; ++p) {
+        const int i[3] = {p % N + lo, (p / N) % N + lo, p / (N * N) + lo};
+        counts[p] = 0;
+        if (i[0] >= 0 && i[0] < 8 && i[1] >= 0 && i[1] < 8 && i[2] >= 0 && i[2] < 8) continue;
+        T.n = 0;
+        collect(&t, order, L, 8LL * leaves[4 * leaf + 1] + i[0], 8LL * leaves[4 * leaf + 2] + i[1],
+                8LL * leaves[4 * leaf + 3] + i[2], 1.0, &T);
+        counts[p] = T.n;
+        for (int q = 0; q < T.n; ++q, ++total) {
+            if (total >= cap) continue;
+            src[total] = (int)T.v[q].src;
+            w[total] = is_f32 ? (double)(float)T.v[q].w : T.v[q].w;
+        }
+    }
+    free(T.v);
+    tree_free(&t);
+    return total;
+}
+
 /* BlockFWT::_filter / _fwt (MRAG/MRAGcore/MRAGBlockFWT.h:219-298) in Real;
  * report = max |value| over the wavelet codes 1..7 (:109-120) */
 #define FWT_BODY(REAL)                                                                               \

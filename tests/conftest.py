@@ -55,6 +55,10 @@ def _load_oracle():
         "gho_amr_rd_steps_f32": (None, [_i, _vp, _i, _vp, _vp, _vp, C.c_float, C.c_float, C.c_float, _d, _i64]),
         "gho_amr_rd_steps_f64": (None, [_i, _vp, _i, _vp, _vp, _vp, _d, _d, _d, _d, _i64]),
         "gho_amr_resample": (None, [_i, _vp, _vp, _i, _i, _vp]),
+        "gho_amr_block_terms": (_i, [_i, _vp, _i, _i, _i, _i, _vp, _vp, _vp]),
+        "gho_mrag_lab_f32": (_i, [_i, _vp, _i, _vp, _i, _vp]),
+        "gho_mrag_lab_f64": (_i, [_i, _vp, _i, _vp, _i, _vp]),
+        "gho_mrag_block_terms": (_i, [_i, _vp, _i, _i, _i, _i, _vp, _vp, _vp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -156,6 +160,31 @@ class Oracle:
         fn = self.lib.gho_amr_lab_f32 if f.dtype == np.float32 else self.lib.gho_amr_lab_f64
         fn(len(r), self.p(r), order, self.p(f), leaf, self.p(lab))
         return lab.reshape(n, n, n)
+
+    def mrag_lab(self, leaves, order, field, leaf):
+        """The lab as MRAG's instruction lists give it (gh_oracle_mrag.c)."""
+        r = self._rows(leaves)
+        f = np.ascontiguousarray(field)
+        n = 13 if order == 4 else 10
+        lab = np.zeros(n ** 3, f.dtype)
+        fn = self.lib.gho_mrag_lab_f32 if f.dtype == np.float32 else self.lib.gho_mrag_lab_f64
+        rc = fn(len(r), self.p(r), order, self.p(f), leaf, self.p(lab))
+        assert rc == 0, "MRAG restatement refused the grid around leaf %d" % leaf
+        return lab.reshape(n, n, n)
+
+    def block_terms(self, which, leaves, order, leaf, f32=True, cap=400000):
+        """Per lab position of `leaf` (N^3, x fastest): counts, then the
+        sources and Real weights of all lists in order; which = 'amr' (the
+        recursion) or 'mrag' (MRAG's instruction lists)."""
+        r = self._rows(leaves)
+        n = 13 if order == 4 else 10
+        counts = np.zeros(n ** 3, np.int32)
+        src = np.zeros(cap, np.int32)
+        w = np.zeros(cap, np.float64)
+        fn = self.lib.gho_amr_block_terms if which == "amr" else self.lib.gho_mrag_block_terms
+        tot = fn(len(r), self.p(r), order, leaf, int(f32), cap, self.p(counts), self.p(src), self.p(w))
+        assert 0 <= tot <= cap, tot
+        return counts, src[:tot].copy(), w[:tot].copy()
 
     def amr_fwt(self, order, lab):
         lab = np.ascontiguousarray(lab)
