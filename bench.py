@@ -24,9 +24,20 @@ batched    = one TMCMC stage (forward + likelihood + adjoint of every sample,
              the stage's cell-steps); FP32 loss/gradient error against the
              FP64 solve of the first samples
 cpu_baseline = the reference's ReactionDiffusionOperator + UpdateTumor
-             (oracle/_ref/rd_ref_f32, built from the reference sources) on the
-             host cores, on a bounded sample of the same workload; the oracle
-             restatement (single thread) when that build is absent.
+             (oracle/_ref/rd_ref_f32, built from the reference sources) over
+             MRAG-shaped data -- 27-field AoS Cells, per-block labs loaded from
+             ghost instructions -- on the host cores given to one GPU (16), on
+             a bounded sample of the same workload; the oracle restatement
+             (single thread) when that build is absent.
+accuracy   = the FP32 solve against the FP64 solve of the same phantom, IC and
+             parameters (each with its own Real schedule): rel-L2 and max-abs
+             of phi every 50 days to T = 400 (the FP64 GPU solve equals the
+             FP64 CPU oracle bit for bit, tests/test_rd_gpu.py)
+uniform256 = the same RD step on a 256^3 grid, whose 400 MB working set
+             exceeds the 256 MB Infinity Cache: the HBM-resident rate
+adaptive   = the wavelet-adaptive run (levels 3-5 of 8^3 blocks, 256^3
+             finest, order-4 wavelets, refine 1e-4 / compress 1e-6 every 10
+             steps) over its first epochs: leaf cell-updates/s
 
 Prints exactly one JSON line on stdout; progress goes to stderr.
 """
@@ -55,10 +66,24 @@ def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
 
 
+HOST_SHARE = 16  # host threads a one-GPU job may use on the GPU box
+
+
+def cpu_threads():
+    """The host threads of one GPU's share: at most 16 on the GPU box (it
+    exposes more CPUs than one GPU's job may load), fewer if this process is
+    pinned to fewer."""
+    try:
+        visible = len(os.sched_getaffinity(0))
+    except AttributeError:
+        visible = os.cpu_count() or 1
+    return max(1, min(HOST_SHARE, visible))
+
+
 def cpu_baseline(case, pw, pg, phi0, target_s=12.0):
     """Bounded sample of the same workload on the host CPU."""
     ref = os.path.join(ROOT, "oracle", "_ref", "rd_ref_f32")
-    threads = min(os.cpu_count() or 1, 16)
+    threads = cpu_threads()
     if os.path.exists(ref):
         with tempfile.TemporaryDirectory() as tmp:
             inp, out = os.path.join(tmp, "in.bin"), os.path.join(tmp, "out.bin")
@@ -78,9 +103,11 @@ def cpu_baseline(case, pw, pg, phi0, target_s=12.0):
             k = int(max(2, min(20000, target_s / max(t2 / 2, 1e-6))))
             el = run(k)
             return {"value": case.ncells * k / el, "unit": "cell-updates/s", "cores": threads,
-                    "kind": "reference",
-                    "sample": "%d Euler steps of the 128^3 FP32 workload (%.1f s), reference operator over 8^3 "
-                              "blocks split across %d threads" % (k, el, threads)}
+                    "host_cpus_visible": os.cpu_count(), "kind": "reference",
+                    "sample": "%d Euler steps of the 128^3 FP32 workload (%.1f s): the reference's "
+                              "ReactionDiffusionOperator + UpdateTumor over 8^3 blocks of 27-field Cells, labs "
+                              "loaded from ghost instructions, block ranges on %d threads (one GPU's share of "
+                              "the %d host CPUs)" % (k, el, threads, os.cpu_count() or 0)}
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     from conftest import Oracle  # checker library, used here only as the CPU baseline
     o = Oracle()
@@ -93,6 +120,70 @@ def cpu_baseline(case, pw, pg, phi0, target_s=12.0):
     el = time.perf_counter() - t0
     return {"value": case.ncells * k / el, "unit": "cell-updates/s", "cores": 1, "kind": "port",
             "sample": "%d Euler steps of the 128^3 FP32 workload (%.1f s), oracle restatement, 1 thread" % (k, el)}
+
+
+def accuracy_leg(case, marks=(50, 100, 150, 200, 250, 300, 350, 400)):
+    """FP32 state against FP64 state of the same phantom, IC and parameters,
+    each advanced with its own Real schedule to every mark (days): rel-L2 and
+    max-abs of phi.  The FP64 solve runs through the same kernels in FP64,
+    which equal the FP64 CPU oracle bit for bit."""
+    import gliomasolver_amd as gh
+    from gliomasolver_amd import FIELD_PHI
+    from gliomasolver_amd import workloads as wl
+
+    c64 = wl.make_case(case.n, "f64", case.seed, Dw_phys=case.Dw_phys, rho=case.rho, tend=case.tend, cfl=case.cfl)
+    x32, x64 = wl.load_context(case), wl.load_context(c64)
+    done32 = done64 = 0
+    rows = []
+    for T in marks:
+        n32 = gh.schedule("f32", case.Dw_phys, wl.L_CUBE, float(T), case.n, case.cfl)[3]
+        n64 = gh.schedule("f64", case.Dw_phys, wl.L_CUBE, float(T), case.n, case.cfl)[3]
+        x32.rd_steps(n32 - done32, case.dt)
+        x64.rd_steps(n64 - done64, c64.dt)
+        done32, done64 = n32, n64
+        u32 = x32.get_field(FIELD_PHI).astype(np.float64)
+        u64 = x64.get_field(FIELD_PHI)
+        d = u32 - u64
+        rows.append({"t_days": T, "steps_f32": n32, "steps_f64": n64,
+                     "rel_l2": float(np.linalg.norm(d) / np.linalg.norm(u64)), "max_abs": float(np.abs(d).max())})
+    x32.close()
+    x64.close()
+    out = {"what": "phi of the FP32 solve vs the FP64 solve (same phantom, IC, Dw, rho; each precision's own dt and "
+                   "step count) every 50 days; the FP64 GPU solve is the FP64 CPU oracle's result bit for bit",
+           "bar_rel_l2": 1e-3, "max_rel_l2": max(r["rel_l2"] for r in rows), "marks": rows}
+    log("accuracy", json.dumps(out))
+    return out
+
+
+def uniform256_leg(case, steps=1000):
+    """The RD step on 256^3 (32,768 blocks): two states (134 MB) and the 16 B
+    records (268 MB) exceed the 256 MB Infinity Cache, so each launch streams
+    from HBM.  Same parameters as the headline case; a fixed step count."""
+    from gliomasolver_amd import workloads as wl
+
+    c = wl.make_case(256, "f32", case.seed, Dw_phys=case.Dw_phys, rho=case.rho, tend=case.tend, cfl=case.cfl)
+    ctx = wl.load_context(c)
+    ctx.checkpoint_save(0)
+    tile = ctx.rd_autotune(c.dt, 20)
+    ctx.checkpoint_load(0)
+    ctx.rd_steps(100, c.dt)  # warm-up
+    ctx.sync()
+    ctx.checkpoint_load(0)
+    ctx.sync()
+    t0 = time.perf_counter()
+    kernel_ms = ctx.rd_steps(steps, c.dt, timed=True)
+    ctx.sync()
+    wall = time.perf_counter() - t0
+    ctx.close()
+    updates = c.ncells * steps
+    achieved = BYTES_PER_CELL_UPDATE_F32 * updates / (kernel_ms * 1e-3) / 1e9
+    out = {"workload": "uniform 256^3 RD, FP32, %d Euler steps (working set 400 MB > 256 MB Infinity Cache)" % steps,
+           "value": updates / wall, "unit": "cell-updates/s", "schedule": "two Euler steps per launch" if tile == 0
+           else "%d blocks per workgroup" % tile,
+           "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": round(achieved / HBM_PEAK_GBS, 4)}}
+    log("uniform256", json.dumps(out))
+    return out
 
 
 BATCH_LIK = (0.0025, 1.0, 0.7, 0.25, 2.0)  # PET noise sigma 0.05, PET scale 1; T1Gd/FLAIR thresholds 0.7/0.25
@@ -237,6 +328,8 @@ def batched_leg(nsamples, K, budget_gb, seq_samples, acc_samples, chunk_cells=0)
     b64.close()
     rel_loss = np.abs(b_loss[accs, 0] - loss64[:, 0]) / np.abs(loss64[:, 0])
     rel_grad = np.abs(b_grad[accs] - grad64) / np.maximum(np.abs(grad64), 1e-300)
+    worst = accs[int(np.argmax(rel_loss))]
+    gap = loss_gap(st, worst, obs[st.anatomy[worst]], make_batch)
     # work of the sequential sample against the whole stage's: evals/s of the plain path
     seq_share = float((cells[seq] * st.nsteps[seq]).sum()) / float((cells * st.nsteps).sum())
     seq_eval_s = len(seq) / seq_wall
@@ -257,9 +350,41 @@ def batched_leg(nsamples, K, budget_gb, seq_samples, acc_samples, chunk_cells=0)
                              "bit-identical results to the batch: %s" % (len(seq), 100 * seq_share, identical),
         "accuracy_vs_fp64": {"samples": len(accs), "loss_rel_max": float(rel_loss.max()),
                              "loss_rel_median": float(np.median(rel_loss)),
-                             "grad_rel_median": float(np.median(rel_grad))},
+                             "grad_rel_median": float(np.median(rel_grad)), "worst_sample": gap},
     }
     return out
+
+
+def loss_gap(st, i, obs, make_batch):
+    """Why one sample's FP32 loss differs from its FP64 loss: the likelihood's
+    Bernoulli terms jump where u crosses T1uc or T2uc (omega^2 = 1 above uc,
+    (u - uc)^2 below, HGG_Likelihood.cpp:136-156), so a voxel whose FP32 and
+    FP64 states fall on opposite sides moves the loss by a finite amount.
+    Counts those voxels and evaluates the loss of the FP32 state with them
+    set to their FP64 values: what is left is the smooth part of the gap."""
+    import gliomasolver_amd as gh
+
+    states = []
+    for real in ("f32", "f64"):
+        bt, ids = make_batch(real)
+        bt.set_adjoint_samples([st.anatomy[i]], [st.params[i]], [st.nsteps[i]])
+        bt.run()
+        states.append(bt.get_phi(0).astype(np.float64))
+        bt.close()
+    u32, u64 = states
+    t1, fl, pet = obs
+    uc1, uc2 = BATCH_LIK[2], BATCH_LIK[3]
+    cross1 = (u32 > uc1) != (u64 > uc1)
+    cross2 = (u32 > uc2) != (u64 > uc2)
+    mixed = np.where(cross1 | cross2, u64, u32)
+    with gh.Context(0, "f64") as ctx:
+        l32 = ctx.likelihood_dense(u32, t1, fl, pet, params=BATCH_LIK)[0]
+        l64 = ctx.likelihood_dense(u64, t1, fl, pet, params=BATCH_LIK)[0]
+        lmix = ctx.likelihood_dense(mixed, t1, fl, pet, params=BATCH_LIK)[0]
+    return {"sample": int(i), "grid": int(st.sizes[st.anatomy[i]]), "loss_f64": l64,
+            "loss_rel": abs(l32 - l64) / abs(l64), "voxels_crossing_T1uc": int(cross1.sum()),
+            "voxels_crossing_T2uc": int(cross2.sum()), "state_max_abs": float(np.abs(u32 - u64).max()),
+            "loss_rel_without_crossings": abs(lmix - l64) / abs(l64)}
 
 
 def adaptive_leg(tend, epoch_steps, max_epochs):
@@ -343,6 +468,9 @@ def main():
     ap.add_argument("--workload", default="uniform128", choices=["uniform128", "adaptive"],
                     help="adaptive: the levels 3-5 wavelet-adaptive run instead (its own JSON line)")
     ap.add_argument("--adaptive-epochs", type=int, default=200)
+    ap.add_argument("--no-adaptive", action="store_true", help="skip the adaptive leg of the default line")
+    ap.add_argument("--no-accuracy", action="store_true", help="skip the FP32-vs-FP64 accuracy report")
+    ap.add_argument("--no-256", action="store_true", help="skip the 256^3 (HBM-resident) uniform leg")
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path (samples shard with no "
@@ -392,15 +520,18 @@ def main():
     # algorithmic bytes of all cell-updates over the device time of all launches
     achieved = BYTES_PER_CELL_UPDATE_F32 * updates / (kernel_ms * 1e-3) / 1e9
     kbase = ("rd_step2f_kernel" if clamped else "rd_step2_kernel") if tile == 0 else "rd_step_kernel"
-    traffic = None
+    traffic, traffic_src = None, None
     if os.path.exists(TRAFFIC_FILE):
         tf = json.load(open(TRAFFIC_FILE))
         if tf.get("kernel") == kbase:
             traffic = tf.get("bytes_per_launch")
+            traffic_src = ("stored rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs of this kernel on this workload "
+                           "(%s, copy-kernel calibrated; per-launch time there %.3f us)" %
+                           (os.path.relpath(TRAFFIC_FILE, ROOT), tf.get("avg_launch_us", float("nan"))))
     kname = ("%s<float> (2 Euler steps per launch)" % kbase if tile == 0 else
              "rd_step_kernel<float,true,%d>" % tile)
     roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "traffic_source": traffic_src,
             "kernel": kname, "avg_launch_us": round(avg_launch_s * 1e6, 3), "steps_per_launch": steps_per_launch,
             "algorithmic_bytes_per_launch": BYTES_PER_CELL_UPDATE_F32 * case.ncells * steps_per_launch}
     cpu = None
@@ -418,8 +549,17 @@ def main():
             "roofline": roof, "cpu_baseline": cpu}
     if cpu:
         line["gpu_over_cpu"] = round(value / cpu["value"], 1)
+    if not a.no_accuracy:
+        log("accuracy: FP32 vs FP64 every 50 days ...")
+        line["accuracy"] = accuracy_leg(case)
+    if not a.no_256:
+        log("uniform 256^3 ...")
+        line["uniform256"] = uniform256_leg(case)
     if adj:
         line["adjoint"] = adj
+    if not a.no_adaptive:
+        log("adaptive run ...")
+        line["adaptive"] = adaptive_leg(500.0, 10, a.adaptive_epochs)
     if not a.no_batch:
         log("batched TMCMC stage ...")
         line["batched"] = batched_leg(a.batch_samples, a.batch_ckpt, a.batch_budget_gb, a.batch_seq, a.batch_acc,

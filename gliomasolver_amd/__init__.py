@@ -200,6 +200,22 @@ class Context:
         check(self.lib.gh_compress(self.h, tol, max_loops, C.byref(n)))
         return n.value
 
+    def refine_blocks(self, block_ids) -> int:
+        """Grid::refine(set<int>): split the listed blocks with the refiner's
+        plan (space extension, maxLevel, 2:1 chain reaction)."""
+        ids = np.ascontiguousarray(np.unique(np.asarray(block_ids, np.int64)), np.int32)
+        n = C.c_int()
+        check(self.lib.gh_grid_refine_blocks(self.h, len(ids), _ptr(ids), C.byref(n)))
+        return n.value
+
+    def compress_blocks(self, block_ids) -> int:
+        """Grid::compress(set<int>, int&): collapse every parent whose eight
+        children are listed and pass the compressor's level-jump test."""
+        ids = np.ascontiguousarray(np.unique(np.asarray(block_ids, np.int64)), np.int32)
+        n = C.c_int()
+        check(self.lib.gh_grid_compress_blocks(self.h, len(ids), _ptr(ids), C.byref(n)))
+        return n.value
+
     ADAPT_PHASES = ("ref_fwt", "ref", "comp_fwt", "comp", "plan", "move", "topology", "statics")
 
     def adapt_profile(self, reset: bool = False) -> dict:
@@ -276,6 +292,12 @@ class Context:
         tile, cp = C.c_int(), C.c_int()
         check(self.lib.gh_rd_schedule_info(self.h, C.byref(tile), C.byref(cp)))
         return tile.value, bool(cp.value)
+
+    def rd_zero_flux_blocks(self) -> int:
+        """Blocks without diffusion, advanced by the reaction update alone."""
+        n = C.c_int()
+        check(self.lib.gh_rd_zero_flux_blocks(self.h, C.byref(n)))
+        return n.value
 
     # ---- likelihood (HGG_Likelihood) ----
     LIK_DEFAULT = (0.000361, 0.85, 0.7, 0.25, 2.0)  # LikelihoodInput.txt

@@ -41,6 +41,8 @@ SIGNATURES = {
     "gh_fwt_details": (_i, [_vp, _vp]),
     "gh_refine": (_i, [_vp, _d, _i, _pi]),
     "gh_compress": (_i, [_vp, _d, _i, _pi]),
+    "gh_grid_refine_blocks": (_i, [_vp, _i, _vp, _pi]),
+    "gh_grid_compress_blocks": (_i, [_vp, _i, _vp, _pi]),
     "gh_adapt_profile": (_i, [_vp, _vp, _i]),
     "gh_set_field_blocks": (_i, [_vp, _i, _vp]),
     "gh_get_field_blocks": (_i, [_vp, _i, _vp]),
@@ -60,6 +62,7 @@ SIGNATURES = {
     "gh_rd_set_tile": (_i, [_vp, _i]),
     "gh_rd_autotune": (_i, [_vp, _d, _i, _pi]),
     "gh_rd_schedule_info": (_i, [_vp, _pi, _pi]),
+    "gh_rd_zero_flux_blocks": (_i, [_vp, _pi]),
     "gh_rd_thresholds": (_i, [_i, _i64, _vp, _vp, _vp, _vp]),
     "gh_lik_set_data": (_i, [_vp, _i, _vp, _vp, _vp, _vp, _i]),
     "gh_likelihood": (_i, [_vp, _pd, _i, _i, _i, _pd, _pi64]),

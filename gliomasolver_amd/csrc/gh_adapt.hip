@@ -265,9 +265,12 @@ __global__ __launch_bounds__(1024) void scan_chunk_apply(int n, int32_t* __restr
 
 // Face-ghost table on the device, pass 1: term count of ghost g = 64 * k + p
 // of jump face k (faces[k] = 6 * slot + f), p = v * 8 + u.
+// pkt (may be null): per face, 64 x the largest count of its 64 ghosts, which
+// are the 64 lanes of one wave (n is a multiple of 64, blocks of 256 lanes)
 __global__ __launch_bounds__(256) void face_count_kernel(int n, const int32_t* __restrict__ faces,
                                                          const int4* __restrict__ blk, LevelMaps m, int order,
-                                                         int32_t* __restrict__ count, int* __restrict__ bad) {
+                                                         int32_t* __restrict__ count, int32_t* __restrict__ pkt,
+                                                         int* __restrict__ bad) {
     const int g = blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= n) return;
     const int sf = faces[g >> 6], s = sf / 6, f = sf % 6, p = g & 63;
@@ -279,29 +282,58 @@ __global__ __launch_bounds__(256) void face_count_kernel(int n, const int32_t* _
     ghost_visit<2>(m, order, b.x, 8 * b.y + i[0], 8 * b.z + i[1], 8 * b.w + i[2], 1.0, visit, my_bad);
     count[g] = cnt;
     if (my_bad) atomicOr(bad, 1);
+    if (pkt) {
+        int mx = cnt;
+        for (int o = 32; o > 0; o >>= 1) mx = max(mx, __shfl_xor(mx, o, 64));
+        if (p == 0) pkt[g >> 6] = 64 * mx;
+    }
 }
 
-// pass 2: entries {dst = slot * 384 + f * 64 + p, start, count} and terms
+// pass 2 in packets: the terms of ghost p of face k at base[k] + 64 j + p,
+// j < its count, then src -1 as a terminator when the column is longer
 template <typename Real>
-__global__ __launch_bounds__(256) void face_fill_kernel(int n, const int32_t* __restrict__ faces,
-                                                        const int4* __restrict__ blk, LevelMaps m, int order,
-                                                        const int32_t* __restrict__ start, GhostEntry* __restrict__ ent,
-                                                        int32_t* __restrict__ tsrc, Real* __restrict__ tw) {
+__global__ __launch_bounds__(256) void face_fill_packet_kernel(int n, const int32_t* __restrict__ faces,
+                                                               const int4* __restrict__ blk, LevelMaps m, int order,
+                                                               const int32_t* __restrict__ base,
+                                                               int32_t* __restrict__ tsrc, Real* __restrict__ tw) {
     const int g = blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= n) return;
-    const int sf = faces[g >> 6], s = sf / 6, f = sf % 6, p = g & 63;
+    const int k = g >> 6, sf = faces[k], s = sf / 6, f = sf % 6, p = g & 63;
     const int4 b = blk[s];
     int i[3];
     face_point(f, p & 7, p >> 3, i);
-    const int k0 = start[g];
-    int k = k0, my_bad = 0;
+    const int b0 = base[k] + p, len = (base[k + 1] - base[k]) >> 6;
+    int j = 0, my_bad = 0;
     auto visit = [&](int32_t src, double w) {
-        tsrc[k] = src;
-        tw[k] = (Real)w;
-        ++k;
+        tsrc[b0 + 64 * j] = src;
+        tw[b0 + 64 * j] = (Real)w;
+        ++j;
     };
     ghost_visit<2>(m, order, b.x, 8 * b.y + i[0], 8 * b.z + i[1], 8 * b.w + i[2], 1.0, visit, my_bad);
-    ent[g] = GhostEntry{s * 384 + f * 64 + p, k0, k - k0};
+    if (j < len) tsrc[b0 + 64 * j] = -1;
+}
+
+// face ghosts from packets: out[64 * faces[k] + p] = sum of f[src] * w over
+// ghost p's terms in order (the GhostEntry kernel's arithmetic), one wave per
+// face, each term a contiguous 64-lane load
+template <typename Real>
+__global__ __launch_bounds__(256) void face_packet_kernel(int n, const int32_t* __restrict__ faces,
+                                                          const int32_t* __restrict__ base,
+                                                          const int32_t* __restrict__ tsrc,
+                                                          const Real* __restrict__ tw, const Real* __restrict__ f,
+                                                          Real* __restrict__ out) {
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= n) return;
+    const int k = g >> 6, p = g & 63;
+    const int b0 = base[k] + p, len = (base[k + 1] - base[k]) >> 6;
+    Real acc = (Real)0;
+    for (int j = 0; j < len; ++j) {  // lanes leave at their terminator: no loads past their own terms
+        const int src = tsrc[b0 + 64 * j];
+        if (src < 0) break;
+        const Real prod = f[src] * tw[b0 + 64 * j];
+        acc = acc + prod;
+    }
+    out[(size_t)faces[k] * 64 + p] = acc;
 }
 
 // Full labs of the listed slots (slots == nullptr: every slot j), ghosts
@@ -774,6 +806,7 @@ HostTable build_lab_table(const Octree& t, const std::vector<int>& slots, const 
 // recursion (the caller then builds it on the host).
 int device_face_table(Ctx& c, bool& done) {
     done = false;
+    c.nFacePk = 0;
     // host side of an asynchronous upload: a context member, so it outlives
     // this call until apply_topology synchronizes
     std::vector<int32_t>& faces = c.hFaces;
@@ -782,8 +815,9 @@ int device_face_table(Ctx& c, bool& done) {
     for (int s = 0; s < nb; ++s)
         for (int f = 0; f < 6; ++f)
             if (c.topo.nbr[6 * (size_t)s + f] < 0) faces.push_back(6 * s + f);
-    const int n = (int)faces.size() * 64;
-    c.nFaceEnt = n;
+    const int nf = (int)faces.size();
+    const int n = nf * 64;
+    c.nFaceEnt = 0;
     if (n == 0) {
         done = true;
         return GH_OK;
@@ -793,22 +827,28 @@ int device_face_table(Ctx& c, bool& done) {
     DevBuf& dfaces = c.faceList;  // kept across calls: no allocation per adaptation
     DevBuf& dcount = c.faceCount;
     int rc;
-    if ((rc = dfaces.ensure_grow(faces.size() * 4)) || (rc = dcount.ensure_grow((size_t)n * 4))) return rc;
+    if ((rc = dfaces.ensure_grow(faces.size() * 4)) || (rc = dcount.ensure_grow((size_t)n * 4)) ||
+        (rc = c.facePkBase.ensure_grow(((size_t)nf + 1) * 4)))
+        return rc;
     GH_HIP_CHECK(hipMemcpyAsync(dfaces.p, faces.data(), faces.size() * 4, hipMemcpyHostToDevice, c.stream));
     GH_HIP_CHECK(hipMemsetAsync(c.dBad.p, 0, sizeof(int), c.stream));
+    GH_HIP_CHECK(hipMemsetAsync(c.facePkBase.as<int32_t>() + nf, 0, sizeof(int32_t), c.stream));
     const unsigned grid = (unsigned)((n + 255) / 256);
     hipLaunchKernelGGL(face_count_kernel, dim3(grid), dim3(256), 0, c.stream, n, dfaces.as<int32_t>(),
-                       c.dBlk.as<int4>(), m, order, dcount.as<int32_t>(), c.dBad.as<int>());
+                       c.dBlk.as<int4>(), m, order, dcount.as<int32_t>(), c.facePkBase.as<int32_t>(),
+                       c.dBad.as<int>());
     GH_HIP_CHECK(hipGetLastError());
-    // start of each entry's terms: exclusive prefix sum of the counts, in place
-    const int nchunk = (n + 1023) / 1024;
+    // packet starts: exclusive prefix sum of the packet sizes, in place, the
+    // total in the extra last entry
+    const int np = nf + 1;
+    const int nchunk = (np + 1023) / 1024;
     if ((rc = c.scanBuf.ensure_grow(((size_t)2 * nchunk + 1) * sizeof(long long)))) return rc;
     long long* sums = c.scanBuf.as<long long>();
-    hipLaunchKernelGGL(scan_chunk_sums, dim3(nchunk), dim3(1024), 0, c.stream, n, dcount.as<int32_t>(), sums);
+    int32_t* pk = c.facePkBase.as<int32_t>();
+    hipLaunchKernelGGL(scan_chunk_sums, dim3(nchunk), dim3(1024), 0, c.stream, np, pk, sums);
     hipLaunchKernelGGL(scan_chunk_offsets, dim3(1), dim3(1024), 0, c.stream, nchunk, sums, sums + nchunk,
                        sums + 2 * nchunk);
-    hipLaunchKernelGGL(scan_chunk_apply, dim3(nchunk), dim3(1024), 0, c.stream, n, dcount.as<int32_t>(),
-                       sums + nchunk);
+    hipLaunchKernelGGL(scan_chunk_apply, dim3(nchunk), dim3(1024), 0, c.stream, np, pk, sums + nchunk);
     GH_HIP_CHECK(hipGetLastError());
     int bad = 1;
     long long total = 0;
@@ -818,19 +858,19 @@ int device_face_table(Ctx& c, bool& done) {
     if (bad) return GH_OK;
     if (total > INT32_MAX) return set_error(GH_EINVAL, "face ghost table: too many terms");
     const size_t rs = c.real_size;
-    if ((rc = c.faceEnt.ensure_grow((size_t)n * sizeof(GhostEntry))) ||
-        (rc = c.faceSrc.ensure_grow(std::max<int64_t>(1, total) * 4)) ||
+    if ((rc = c.faceSrc.ensure_grow(std::max<int64_t>(1, total) * 4)) ||
         (rc = c.faceW.ensure_grow(std::max<int64_t>(1, total) * rs)))
         return rc;
     if (c.real_kind == GH_REAL_F32)
-        hipLaunchKernelGGL(face_fill_kernel<float>, dim3(grid), dim3(256), 0, c.stream, n, dfaces.as<int32_t>(),
-                           c.dBlk.as<int4>(), m, order, dcount.as<int32_t>(), c.faceEnt.as<GhostEntry>(),
-                           c.faceSrc.as<int32_t>(), c.faceW.as<float>());
+        hipLaunchKernelGGL(face_fill_packet_kernel<float>, dim3(grid), dim3(256), 0, c.stream, n,
+                           dfaces.as<int32_t>(), c.dBlk.as<int4>(), m, order, pk, c.faceSrc.as<int32_t>(),
+                           c.faceW.as<float>());
     else
-        hipLaunchKernelGGL(face_fill_kernel<double>, dim3(grid), dim3(256), 0, c.stream, n, dfaces.as<int32_t>(),
-                           c.dBlk.as<int4>(), m, order, dcount.as<int32_t>(), c.faceEnt.as<GhostEntry>(),
-                           c.faceSrc.as<int32_t>(), c.faceW.as<double>());
+        hipLaunchKernelGGL(face_fill_packet_kernel<double>, dim3(grid), dim3(256), 0, c.stream, n,
+                           dfaces.as<int32_t>(), c.dBlk.as<int4>(), m, order, pk, c.faceSrc.as<int32_t>(),
+                           c.faceW.as<double>());
     GH_HIP_CHECK(hipGetLastError());
+    c.nFacePk = nf;
     done = true;  // the upload above reads c.hFaces until apply_topology synchronizes
     return GH_OK;
 }
@@ -899,6 +939,7 @@ int apply_topology(Ctx& c) {
         return rc;
     }
     if (!on_device) {
+        c.nFacePk = 0;
         HostTable ft = build_face_table(t, c.topo.nbr);
         c.nFaceEnt = (int)ft.ent.size();
         rc = c.real_kind == GH_REAL_F32 ? upload_table<float>(c, ft, c.faceEnt, c.faceSrc, c.faceW)
@@ -920,6 +961,14 @@ namespace {
 
 template <typename Real>
 int face_ghosts(Ctx& c, const Real* field, Real* out) {
+    if (c.nFacePk > 0) {
+        const int n = c.nFacePk * 64;
+        hipLaunchKernelGGL(face_packet_kernel<Real>, dim3((n + 255) / 256), dim3(256), 0, c.stream, n,
+                           c.faceList.as<int32_t>(), c.facePkBase.as<int32_t>(), c.faceSrc.as<int32_t>(),
+                           c.faceW.as<Real>(), field, out);
+        GH_HIP_CHECK(hipGetLastError());
+        return GH_OK;
+    }
     if (c.nFaceEnt == 0) return GH_OK;
     hipLaunchKernelGGL(ghost_entry_kernel<Real>, dim3((c.nFaceEnt + 255) / 256), dim3(256), 0, c.stream, c.nFaceEnt,
                        c.faceEnt.as<GhostEntry>(), c.faceSrc.as<int32_t>(), c.faceW.as<Real>(), field, out);
@@ -1438,6 +1487,54 @@ int gh_compress(gh_ctx* ctx, double tol, int max_loops, int* ncollapsed) {
         if (max_loops > 0 && loops >= max_loops) break;
     }
     if (ncollapsed) *ncollapsed = total;
+    return GH_OK;
+}
+
+// the listed blocks as per-slot flags; ids outside [0, nblocks) are refused
+static int flags_of(Ctx* c, int n, const int* ids, std::vector<char>& flagged) {
+    if (n < 0 || (n > 0 && !ids)) return set_error(GH_EINVAL, "block list: bad size or null pointer");
+    const int nb = c->topo.nblocks();
+    flagged.assign(nb, 0);
+    for (int i = 0; i < n; ++i) {
+        if (ids[i] < 0 || ids[i] >= nb) return set_error(GH_EINVAL, "block list: block id out of range");
+        flagged[ids[i]] = 1;
+    }
+    return GH_OK;
+}
+
+// Grid::refine(const set<int>&) (MRAGrid.inl:247-355)
+int gh_grid_refine_blocks(gh_ctx* ctx, int n, const int* block_ids, int* nrefined) {
+    Ctx* c = as_ctx(ctx);
+    if (!c) return set_error(GH_EINVAL, "null context");
+    if (c->topo.nblocks() == 0) return set_error(GH_ESTATE, "gh_grid_refine_blocks: no grid");
+    std::vector<char> flagged;
+    int rc = flags_of(c, n, block_ids, flagged);
+    if (rc) return rc;
+    GH_HIP_CHECK(hipSetDevice(c->device));
+    int nr = 0;
+    {
+        ProfScope ref(c->prof[P_REF]);
+        if ((rc = grid_refine(*c, flagged, &nr))) return rc;
+    }
+    if (nrefined) *nrefined = nr;
+    return GH_OK;
+}
+
+// Grid::compress(const set<int>&, int& nCollapsedBlocks) (MRAGrid.inl:383-486)
+int gh_grid_compress_blocks(gh_ctx* ctx, int n, const int* block_ids, int* ncollapsed) {
+    Ctx* c = as_ctx(ctx);
+    if (!c) return set_error(GH_EINVAL, "null context");
+    if (c->topo.nblocks() == 0) return set_error(GH_ESTATE, "gh_grid_compress_blocks: no grid");
+    std::vector<char> flagged;
+    int rc = flags_of(c, n, block_ids, flagged);
+    if (rc) return rc;
+    GH_HIP_CHECK(hipSetDevice(c->device));
+    int nc = 0;
+    {
+        ProfScope comp(c->prof[P_COMP]);
+        if ((rc = grid_compress(*c, flagged, &nc))) return rc;
+    }
+    if (ncollapsed) *ncollapsed = nc;
     return GH_OK;
 }
 

@@ -64,6 +64,11 @@ struct Ctx {
     std::vector<int32_t> hFaces;     // the face list's host side
     DevBuf scanBuf;                  // face-table prefix sum: chunk sums, offsets, total
     int nFaceEnt = 0;
+    // device-built face table in packets: per jump face, its 64 ghosts' term
+    // lists interleaved (term k of ghost p at facePkBase[face] + 64 k + p,
+    // src -1 past a ghost's own count), so a wave's loads are contiguous
+    DevBuf facePkBase;               // int32 per face + 1: packet starts (exclusive scan)
+    int nFacePk = 0;                 // faces in packets (0: the GhostEntry table is used)
     DevBuf fgPhi, fgTau, fgDm;       // per slot: 6*64 ghost phi, 6*64 ghost tau, 3*64 minus-face D
     DevBuf fgPw, fgPg;               // per slot: 6*64 ghost p_w, p_g (statics)
     // FWT / split labs (compact, one per listed slot)
@@ -96,6 +101,9 @@ struct Ctx {
     DevBuf statF;       // 4 * ncells: {D+x, D+y, D+z, tau_enc} of rd_step2f_kernel (valid iff fast_ok)
     DevBuf flag;        // small scratch word for device-side counts
     bool fast_ok = false;  // every cell has p_w + p_g >= 0: clamped states may take rd_step2f_kernel
+    DevBuf zeroFlux;       // per slot: 1 = block without diffusion (zero_flux_kernel), uniform grids
+    int nZeroFlux = 0;     // blocks without diffusion
+    int split2f[9] = {0};  // rd_step2f_kernel's per-XCD slot ranges, balanced by block cost
     DevBuf nbr;         // 6 * nblocks int32
     std::vector<DevBuf> ckpt;  // device-side copies of phi (gh_checkpoint_*)
     std::vector<uint64_t> ckptVersion;  // tree_version each checkpoint was saved on

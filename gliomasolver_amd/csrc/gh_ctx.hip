@@ -60,11 +60,11 @@ int Ctx::alloc_fields() {
 }
 
 void Ctx::release_all() {
-    DevBuf* bufs[] = {&dLevel,   &dNbr27,  &faceEnt,    &faceSrc,   &faceW,     &faceList,    &faceCount, &scanBuf,     &fgPhi,       &fgTau,   &fgDm,
+    DevBuf* bufs[] = {&dLevel,   &dNbr27,  &faceEnt,    &faceSrc,   &faceW,     &faceList,    &faceCount, &facePkBase, &scanBuf,     &fgPhi,       &fgTau,   &fgDm,
                       &fgPw,     &fgPg,    &labBuf,     &labSlots,  &labEnt,    &labSrc,      &labW,    &labPerm,  &labAllEnt,
                       &labAllSrc, &labAllW, &labAllSlots, &dDense, &dBlk, &dBad, &spare[0], &spare[1], &spare[2], &spare[3],
                       &labSplit[0], &labSplit[1], &labSplit[2], &labSplit[3], &remapBuf, &detBuf, &phi[0],  &phi[1],    &dphidt,      &pw,      &pg,
-                      &pcsf,     &statD,   &statSig,    &statF,     &flag,      &nbr,         &obsT1,   &obsFl,
+                      &pcsf,     &statD,   &statSig,    &statF,     &flag,      &zeroFlux,      &nbr,         &obsT1,   &obsFl,
                       &obsPet,   &obsRoi,  &likPart,    &adjCk,     &adjSeg,    &adjLam[0],   &adjLam[1], &adjPart};
     for (DevBuf* b : bufs) b->release();
     for (DevBuf& b : ckpt) b.release();
@@ -447,6 +447,14 @@ int gh_rd_schedule_info(gh_ctx* ctx, int* tile, int* clamped_pairs) {
     return GH_OK;
 }
 
+int gh_rd_zero_flux_blocks(gh_ctx* ctx, int* nblocks) {
+    Ctx* c = as_ctx(ctx);
+    if (!c || !nblocks) return set_error(GH_EINVAL, "gh_rd_zero_flux_blocks: NULL argument");
+    if (!c->prepared) return set_error(GH_ESTATE, "gh_rd_zero_flux_blocks: call gh_rd_prepare first");
+    *nblocks = c->nZeroFlux;
+    return GH_OK;
+}
+
 int gh_rd_autotune(gh_ctx* ctx, double dt, int trial_steps, int* best) {
     Ctx* c = as_ctx(ctx);
     if (!c) return set_error(GH_EINVAL, "null context");
@@ -478,9 +486,9 @@ int gh_checkpoint_load(gh_ctx* ctx, int slot) {
     Ctx* c = as_ctx(ctx);
     if (!c) return set_error(GH_EINVAL, "null context");
     const size_t nb = c->ncells() * c->real_size;
-    if (slot < 0 || slot >= (int)c->ckpt.size() || c->ckpt[slot].p == nullptr || c->ckpt[slot].bytes < nb)
-        return set_error(GH_EINVAL, "gh_checkpoint_load: slot was never saved for this grid");
-    if (c->ckptVersion[slot] != c->tree_version)
+    if (slot < 0 || slot >= (int)c->ckpt.size() || c->ckpt[slot].p == nullptr)
+        return set_error(GH_EINVAL, "gh_checkpoint_load: slot was never saved");
+    if (c->ckptVersion[slot] != c->tree_version || c->ckpt[slot].bytes < nb)
         return set_error(GH_ESTATE, "gh_checkpoint_load: the grid changed since the slot was saved");
     GH_HIP_CHECK(hipSetDevice(c->device));
     GH_HIP_CHECK(hipMemcpyAsync(c->phi[c->cur].p, c->ckpt[slot].p, nb, hipMemcpyDeviceToDevice, c->stream));

@@ -73,6 +73,42 @@ inline double block_h(const Ctx& c) {
     return std::ldexp(1.0, -c.topo.blocks[0].level) / BS;
 }
 
+// zero-flux flags of the uniform grid and the XCD split that balances them:
+// contiguous slot ranges (halos stay in one L2) of equal total cost
+template <typename Real>
+int plan_step2f(Ctx& c) {
+    const int ns = c.topo.nblocks();
+    XcdSplit even = even_split(ns);
+    std::copy(even.start, even.start + 9, c.split2f);
+    c.nZeroFlux = 0;
+    if (!c.fast_ok || c.topo.uniform_bpd <= 0 || ns == 0) return GH_OK;
+    int rc;
+    if ((rc = c.zeroFlux.ensure_grow(ns))) return rc;
+    const UniformStepParams<Real> prm{(Real)c.rho, 0., 0., c.topo.uniform_bpd, bpd_magic(c.topo.uniform_bpd)};
+    hipLaunchKernelGGL((zero_flux_kernel<Real, UniformStepParams<Real>>), dim3(ns), dim3(256), 0, c.stream, ns,
+                       c.statF.as<Stat4<Real>>(), prm, c.zeroFlux.as<unsigned char>());
+    GH_HIP_CHECK(hipGetLastError());
+    std::vector<unsigned char> z(ns);
+    GH_HIP_CHECK(hipMemcpyAsync(z.data(), c.zeroFlux.p, ns, hipMemcpyDeviceToHost, c.stream));
+    GH_HIP_CHECK(hipStreamSynchronize(c.stream));
+    double total = 0;
+    for (int s = 0; s < ns; ++s) {
+        c.nZeroFlux += z[s];
+        total += z[s] ? kZeroFluxCost : 1.0;
+    }
+    // split point i: the first slot whose prefix cost reaches i/8 of the total
+    double acc = 0;
+    int x = 1;
+    c.split2f[0] = 0;
+    for (int s = 0; s < ns && x < 8; ++s) {
+        acc += z[s] ? kZeroFluxCost : 1.0;
+        while (x < 8 && acc >= total * x / 8) c.split2f[x++] = s + 1;
+    }
+    for (; x < 8; ++x) c.split2f[x] = ns;
+    c.split2f[8] = ns;
+    return GH_OK;
+}
+
 template <typename Real>
 int prepare_t(Ctx& c, double Dw, double Dg, double rho) {
     const int ns = c.topo.nblocks();
@@ -85,6 +121,7 @@ int prepare_t(Ctx& c, double Dw, double Dg, double rho) {
     int rc;
     if ((rc = build_stat_fast(c.stream, c.real_kind, nc, c.statD.p, c.statSig.p, c.statF, c.flag, &c.fast_ok)))
         return rc;
+    if ((rc = plan_step2f<Real>(c))) return rc;
     c.Dw = Dw;
     c.Dg = Dg;
     c.rho = rho;
@@ -127,8 +164,15 @@ int launch_step2f(Ctx& c, const Real* in, Real* out, double dt, Real* mid = null
     const double h = block_h(c);
     const double ih2 = 1. / (h * h);
     const UniformStepParams<Real> prm{(Real)c.rho, ih2, dt, c.topo.uniform_bpd, bpd_magic(c.topo.uniform_bpd)};
-    hipLaunchKernelGGL((rd_step2f_kernel<Real, UniformStepParams<Real>>), dim3(8 * per_xcd), dim3(512), 0, c.stream,
-                       ns, per_xcd, in, out, mid, c.statF.as<Stat4<Real>>(), prm);
+    XcdSplit split{};
+    int widest = 0;
+    for (int i = 0; i <= 8; ++i) split.start[i] = c.split2f[i];
+    for (int i = 0; i < 8; ++i) widest = std::max(widest, split.start[i + 1] - split.start[i]);
+    (void)per_xcd;
+    if (split.start[8] != ns) return set_error(GH_ESTATE, "two-step launch: stale work split (call gh_rd_prepare)");
+    hipLaunchKernelGGL((rd_step2f_kernel<Real, UniformStepParams<Real>>), dim3(8 * widest), dim3(512), 0, c.stream,
+                       split, in, out, mid, c.statF.as<Stat4<Real>>(),
+                       c.nZeroFlux > 0 ? c.zeroFlux.as<unsigned char>() : (const unsigned char*)nullptr, prm);
     GH_HIP_CHECK(hipGetLastError());
     return GH_OK;
 }

@@ -8,6 +8,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cfloat>
 #include <cmath>
 #include <cstdint>
@@ -482,6 +483,66 @@ __global__ __launch_bounds__(256) void stat_fast_kernel(size_t n, const Stat4<Re
     if (!(sg <= (Real)0.)) atomicAdd(bad, 1u);
 }
 
+// Blocks without diffusion.  When every face coefficient a block's cells use
+// is +0 -- no tissue in the block, p_w*Dw + p_g*Dg == 0 in every cell, makes
+// each harmonic mean with such a cell +0 -- the operator's flux sums are exact
+// zeros: sum df_i*phi_i = +0, fin = ih2*(+0) = +0, fout = -(+0*ih2) = -0 and
+// fout + fin = +0 (ReactionDiffusionOperator.h:125-135), so dphidt reduces to
+// (Real)(+0 + reaction) in the domain and 0 outside it.  zero_flux_kernel
+// marks such blocks from the clamped-pair records: the three plus-face
+// coefficients of all 512 cells and the minus-face coefficients across the
+// block's -x, -y, -z faces (the neighbours' plus-face values) must all be +0.
+template <typename Real, typename Params>
+__global__ __launch_bounds__(256) void zero_flux_kernel(int nslots, const Stat4<Real>* __restrict__ stf, Params prm,
+                                                        unsigned char* __restrict__ out) {
+    const int s = blockIdx.x;
+    if (s >= nslots) return;
+    const Stat4<Real>* r = stf + (size_t)s * BCELLS;
+    bool nz = false;
+    for (int c = threadIdx.x; c < BCELLS; c += 256)
+        nz |= (r[c].dx != (Real)0 || sign_of(r[c].dx)) || (r[c].dy != (Real)0 || sign_of(r[c].dy)) ||
+              (r[c].dz != (Real)0 || sign_of(r[c].dz));
+    if (threadIdx.x < 64) {
+        const BlockPos bp = prm.pos(s);
+        const int a = threadIdx.x & 7, b = threadIdx.x >> 3;
+        const Real mx = stf[(size_t)bp.nb(-1, 0, 0) * BCELLS + 7 + 8 * a + 64 * b].dx;
+        const Real my = stf[(size_t)bp.nb(0, -1, 0) * BCELLS + a + 8 * 7 + 64 * b].dy;
+        const Real mz = stf[(size_t)bp.nb(0, 0, -1) * BCELLS + a + 8 * b + 64 * 7].dz;
+        nz |= (mx != (Real)0 || sign_of(mx)) || (my != (Real)0 || sign_of(my)) || (mz != (Real)0 || sign_of(mz));
+    }
+    nz = __syncthreads_or(nz);
+    if (threadIdx.x == 0) out[s] = nz ? 0 : 1;
+}
+
+// dphidt of a cell of a block without diffusion (see zero_flux_kernel)
+template <typename Real>
+__device__ inline Real rd_rhs_reaction_only(Real pc, bool dom, Real rho) {
+    const Real rp = rho * pc;
+    const double react = (double)rp * (1. - (double)pc);
+    const Real dphidt = (Real)(0.0 + react);
+    return dom ? dphidt : (Real)0.;
+}
+
+// Relative cost of a block without diffusion in rd_step2f_kernel (its own
+// 24 B per cell, no halo, no LDS) against a full block (about 2.75 stencil
+// evaluations per cell): the weight the XCD split gives it.
+constexpr double kZeroFluxCost = 0.15;
+
+// Work split of the two-step kernel over the eight XCDs: workgroup w runs on
+// XCD w % 8 and serves block start[w % 8] + w / 8 while that is below
+// start[w % 8 + 1]; contiguous slot ranges keep halos in one L2, and the split
+// points balance the ranges' work when blocks differ in cost.
+struct XcdSplit {
+    int start[9];
+};
+// equal contiguous ranges of ceil(n / 8) blocks (grid: 8 * ceil(n / 8))
+inline XcdSplit even_split(int n) {
+    XcdSplit x{};
+    const int per = (n + 7) / 8;
+    for (int i = 0; i <= 8; ++i) x.start[i] = std::min(i * per, n);
+    return x;
+}
+
 // Where cell (a, b) of face f at depth d (1 or 2) outside the block sits:
 // LDS offsets uu (state tile) and dm (coefficient tile) and the cell index in
 // the neighbour block across the face, each base + a * step_a + b * step_b
@@ -504,16 +565,32 @@ __device__ inline FaceMap face_map(int f, int d) {
 }
 __device__ inline int mad24(int a, int b, int c) { return (int)__umul24((unsigned)a, (unsigned)b) + c; }
 
+// zflux (may be null): per slot, 1 = a block without diffusion, whose two
+// steps are each cell's reaction update alone (same bits, no halo).
 template <typename Real, typename Params>
-__global__ __launch_bounds__(512) void rd_step2f_kernel(int nslots, int per_xcd, const Real* __restrict__ phi,
+__global__ __launch_bounds__(512) void rd_step2f_kernel(XcdSplit split, const Real* __restrict__ phi,
                                                         Real* __restrict__ out, Real* __restrict__ mid,
-                                                        const Stat4<Real>* __restrict__ stf, Params prm) {
-    const int s = (int)(blockIdx.x % 8) * per_xcd + (int)(blockIdx.x / 8);
-    if (s >= nslots) return;
+                                                        const Stat4<Real>* __restrict__ stf,
+                                                        const unsigned char* __restrict__ zflux, Params prm) {
+    const int x = (int)(blockIdx.x % 8);
+    const int s = split.start[x] + (int)(blockIdx.x / 8);
+    if (s >= split.start[x + 1]) return;
     __shared__ Real s_uu[UU_SIZE];
     __shared__ Real s_dmx[1000], s_dmy[1000], s_dmz[1000];
 
     const int t = threadIdx.x;
+    if (zflux && zflux[s]) {  // one block per workgroup: the branch is uniform
+        const size_t bc = (size_t)s * BCELLS + (unsigned)t;
+        const Real u0 = phi[bc];
+        const bool in0 = sign_of(stf[bc].tau) != 0;
+        Real rho0;
+        double ih20, dt0;
+        prm.get(s, rho0, ih20, dt0);
+        const Real u1 = rd_update(u0, rd_rhs_reaction_only(u0, in0 || u0 > (Real)0., rho0), dt0);
+        if (mid) mid[bc] = u1;
+        out[bc] = rd_update(u1, rd_rhs_reaction_only(u1, in0 || u1 > (Real)0., rho0), dt0);
+        return;
+    }
     const int w = __builtin_amdgcn_readfirstlane(t >> 6);  // wave = interior plane z; halo role
     const int a = t & 7, b = (t >> 3) & 7;
     const BlockPos bp = prm.pos(s);

@@ -77,6 +77,20 @@ int gh_grid_init_leaves(gh_ctx* ctx, int nleaves, const int* level, const int* i
 int gh_fwt_details(gh_ctx* ctx, double* detail);
 int gh_refine(gh_ctx* ctx, double tol, int max_loops, int* nrefined);
 int gh_compress(gh_ctx* ctx, double tol, int max_loops, int* ncollapsed);
+/* gh_grid_refine_blocks: Grid::refine(const set<int>& blocksToRefine)
+ *   (MRAG/MRAGcore/MRAGrid.h:85, MRAGrid.inl:247-355) for a plan the caller
+ *   made: the n listed blocks (ids in gh_grid_blocks() order, duplicates
+ *   ignored) with the Refiner_SpaceExtension rules (space extension, maxLevel,
+ *   2:1 chain reaction) are split; *nrefined = split blocks
+ *   (RefinementResult::nCollapsedParentBlocks).
+ * gh_grid_compress_blocks: Grid::compress(const set<int>&, int&
+ *   nCollapsedBlocks) (MRAGrid.h:82, MRAGrid.inl:383-486): every parent whose
+ *   eight children are listed leaves and pass the Compressor's level-jump test
+ *   (MRAGCompressor.h:67-146, and the min_level floor) is collapsed;
+ *   *ncollapsed = new parent blocks.  Both move the fields as gh_refine /
+ *   gh_compress do; block ids change afterwards. */
+int gh_grid_refine_blocks(gh_ctx* ctx, int n, const int* block_ids, int* nrefined);
+int gh_grid_compress_blocks(gh_ctx* ctx, int n, const int* block_ids, int* ncollapsed);
 /* gh_adapt_profile: host wall time (ms) spent by gh_refine / gh_compress
  * since the last reset, per phase -- the agents of MRAG's Profiler in
  * Science::AutomaticRefinement/Compression (MRAG/MRAGscience/
@@ -166,6 +180,13 @@ int gh_rd_autotune(gh_ctx* ctx, double dt, int trial_steps, int* best);
  * after one single step from the caller's state every pair of steps takes
  * it; 0: the general two-step kernel). */
 int gh_rd_schedule_info(gh_ctx* ctx, int* tile, int* clamped_pairs);
+/* gh_rd_zero_flux_blocks: no counterpart in the reference (a schedule
+ *   diagnostic).  Blocks of a uniform grid whose cells carry only +0 face
+ *   coefficients (no tissue, p_w*Dw + p_g*Dg == 0), which the two-step kernel
+ *   advances by each cell's reaction update alone: the operator's flux sums
+ *   are exact zeros there (ReactionDiffusionOperator.h:125-135), so the bits
+ *   are the same.  Valid after gh_rd_prepare. */
+int gh_rd_zero_flux_blocks(gh_ctx* ctx, int* nblocks);
 /* gh_rd_thresholds (host-side, no device): the per-cell predicates that
  * gh_rd_prepare stores, for inspection and testing.  tau: the neighbour test
  * "phi + p_w + p_g < 0.1" of ReactionDiffusionOperator.h:115-120,166-181 holds

@@ -30,6 +30,7 @@
 #include <fstream>
 #include <stdexcept>
 #include <string>
+#include <set>
 #include <vector>
 
 #include "gliomahip.h"
@@ -202,6 +203,17 @@ struct Compressor {
     int resJump;
     explicit Compressor(int rj) : resJump(rj) {}
 };
+/* Outcome of Grid::refine / Grid::compress (MRAGRefinementPlan.h:86-104,
+ * MRAGCompressionPlan.h:45-49). */
+struct RefinementResult {
+    int nChildrenBlocks = 0;
+    int nCollapsedParentBlocks = 0;
+    bool hasFailed() const { return false; }
+};
+struct CompressionResult {
+    int nCollapsedBlocks = 0;
+    int nNewBlocks = 0;
+};
 /* BlockFWT<W,B,RD_Projector_Wavelets>: details of phi (Glioma_Types.h:247-255). */
 struct BlockFWT {};
 struct Profiler {};
@@ -270,6 +282,30 @@ public:
     }
     void setCompressor(Compressor* c) {
         if (c && c->resJump != 1) throw Error(GH_EINVAL, "Grid: only resJump 1 is supported");
+    }
+
+    /* Grid::refine(const set<int>&) and Grid::compress(const set<int>&, int&)
+     * (MRAG/MRAGcore/MRAGrid.h:82-85): apply a plan made by the caller over
+     * blockIDs of getBlocksInfo(); the results carry the reference's counters
+     * (MRAGRefinementPlan.h:86-104, MRAGCompressionPlan.h:45-49). */
+    RefinementResult refine(const std::set<int>& blocksToRefine) {
+        std::vector<int> ids(blocksToRefine.begin(), blocksToRefine.end());
+        int n = 0;
+        check(gh_grid_refine_blocks(ctx_.get(), (int)ids.size(), ids.data(), &n));
+        RefinementResult r;
+        r.nCollapsedParentBlocks = n;
+        r.nChildrenBlocks = 8 * n;
+        return r;
+    }
+    CompressionResult compress(const std::set<int>& blocksToCompress, int& nCollapsedBlocks) {
+        std::vector<int> ids(blocksToCompress.begin(), blocksToCompress.end());
+        int n = 0;
+        check(gh_grid_compress_blocks(ctx_.get(), (int)ids.size(), ids.data(), &n));
+        nCollapsedBlocks = n;
+        CompressionResult r;
+        r.nCollapsedBlocks = n;
+        r.nNewBlocks = n;
+        return r;
     }
 
     /* Field access at the uniform finest resolution (uniform grids only). */

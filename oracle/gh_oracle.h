@@ -83,6 +83,10 @@ void gho_amr_lab_f32(int nleaves, const int* leaves, int order, const float* fie
 void gho_amr_lab_f64(int nleaves, const int* leaves, int order, const double* field, int leaf, double* lab);
 double gho_amr_fwt_f32(int order, const float* lab);
 double gho_amr_fwt_f64(int order, const double* lab);
+void gho_amr_details_f32(int nleaves, const int* leaves, int order, const float* field, int nlist, const int* list,
+                         double* out);
+void gho_amr_details_f64(int nleaves, const int* leaves, int order, const double* field, int nlist, const int* list,
+                         double* out);
 void gho_amr_split_f32(int order, const float* lab, int octant, float* child);
 void gho_amr_split_f64(int order, const double* lab, int octant, double* child);
 void gho_amr_rd_steps_f32(int nleaves, const int* leaves, int order, float* phi, const float* p_w,

@@ -259,6 +259,31 @@ int gho_amr_block_terms(int nleaves, const int* leaves, int order, int leaf, int
 double gho_amr_fwt_f32(int order, const float* lab) { FWT_BODY(float) }
 double gho_amr_fwt_f64(int order, const double* lab) { FWT_BODY(double) }
 
+/* FWT details (the report of BlockFWT, max |detail| over codes 1..7) of the
+ * nlist listed leaves, the tree built once */
+#define DETAILS_BODY(REAL, FILL, FWT)                                                               \
+    tree_t t;                                                                                      \
+    terms_t T = {0, 0, 0};                                                                         \
+    tree_build(&t, nleaves, leaves);                                                               \
+    const int N = lab_n(order);                                                                    \
+    REAL* lab = (REAL*)malloc(sizeof(REAL) * N * N * N);                                           \
+    for (int i = 0; i < nlist; ++i) {                                                              \
+        FILL(&t, leaves, order, field, list[i], lab, 0, &T);                                       \
+        out[i] = FWT(order, lab);                                                                  \
+    }                                                                                              \
+    free(lab);                                                                                     \
+    free(T.v);                                                                                     \
+    tree_free(&t);
+
+void gho_amr_details_f32(int nleaves, const int* leaves, int order, const float* field, int nlist, const int* list,
+                         double* out) {
+    DETAILS_BODY(float, lab_fill_f32, gho_amr_fwt_f32)
+}
+void gho_amr_details_f64(int nleaves, const int* leaves, int order, const double* field, int nlist, const int* list,
+                         double* out) {
+    DETAILS_BODY(double, lab_fill_f64, gho_amr_fwt_f64)
+}
+
 /* BlockSplitter::_splitBlocks (MRAG/MRAGcore/MRAGBlockSplitter.h:24-99): the
  * child at `octant` of the parent whose lab is given; contributions in source
  * order (sz, sy, sx), weight (Real)(wZ*wY*Hs_x) with wZ, wY in double */

@@ -2,13 +2,20 @@
 // and UpdateTumor (Glioma/Operators/ReactionDiffusionOperator.h, compiled from
 // the reference tree by oracle/Makefile) over a uniform periodic grid of 8^3
 // blocks.  TEST INFRASTRUCTURE ONLY: its output is the golden answer that the
-// oracle restatement (gh_oracle.c) and the HIP path are compared against.
+// oracle restatement (gh_oracle.c) and the HIP path are compared against, and
+// its timed loop is the CPU baseline of the benchmark.
 //
 // What the harness supplies is the part of MRAG that a uniform grid reduces
-// to: a lab of (8+2)^3 cells whose ghosts are same-level copies with weight 1
-// (MRAG/MRAGcore/MRAGBlockLab.h:256-295 with one instruction per ghost), and
-// BlockInfo with h = 1/(8*bpd) (MRAG/MRAGcore/MRAGrid.inl:200-214).  Real is
-// float unless -D_MRAG_REAL_=double (MRAG/MRAGcore/MRAGCommon.h:43-48).
+// to, with MRAG's data shapes so that the timing is the reference's:
+//   * Cell: the 27 Real fields of Glioma/Glioma_Types.h:21-201 in their order
+//     (108 B in FP32), zero-initialised, with Cell * Real and Cell += Cell
+//     applied field by field as there;
+//   * BlockLab::load (MRAG/MRAGcore/MRAGBlockLab.h:154-301): the interior rows
+//     copied with memcpy, then every ghost evaluated from its instruction list
+//     -- on a uniform grid one instruction (the same-level source, weight
+//     pool entry 1.0) -- as ghost = Cell(); ghost += value * w;
+//   * BlockInfo with h = 1/(8*bpd) (MRAG/MRAGcore/MRAGrid.inl:200-214).
+// Real is float unless -D_MRAG_REAL_=double (MRAG/MRAGcore/MRAGCommon.h:43-48).
 //
 // Input file (binary): int32 n, int64 nsteps, double Dw, Dg, rho, h, dt, then
 // p_w[n^3], p_g[n^3], phi[n^3] as Real, x-fastest.  Output: phi[n^3] as Real.
@@ -25,14 +32,41 @@
 #include <cstdio>
 #include <cstdint>
 #include <cstdlib>
+#include <cstring>
 #include <vector>
 using namespace std;
 using namespace MRAG;
 
 struct Cell {
-    Real phi, dphidt, p_g, p_w;
-    Cell() : phi(0), dphidt(0), p_g(0), p_w(0) {}
+    Real phi, dphidt;
+    Real p_g, p_w, p_csf;
+    Real wm, gm, csf;
+    Real dwmdt, dgmdt, dcsfdt;
+    Real ux, uy, uz;
+    Real omega, domegadt;
+    Real p, dpdt;
+    Real f;
+    Real chi;
+    Real pff, dpffdt;
+    Real exact;
+    Real tmp;
+    Real vp;
+    Real mob, mu;
+    Cell() { std::memset(this, 0, sizeof(Cell)); }
+    void operator+=(const Cell& t) {
+        Real* a = &phi;
+        const Real* b = &t.phi;
+        for (int k = 0; k < 27; ++k) a[k] += b[k];
+    }
 };
+static_assert(sizeof(Cell) == 27 * sizeof(Real), "Cell holds the reference's 27 Real fields");
+inline Cell operator*(const Cell& c, Real v) {
+    Cell r;
+    const Real* a = &c.phi;
+    Real* o = &r.phi;
+    for (int k = 0; k < 27; ++k) o[k] = a[k] * v;
+    return r;
+}
 
 struct HBlock {
     static const int sizeX = 8, sizeY = 8, sizeZ = 8;
@@ -79,17 +113,25 @@ int main(int argc, char** argv) {
     const ReactionDiffusionOperator<3> rhs((Real)Dw, (Real)Dg, (Real)rho);
     const UpdateTumor<3> update(dt);
     const Real hr = (Real)h;
+    const Real weight_pool[1] = {(Real)1.0};  // BoundaryInfoBlock::weightsPool[0]
     const int nb = bpd * bpd * bpd;
     auto rhs_range = [&](int b0, int b1) {
         HLab* lab = new HLab;
         for (int b = b0; b < b1; ++b) {
             const int bx = b % bpd, by = (b / bpd) % bpd, bz = b / (bpd * bpd);
-            // lab load: interior copy plus same-level ghost copies (periodic)
+            // lab load: interior rows by memcpy, then each ghost from its
+            // one-instruction list (same-level source, weight 1.0)
+            for (int iz = 0; iz < 8; ++iz)
+                for (int iy = 0; iy < 8; ++iy) std::memcpy(&(*lab)(0, iy, iz), &blocks[b](0, iy, iz), 8 * sizeof(Cell));
             for (int iz = -1; iz < 9; ++iz) for (int iy = -1; iy < 9; ++iy) for (int ix = -1; ix < 9; ++ix) {
+                if (ix >= 0 && ix < 8 && iy >= 0 && iy < 8 && iz >= 0 && iz < 8) continue;
                 int gi = 8 * bx + ix, gj = 8 * by + iy, gk = 8 * bz + iz;
                 gi = (gi + n) % n; gj = (gj + n) % n; gk = (gk + n) % n;
                 HBlock& src = blocks[(gi / 8) + bpd * ((gj / 8) + bpd * (gk / 8))];
-                (*lab)(ix, iy, iz) = src(gi % 8, gj % 8, gk % 8);
+                const Real w = weight_pool[0] * weight_pool[0] * weight_pool[0];
+                Cell ghost = Cell();
+                ghost += src(gi % 8, gj % 8, gk % 8) * w;
+                (*lab)(ix, iy, iz) = ghost;
             }
             const int idx[3] = {bx, by, bz};
             const Real pos[3] = {(Real)(bx * 8 * h), (Real)(by * 8 * h), (Real)(bz * 8 * h)};

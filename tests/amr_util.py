@@ -4,7 +4,9 @@ direction walk):
 
   neighbourhood  MRAG/MRAGcore/MRAGrid.inl:840-1149, MRAGridHelpers.h:77-84 --
                  leaves whose finest-level boxes touch (faces, edges, corners),
-                 periodic images included and marked as such
+                 periodic images included and marked as such (brute force;
+                 neighbours_by_ring gives the same from an ownership table and
+                 serves the plan restatements on large grids)
   refinement     MRAG/MRAGscience/MRAGRefiner_SpaceExtension.h:25-67 and
                  MRAG/MRAGcore/MRAGRefiner.h:43-197
   compression    MRAG/MRAGcore/MRAGCompressor.h:67-146
@@ -42,9 +44,47 @@ def neighbours(rows):
     return out
 
 
+_RING = {}
+
+
+def _ring_offsets(size):
+    """Cells of the one-cell shell around a cube of `size` cells."""
+    if size not in _RING:
+        g = np.indices((size + 2,) * 3).reshape(3, -1).T - 1
+        inner = np.all((g >= 0) & (g < size), axis=1)
+        _RING[size] = g[~inner][:, ::-1]  # (x, y, z) order
+    return _RING[size]
+
+
+def neighbours_by_ring(rows):
+    """neighbours() computed from a finest-level ownership table: a leaf
+    touches every leaf owning a cell of the one-cell shell around it (a ring
+    cell outside the domain is a periodic image).  Same result as the
+    brute-force box test, in time linear in the grid size."""
+    r = canonical(rows)
+    M = int(r[:, 0].max())
+    N = 1 << M
+    owner = np.full((N, N, N), -1, np.int64)  # [z, y, x]
+    lo = (r[:, 1:].astype(np.int64) << (M - r[:, :1])).astype(np.int64)
+    size = (1 << (M - r[:, 0])).astype(np.int64)
+    for a in range(len(r)):
+        x, y, z = lo[a]
+        s = size[a]
+        owner[z:z + s, y:y + s, x:x + s] = a
+    out = []
+    for a in range(len(r)):
+        c = lo[a][None, :] + _ring_offsets(int(size[a]))
+        wrapped = np.any((c < 0) | (c >= N), axis=1)
+        c = c % N
+        j = owner[c[:, 2], c[:, 1], c[:, 0]]
+        pairs = {(int(b), bool(w)) for b, w in zip(j.tolist(), wrapped.tolist()) if not (b == a and not w)}
+        out.append(sorted(pairs))
+    return out
+
+
 def refine_victims(rows, flagged, max_level):
     r = canonical(rows)
-    nb = neighbours(r)
+    nb = neighbours_by_ring(r)
     flag = list(bool(f) for f in flagged)
     lev = r[:, 0]
     for s in range(len(r)):
@@ -68,7 +108,7 @@ def refine_victims(rows, flagged, max_level):
 
 def collapse_parents(rows, flagged, min_level=0):
     r = canonical(rows)
-    nb = neighbours(r)
+    nb = neighbours_by_ring(r)
     index = {tuple(x): i for i, x in enumerate(r.tolist())}
     out = []
     parents = {(L - 1, x >> 1, y >> 1, z >> 1) for L, x, y, z in r.tolist() if L > 0}
@@ -116,7 +156,7 @@ def balanced_random_grid(rng, base_level=1, max_level=3, nrefine=4):
             break
         rows = apply_refine(rows, [int(rng.choice(cand))])
         while True:
-            nb = neighbours(rows)
+            nb = neighbours_by_ring(rows)
             bad = sorted({j for s in range(len(rows)) for j, _ in nb[s] if rows[s, 0] - rows[j, 0] > 1})
             if not bad:
                 break

@@ -55,6 +55,8 @@ def _load_oracle():
         "gho_amr_rd_steps_f32": (None, [_i, _vp, _i, _vp, _vp, _vp, C.c_float, C.c_float, C.c_float, _d, _i64]),
         "gho_amr_rd_steps_f64": (None, [_i, _vp, _i, _vp, _vp, _vp, _d, _d, _d, _d, _i64]),
         "gho_amr_resample": (None, [_i, _vp, _vp, _i, _i, _vp]),
+        "gho_amr_details_f32": (None, [_i, _vp, _i, _vp, _i, _vp, _vp]),
+        "gho_amr_details_f64": (None, [_i, _vp, _i, _vp, _i, _vp, _vp]),
         "gho_amr_block_terms": (_i, [_i, _vp, _i, _i, _i, _i, _vp, _vp, _vp]),
         "gho_mrag_lab_f32": (_i, [_i, _vp, _i, _vp, _i, _vp]),
         "gho_mrag_lab_f64": (_i, [_i, _vp, _i, _vp, _i, _vp]),
@@ -185,6 +187,16 @@ class Oracle:
         tot = fn(len(r), self.p(r), order, leaf, int(f32), cap, self.p(counts), self.p(src), self.p(w))
         assert 0 <= tot <= cap, tot
         return counts, src[:tot].copy(), w[:tot].copy()
+
+    def amr_details(self, leaves, order, field, subset):
+        """FWT details of the listed leaves (lab + BlockFWT report each)."""
+        r = self._rows(leaves)
+        f = np.ascontiguousarray(field)
+        lst = np.ascontiguousarray(subset, np.int32)
+        out = np.zeros(len(lst), np.float64)
+        fn = self.lib.gho_amr_details_f32 if f.dtype == np.float32 else self.lib.gho_amr_details_f64
+        fn(len(r), self.p(r), order, self.p(f), len(lst), self.p(lst), self.p(out))
+        return out
 
     def amr_fwt(self, order, lab):
         lab = np.ascontiguousarray(lab)

@@ -131,7 +131,20 @@ def test_gpu_multilevel_rd_steps_bit_exact(oracle, real, order, grid):
 # --- oracle-driven simulation of AutomaticRefinement / AutomaticCompression --
 
 def sim_details(oracle, rows, order, phi, subset):
-    return {b: oracle.amr_fwt(order, oracle.amr_lab(rows, order, phi, b)) for b in subset}
+    subset = list(subset)
+    return dict(zip(subset, oracle.amr_details(rows, order, phi, subset).tolist())) if subset else {}
+
+
+def _collapse_gather():
+    """Per parent cell: the child octant and the child cell of the coincident
+    fine vertex (Ha = {1}, MRAG/MRAGcore/MRAGBlockCollapser.h:24-85)."""
+    c = np.arange(512)
+    cx, cy, cz = c & 7, (c >> 3) & 7, c >> 6
+    octant = (cx >> 2) | ((cy >> 2) << 1) | ((cz >> 2) << 2)
+    return octant, 2 * (cx & 3) + 8 * 2 * (cy & 3) + 64 * 2 * (cz & 3)
+
+
+COLLAPSE_OCTANT, COLLAPSE_SRC = _collapse_gather()
 
 
 def sim_refine(oracle, rows, fields, order, tol, max_level, max_loops=1):
@@ -197,15 +210,10 @@ def sim_compress(oracle, rows, fields, order, tol, min_level, max_loops=-1):
                 for f in FIELDS:
                     nf[f][s] = fields[f][old_index[(L, x, y, z)]]
             else:
+                kids = np.array([old_index[(L + 1, 2 * x + (o & 1), 2 * y + ((o >> 1) & 1), 2 * z + ((o >> 2) & 1))]
+                                 for o in range(8)])
                 for f in FIELDS:
-                    blk = np.zeros(512, fields[f].dtype)
-                    for c in range(512):
-                        cx, cy, cz = c & 7, (c >> 3) & 7, c >> 6
-                        o = (cx >> 2) | ((cy >> 2) << 1) | ((cz >> 2) << 2)
-                        kid = old_index[(L + 1, 2 * x + (o & 1), 2 * y + ((o >> 1) & 1), 2 * z + ((o >> 2) & 1))]
-                        src = 2 * (cx & 3) + 8 * 2 * (cy & 3) + 64 * 2 * (cz & 3)
-                        blk[c] = fields[f].dtype.type(0) + fields[f][kid, src]
-                    nf[f][s] = blk
+                    nf[f][s] = fields[f].dtype.type(0) + fields[f][kids[COLLAPSE_OCTANT], COLLAPSE_SRC]
         rows, fields = new, nf
         total += len(parents)
         loops += 1
@@ -314,3 +322,151 @@ def test_gpu_adapt_profile_counts_and_resets():
     assert all(v >= 0 for v in prof.values()) and prof["ref_fwt"] > 0 and prof["comp_fwt"] > 0
     assert all(v == 0 for v in ctx.adapt_profile().values())
     ctx.close()
+
+
+def phantom_blocks(oracle, n, seed, sigma, peak):
+    """Seeded phantom of n^3 cells as uniform-level block rows and fields
+    (FP32 anatomy by the oracle's _ic vote, Gaussian IC at the WM seed voxel)."""
+    gm, wm, csf, sv = gh.phantom(n, seed)
+    pw, pg, _ = oracle.anatomy(gm, wm, csf, "f32")
+    phi = gh.gaussian_ic("f32", pw, pg, sv, sigma, peak)
+    bpd = n // 8
+    L = bpd.bit_length() - 1
+    rows = canonical([[L, x, y, z] for z in range(bpd) for y in range(bpd) for x in range(bpd)])
+
+    def to_blocks(a):
+        return np.ascontiguousarray(a.reshape(bpd, 8, bpd, 8, bpd, 8).transpose(0, 2, 4, 1, 3, 5).reshape(-1, 512))
+
+    fields = {FIELD_PHI: to_blocks(phi), FIELD_PW: to_blocks(pw), FIELD_PG: to_blocks(pg),
+              FIELD_PCSF: np.zeros((bpd ** 3, 512), np.float32)}
+    return rows, fields
+
+
+def test_gpu_adaptive_refine_and_compress_every_10_steps_matches_oracle(oracle):
+    """The adaptive configuration at half size: a 128^3 phantom starting on its
+    finest level (4), levels 2-4, order-4 interpolating wavelets, FP32 state,
+    Dw = 0.013 cm^2/day, rho = 0.025/day, Gaussian IC (sigma 1 cell, peak
+    0.1).  Initial refine (1e-4, one loop) + compress (1e-6, all loops), then
+    15 epochs of 10 Euler steps, each followed by refine 1e-4 and compress
+    1e-6.  Block lists and all four fields bit-exact against the
+    oracle-driven simulation after every step batch and every adaptation,
+    with compression inside the loop exercised."""
+    t = np.float32
+    n, top, bottom = 128, 4, 2
+    rows, fields = phantom_blocks(oracle, n, 21, 1.0, 0.1)
+    Dw, Dg, dt, _ = gh.schedule("f32", 0.013, 17.6, 300.0, n, 0.9)
+    rho = 0.025
+    ctx = make_ctx("f32", rows, 4, fields, max_level=top, min_level=bottom)
+    ctx.rd_prepare(Dw, Dg, rho)
+    say("adaptive 128^3 levels 2-4: initial adaptation")
+    ctx.refine(1e-4, 1)
+    ctx.compress(1e-6, -1)
+    rows, fields, _ = sim_refine(oracle, rows, fields, 4, 1e-4, top, 1)
+    rows, fields, _ = sim_compress(oracle, rows, fields, 4, 1e-6, bottom, -1)
+    assert np.array_equal(rows_of(ctx), rows)
+    for f in FIELDS:
+        assert bits_equal(ctx.get_field_blocks(f).reshape(len(rows), 512), fields[f]), f
+    refined = collapsed = 0
+    levels_seen = set(rows[:, 0].tolist())
+    for epoch in range(15):
+        ctx.rd_steps(10, dt)
+        fields[FIELD_PHI] = oracle.amr_rd_steps(rows, 4, fields[FIELD_PHI], fields[FIELD_PW], fields[FIELD_PG], t(Dw),
+                                                t(Dg), t(rho), dt, 10)
+        assert bits_equal(ctx.get_field_blocks(FIELD_PHI).reshape(len(rows), 512), fields[FIELD_PHI]), epoch
+        nr = ctx.refine(1e-4, 1)
+        nc = ctx.compress(1e-6, -1)
+        rows, fields, snr = sim_refine(oracle, rows, fields, 4, 1e-4, top, 1)
+        rows, fields, snc = sim_compress(oracle, rows, fields, 4, 1e-6, bottom, -1)
+        say("epoch", epoch, "blocks", len(rows), "refined", nr, "collapsed", nc)
+        assert (nr, nc) == (snr, snc)
+        assert np.array_equal(rows_of(ctx), rows), epoch
+        for f in FIELDS:
+            assert bits_equal(ctx.get_field_blocks(f).reshape(len(rows), 512), fields[f]), (epoch, f)
+        refined += nr
+        collapsed += nc
+        levels_seen |= set(rows[:, 0].tolist())
+    ctx.close()
+    assert levels_seen == {2, 3, 4} and refined > 0 and collapsed > 0
+
+
+def level_jumps(rows):
+    """Largest |level difference| between touching leaves (faces, edges,
+    corners, periodic), from a finest-level ownership table: the property of
+    MRAG/MRAGcore/MRAG_STDTestL1_Refinement.h:62 and
+    MRAG_STDTestL1_Compression.h:63."""
+    from amr_util import neighbours_by_ring
+    lev = rows[:, 0]
+    return max(abs(int(lev[a]) - int(lev[j])) for a, nb in enumerate(neighbours_by_ring(rows)) for j, _ in nb)
+
+
+def test_gpu_adaptive_levels_3_to_5_full_size_properties():
+    """The adaptive configuration at full size: 256^3 phantom starting on level
+    5, levels 3-5, order-4 wavelets, FP32, refine 1e-4 + compress 1e-6 every
+    10 steps for 60 epochs.  After every 5 epochs: levels within [3, 5], no two
+    touching blocks more than one level apart, 0 <= phi <= 1, the blocks tile
+    the domain, and the tumour mass stays positive and finite."""
+    n = 256
+    gm, wm, csf, sv = gh.phantom(n, 3)
+    Dw, Dg, dt, _ = gh.schedule("f32", 0.013, 17.6, 500.0, n, 0.9)
+    ctx = gh.Context(0, "f32")
+    ctx.grid_configure(4, 5, 3)
+    ctx.grid_init_uniform(n // 8)
+    ctx.set_anatomy(gm, wm, csf)
+    del gm, wm, csf
+    ctx.set_field(FIELD_PHI, gh.gaussian_ic("f32", ctx.get_field(FIELD_PW), ctx.get_field(FIELD_PG), sv, 1.0, 0.1))
+    ctx.rd_prepare(Dw, Dg, 0.025)
+    say("adaptive 256^3 levels 3-5: initial compression")
+    ctx.compress(1e-6, -1)
+    refined = collapsed = 0
+    for epoch in range(60):
+        ctx.rd_steps(10, dt)
+        refined += ctx.refine(1e-4, 1)
+        collapsed += ctx.compress(1e-6, -1)
+        if (epoch + 1) % 5:
+            continue
+        rows = rows_of(ctx)
+        phi = ctx.get_field_blocks(FIELD_PHI)
+        vol = float(np.sum(np.ldexp(1.0, -3 * rows[:, 0].astype(np.int64))))
+        mass = float(np.sum(phi.reshape(len(rows), 512).astype(np.float64).sum(1) * np.ldexp(1.0, -3 * rows[:, 0])))
+        say("epoch", epoch + 1, "blocks", len(rows), "levels", np.bincount(rows[:, 0]).tolist(), "mass", mass)
+        assert rows[:, 0].min() >= 3 and rows[:, 0].max() <= 5
+        assert vol == 1.0
+        assert level_jumps(rows) <= 1
+        assert np.all(np.isfinite(phi)) and phi.min() >= 0 and phi.max() <= 1
+        assert mass > 0
+    ctx.close()
+    assert refined > 0 and collapsed > 0
+
+
+@pytest.mark.parametrize("real", ["f32", "f64"])
+def test_gpu_refine_and_compress_caller_plans_equal_automatic(real):
+    """Grid::refine(set) / Grid::compress(set, int&) with the sets that
+    Science::AutomaticRefinement / AutomaticCompression would form (detail >=
+    tol, detail < tol; one loop each) give the same grid and the same bits as
+    gh_refine / gh_compress on a twin context; ids outside the grid are refused."""
+    t = np.float32 if real == "f32" else np.float64
+    rows = canonical([[2, x, y, z] for z in range(4) for y in range(4) for x in range(4)])
+    fields = blob_fields(rows, t)
+    a = make_ctx(real, rows, 4, fields, max_level=4, min_level=1)
+    b = make_ctx(real, rows, 4, fields, max_level=4, min_level=1)
+    say("caller plans", real)
+    det = b.fwt_details()
+    na = a.refine(1e-3, 1)
+    nb = b.refine_blocks(np.nonzero(det >= 1e-3)[0])
+    assert na == nb > 0
+    assert np.array_equal(rows_of(a), rows_of(b))
+    for f in FIELDS:
+        assert bits_equal(a.get_field_blocks(f), b.get_field_blocks(f)), f
+    det = b.fwt_details()
+    na = a.compress(2e-2, 1)
+    nb = b.compress_blocks(np.nonzero(det < 2e-2)[0])
+    assert na == nb > 0
+    assert np.array_equal(rows_of(a), rows_of(b))
+    for f in FIELDS:
+        assert bits_equal(a.get_field_blocks(f), b.get_field_blocks(f)), f
+    from gliomasolver_amd._capi import GHError
+    with pytest.raises(GHError):
+        b.refine_blocks([b.nblocks()])
+    assert b.refine_blocks([]) == 0 and b.compress_blocks([]) == 0
+    a.close()
+    b.close()

@@ -8,7 +8,7 @@ import pytest
 
 import gliomasolver_amd as gh
 from amr_util import (apply_refine, balanced_random_grid, canonical, cell_positions, collapse_parents, neighbours,
-                      refine_victims)
+                      neighbours_by_ring, refine_victims)
 
 SEEDS = [0, 1, 2, 3]
 
@@ -19,6 +19,16 @@ def test_neighbourhood_matches_brute_force(seed):
     ref = neighbours(rows)
     for leaf in range(len(rows)):
         assert gh.plan_neighbours(rows, leaf) == ref[leaf], leaf
+
+
+@pytest.mark.parametrize("base,top", [(0, 2), (1, 3), (2, 4)])
+def test_ring_neighbourhood_equals_brute_force(base, top):
+    """The linear-time neighbourhood used by the plan restatements on large
+    grids gives the brute-force box test's (leaf, periodic image) pairs,
+    including the tiny periodic grids where a leaf touches itself."""
+    for seed in range(3):
+        rows = balanced_random_grid(np.random.default_rng(40 + seed), base_level=base, max_level=top, nrefine=3)
+        assert neighbours_by_ring(rows) == neighbours(rows)
 
 
 @pytest.mark.parametrize("seed", SEEDS)

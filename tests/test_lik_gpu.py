@@ -86,26 +86,26 @@ def test_gpu_likelihood_refuses_stale_observations():
     old grid is refused too."""
     from gliomasolver_amd._capi import GHError
     GH_ESTATE = -3
-    case = wl.make_case(16, "f32", 5, Dw_phys=0.013, rho=0.1, tend=20.0)
-    say("likelihood state guard 16^3 -> 32^3 -> refine")
+    case = wl.make_case(32, "f32", 5, Dw_phys=0.013, rho=0.1, tend=20.0)
+    say("likelihood state guard 32^3 -> 64^3 -> refine")
     ctx = wl.load_context(case)
     u = ctx.get_field(FIELD_PHI).astype(np.float64)
     t1, fl, pet = synthetic_observations(u, 3)
     ctx.lik_set_data(t1, fl, pet)
     ctx.likelihood()
     ctx.checkpoint_save(0)
-    ctx.grid_init_uniform(4)  # 32^3: the 16^3 observations no longer fit
-    for call in (lambda: ctx.likelihood(), lambda: ctx.rd_adjoint(4, 1.0, (8, 8, 8)), lambda: ctx.checkpoint_load(0)):
+    ctx.grid_init_uniform(8)  # 64^3: the 32^3 observations no longer fit
+    for call in (lambda: ctx.likelihood(), lambda: ctx.rd_adjoint(4, 1.0, (16, 16, 16)), lambda: ctx.checkpoint_load(0)):
         with pytest.raises(GHError) as e:
             call()
         assert e.value.code == GH_ESTATE
     with pytest.raises(ValueError):
-        ctx.lik_set_data(t1, fl, pet)  # 16^3 arrays for a 32^3 grid: refused before the C call
-    z = np.zeros((32, 32, 32))
+        ctx.lik_set_data(t1, fl, pet)  # 32^3 arrays for a 64^3 grid: refused before the C call
+    z = np.zeros((64, 64, 64))
     ctx.lik_set_data(z, z, z)
     ctx.likelihood()
-    ctx.grid_configure(2, 3, 0)
-    ctx.set_field(FIELD_PHI, np.zeros((32, 32, 32), np.float32))
+    ctx.grid_configure(2, 4, 0)
+    ctx.set_field(FIELD_PHI, np.zeros((64, 64, 64), np.float32))
     ctx.refine(-1.0, 1)  # every block is flagged: the grid changes
     with pytest.raises(GHError) as e:
         ctx.likelihood()

@@ -271,3 +271,51 @@ def test_gpu_schedule_info_reports_the_clamped_pair_kernel():
         ctx.rd_set_tile(0)
         assert ctx.rd_schedule_info() == (0, expect)
         ctx.close()
+
+
+@pytest.mark.parametrize("real", ["f32", "f64"])
+def test_gpu_zero_flux_blocks_match_oracle(oracle, real):
+    """Blocks without tissue (every face coefficient +0) take the two-step
+    kernel's reaction-only path: the count equals the tissue-free blocks, and
+    the state, tumour in those blocks included (phi > 0 there grows by the
+    logistic term alone), equals the oracle bit for bit.  Blocks that hold an
+    adjacent pair of tissue cells keep the full stencil."""
+    n, nsteps = 32, 11
+    dt_ = np.float32 if real == "f32" else np.float64
+    rng = np.random.default_rng(5)
+    pw = rng.uniform(0, 1, (n, n, n))
+    pg = rng.uniform(0, 1, (n, n, n)) * (1 - pw)
+    dead = 0
+    for bz in range(4):
+        for by in range(4):
+            for bx in range(4):
+                if (bx + 2 * by + bz) % 3:
+                    continue
+                sl = np.s_[8 * bz:8 * bz + 8, 8 * by:8 * by + 8, 8 * bx:8 * bx + 8]
+                pw[sl] = 0
+                pg[sl] = 0
+                if (bx + by) % 2:  # a tissue pair inside: not zero-flux
+                    pw[8 * bz + 3, 8 * by + 4, 8 * bx + 2:8 * bx + 4] = 0.6
+                else:
+                    dead += 1
+    pw, pg = pw.astype(dt_), pg.astype(dt_)
+    phi = rng.uniform(0, 1, (n, n, n)).astype(dt_)
+    phi[rng.uniform(size=phi.shape) < 0.3] = 0
+    say("zero-flux blocks", real, "expect", dead)
+    ctx = gh.Context(0, real)
+    ctx.grid_init_uniform(n // 8)
+    ctx.set_field(FIELD_PW, pw)
+    ctx.set_field(FIELD_PG, pg)
+    ctx.set_field(FIELD_PHI, phi)
+    Dw, Dg, rho = 0.0021, 0.00021, 0.31
+    dt = 0.2 / (n * n) / Dw
+    ctx.rd_prepare(Dw, Dg, rho)
+    ctx.rd_set_tile(0)
+    assert ctx.rd_schedule_info() == (0, True)
+    assert ctx.rd_zero_flux_blocks() == dead > 0
+    ctx.rd_steps(nsteps, dt)
+    out = ctx.get_field(FIELD_PHI)
+    ctx.close()
+    ref = oracle.rd_steps(phi, pw, pg, dt_(Dw), dt_(Dg), dt_(rho), dt_(1.0 / n), dt, nsteps)
+    assert bits_equal(out, ref)
+    assert not np.array_equal(out[pw + pg == 0], phi[pw + pg == 0])  # the reaction did act there
