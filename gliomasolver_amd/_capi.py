@@ -67,6 +67,7 @@ SIGNATURES = {
     "gh_lik_set_data": (_i, [_vp, _i, _vp, _vp, _vp, _vp, _i]),
     "gh_likelihood": (_i, [_vp, _pd, _i, _i, _i, _pd, _pi64]),
     "gh_likelihood_dense": (_i, [_vp, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _i, _pd, _i, _i, _pd, _pi64]),
+    "gh_likelihood_noalloc": (_i, [_vp, _pd, _i, _i, _i, _pd, _pi64]),
     "gh_rd_adjoint": (_i, [_vp, _i64, _d, _i, _pd, _i, _i, _i, _pd, _d, _pd, _pd]),
     "gh_checkpoint_save": (_i, [_vp, _i]),
     "gh_checkpoint_load": (_i, [_vp, _i]),

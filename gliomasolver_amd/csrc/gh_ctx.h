@@ -160,5 +160,8 @@ int lik_eval_current(Ctx& c, const double* params, int stepPET, int stepTi, int 
                      int64_t* npet);
 int lik_seed_gradient(Ctx& c, const void* u, double* g, const double* params, int stepPET, int stepTi,
                       int use_roi);
+// resolves the device address of gh_likelihood_noalloc's scratch on `device`
+// (current); once per device
+int lik_scratch_resolve(int device);
 
 }  // namespace gh

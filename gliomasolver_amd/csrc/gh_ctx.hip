@@ -260,6 +260,10 @@ int gh_ctx_create(int device, int real_kind, gh_ctx** out) {
         delete c;
         return set_error(GH_EHIP, std::string("gh_ctx_create: ") + hipGetErrorString(e));
     }
+    // device address of the allocation-free likelihood's scratch on this
+    // device (once per device); a failure here leaves every other entry point
+    // working and gh_likelihood_noalloc retries and reports it
+    (void)lik_scratch_resolve(device);
     *out = reinterpret_cast<gh_ctx*>(c);
     return GH_OK;
 }

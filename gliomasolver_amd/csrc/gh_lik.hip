@@ -11,10 +11,39 @@
 // The reference accumulates in long double; here every per-voxel term is
 // double and the sums are FP64 with a fixed-order two-stage reduction, so a
 // result is reproducible run to run.
+#include <mutex>
+
 #include "gh_ctx.h"
 #include "gh_lik_common.h"
 
 namespace gh {
+
+// Partials of gh_likelihood_noalloc: 4 sums per workgroup of the fixed
+// LIK_GRID partition plus the 4 final sums (32.8 KB).  Each GPU holds its own
+// copy until the process ends; in host code the name is only a stand-in, the
+// device address comes from hipGetSymbolAddress (lik_scratch_resolve).
+constexpr int LIK_SCRATCH_DOUBLES = 4 * (LIK_GRID + 1);
+static __device__ __align__(256) double scratch[LIK_SCRATCH_DOUBLES];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[kMaxDevices] = {nullptr};  // scratch's address on each device
+}  // namespace
+
+// Resolves scratch's address on `device`, which the caller has made current;
+// once per device (gh_ctx_create calls it).
+int lik_scratch_resolve(int device) {
+    if (device < 0 || device >= kMaxDevices) return set_error(GH_EINVAL, "device id out of range");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch_ptr[device]) return GH_OK;
+    void* ptr = nullptr;
+    hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr)
+        return set_error(GH_EHIP, std::string("hipGetSymbolAddress(likelihood scratch): ") + hipGetErrorString(e));
+    g_scratch_ptr[device] = static_cast<double*>(ptr);
+    return GH_OK;
+}
 
 namespace {
 
@@ -35,14 +64,14 @@ __global__ __launch_bounds__(256) void lik_final_kernel(int nparts, const double
     lik_final_body(nparts, partials, out);
 }
 
-// parts: 4 * (LIK_GRID + 1) doubles of device memory (null: the context's own
-// buffer, allocated on first use); every slot is written before it is read
+// parts: 4 * (LIK_GRID + 1) doubles of device scratch (null: the context's
+// own buffer, allocated on first use); every slot is written before it is read
 template <typename T>
 int run_lik(Ctx& c, size_t ncells, Layout lay, const T* u, const unsigned char* t1, const unsigned char* fl,
             const double* pet, const int4* roi, int nroi, const LikParams& p, double out[4], int64_t* npet,
             double* parts = nullptr) {
     if (!parts) {
-        int rc = c.likPart.ensure(sizeof(double) * 4 * (LIK_GRID + 1));
+        int rc = c.likPart.ensure(sizeof(double) * LIK_SCRATCH_DOUBLES);
         if (rc) return rc;
         parts = c.likPart.as<double>();
     }
@@ -191,6 +220,20 @@ int gh_likelihood(gh_ctx* ctx, const double* params, int stepPET, int stepTi, in
     if (!c || !out) return set_error(GH_EINVAL, "null argument");
     GH_HIP_CHECK(hipSetDevice(c->device));
     return lik_eval_current(*c, params, stepPET, stepTi, use_roi, out, npet);
+}
+
+int gh_likelihood_noalloc(gh_ctx* ctx, const double* params, int stepPET, int stepTi, int use_roi, double* out,
+                          int64_t* npet) {
+    Ctx* c = as_ctx(ctx);
+    if (!c || !out) return set_error(GH_EINVAL, "null argument");
+    // the partition is fixed at LIK_GRID workgroups whatever the grid size, so
+    // the scratch bounds the partition, not the grid
+    static_assert(LIK_SCRATCH_DOUBLES == 4 * (LIK_GRID + 1), "scratch holds the LIK_GRID partials + 4 sums");
+    GH_HIP_CHECK(hipSetDevice(c->device));
+    int rc = lik_scratch_resolve(c->device);  // a no-op once gh_ctx_create has resolved it
+    if (rc) return rc;
+    double* parts = g_scratch_ptr[c->device];
+    return lik_eval_into(*c, params, stepPET, stepTi, use_roi, out, npet, parts);
 }
 
 int gh_likelihood_dense(gh_ctx* ctx, int nx, int ny, int nz, const double* model, const double* t1c,

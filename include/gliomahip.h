@@ -214,6 +214,18 @@ int gh_likelihood(gh_ctx* ctx, const double* params, int stepPET, int stepTi, in
 int gh_likelihood_dense(gh_ctx* ctx, int nx, int ny, int nz, const double* model, const double* t1c,
                         const double* flair, const double* pet, const double* roi, int nroi,
                         const double* params, int stepPET, int stepTi, double* out, int64_t* npet);
+/* gh_likelihood_noalloc: no counterpart in the reference.  gh_likelihood for
+ *   callers that cannot allocate device memory (code under HIP graph capture,
+ *   tight sampling loops, harnesses that forbid hipMalloc): same arguments,
+ *   same result bit for bit, but the per-workgroup partials live in a
+ *   file-scope device array of 4 * (1024 + 1) doubles (32.8 KB per GPU, held
+ *   until the process ends) instead of a context buffer, and the call makes no
+ *   hipMalloc/hipFree.  The partition is fixed at 1,024 workgroups for every
+ *   grid size, so no input exceeds the array.  The array is shared by every
+ *   context on a GPU: calls that overlap on one GPU (two threads, two
+ *   streams) race and must be serialized by the caller. */
+int gh_likelihood_noalloc(gh_ctx* ctx, const double* params, int stepPET, int stepTi, int use_roi, double* out,
+                          int64_t* npet);
 
 /* ---- adjoint (no reference counterpart: the reference has no derivatives) --
  * gh_rd_adjoint: forward solve of nsteps fused steps with dt = T/nsteps from
