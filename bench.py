@@ -346,6 +346,17 @@ def batched_leg(nsamples, K, budget_gb, seq_samples, acc_samples, chunk_cells=0)
         "sequential_evals_per_s": seq_eval_s,
         "sequential_stage_s_est": seq_stage_s,
         "speedup_vs_sequential": seq_stage_s / wall,
+        "roofline": {"bytes_per_cell_step": ADJ_BYTES_PER_CELL_STEP_F32,
+                     "stage_bytes": ADJ_BYTES_PER_CELL_STEP_F32 * float(st.cell_steps()),
+                     "stage_s_at_peak": ADJ_BYTES_PER_CELL_STEP_F32 * float(st.cell_steps()) / (HBM_PEAK_GBS * 1e9),
+                     "batched_frac": ADJ_BYTES_PER_CELL_STEP_F32 * float(st.cell_steps()) / (HBM_PEAK_GBS * 1e9) / wall,
+                     "sequential_frac": ADJ_BYTES_PER_CELL_STEP_F32 * float(st.cell_steps()) /
+                                        (HBM_PEAK_GBS * 1e9) / seq_stage_s,
+                     "max_speedup_over_sequential": seq_stage_s /
+                                                    (ADJ_BYTES_PER_CELL_STEP_F32 * float(st.cell_steps()) /
+                                                     (HBM_PEAK_GBS * 1e9)),
+                     "what": "forward 16 + reverse 20 + recompute 16 B per forward cell-step at 8 TB/s: no batching "
+                             "can beat the sequential solves by more than their distance from this bound"},
         "sequential_sample": "first %d samples of the stage, one gh_batch call each (%.1f%% of its cell-steps), "
                              "bit-identical results to the batch: %s" % (len(seq), 100 * seq_share, identical),
         "accuracy_vs_fp64": {"samples": len(accs), "loss_rel_max": float(rel_loss.max()),

@@ -290,7 +290,7 @@ __global__ __launch_bounds__(256) void face_count_kernel(int n, const int32_t* _
 }
 
 // pass 2 in packets: the terms of ghost p of face k at base[k] + 64 j + p,
-// j < its count, then src -1 as a terminator when the column is longer
+// j < its count (the rest of the packet's column is never read)
 template <typename Real>
 __global__ __launch_bounds__(256) void face_fill_packet_kernel(int n, const int32_t* __restrict__ faces,
                                                                const int4* __restrict__ blk, LevelMaps m, int order,
@@ -310,27 +310,28 @@ __global__ __launch_bounds__(256) void face_fill_packet_kernel(int n, const int3
         ++j;
     };
     ghost_visit<2>(m, order, b.x, 8 * b.y + i[0], 8 * b.z + i[1], 8 * b.w + i[2], 1.0, visit, my_bad);
-    if (j < len) tsrc[b0 + 64 * j] = -1;
+    (void)len;
 }
 
 // face ghosts from packets: out[64 * faces[k] + p] = sum of f[src] * w over
-// ghost p's terms in order (the GhostEntry kernel's arithmetic), one wave per
-// face, each term a contiguous 64-lane load
+// ghost p's count[g] terms in order (the GhostEntry kernel's arithmetic), one
+// wave per face, each term a contiguous 64-lane load; the loop bound is known
+// up front, so the term loads of several iterations are in flight together
 template <typename Real>
 __global__ __launch_bounds__(256) void face_packet_kernel(int n, const int32_t* __restrict__ faces,
                                                           const int32_t* __restrict__ base,
+                                                          const int32_t* __restrict__ count,
                                                           const int32_t* __restrict__ tsrc,
                                                           const Real* __restrict__ tw, const Real* __restrict__ f,
                                                           Real* __restrict__ out) {
     const int g = blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= n) return;
     const int k = g >> 6, p = g & 63;
-    const int b0 = base[k] + p, len = (base[k + 1] - base[k]) >> 6;
+    const int b0 = base[k] + p, len = count[g];
     Real acc = (Real)0;
-    for (int j = 0; j < len; ++j) {  // lanes leave at their terminator: no loads past their own terms
-        const int src = tsrc[b0 + 64 * j];
-        if (src < 0) break;
-        const Real prod = f[src] * tw[b0 + 64 * j];
+#pragma unroll 4
+    for (int j = 0; j < len; ++j) {
+        const Real prod = f[tsrc[b0 + 64 * j]] * tw[b0 + 64 * j];
         acc = acc + prod;
     }
     out[(size_t)faces[k] * 64 + p] = acc;
@@ -964,8 +965,8 @@ int face_ghosts(Ctx& c, const Real* field, Real* out) {
     if (c.nFacePk > 0) {
         const int n = c.nFacePk * 64;
         hipLaunchKernelGGL(face_packet_kernel<Real>, dim3((n + 255) / 256), dim3(256), 0, c.stream, n,
-                           c.faceList.as<int32_t>(), c.facePkBase.as<int32_t>(), c.faceSrc.as<int32_t>(),
-                           c.faceW.as<Real>(), field, out);
+                           c.faceList.as<int32_t>(), c.facePkBase.as<int32_t>(), c.faceCount.as<int32_t>(),
+                           c.faceSrc.as<int32_t>(), c.faceW.as<Real>(), field, out);
         GH_HIP_CHECK(hipGetLastError());
         return GH_OK;
     }

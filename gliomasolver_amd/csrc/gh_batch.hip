@@ -254,25 +254,6 @@ struct Batch {
     DevBuf statF, flag;  // rd_step2f_kernel records (valid iff fast_ok), count scratch
     bool fast_ok = false;
     DevBuf zeroFlux;                 // per slot: 1 = block without diffusion (zero_flux_kernel)
-    std::vector<double> costPrefix;  // per slot prefix of rd_step2f_kernel block costs
-    // rd_step2f_kernel's XCD split of the first nsl slots (contiguous ranges of
-    // equal cost); *widest = the largest range
-    XcdSplit split2f(int nsl, int* widest) const {
-        XcdSplit x = even_split(nsl);
-        if ((int)costPrefix.size() > nsl && nsl > 0) {
-            const double total = costPrefix[nsl];
-            for (int i = 1; i < 8; ++i) {
-                const double target = total * i / 8;
-                x.start[i] = (int)(std::lower_bound(costPrefix.begin() + 1, costPrefix.begin() + nsl + 1, target) -
-                                   costPrefix.begin());
-                x.start[i] = std::max(x.start[i - 1], std::min(x.start[i], nsl));
-            }
-            x.start[8] = nsl;
-        }
-        *widest = 0;
-        for (int i = 0; i < 8; ++i) *widest = std::max(*widest, x.start[i + 1] - x.start[i]);
-        return x;
-    }
     DevBuf ck, tape, lam[2], apart, icpart, adjJobs;  // adjoint
     double ic_is2 = 1.0;                                 // adjoint: 1 / sigma^2 of the Gaussian IC
     std::vector<int> final_buf;                          // per position: phi buffer of the final state
@@ -374,18 +355,12 @@ int build_layout_t(Batch& b) {
     if ((rc = build_stat_fast(b.stream, sizeof(Real) == 4 ? GH_REAL_F32 : GH_REAL_F64, nc, b.statD.p, b.statSig.p,
                               b.statF, b.flag, &b.fast_ok)))
         return rc;
-    b.costPrefix.clear();
-    if (b.fast_ok && nsl > 0) {  // blocks without diffusion and the cost-balanced XCD split
+    if (b.fast_ok && nsl > 0) {  // blocks without diffusion
         if ((rc = b.zeroFlux.ensure_grow(nsl))) return rc;
         const BatchStepParams<Real> prm{b.slotPos.as<int32_t>(), b.range.as<SlotRange<Real>>()};
         hipLaunchKernelGGL((zero_flux_kernel<Real, BatchStepParams<Real>>), dim3((unsigned)nsl), dim3(256), 0,
                            b.stream, (int)nsl, b.statF.as<Stat4<Real>>(), prm, b.zeroFlux.as<unsigned char>());
         GH_HIP_CHECK(hipGetLastError());
-        std::vector<unsigned char> z(nsl);
-        GH_HIP_CHECK(hipMemcpyAsync(z.data(), b.zeroFlux.p, nsl, hipMemcpyDeviceToHost, b.stream));
-        GH_HIP_CHECK(hipStreamSynchronize(b.stream));
-        b.costPrefix.assign(nsl + 1, 0.0);
-        for (size_t i = 0; i < nsl; ++i) b.costPrefix[i + 1] = b.costPrefix[i] + (z[i] ? kZeroFluxCost : 1.0);
     }
     GH_HIP_CHECK(hipMemsetAsync(b.phi[0].p, 0, nc * sizeof(Real), b.stream));
     GH_HIP_CHECK(hipStreamSynchronize(b.stream));
@@ -535,12 +510,10 @@ struct ForwardSweep {
         Real* out = b.phi[1 - cur].as<Real>();
         if (two) {
             const int per_xcd = (nsl + 7) / 8;
-            int widest = 0;
-            const XcdSplit sp = b.split2f(nsl, &widest);
             if (b.fast_ok)
-                hipLaunchKernelGGL((rd_step2f_kernel<Real, BatchStepParams<Real>>), dim3(8 * widest), dim3(512), 0,
-                                   b.stream, sp, in, out, (Real*)nullptr, b.statF.as<Stat4<Real>>(),
-                                   b.zeroFlux.as<unsigned char>(), prm);
+                hipLaunchKernelGGL((rd_step2f_kernel<Real, BatchStepParams<Real>>), dim3(8 * per_xcd), dim3(512), 0,
+                                   b.stream, even_split(nsl), (const int2*)nullptr, in, out, (Real*)nullptr,
+                                   b.statF.as<Stat4<Real>>(), b.zeroFlux.as<unsigned char>(), prm);
             else
                 hipLaunchKernelGGL((rd_step2_kernel<Real, BatchStepParams<Real>>), dim3(8 * per_xcd), dim3(512), 0,
                                    b.stream, nsl, per_xcd, in, out, b.statD.as<Stat4<Real>>(), b.statSig.as<Real>(),
@@ -723,10 +696,9 @@ int adjoint_t(Batch& b, int K, const double* lik, int stepPET, int stepTi, int u
             char* in = tape + (size_t)(k - k0) * nsl * slot_bytes;
             const int nslots = (int)b.slot0[active(k)];
             if (b.fast_ok && k >= 1 && k + 2 <= k1) {
-                int widest = 0;
-                const XcdSplit sp = b.split2f(nslots, &widest);
-                hipLaunchKernelGGL((rd_step2f_kernel<Real, BatchStepParams<Real>>), dim3(8 * widest), dim3(512), 0,
-                                   b.stream, sp, (const Real*)in,
+                const int per_xcd = (nslots + 7) / 8;
+                hipLaunchKernelGGL((rd_step2f_kernel<Real, BatchStepParams<Real>>), dim3(8 * per_xcd), dim3(512), 0,
+                                   b.stream, even_split(nslots), (const int2*)nullptr, (const Real*)in,
                                    (Real*)(in + 2 * (size_t)nsl * slot_bytes), (Real*)(in + (size_t)nsl * slot_bytes),
                                    b.statF.as<Stat4<Real>>(), b.zeroFlux.as<unsigned char>(), fprm);
                 k += 2;

@@ -103,7 +103,9 @@ struct Ctx {
     bool fast_ok = false;  // every cell has p_w + p_g >= 0: clamped states may take rd_step2f_kernel
     DevBuf zeroFlux;       // per slot: 1 = block without diffusion (zero_flux_kernel), uniform grids
     int nZeroFlux = 0;     // blocks without diffusion
-    int split2f[9] = {0};  // rd_step2f_kernel's per-XCD slot ranges, balanced by block cost
+    DevBuf items2f;        // rd_step2f_kernel's work items {slot, blocks without diffusion} (plan_step2f)
+    int nItems2f = 0;      // 0: one item per slot
+    int split2f[9] = {0};  // per-XCD item ranges, balanced by estimated cost
     DevBuf nbr;         // 6 * nblocks int32
     std::vector<DevBuf> ckpt;  // device-side copies of phi (gh_checkpoint_*)
     std::vector<uint64_t> ckptVersion;  // tree_version each checkpoint was saved on

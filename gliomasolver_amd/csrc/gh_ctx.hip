@@ -64,7 +64,7 @@ void Ctx::release_all() {
                       &fgPw,     &fgPg,    &labBuf,     &labSlots,  &labEnt,    &labSrc,      &labW,    &labPerm,  &labAllEnt,
                       &labAllSrc, &labAllW, &labAllSlots, &dDense, &dBlk, &dBad, &spare[0], &spare[1], &spare[2], &spare[3],
                       &labSplit[0], &labSplit[1], &labSplit[2], &labSplit[3], &remapBuf, &detBuf, &phi[0],  &phi[1],    &dphidt,      &pw,      &pg,
-                      &pcsf,     &statD,   &statSig,    &statF,     &flag,      &zeroFlux,      &nbr,         &obsT1,   &obsFl,
+                      &pcsf,     &statD,   &statSig,    &statF,     &flag,      &zeroFlux,   &items2f,      &nbr,         &obsT1,   &obsFl,
                       &obsPet,   &obsRoi,  &likPart,    &adjCk,     &adjSeg,    &adjLam[0],   &adjLam[1], &adjPart};
     for (DevBuf* b : bufs) b->release();
     for (DevBuf& b : ckpt) b.release();

@@ -73,14 +73,17 @@ inline double block_h(const Ctx& c) {
     return std::ldexp(1.0, -c.topo.blocks[0].level) / BS;
 }
 
-// zero-flux flags of the uniform grid and the XCD split that balances them:
-// contiguous slot ranges (halos stay in one L2) of equal total cost
+// Work items of rd_step2f_kernel on a uniform grid: each block with diffusion
+// is an item; runs of consecutive blocks without diffusion (zero_flux_kernel)
+// are items of up to ZF_GROUP blocks, so those workgroups keep several blocks'
+// loads in flight.  The eight XCDs get contiguous item ranges (halos stay in
+// one L2) of equal estimated cost: a full block 1, a block without diffusion
+// kZeroFluxCost.
 template <typename Real>
 int plan_step2f(Ctx& c) {
     const int ns = c.topo.nblocks();
-    XcdSplit even = even_split(ns);
-    std::copy(even.start, even.start + 9, c.split2f);
     c.nZeroFlux = 0;
+    c.nItems2f = 0;
     if (!c.fast_ok || c.topo.uniform_bpd <= 0 || ns == 0) return GH_OK;
     int rc;
     if ((rc = c.zeroFlux.ensure_grow(ns))) return rc;
@@ -91,21 +94,39 @@ int plan_step2f(Ctx& c) {
     std::vector<unsigned char> z(ns);
     GH_HIP_CHECK(hipMemcpyAsync(z.data(), c.zeroFlux.p, ns, hipMemcpyDeviceToHost, c.stream));
     GH_HIP_CHECK(hipStreamSynchronize(c.stream));
-    double total = 0;
-    for (int s = 0; s < ns; ++s) {
-        c.nZeroFlux += z[s];
-        total += z[s] ? kZeroFluxCost : 1.0;
+    std::vector<int2> items;
+    std::vector<double> cost;
+    for (int s = 0; s < ns;) {
+        if (!z[s]) {
+            items.push_back(make_int2(s, 0));
+            cost.push_back(1.0);
+            ++s;
+            continue;
+        }
+        int k = 0;
+        while (s + k < ns && z[s + k] && k < ZF_GROUP) ++k;
+        items.push_back(make_int2(s, k));
+        cost.push_back(kZeroFluxCost * k);
+        c.nZeroFlux += k;
+        s += k;
     }
-    // split point i: the first slot whose prefix cost reaches i/8 of the total
+    const int ni = (int)items.size();
+    double total = 0;
+    for (double v : cost) total += v;
+    // split point x: the first item whose prefix cost reaches x/8 of the total
     double acc = 0;
     int x = 1;
     c.split2f[0] = 0;
-    for (int s = 0; s < ns && x < 8; ++s) {
-        acc += z[s] ? kZeroFluxCost : 1.0;
-        while (x < 8 && acc >= total * x / 8) c.split2f[x++] = s + 1;
+    for (int i = 0; i < ni && x < 8; ++i) {
+        acc += cost[i];
+        while (x < 8 && acc >= total * x / 8) c.split2f[x++] = i + 1;
     }
-    for (; x < 8; ++x) c.split2f[x] = ns;
-    c.split2f[8] = ns;
+    for (; x <= 8; ++x) c.split2f[x] = ni;
+    if ((rc = c.items2f.ensure_grow((size_t)ni * sizeof(int2)))) return rc;
+    GH_HIP_CHECK(hipMemcpyAsync(c.items2f.p, items.data(), (size_t)ni * sizeof(int2), hipMemcpyHostToDevice,
+                                c.stream));
+    GH_HIP_CHECK(hipStreamSynchronize(c.stream));
+    c.nItems2f = ni;
     return GH_OK;
 }
 
@@ -164,15 +185,17 @@ int launch_step2f(Ctx& c, const Real* in, Real* out, double dt, Real* mid = null
     const double h = block_h(c);
     const double ih2 = 1. / (h * h);
     const UniformStepParams<Real> prm{(Real)c.rho, ih2, dt, c.topo.uniform_bpd, bpd_magic(c.topo.uniform_bpd)};
-    XcdSplit split{};
-    int widest = 0;
-    for (int i = 0; i <= 8; ++i) split.start[i] = c.split2f[i];
-    for (int i = 0; i < 8; ++i) widest = std::max(widest, split.start[i + 1] - split.start[i]);
     (void)per_xcd;
-    if (split.start[8] != ns) return set_error(GH_ESTATE, "two-step launch: stale work split (call gh_rd_prepare)");
+    XcdSplit split = even_split(ns);
+    const int2* items = nullptr;
+    if (c.nItems2f > 0) {  // work items of the current prepare (plan_step2f)
+        for (int i = 0; i <= 8; ++i) split.start[i] = c.split2f[i];
+        items = c.items2f.as<int2>();
+    }
+    int widest = 0;
+    for (int i = 0; i < 8; ++i) widest = std::max(widest, split.start[i + 1] - split.start[i]);
     hipLaunchKernelGGL((rd_step2f_kernel<Real, UniformStepParams<Real>>), dim3(8 * widest), dim3(512), 0, c.stream,
-                       split, in, out, mid, c.statF.as<Stat4<Real>>(),
-                       c.nZeroFlux > 0 ? c.zeroFlux.as<unsigned char>() : (const unsigned char*)nullptr, prm);
+                       split, items, in, out, mid, c.statF.as<Stat4<Real>>(), (const unsigned char*)nullptr, prm);
     GH_HIP_CHECK(hipGetLastError());
     return GH_OK;
 }

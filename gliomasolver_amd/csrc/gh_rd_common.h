@@ -526,7 +526,7 @@ __device__ inline Real rd_rhs_reaction_only(Real pc, bool dom, Real rho) {
 // Relative cost of a block without diffusion in rd_step2f_kernel (its own
 // 24 B per cell, no halo, no LDS) against a full block (about 2.75 stencil
 // evaluations per cell): the weight the XCD split gives it.
-constexpr double kZeroFluxCost = 0.15;
+constexpr double kZeroFluxCost = 0.3;
 
 // Work split of the two-step kernel over the eight XCDs: workgroup w runs on
 // XCD w % 8 and serves block start[w % 8] + w / 8 while that is below
@@ -565,30 +565,53 @@ __device__ inline FaceMap face_map(int f, int d) {
 }
 __device__ inline int mad24(int a, int b, int c) { return (int)__umul24((unsigned)a, (unsigned)b) + c; }
 
-// zflux (may be null): per slot, 1 = a block without diffusion, whose two
-// steps are each cell's reaction update alone (same bits, no halo).
+// Work items: with items (may be null), workgroup item i is {slot, k}: k = 0
+// a full block, k >= 1 the k consecutive blocks from slot on, all without
+// diffusion (their two steps are each cell's reaction update alone: same
+// bits, no halo).  Without items, item i is slot i, and zflux (may be null)
+// marks the blocks without diffusion (k = 1).
+constexpr int ZF_GROUP = 8;  // blocks without diffusion per workgroup
 template <typename Real, typename Params>
-__global__ __launch_bounds__(512) void rd_step2f_kernel(XcdSplit split, const Real* __restrict__ phi,
-                                                        Real* __restrict__ out, Real* __restrict__ mid,
-                                                        const Stat4<Real>* __restrict__ stf,
+__global__ __launch_bounds__(512) void rd_step2f_kernel(XcdSplit split, const int2* __restrict__ items,
+                                                        const Real* __restrict__ phi, Real* __restrict__ out,
+                                                        Real* __restrict__ mid, const Stat4<Real>* __restrict__ stf,
                                                         const unsigned char* __restrict__ zflux, Params prm) {
     const int x = (int)(blockIdx.x % 8);
-    const int s = split.start[x] + (int)(blockIdx.x / 8);
-    if (s >= split.start[x + 1]) return;
+    const int it = split.start[x] + (int)(blockIdx.x / 8);
+    if (it >= split.start[x + 1]) return;
+    int s = it, nzf = 0;
+    if (items) {
+        const int2 v = items[it];
+        s = v.x;
+        nzf = v.y;
+    } else if (zflux && zflux[s]) {
+        nzf = 1;
+    }
     __shared__ Real s_uu[UU_SIZE];
     __shared__ Real s_dmx[1000], s_dmy[1000], s_dmz[1000];
 
     const int t = threadIdx.x;
-    if (zflux && zflux[s]) {  // one block per workgroup: the branch is uniform
-        const size_t bc = (size_t)s * BCELLS + (unsigned)t;
-        const Real u0 = phi[bc];
-        const bool in0 = sign_of(stf[bc].tau) != 0;
-        Real rho0;
-        double ih20, dt0;
-        prm.get(s, rho0, ih20, dt0);
-        const Real u1 = rd_update(u0, rd_rhs_reaction_only(u0, in0 || u0 > (Real)0., rho0), dt0);
-        if (mid) mid[bc] = u1;
-        out[bc] = rd_update(u1, rd_rhs_reaction_only(u1, in0 || u1 > (Real)0., rho0), dt0);
+    if (nzf > 0) {  // uniform per workgroup: nzf blocks, all loads issued before the updates
+        Real u0[ZF_GROUP];
+        bool in0[ZF_GROUP];
+#pragma unroll
+        for (int k = 0; k < ZF_GROUP; ++k)
+            if (k < nzf) {
+                const size_t bc = (size_t)(s + k) * BCELLS + (unsigned)t;
+                u0[k] = phi[bc];
+                in0[k] = sign_of(stf[bc].tau) != 0;
+            }
+#pragma unroll
+        for (int k = 0; k < ZF_GROUP; ++k)
+            if (k < nzf) {
+                const size_t bc = (size_t)(s + k) * BCELLS + (unsigned)t;
+                Real rho0;
+                double ih20, dt0;
+                prm.get(s + k, rho0, ih20, dt0);
+                const Real u1 = rd_update(u0[k], rd_rhs_reaction_only(u0[k], in0[k] || u0[k] > (Real)0., rho0), dt0);
+                if (mid) mid[bc] = u1;
+                out[bc] = rd_update(u1, rd_rhs_reaction_only(u1, in0[k] || u1 > (Real)0., rho0), dt0);
+            }
         return;
     }
     const int w = __builtin_amdgcn_readfirstlane(t >> 6);  // wave = interior plane z; halo role
