@@ -318,6 +318,7 @@ def batched_leg(nsamples, K, budget_gb, seq_samples, acc_samples, chunk_cells=0)
         identical &= bool(np.array_equal(loss[0], b_loss[i]) and np.array_equal(grad[0], b_grad[i]))
     seq_wall = time.perf_counter() - t0
     log("sequential: %d samples in %.2f s (identical to the batch: %s)" % (len(seq), seq_wall, identical))
+    anat32 = [b.get_anatomy(aids[a]) for a in range(len(st.sizes))]
     b.close()
     # FP32 state against the FP64 solve of the same samples
     accs = list(range(min(acc_samples, nsamples)))
@@ -325,11 +326,21 @@ def batched_leg(nsamples, K, budget_gb, seq_samples, acc_samples, chunk_cells=0)
     for a in range(len(st.sizes)):
         b64.set_observations(ids64[a], *obs[a])
     loss64, grad64, _ = solve(b64, accs)
+    # the grids the two precisions run on: _ic places cells and picks voxels in
+    # Real, so where 8*bpd is not a power of two FP32 and FP64 map some cells to
+    # neighbouring voxels (the reference's own behaviour)
+    map_diff = []
+    for a in range(len(st.sizes)):
+        pw64, pg64 = b64.get_anatomy(ids64[a])
+        pw32, pg32 = anat32[a]
+        map_diff.append(int(((np.abs(pw32 - pw64) > 1e-6) | (np.abs(pg32 - pg64) > 1e-6)).sum()))
     b64.close()
     rel_loss = np.abs(b_loss[accs, 0] - loss64[:, 0]) / np.abs(loss64[:, 0])
     rel_grad = np.abs(b_grad[accs] - grad64) / np.maximum(np.abs(grad64), 1e-300)
     worst = accs[int(np.argmax(rel_loss))]
     gap = loss_gap(st, worst, obs[st.anatomy[worst]], make_batch)
+    gap["anatomy_cells_differing_f32_f64"] = map_diff[st.anatomy[worst]]
+    same = [k for k, i in enumerate(accs) if map_diff[st.anatomy[i]] == 0]
     # work of the sequential sample against the whole stage's: evals/s of the plain path
     seq_share = float((cells[seq] * st.nsteps[seq]).sum()) / float((cells * st.nsteps).sum())
     seq_eval_s = len(seq) / seq_wall
@@ -361,7 +372,11 @@ def batched_leg(nsamples, K, budget_gb, seq_samples, acc_samples, chunk_cells=0)
                              "bit-identical results to the batch: %s" % (len(seq), 100 * seq_share, identical),
         "accuracy_vs_fp64": {"samples": len(accs), "loss_rel_max": float(rel_loss.max()),
                              "loss_rel_median": float(np.median(rel_loss)),
-                             "grad_rel_median": float(np.median(rel_grad)), "worst_sample": gap},
+                             "grad_rel_median": float(np.median(rel_grad)), "worst_sample": gap,
+                             "anatomy_cells_differing_f32_f64": dict(zip([str(n) for n in st.sizes], map_diff)),
+                             "samples_on_identical_grids": len(same),
+                             "loss_rel_max_identical_grids": float(rel_loss[same].max()) if same else None,
+                             "grad_rel_median_identical_grids": float(np.median(rel_grad[same])) if same else None},
     }
     return out
 

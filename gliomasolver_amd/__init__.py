@@ -460,6 +460,15 @@ class Batch:
         check(self.lib.gh_batch_get_phi(self.h, i, _ptr(out)))
         return out
 
+    def get_anatomy(self, anatomy: int):
+        """(p_w, p_g) of an anatomy on its grid, (n, n, n) each."""
+        if not 0 <= anatomy < len(self.sizes):
+            raise ValueError("get_anatomy: unknown anatomy %d" % anatomy)
+        n = self.sizes[anatomy]
+        pw, pg = np.zeros((n, n, n), self.dtype), np.zeros((n, n, n), self.dtype)
+        check(self.lib.gh_batch_get_anatomy(self.h, anatomy, _ptr(pw), _ptr(pg)))
+        return pw, pg
+
 
 class Tmcmc:
     """The TMCMC sampler (gh_tmcmc): ask for candidates, evaluate them (e.g.

@@ -82,6 +82,7 @@ SIGNATURES = {
     "gh_batch_adjoint": (_i, [_vp, _i, _vp, _i, _i, _i, _vp, _vp, _pd]),
     "gh_batch_sample_info": (_i, [_vp, _i, _pd, _pd, _pd, _pi64]),
     "gh_batch_get_phi": (_i, [_vp, _i, _vp]),
+    "gh_batch_get_anatomy": (_i, [_vp, _i, _vp, _vp]),
     "gh_tmcmc_create": (_i, [_i, _vp, _vp, _i, _i, _i, _i, _d, _d, _u64, C.POINTER(_vp)]),
     "gh_tmcmc_destroy": (_i, [_vp]),
     "gh_tmcmc_candidates": (_i, [_vp, _pi, _vp]),

@@ -982,4 +982,25 @@ int gh_batch_get_phi(gh_batch* bh, int sample, void* dense) {
     return GH_OK;
 }
 
+int gh_batch_get_anatomy(gh_batch* bh, int anatomy_id, void* p_w, void* p_g) {
+    Batch* b = as_batch(bh);
+    if (!b || anatomy_id < 0 || anatomy_id >= (int)b->anat.size())
+        return set_error(GH_EINVAL, "gh_batch_get_anatomy: no such anatomy");
+    GH_HIP_CHECK(hipSetDevice(b->device));
+    const BatchAnatomy& a = *b->anat[anatomy_id];
+    const size_t nc = a.nslots * BCELLS;
+    std::vector<unsigned char> bm(nc * b->real_size);
+    const DevBuf* src[2] = {&a.pw, &a.pg};
+    void* dst[2] = {p_w, p_g};
+    for (int f = 0; f < 2; ++f) {
+        if (!dst[f]) continue;
+        GH_HIP_CHECK(hipMemcpyAsync(bm.data(), src[f]->p, bm.size(), hipMemcpyDeviceToHost, b->stream));
+        GH_HIP_CHECK(hipStreamSynchronize(b->stream));
+        for (size_t c = 0; c < nc; ++c)
+            std::memcpy((unsigned char*)dst[f] + c * b->real_size, bm.data() + block_index(a.bpd, c) * b->real_size,
+                        b->real_size);
+    }
+    return GH_OK;
+}
+
 }  // extern "C"

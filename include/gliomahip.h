@@ -273,7 +273,12 @@ int gh_checkpoint_load(gh_ctx* ctx, int slot);
  * gh_batch_likelihood: lik_params = 5 doubles per sample (LikelihoodInput.txt
  *   order); out = 4 doubles per sample as gh_likelihood.
  * gh_batch_sample_info / gh_batch_get_phi: a sample's schedule and state
- *   (n^3 Real, x fastest). */
+ *   (n^3 Real, x fastest).
+ * gh_batch_get_anatomy: the p_w and p_g an anatomy's samples run on, as _ic
+ *   left them on the grid (Glioma_ReactionDiffusion.cpp:103-150: cell
+ *   position and voxel index in Real, so the FP32 and FP64 grids of one data
+ *   set can differ where 8*bpd is not a power of two); n^3 Real each, either
+ *   pointer may be NULL. */
 typedef struct gh_batch gh_batch;
 int gh_batch_create(int device, int real_kind, gh_batch** out);
 int gh_batch_destroy(gh_batch* b);
@@ -302,6 +307,7 @@ int gh_batch_adjoint(gh_batch* b, int ckpt_every, const double* lik_params, int 
                      double* loss, double* grad, double* elapsed_ms);
 int gh_batch_sample_info(gh_batch* b, int sample, double* Dw, double* Dg, double* dt, int64_t* nsteps);
 int gh_batch_get_phi(gh_batch* b, int sample, void* dense);
+int gh_batch_get_anatomy(gh_batch* b, int anatomy_id, void* p_w, void* p_g);
 
 /* ---- TMCMC sampler (host) ----------------------------------------------------
  * The sampler of tools/pi4u_lite/Inference/engine_tmcmc.c (main loop

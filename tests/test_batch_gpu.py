@@ -163,3 +163,32 @@ def test_tmcmc_glioma_app(tmp_path):
             nll, _, _ = ctx.likelihood(tuple(lk), stepPET=1, stepTi=1)
         F = gh.glioma_fitness(np.array([nll]))[0]
         assert abs(F - db0[i, 11]) <= 1e-9 * max(1.0, abs(F)), (i, F, db0[i, 11])
+
+
+@pytest.mark.parametrize("real", ["f32", "f64"])
+def test_batch_96_grid_long_run_matches_oracle(real, oracle):
+    """The 96^3 grid of the TMCMC stage (a bpd that is not a power of two, the
+    stage's most common size) over 240 Euler steps from a Gaussian seed near a
+    tissue boundary: the state equals the oracle bit for bit, and the grid the
+    batch runs on is the oracle's _ic mapping of the same data in the same
+    precision (gh_batch_get_anatomy)."""
+    from gliomasolver_amd import workloads as wl
+
+    t = np.float32 if real == "f32" else np.float64
+    gm, wm, csf, _ = gh.phantom(96, 5)
+    params = (1.3e-5, 0.08, 150.0, 80.0, 51.0, 37.0)
+    N = 240
+    say("batch 96^3", real, "%d steps" % N)
+    b = gh.Batch(0, real)
+    aid = b.add_anatomy(12, gm, wm, csf)
+    b.set_adjoint_samples([aid], [params], [N])
+    b.run()
+    phi = b.get_phi(0)
+    pw_b, pg_b = b.get_anatomy(aid)
+    b.close()
+    pw, pg, _, _ = oracle.ic(12, gm, wm, csf, None, real)
+    assert np.array_equal(pw_b, pw) and np.array_equal(pg_b, pg)
+    u0 = gh.gaussian_ic(real, pw, pg, params[3:], 1.0, 0.1)
+    ref = oracle.rd_steps(u0, pw, pg, t(params[0]), t(params[0] / 10.0), t(params[1]), t(1.0 / 96), params[2] / N, N)
+    assert np.array_equal(phi, ref)
+    assert phi.max() > 0.5                   # the tumour grew
