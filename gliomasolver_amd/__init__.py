@@ -225,6 +225,13 @@ class Context:
         check(self.lib.gh_adapt_profile(self.h, _ptr(ms), int(reset)))
         return dict(zip(self.ADAPT_PHASES, ms.tolist()))
 
+    def adapt_counters(self, reset: bool = False) -> dict:
+        """How ghosts were built since the last reset: lab batches on the
+        device / from host tables, face tables on the device / on the host."""
+        n = np.zeros(4, np.int64)
+        check(self.lib.gh_adapt_counters(self.h, _ptr(n), int(reset)))
+        return dict(zip(("labs_device", "labs_host", "faces_device", "faces_host"), n.tolist()))
+
     def set_field_blocks(self, field: int, values: np.ndarray):
         v = np.ascontiguousarray(values, self.dtype).reshape(self.nblocks(), 8, 8, 8)
         check(self.lib.gh_set_field_blocks(self.h, field, _ptr(v)))

@@ -44,6 +44,7 @@ SIGNATURES = {
     "gh_grid_refine_blocks": (_i, [_vp, _i, _vp, _pi]),
     "gh_grid_compress_blocks": (_i, [_vp, _i, _vp, _pi]),
     "gh_adapt_profile": (_i, [_vp, _vp, _i]),
+    "gh_adapt_counters": (_i, [_vp, _vp, _i]),
     "gh_set_field_blocks": (_i, [_vp, _i, _vp]),
     "gh_get_field_blocks": (_i, [_vp, _i, _vp]),
     "gh_uq_resample": (_i, [_vp, _i, _vp]),

@@ -122,18 +122,43 @@ __global__ __launch_bounds__(256) void ghost_entry_kernel(int n, const GhostEntr
 
 // Per-level block maps (Octree::dense on the device): map[off[l] + x +
 // (y << l) + (z << 2l)] = slot of a leaf at (l, x, y, z), -2 for a node with
-// children, -1 for a point inside a coarser leaf; levels >= nlev hold no node.
+// children, -1 for a point inside a coarser leaf, for levels < nlev; levels
+// nlev..top in the open-addressing table hkey/hval (Octree::hash_keys, 2^hbits
+// slots, linear probing, absent = -1); levels > top hold no node.
 struct LevelMaps {
     const int32_t* __restrict__ map;
     int off[Octree::MAXL + 1];
     int nlev;
+    int top;
+    const uint64_t* __restrict__ hkey;
+    const int32_t* __restrict__ hval;
+    int hbits;
 };
+
+// value of block position (L, x, y, z) in the hashed levels (Octree::node_hash;
+// the table is at most half full, so the probe reaches a free slot)
+__device__ inline int32_t hashed_node(const LevelMaps& m, int L, int x, int y, int z) {
+    const uint64_t k = ((uint64_t)L << 48) | ((uint64_t)(uint32_t)x << 32) | ((uint64_t)(uint32_t)y << 16) |
+                       (uint64_t)(uint32_t)z;
+    const uint32_t mask = (1u << m.hbits) - 1u;
+    uint32_t h = (uint32_t)((k * 0x9E3779B97F4A7C15ull) >> (64 - m.hbits));
+    for (;;) {
+        const uint64_t e = m.hkey[h];
+        if (e == k) return m.hval[h];
+        if (e == Octree::kEmptyKey) return -1;
+        h = (h + 1) & mask;
+    }
+}
 
 inline LevelMaps level_maps(const Ctx& c) {
     LevelMaps m{};
     m.map = c.dDense.as<int32_t>();
     for (int l = 0; l <= Octree::MAXL; ++l) m.off[l] = c.denseOff[l];
     m.nlev = c.nDense;
+    m.top = c.topLevel;
+    m.hkey = c.dHashKey.as<uint64_t>();
+    m.hval = c.dHashVal.as<int32_t>();
+    m.hbits = c.hashBits;
     return m;
 }
 
@@ -160,9 +185,9 @@ __device__ inline void ghost_visit(const LevelMaps& m, int order, int L, int qx,
         qx &= mask;
         qy &= mask;
         qz &= mask;
-        if (L >= m.nlev) break;  // inside a coarser leaf
+        if (L > m.top) break;  // inside a coarser leaf
         const int bx = (int)(qx >> 3), by = (int)(qy >> 3), bz = (int)(qz >> 3);
-        const int32_t v = m.map[m.off[L] + bx + (by << L) + (bz << (2 * L))];
+        const int32_t v = L < m.nlev ? m.map[m.off[L] + bx + (by << L) + (bz << (2 * L))] : hashed_node(m, L, bx, by, bz);
         if (v >= 0) {
             visit((int32_t)(v * BCELLS + (qx & 7) + 8 * (qy & 7) + 64 * (qz & 7)), w);
             return;
@@ -302,7 +327,7 @@ __global__ __launch_bounds__(256) void face_fill_packet_kernel(int n, const int3
     const int4 b = blk[s];
     int i[3];
     face_point(f, p & 7, p >> 3, i);
-    const int b0 = base[k] + p, len = (base[k + 1] - base[k]) >> 6;
+    const int b0 = base[k] + p;
     int j = 0, my_bad = 0;
     auto visit = [&](int32_t src, double w) {
         tsrc[b0 + 64 * j] = src;
@@ -310,7 +335,6 @@ __global__ __launch_bounds__(256) void face_fill_packet_kernel(int n, const int3
         ++j;
     };
     ghost_visit<2>(m, order, b.x, 8 * b.y + i[0], 8 * b.z + i[1], 8 * b.w + i[2], 1.0, visit, my_bad);
-    (void)len;
 }
 
 // face ghosts from packets: out[64 * faces[k] + p] = sum of f[src] * w over
@@ -340,8 +364,9 @@ __global__ __launch_bounds__(256) void face_packet_kernel(int n, const int32_t* 
 // Full labs of the listed slots (slots == nullptr: every slot j), ghosts
 // evaluated on the device; *bad is set when a ghost needs more than two
 // synthesis levels (the caller then uses host tables).
+constexpr int LAB_THREADS = 512;  // lanes per lab (256: 12 % slower epochs, 1024: no gain)
 template <typename Real>
-__global__ __launch_bounds__(256) void lab_kernel(int nlist, const int32_t* __restrict__ slots, int LS, int LAB,
+__global__ __launch_bounds__(LAB_THREADS) void lab_kernel(int nlist, const int32_t* __restrict__ slots, int LS, int LAB,
                                                   int order, const int32_t* __restrict__ nbr27,
                                                   const int4* __restrict__ blk, LevelMaps m,
                                                   const int16_t* __restrict__ perm, const Real* __restrict__ f,
@@ -530,10 +555,24 @@ __global__ __launch_bounds__(256) void statics_ghosted_kernel(int nslots, const 
     }
 }
 
+// Face-ghost packets as the step kernel reads them: packet of jump face
+// (slot, f) = pkOf[6 * slot + f] (-1: no jump face), its terms as in
+// face_packet_kernel
+template <typename Real>
+struct FacePackets {
+    const int32_t* __restrict__ pkOf;
+    const int32_t* __restrict__ base;
+    const int32_t* __restrict__ count;
+    const int32_t* __restrict__ tsrc;
+    const Real* __restrict__ tw;
+};
+
 // RD step on a multi-level grid: one workgroup per block, h of the block's
-// level; a face's halo comes from the same-level neighbour or from the
-// interpolated face ghosts
-template <typename Real, bool FUSED>
+// level; a face's halo comes from the same-level neighbour or, across a level
+// jump, from the interpolated face ghosts -- precomputed in fgPhi, or (PK)
+// summed here from the face's packet by the wave that owns the face, term by
+// term in face_packet_kernel's order, so both give the same bits
+template <typename Real, bool FUSED, bool PK = false>
 __global__ __launch_bounds__(512) void rd_step_ghosted_kernel(int nslots, int per_xcd, const int32_t* __restrict__ nbr,
                                                               const int32_t* __restrict__ level,
                                                               const Real* __restrict__ phi, Real* __restrict__ out,
@@ -541,7 +580,8 @@ __global__ __launch_bounds__(512) void rd_step_ghosted_kernel(int nslots, int pe
                                                               const Real* __restrict__ sig,
                                                               const Real* __restrict__ fgPhi,
                                                               const Real* __restrict__ fgTau,
-                                                              const Real* __restrict__ fgDm, Real rho, double dt) {
+                                                              const Real* __restrict__ fgDm, Real rho, double dt,
+                                                              FacePackets<Real> fp) {
     const int s = (int)(blockIdx.x % 8) * per_xcd + (int)(blockIdx.x / 8);
     if (s >= nslots) return;
     __shared__ Real s_phi[1000];
@@ -578,7 +618,20 @@ __global__ __launch_bounds__(512) void rd_step_ghosted_kernel(int nslots, int pe
             tau = hs.tau;
             dm = f == FXM ? hs.dx : (f == FYM ? hs.dy : hs.dz);
         } else {
-            ph = fgPhi[(size_t)s * 384 + t];
+            if (PK) {
+                const int k = fp.pkOf[6 * s + f];
+                const int p = t & 63;
+                const int b0 = fp.base[k] + p, len = fp.count[64 * k + p];
+                Real acc = (Real)0;
+#pragma unroll 4
+                for (int j = 0; j < len; ++j) {
+                    const Real prod = phi[fp.tsrc[b0 + 64 * j]] * fp.tw[b0 + 64 * j];
+                    acc = acc + prod;
+                }
+                ph = acc;
+            } else {
+                ph = fgPhi[(size_t)s * 384 + t];
+            }
             tau = fgTau[(size_t)s * 384 + t];
             if (f == FXM || f == FYM || f == FZM) dm = fgDm[(size_t)s * 192 + (f >> 1) * 64 + v * 8 + u];
         }
@@ -813,9 +866,14 @@ int device_face_table(Ctx& c, bool& done) {
     std::vector<int32_t>& faces = c.hFaces;
     faces.clear();
     const int nb = c.topo.nblocks();
+    std::vector<int32_t>& pk_of = c.hFacePkOf;
+    pk_of.assign(6 * (size_t)nb, -1);
     for (int s = 0; s < nb; ++s)
         for (int f = 0; f < 6; ++f)
-            if (c.topo.nbr[6 * (size_t)s + f] < 0) faces.push_back(6 * s + f);
+            if (c.topo.nbr[6 * (size_t)s + f] < 0) {
+                pk_of[6 * (size_t)s + f] = (int32_t)faces.size();
+                faces.push_back(6 * s + f);
+            }
     const int nf = (int)faces.size();
     const int n = nf * 64;
     c.nFaceEnt = 0;
@@ -829,9 +887,10 @@ int device_face_table(Ctx& c, bool& done) {
     DevBuf& dcount = c.faceCount;
     int rc;
     if ((rc = dfaces.ensure_grow(faces.size() * 4)) || (rc = dcount.ensure_grow((size_t)n * 4)) ||
-        (rc = c.facePkBase.ensure_grow(((size_t)nf + 1) * 4)))
+        (rc = c.facePkBase.ensure_grow(((size_t)nf + 1) * 4)) || (rc = c.facePkOf.ensure_grow(pk_of.size() * 4)))
         return rc;
     GH_HIP_CHECK(hipMemcpyAsync(dfaces.p, faces.data(), faces.size() * 4, hipMemcpyHostToDevice, c.stream));
+    GH_HIP_CHECK(hipMemcpyAsync(c.facePkOf.p, pk_of.data(), pk_of.size() * 4, hipMemcpyHostToDevice, c.stream));
     GH_HIP_CHECK(hipMemsetAsync(c.dBad.p, 0, sizeof(int), c.stream));
     GH_HIP_CHECK(hipMemsetAsync(c.facePkBase.as<int32_t>() + nf, 0, sizeof(int32_t), c.stream));
     const unsigned grid = (unsigned)((n + 255) / 256);
@@ -918,9 +977,21 @@ int apply_topology(Ctx& c) {
     // level maps and block coordinates for the device-side lab ghosts
     std::vector<int32_t> dense;
     c.nDense = t.dense_levels;
+    c.topLevel = t.top_level;
+    c.hashBits = t.hash_bits;
     for (int l = 0; l < c.nDense; ++l) {
         c.denseOff[l] = (int)dense.size();
         dense.insert(dense.end(), t.dense[l].begin(), t.dense[l].end());
+    }
+    if (c.hashBits > 0) {  // the tree outlives the asynchronous copies (synchronized below)
+        int rc2;
+        if ((rc2 = c.dHashKey.ensure_grow(t.hash_keys.size() * 8)) ||
+            (rc2 = c.dHashVal.ensure_grow(t.hash_vals.size() * 4)))
+            return rc2;
+        GH_HIP_CHECK(hipMemcpyAsync(c.dHashKey.p, t.hash_keys.data(), t.hash_keys.size() * 8, hipMemcpyHostToDevice,
+                                    c.stream));
+        GH_HIP_CHECK(hipMemcpyAsync(c.dHashVal.p, t.hash_vals.data(), t.hash_vals.size() * 4, hipMemcpyHostToDevice,
+                                    c.stream));
     }
     std::vector<int4> blk(nb);
     for (int s = 0; s < nb; ++s)
@@ -939,6 +1010,7 @@ int apply_topology(Ctx& c) {
         (void)hipStreamSynchronize(c.stream);  // host vectors above are being uploaded
         return rc;
     }
+    ++c.adapt_count[on_device ? 2 : 3];
     if (!on_device) {
         c.nFacePk = 0;
         HostTable ft = build_face_table(t, c.topo.nbr);
@@ -995,14 +1067,24 @@ int prepare_ghosted_t(Ctx& c, double Dw, double Dg) {
 
 template <typename Real, bool FUSED>
 int step_ghosted_t(Ctx& c, const Real* in, Real* out, double dt) {
-    int rc = face_ghosts<Real>(c, in, c.fgPhi.as<Real>());
-    if (rc) return rc;
     const int ns = c.topo.nblocks();
     const int per_xcd = (ns + 7) / 8;
+    if (c.nFacePk > 0) {  // ghosts summed inside the step kernel
+        const FacePackets<Real> fp{c.facePkOf.as<int32_t>(), c.facePkBase.as<int32_t>(), c.faceCount.as<int32_t>(),
+                                   c.faceSrc.as<int32_t>(), c.faceW.as<Real>()};
+        hipLaunchKernelGGL((rd_step_ghosted_kernel<Real, FUSED, true>), dim3(8 * per_xcd), dim3(512), 0, c.stream, ns,
+                           per_xcd, c.nbr.as<int32_t>(), c.dLevel.as<int32_t>(), in, out, c.statD.as<Stat4<Real>>(),
+                           c.statSig.as<Real>(), (const Real*)nullptr, c.fgTau.as<Real>(), c.fgDm.as<Real>(),
+                           (Real)c.rho, dt, fp);
+        GH_HIP_CHECK(hipGetLastError());
+        return GH_OK;
+    }
+    int rc = face_ghosts<Real>(c, in, c.fgPhi.as<Real>());
+    if (rc) return rc;
     hipLaunchKernelGGL((rd_step_ghosted_kernel<Real, FUSED>), dim3(8 * per_xcd), dim3(512), 0, c.stream, ns, per_xcd,
                        c.nbr.as<int32_t>(), c.dLevel.as<int32_t>(), in, out, c.statD.as<Stat4<Real>>(),
                        c.statSig.as<Real>(), c.fgPhi.as<Real>(), c.fgTau.as<Real>(), c.fgDm.as<Real>(), (Real)c.rho,
-                       dt);
+                       dt, FacePackets<Real>{});
     GH_HIP_CHECK(hipGetLastError());
     return GH_OK;
 }
@@ -1080,7 +1162,7 @@ int device_labs(Ctx& c, int nlist, const int32_t* dslots, const Real* field, Dev
     if (nlist == 0) return GH_OK;
     if ((rc = lab_order(c, order))) return rc;
     const LevelMaps m = level_maps(c);
-    hipLaunchKernelGGL(lab_kernel<Real>, dim3((unsigned)nlist), dim3(256), 0, c.stream, nlist, dslots, LS, LAB, order,
+    hipLaunchKernelGGL(lab_kernel<Real>, dim3((unsigned)nlist), dim3(LAB_THREADS), 0, c.stream, nlist, dslots, LS, LAB, order,
                        c.dNbr27.as<int32_t>(), c.dBlk.as<int4>(), m, c.labPerm.as<int16_t>(), field, out.as<Real>(),
                        c.dBad.as<int>());
     GH_HIP_CHECK(hipGetLastError());
@@ -1130,8 +1212,12 @@ int fwt_details_t(Ctx& c, const std::vector<int>& slots, std::vector<double>& de
         if (e == hipSuccess) e = hipMemcpyAsync(&bad, c.dBad.p, sizeof(int), hipMemcpyDeviceToHost, c.stream);
         if (e == hipSuccess) e = hipStreamSynchronize(c.stream);
         if (e != hipSuccess) return set_error(GH_EHIP, std::string("fwt: ") + hipGetErrorString(e));
-        if (bad == 0) return GH_OK;
+        if (bad == 0) {
+            ++c.adapt_count[0];
+            return GH_OK;
+        }
     }
+    ++c.adapt_count[1];
     // host-built lab table of every slot (cached per tree version)
     std::vector<int> all;  // source of an async upload: lives until the sync below
     if (c.labAllVersion != c.tree_version) {
@@ -1163,7 +1249,7 @@ int fwt_details_t(Ctx& c, const std::vector<int>& slots, std::vector<double>& de
 // old parent's lab) or a collapse of 8 old leaves; field arrays are rebuilt
 // on the device and the per-solve constants recomputed
 template <typename Real>
-int apply_new_tree(Ctx& c, const Octree& nt, const std::vector<Remap>& rm, const std::vector<int>& split_parents) {
+int apply_new_tree(Ctx& c, Octree& nt, const std::vector<Remap>& rm, const std::vector<int>& split_parents) {
     const int order = c.tree.order;
     const int LS = lab_start(order), LAB = lab_size(order);
     int rc;
@@ -1182,6 +1268,7 @@ int apply_new_tree(Ctx& c, const Octree& nt, const std::vector<Remap>& rm, const
                     return rc;
             on_device = device_labs_ok(c);
         }
+        ++c.adapt_count[on_device ? 0 : 1];
         if (!on_device) {
             HostTable tab = build_lab_table(c.tree, split_parents, c.topo.nbr27, order);
             if ((rc = upload_table<Real>(c, tab, c.labEnt, c.labSrc, c.labW))) return rc;
@@ -1206,7 +1293,7 @@ int apply_new_tree(Ctx& c, const Octree& nt, const std::vector<Remap>& rm, const
     c.prof[P_MOVE] += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_move).count();
     {
         ProfScope topo(c.prof[P_TOPO]);
-        c.tree = nt;
+        c.tree = std::move(nt);  // nt is the caller's scratch tree
         if ((rc = apply_topology(c))) return rc;
     }
     // the other state buffers follow the new size
@@ -1268,12 +1355,12 @@ int grid_refine(Ctx& c, const std::vector<char>& flagged, int* nrefined) {
     nt.rebuild_index();
     std::vector<Remap> rm(nt.leaves.size());
     for (int s = 0; s < (int)nt.leaves.size(); ++s) {
-        const uint64_t k = Octree::key(nt.leaves[s]);
+        const BlockRec& nb = nt.leaves[s];
         Remap r{};
-        auto it = t.leaf_slot.find(k);
-        if (it != t.leaf_slot.end()) {
+        int old_slot = -1;
+        if (t.kind(nb.level, nb.index[0], nb.index[1], nb.index[2], &old_slot) == Octree::LEAF) {
             r.kind = 0;
-            r.src = it->second;
+            r.src = old_slot;
         } else {  // a child of a split leaf: the parent is the old leaf one level up
             const BlockRec& ch = nt.leaves[s];
             int ps = -1;
@@ -1322,12 +1409,13 @@ int grid_compress(Ctx& c, const std::vector<char>& flagged, int* ncollapsed) {
     nt.rebuild_index();
     std::vector<Remap> rm(nt.leaves.size());
     for (int s = 0; s < (int)nt.leaves.size(); ++s) {
-        const uint64_t k = Octree::key(nt.leaves[s]);
+        const BlockRec& nb = nt.leaves[s];
+        const uint64_t k = Octree::key(nb);
         Remap r{};
-        auto it = t.leaf_slot.find(k);
-        if (it != t.leaf_slot.end()) {
+        int old_slot = -1;
+        if (t.kind(nb.level, nb.index[0], nb.index[1], nb.index[2], &old_slot) == Octree::LEAF) {
             r.kind = 0;
-            r.src = it->second;
+            r.src = old_slot;
         } else {
             r.kind = 2;
             const auto& kk = kids.at(k);
@@ -1380,7 +1468,9 @@ int gh_grid_init_leaves(gh_ctx* ctx, int nleaves, const int* level, const int* i
                                           index[3 * i + 2] >> (L - l)));
     }
     t.rebuild_index();
-    if ((int)t.leaf_slot.size() != nleaves) return set_error(GH_EINVAL, "gh_grid_init_leaves: duplicate leaf");
+    for (int s = 1; s < nleaves; ++s)  // canonical order puts equal leaves next to each other
+        if (Octree::key(t.leaves[s]) == Octree::key(t.leaves[s - 1]))
+            return set_error(GH_EINVAL, "gh_grid_init_leaves: duplicate leaf");
     for (auto& b : t.leaves)
         if (t.interior.count(Octree::key(b)))
             return set_error(GH_EINVAL, "gh_grid_init_leaves: a leaf contains another leaf");
@@ -1536,6 +1626,16 @@ int gh_grid_compress_blocks(gh_ctx* ctx, int n, const int* block_ids, int* ncoll
         if ((rc = grid_compress(*c, flagged, &nc))) return rc;
     }
     if (ncollapsed) *ncollapsed = nc;
+    return GH_OK;
+}
+
+int gh_adapt_counters(gh_ctx* ctx, int64_t* counts, int reset) {
+    Ctx* c = as_ctx(ctx);
+    if (!c) return set_error(GH_EINVAL, "null context");
+    if (counts)
+        for (int i = 0; i < 4; ++i) counts[i] = c->adapt_count[i];
+    if (reset)
+        for (int64_t& v : c->adapt_count) v = 0;
     return GH_OK;
 }
 

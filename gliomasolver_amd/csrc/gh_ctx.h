@@ -65,9 +65,11 @@ struct Ctx {
     DevBuf scanBuf;                  // face-table prefix sum: chunk sums, offsets, total
     int nFaceEnt = 0;
     // device-built face table in packets: per jump face, its 64 ghosts' term
-    // lists interleaved (term k of ghost p at facePkBase[face] + 64 k + p,
-    // src -1 past a ghost's own count), so a wave's loads are contiguous
+    // lists interleaved (term k of ghost p at facePkBase[face] + 64 k + p for
+    // k below its own term count in faceCount), so a wave's loads are contiguous
     DevBuf facePkBase;               // int32 per face + 1: packet starts (exclusive scan)
+    DevBuf facePkOf;                 // int32 per slot face: its packet (index in the face list), -1 if none
+    std::vector<int32_t> hFacePkOf;  // its host side (uploaded asynchronously, like hFaces)
     int nFacePk = 0;                 // faces in packets (0: the GhostEntry table is used)
     DevBuf fgPhi, fgTau, fgDm;       // per slot: 6*64 ghost phi, 6*64 ghost tau, 3*64 minus-face D
     DevBuf fgPw, fgPg;               // per slot: 6*64 ghost p_w, p_g (statics)
@@ -85,12 +87,18 @@ struct Ctx {
     DevBuf dDense, dBlk, dBad;
     int denseOff[Octree::MAXL + 1] = {0};
     int nDense = 0;
+    int topLevel = -1;           // finest leaf level
+    DevBuf dHashKey, dHashVal;   // nodes of levels nDense..topLevel (Octree::hash_keys / hash_vals)
+    int hashBits = 0;            // 0: no hashed levels
     double last_fwt_ms = 0;
     // adaptation scratch kept across epochs: spare field arrays swapped with
     // the live ones by apply_new_tree, split labs, remap list, FWT details
     DevBuf spare[4], labSplit[4], remapBuf, detBuf;
     // adaptation profile (gh_adapt_profile), host wall time in ms per phase
     double prof[8] = {0};
+    // how ghosts were built (gh_adapt_counters): lab batches on the device,
+    // lab batches from host tables, face tables on the device, from the host
+    int64_t adapt_count[4] = {0};
 
     DevBuf phi[2];      // ping-pong tumor density
     int cur = 0;        // phi[cur] holds the current state

@@ -3,7 +3,6 @@
 #include "gh_octree.h"
 
 #include <algorithm>
-#include <tuple>
 
 namespace gh {
 
@@ -15,8 +14,7 @@ Octree::Kind Octree::kind(int L, int x, int y, int z, int* slot) const {
     x = wrapi(x, n);
     y = wrapi(y, n);
     z = wrapi(z, n);
-    if (dense_levels > 0) {  // every level with a leaf or a subdivided node is dense
-        if (L >= dense_levels) return COARSER;
+    if (L < dense_levels) {
         const int32_t v = dense[L][(size_t)x + ((size_t)y << L) + ((size_t)z << (2 * L))];
         if (v >= 0) {
             if (slot) *slot = v;
@@ -24,14 +22,19 @@ Octree::Kind Octree::kind(int L, int x, int y, int z, int* slot) const {
         }
         return v == -2 ? FINER : COARSER;
     }
+    if (L > top_level || hash_bits == 0) return COARSER;  // no node below the finest leaf
     const uint64_t k = key(L, x, y, z);
-    auto it = leaf_slot.find(k);
-    if (it != leaf_slot.end()) {
-        if (slot) *slot = it->second;
-        return LEAF;
+    const size_t mask = ((size_t)1 << hash_bits) - 1;
+    for (size_t h = node_hash(k, hash_bits);; h = (h + 1) & mask) {
+        if (hash_keys[h] == kEmptyKey) return COARSER;
+        if (hash_keys[h] != k) continue;
+        const int32_t v = hash_vals[h];
+        if (v >= 0) {
+            if (slot) *slot = v;
+            return LEAF;
+        }
+        return v == -2 ? FINER : COARSER;
     }
-    if (interior.count(k)) return FINER;
-    return COARSER;
 }
 
 void Octree::init_uniform(int level) {
@@ -52,30 +55,62 @@ void Octree::init_uniform(int level) {
 
 // canonical slot order: (level, z, y, x) ascending
 void Octree::rebuild_index() {
-    std::sort(leaves.begin(), leaves.end(), [](const BlockRec& a, const BlockRec& b) {
-        return std::make_tuple(a.level, a.index[2], a.index[1], a.index[0]) <
-               std::make_tuple(b.level, b.index[2], b.index[1], b.index[0]);
-    });
-    leaf_slot.clear();
-    leaf_slot.reserve(leaves.size());
-    for (size_t s = 0; s < leaves.size(); ++s) leaf_slot[key(leaves[s])] = (int)s;
+    auto order_key = [](const BlockRec& b) {
+        return ((uint64_t)b.level << 48) | ((uint64_t)(uint32_t)b.index[2] << 32) |
+               ((uint64_t)(uint32_t)b.index[1] << 16) | (uint64_t)(uint32_t)b.index[0];
+    };
+    std::sort(leaves.begin(), leaves.end(),
+              [&](const BlockRec& a, const BlockRec& b) { return order_key(a) < order_key(b); });
     for (int l = 0; l < MAXL; ++l) dense[l].clear();
     dense_levels = 0;
+    top_level = -1;
+    hash_bits = 0;
+    hash_keys.clear();
+    hash_vals.clear();
+    if (leaves.empty()) return;
     const int top = levels_max();
-    if (leaves.empty() || top > DENSE_MAX) return;
-    for (int l = 0; l <= top; ++l) dense[l].assign((size_t)1 << (3 * l), -1);
+    const int nd = std::min(top, DENSE_MAX);
+    for (int l = 0; l <= nd; ++l) dense[l].assign((size_t)1 << (3 * l), -1);
+    size_t deep = 0;  // nodes below the dense levels
     for (uint64_t k : interior) {
         const int l = (int)(k >> 48);
         if (l > top) continue;
+        if (l > nd) {
+            ++deep;
+            continue;
+        }
         const size_t x = (k >> 32) & 0xffff, y = (k >> 16) & 0xffff, z = k & 0xffff;
         dense[l][x + (y << l) + (z << (2 * l))] = -2;
     }
     for (size_t s = 0; s < leaves.size(); ++s) {
         const BlockRec& b = leaves[s];
+        if (b.level > nd) {
+            ++deep;
+            continue;
+        }
         dense[b.level][(size_t)b.index[0] + ((size_t)b.index[1] << b.level) + ((size_t)b.index[2] << (2 * b.level))] =
             (int32_t)s;
     }
-    dense_levels = top + 1;
+    dense_levels = nd + 1;
+    top_level = top;
+    if (deep == 0) return;
+    hash_bits = 4;
+    while (((size_t)1 << hash_bits) < 2 * deep) ++hash_bits;
+    const size_t cap = (size_t)1 << hash_bits;
+    hash_keys.assign(cap, kEmptyKey);
+    hash_vals.assign(cap, -1);
+    auto put = [&](uint64_t k, int32_t v) {
+        size_t h = node_hash(k, hash_bits);
+        while (hash_keys[h] != kEmptyKey) h = (h + 1) & (cap - 1);
+        hash_keys[h] = k;
+        hash_vals[h] = v;
+    };
+    for (uint64_t k : interior) {
+        const int l = (int)(k >> 48);
+        if (l > nd && l <= top) put(k, -2);
+    }
+    for (size_t s = 0; s < leaves.size(); ++s)
+        if (leaves[s].level > nd) put(key(leaves[s]), (int32_t)s);
 }
 
 int Octree::levels_min() const {
@@ -350,12 +385,13 @@ std::vector<int> Octree::refinement_victims(const std::vector<char>& flagged) co
     // (the targets of a block depend on the tree only, so the closure does not
     // depend on the order the front is processed in)
     std::vector<char> victim(nl, 0);
-    std::vector<int> front, next, tg;
+    std::vector<int> front, next;
     for (int s = 0; s < nl; ++s)
         if (flag[s]) {
             victim[s] = 1;
             front.push_back(s);
         }
+    std::vector<int> tg;
     while (!front.empty()) {
         next.clear();
         for (int v : front) {

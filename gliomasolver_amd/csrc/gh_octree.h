@@ -37,15 +37,27 @@ struct Octree {
     int order = 4;         // interpolating wavelet order: 2 or 4 (MRAGWavelets_Interp{2nd,4th}Order)
     int max_level = 4;     // Refiner_SpaceExtension maxLevel
     int min_level = 0;     // compression floor (0 = MRAG: no floor)
-    std::vector<BlockRec> leaves;                 // slot order
-    std::unordered_map<uint64_t, int> leaf_slot;  // key -> slot
-    std::unordered_set<uint64_t> interior;        // keys of nodes that have children
-    // dense per-level form of the two maps for levels < dense_levels (built by
-    // rebuild_index when the finest leaf level is at most DENSE_MAX): slot of a
+    std::vector<BlockRec> leaves;           // slot order
+    std::unordered_set<uint64_t> interior;  // keys of nodes that have children
+    // dense per-level form of the two maps for levels < dense_levels =
+    // min(finest leaf level, DENSE_MAX) + 1 (built by rebuild_index): slot of a
     // leaf, -2 for a node with children, -1 for a point inside a coarser leaf
     static constexpr int DENSE_MAX = 7;
     int dense_levels = 0;
+    int top_level = -1;  // finest leaf level (-1: no leaves)
     std::vector<int32_t> dense[MAXL];
+    // the nodes of levels dense_levels..top_level (a grid finer than
+    // DENSE_MAX) in an open-addressing table, for kind() and the device: key(node) at
+    // slot node_hash(key) or the first free slot after it (linear probing),
+    // value as in dense; empty slots hold kEmptyKey; 2^hash_bits slots, at
+    // most half of them used
+    static constexpr uint64_t kEmptyKey = ~0ull;
+    int hash_bits = 0;
+    std::vector<uint64_t> hash_keys;
+    std::vector<int32_t> hash_vals;
+    static uint32_t node_hash(uint64_t k, int bits) {
+        return (uint32_t)((k * 0x9E3779B97F4A7C15ull) >> (64 - bits));
+    }
 
     static uint64_t key(int L, int x, int y, int z) {
         return ((uint64_t)L << 48) | ((uint64_t)(uint32_t)x << 32) | ((uint64_t)(uint32_t)y << 16) | (uint64_t)(uint32_t)z;
@@ -58,7 +70,7 @@ struct Octree {
     Kind kind(int L, int x, int y, int z, int* slot = nullptr) const;
 
     // a copy of the configuration and the node sets without the index
-    // (leaf_slot, dense maps): for building a new tree that rebuild_index
+    // (dense maps, hashed levels): for building a new tree that rebuild_index
     // then indexes
     Octree shell() const {
         Octree o;

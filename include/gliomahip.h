@@ -101,6 +101,13 @@ int gh_grid_compress_blocks(gh_ctx* ctx, int n, const int* block_ids, int* ncoll
  * ms[8] = {ref_fwt, ref, comp_fwt, comp, plan, move, topology, statics}.
  * reset != 0 zeroes the counters after reading.  ms may be NULL. */
 int gh_adapt_profile(gh_ctx* ctx, double* ms, int reset);
+/* gh_adapt_counters: how the ghosts of the adaptation were built since the
+ * last reset (no reference counterpart; for tuning and tests):
+ * counts[4] = {lab batches evaluated on the device, lab batches from
+ * host-built tables, face-ghost tables built on the device, on the host}.
+ * Host tables are used only for ghosts that need more than two synthesis
+ * levels; any depth of the tree is served on the device otherwise. */
+int gh_adapt_counters(gh_ctx* ctx, int64_t* counts, int reset);
 int gh_set_field_blocks(gh_ctx* ctx, int field, const void* values);
 int gh_get_field_blocks(gh_ctx* ctx, int field, void* values);
 int gh_uq_resample(gh_ctx* ctx, int gpd, double* out);

@@ -30,8 +30,8 @@ def bits_equal(a, b):
 
 def deep_grid(levels=8):
     """Refinements around one point down to level 8, 2:1-balanced: deeper than
-    the device-side per-level maps (levels <= 7), so labs come from host-built
-    ghost tables."""
+    the device's dense per-level maps (levels <= 7), so the finest level is
+    looked up in the hashed node table."""
     from amr_util import apply_refine, neighbours
     rows = canonical([[1, x, y, z] for z in range(2) for y in range(2) for x in range(2)])
     p = (0.3, 0.45, 0.6)
@@ -98,15 +98,41 @@ def test_gpu_fwt_details_bit_exact(oracle, real, order, grid):
 
 
 @pytest.mark.parametrize("order", [4, 2])
-def test_gpu_fwt_details_host_tables_on_deep_grid(oracle, order):
+def test_gpu_fwt_details_on_deep_grid(oracle, order):
+    """Labs and face ghosts of a level-8 grid evaluated on the device (no
+    host-built table), bit-exact to the oracle."""
     rows = deep_grid()
     fields = random_fields(rows, np.float32, 3)
     say("fwt deep", order, len(rows), "blocks, levels", rows[:, 0].min(), "-", rows[:, 0].max())
     ctx = make_ctx("f32", rows, order, fields, max_level=9)
     det = ctx.fwt_details()
+    counts = ctx.adapt_counters()
     ctx.close()
     ref = np.array([oracle.amr_fwt(order, oracle.amr_lab(rows, order, fields[FIELD_PHI], b)) for b in range(len(rows))])
     assert np.array_equal(det, ref), "max |diff| %g" % np.abs(det - ref).max()
+    assert counts["labs_device"] >= 1 and counts["labs_host"] == 0, counts
+    assert counts["faces_device"] >= 1 and counts["faces_host"] == 0, counts
+
+
+@pytest.mark.parametrize("real", ["f32", "f64"])
+def test_gpu_multilevel_rd_steps_on_deep_grid(oracle, real):
+    """RD steps on the level-8 grid: face ghosts from packets built through the
+    hashed levels, bit-exact to the oracle."""
+    rows = deep_grid()
+    t = np.float32 if real == "f32" else np.float64
+    fields = random_fields(rows, t, 4)
+    say("multilevel rd deep", real, len(rows), "blocks")
+    ctx = make_ctx(real, rows, 4, fields, max_level=9)
+    Dw, Dg, rho, dt, k = 1.5e-7, 1.5e-8, 0.08, 0.02, 5
+    ctx.rd_prepare(Dw, Dg, rho)
+    ctx.rd_steps(k, dt)
+    out = ctx.get_field_blocks(FIELD_PHI).reshape(len(rows), 512)
+    counts = ctx.adapt_counters()
+    ctx.close()
+    ref = oracle.amr_rd_steps(rows, 4, fields[FIELD_PHI], fields[FIELD_PW], fields[FIELD_PG], t(Dw), t(Dg),
+                              t(rho), dt, k)
+    assert bits_equal(out, ref), "max |diff| %g" % np.abs(out.astype(float) - ref).max()
+    assert counts["faces_host"] == 0, counts
 
 
 @pytest.mark.parametrize("real", ["f32", "f64"])
