@@ -413,7 +413,38 @@ def loss_gap(st, i, obs, make_batch):
             "loss_rel_without_crossings": abs(lmix - l64) / abs(l64)}
 
 
-def adaptive_leg(tend, epoch_steps, max_epochs):
+def adaptive_cpu_baseline(rows, phi, pw, pg, Dw, Dg, rho, dt, epoch_steps, target_s=8.0):
+    """Bounded CPU sample of the adaptive workload on the final grid of the GPU
+    run: the oracle's multi-level step (per block and step, labs of phi, p_w and
+    p_g with interpolated ghosts -- the Cells a BlockLab loads, MRAGBlockLab.h
+    :154-301 -- then the operator) and its FWT details of every block (one
+    adaptation pass), on one host thread."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from conftest import Oracle  # checker library, used here only as the CPU baseline
+    o = Oracle()
+    nb = len(rows)
+    t0 = time.perf_counter()
+    o.amr_rd_steps(rows, 4, phi, pw, pg, np.float32(Dw), np.float32(Dg), np.float32(rho), dt, 1)
+    t1 = time.perf_counter() - t0
+    k = int(max(1, min(epoch_steps * 20, (target_s / 2) / max(t1, 1e-6))))
+    t0 = time.perf_counter()
+    o.amr_rd_steps(rows, 4, phi, pw, pg, np.float32(Dw), np.float32(Dg), np.float32(rho), dt, k)
+    el = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    o.amr_details(rows, 4, phi, list(range(nb)))
+    fwt_s = time.perf_counter() - t0
+    steps_rate = nb * 512 * k / el
+    # one epoch = epoch_steps steps + two detail passes (refine, compress)
+    epoch_est = epoch_steps * el / k + 2 * fwt_s
+    return {"value": steps_rate, "unit": "cell-updates/s", "cores": 1, "kind": "port",
+            "sample": "%d multi-level Euler steps (%.1f s) and one FWT-details pass (%.2f s) on the final grid of "
+                      "the GPU run (%d blocks, levels %d-%d), oracle restatement of the MRAG lab + operator, 1 thread"
+                      % (k, el, fwt_s, nb, int(rows[:, 0].min()), int(rows[:, 0].max())),
+            "fwt_details_s": fwt_s, "epoch_s_est": epoch_est,
+            "cell_updates_per_s_with_adaptation": nb * 512 * epoch_steps / epoch_est}
+
+
+def adaptive_leg(tend, epoch_steps, max_epochs, cpu=False):
     """The wavelet-adaptive workload: levels 3-5 of 8^3 blocks (256^3 at the
     finest), 4th-order interpolating wavelets, FP32 state, seeded phantom,
     Dw = 0.013 cm^2/day, rho = 0.025/day, Gaussian IC at a WM voxel; start on
@@ -463,6 +494,14 @@ def adaptive_leg(tend, epoch_steps, max_epochs):
     wall = time.perf_counter() - t0
     prof = {k: round(v, 1) for k, v in ctx.adapt_profile().items()}
     log("adaptive: epoch profile (ms)", json.dumps(prof))
+    cpu_b = None
+    if cpu:
+        lev, idx = ctx.blocks()
+        rows = np.c_[lev, idx].astype(np.int32)
+        fields = [ctx.get_field_blocks(f).reshape(len(rows), 512) for f in (FIELD_PHI, FIELD_PW, FIELD_PG)]
+        log("adaptive: cpu baseline ...")
+        cpu_b = adaptive_cpu_baseline(rows, *fields, Dw, Dg, 0.025, dt, epoch_steps)
+        log("adaptive: cpu baseline", json.dumps(cpu_b))
     ctx.close()
     return {"metric": "adaptive RD leaf cell-updates/s", "value": leaf_updates / wall, "unit": "cell-updates/s",
             "config": {"workload": "wavelet-adaptive levels 3-5 (256^3 finest), order-4 interpolating wavelets, "
@@ -471,7 +510,7 @@ def adaptive_leg(tend, epoch_steps, max_epochs):
                        "blocks_min_mean_max": [min(blocks), float(np.mean(blocks)), max(blocks)]},
             "step_only_cell_updates_per_s": leaf_updates / step_s, "step_s": step_s, "epoch_s": epoch_s,
             "epoch_ms_mean": 1e3 * epoch_s / max(1, epochs), "initial_compress_s": t_init, "wall_s": wall,
-            "epoch_profile_ms": prof}
+            "epoch_profile_ms": prof, "cpu_baseline": cpu_b}
 
 
 def main():
@@ -504,7 +543,7 @@ def main():
         return 2
 
     if a.workload == "adaptive":
-        print(json.dumps(adaptive_leg(500.0, 10, a.adaptive_epochs)), flush=True)
+        print(json.dumps(adaptive_leg(500.0, 10, a.adaptive_epochs, cpu=not a.no_cpu)), flush=True)
         return 0
 
     import gliomasolver_amd as gh
@@ -585,7 +624,7 @@ def main():
         line["adjoint"] = adj
     if not a.no_adaptive:
         log("adaptive run ...")
-        line["adaptive"] = adaptive_leg(500.0, 10, a.adaptive_epochs)
+        line["adaptive"] = adaptive_leg(500.0, 10, a.adaptive_epochs, cpu=not a.no_cpu)
     if not a.no_batch:
         log("batched TMCMC stage ...")
         line["batched"] = batched_leg(a.batch_samples, a.batch_ckpt, a.batch_budget_gb, a.batch_seq, a.batch_acc,
