@@ -365,12 +365,20 @@ __global__ __launch_bounds__(256) void face_packet_kernel(int n, const int32_t* 
 // evaluated on the device; *bad is set when a ghost needs more than two
 // synthesis levels (the caller then uses host tables).
 constexpr int LAB_THREADS = 512;  // lanes per lab (256: 12 % slower epochs, 1024: no gain)
+// up to four fields whose labs are built together: the ghost recursion (map
+// lookups, terms, weights) runs once and each field accumulates its own sum
+// with the single-field arithmetic, so every lab has the bits it has alone
 template <typename Real>
+struct LabFields {
+    const Real* __restrict__ f[4];
+    Real* __restrict__ lab[4];
+};
+template <typename Real, int NF>
 __global__ __launch_bounds__(LAB_THREADS) void lab_kernel(int nlist, const int32_t* __restrict__ slots, int LS, int LAB,
                                                   int order, const int32_t* __restrict__ nbr27,
                                                   const int4* __restrict__ blk, LevelMaps m,
-                                                  const int16_t* __restrict__ perm, const Real* __restrict__ f,
-                                                  Real* __restrict__ lab, int* __restrict__ bad) {
+                                                  const int16_t* __restrict__ perm, LabFields<Real> lf,
+                                                  int* __restrict__ bad) {
     const int j = blockIdx.x;
     if (j >= nlist) return;
     const int s = slots ? slots[j] : j;
@@ -382,20 +390,27 @@ __global__ __launch_bounds__(LAB_THREADS) void lab_kernel(int nlist, const int32
         const int ix = p % LAB + LS, iy = (p / LAB) % LAB + LS, iz = p / (LAB * LAB) + LS;
         const int code = dir_code(ix, iy, iz);
         const int src = code == 13 ? s : nbr27[27 * s + code];
-        Real v;
+        Real v[NF];
         if (src >= 0) {
             const int cx = (ix + 8) & 7, cy = (iy + 8) & 7, cz = (iz + 8) & 7;
-            v = f[(size_t)src * BCELLS + cx + 8 * cy + 64 * cz];
+            const size_t c = (size_t)src * BCELLS + cx + 8 * cy + 64 * cz;
+#pragma unroll
+            for (int k = 0; k < NF; ++k) v[k] = lf.f[k][c];
         } else {
-            Real acc = (Real)0;
+#pragma unroll
+            for (int k = 0; k < NF; ++k) v[k] = (Real)0;
             auto visit = [&](int32_t src, double w) {
-                const Real prod = f[src] * (Real)w;
-                acc = acc + prod;
+                const Real wr = (Real)w;
+#pragma unroll
+                for (int k = 0; k < NF; ++k) {
+                    const Real prod = lf.f[k][src] * wr;
+                    v[k] = v[k] + prod;
+                }
             };
             ghost_visit<2>(m, order, b.x, 8 * b.y + ix, 8 * b.z + iy, 8 * b.w + iz, 1.0, visit, my_bad);
-            v = acc;
         }
-        lab[(size_t)j * lab3 + p] = v;
+#pragma unroll
+        for (int k = 0; k < NF; ++k) lf.lab[k][(size_t)j * lab3 + p] = v[k];
     }
     if (my_bad) atomicOr(bad, 1);
 }
@@ -453,22 +468,31 @@ __global__ __launch_bounds__(512) void fwt_kernel(int nlist, int LS, int LAB, WT
 // new leaf from the old grid: copy, split (Hs synthesis from the parent's
 // lab, BlockSplitter::_splitBlocks) or collapse (Ha injection of the
 // children, BlockCollapser::_collapseBlocks)
+// the four fields of a remap (phi, p_w, p_g, p_csf): old arrays, their split
+// labs and the new arrays
 template <typename Real>
-__global__ __launch_bounds__(512) void remap_kernel(int nnew, const Remap* __restrict__ rm,
-                                                    const Real* __restrict__ old, const Real* __restrict__ lab,
-                                                    int LS, int LAB, WTaps<Real> tp, Real* __restrict__ out) {
+struct RemapFields {
+    const Real* __restrict__ old[4];
+    const Real* __restrict__ lab[4];
+    Real* __restrict__ out[4];
+};
+template <typename Real>
+__global__ __launch_bounds__(512) void remap_kernel(int nnew, const Remap* __restrict__ rm, RemapFields<Real> rf,
+                                                    int LS, int LAB, WTaps<Real> tp) {
     const int ns = blockIdx.x;
     if (ns >= nnew) return;
     const Remap r = rm[ns];
     const int t = threadIdx.x;
     const int x = t & 7, y = (t >> 3) & 7, z = t >> 6;
-    Real v;
+    Real v[4];
     if (r.kind == 0) {
-        v = old[(size_t)r.src * BCELLS + t];
+#pragma unroll
+        for (int f = 0; f < 4; ++f) v[f] = rf.old[f][(size_t)r.src * BCELLS + t];
     } else if (r.kind == 2) {
         const int o = (x >> 2) | ((y >> 2) << 1) | ((z >> 2) << 2);
         const int c = 2 * (x & 3) + 8 * (2 * (y & 3)) + 64 * (2 * (z & 3));
-        v = (Real)0 + old[(size_t)r.kids[o] * BCELLS + c];
+#pragma unroll
+        for (int f = 0; f < 4; ++f) v[f] = (Real)0 + rf.old[f][(size_t)r.kids[o] * BCELLS + c];
     } else {
         // destination in the parent's doubled lattice
         const int d[3] = {8 * (r.octant & 1) + x, 8 * ((r.octant >> 1) & 1) + y, 8 * ((r.octant >> 2) & 1) + z};
@@ -482,8 +506,9 @@ __global__ __launch_bounds__(512) void remap_kernel(int nnew, const Remap* __res
             lo[a] = max(S0, ((d[a] - 1 + tp.hs0) >> 1) + 1);  // arithmetic shift: floor division by 2
             hi[a] = min(E0 - 1, (d[a] - 1 + tp.he0) >> 1);
         }
-        const Real* L = lab + (size_t)r.src * LAB * LAB * LAB;
-        Real acc = (Real)0;
+        const size_t L0 = (size_t)r.src * LAB * LAB * LAB;
+#pragma unroll
+        for (int f = 0; f < 4; ++f) v[f] = (Real)0;
         for (int sz = lo[2]; sz <= hi[2]; ++sz) {
             const int kz = 2 * sz - d[2];
             const double wZ = (double)tp.hs[kz - tp.hs0];
@@ -493,14 +518,18 @@ __global__ __launch_bounds__(512) void remap_kernel(int nnew, const Remap* __res
                 for (int sx = lo[0]; sx <= hi[0]; ++sx) {
                     const int kx = 2 * sx - d[0];
                     const Real w = (Real)(wZwY * (double)tp.hs[kx - tp.hs0]);
-                    const Real prod = L[(sz - LS) * LAB * LAB + (sy - LS) * LAB + (sx - LS)] * w;
-                    acc = acc + prod;
+                    const size_t q = L0 + (sz - LS) * LAB * LAB + (sy - LS) * LAB + (sx - LS);
+#pragma unroll
+                    for (int f = 0; f < 4; ++f) {
+                        const Real prod = rf.lab[f][q] * w;
+                        v[f] = v[f] + prod;
+                    }
                 }
             }
         }
-        v = acc;
     }
-    out[(size_t)ns * BCELLS + t] = v;
+#pragma unroll
+    for (int f = 0; f < 4; ++f) rf.out[f][(size_t)ns * BCELLS + t] = v[f];
 }
 
 // per-cell constants on a multi-level grid: as rd_statics_kernel, with the
@@ -1154,17 +1183,29 @@ int lab_order(Ctx& c, int order) {
 // with the ghosts evaluated on the device; clears and then may set c.dBad
 // (a ghost deeper than the kernel's recursion: use host tables instead).
 template <typename Real>
-int device_labs(Ctx& c, int nlist, const int32_t* dslots, const Real* field, DevBuf& out) {
+int device_labs(Ctx& c, int nlist, const int32_t* dslots, int nf, const Real* const* fields, DevBuf* const* outs) {
     const int order = c.tree.order;
     const int LS = lab_start(order), LAB = lab_size(order), lab3 = LAB * LAB * LAB;
     int rc;
-    if ((rc = out.ensure_grow(std::max<size_t>(1, (size_t)nlist) * lab3 * sizeof(Real)))) return rc;
+    LabFields<Real> lf{};
+    for (int k = 0; k < nf; ++k) {
+        if ((rc = outs[k]->ensure_grow(std::max<size_t>(1, (size_t)nlist) * lab3 * sizeof(Real)))) return rc;
+        lf.f[k] = fields[k];
+        lf.lab[k] = outs[k]->as<Real>();
+    }
     if (nlist == 0) return GH_OK;
     if ((rc = lab_order(c, order))) return rc;
     const LevelMaps m = level_maps(c);
-    hipLaunchKernelGGL(lab_kernel<Real>, dim3((unsigned)nlist), dim3(LAB_THREADS), 0, c.stream, nlist, dslots, LS, LAB, order,
-                       c.dNbr27.as<int32_t>(), c.dBlk.as<int4>(), m, c.labPerm.as<int16_t>(), field, out.as<Real>(),
-                       c.dBad.as<int>());
+    if (nf == 1)
+        hipLaunchKernelGGL((lab_kernel<Real, 1>), dim3((unsigned)nlist), dim3(LAB_THREADS), 0, c.stream, nlist, dslots,
+                           LS, LAB, order, c.dNbr27.as<int32_t>(), c.dBlk.as<int4>(), m, c.labPerm.as<int16_t>(), lf,
+                           c.dBad.as<int>());
+    else if (nf == 4)
+        hipLaunchKernelGGL((lab_kernel<Real, 4>), dim3((unsigned)nlist), dim3(LAB_THREADS), 0, c.stream, nlist, dslots,
+                           LS, LAB, order, c.dNbr27.as<int32_t>(), c.dBlk.as<int4>(), m, c.labPerm.as<int16_t>(), lf,
+                           c.dBad.as<int>());
+    else
+        return set_error(GH_EINVAL, "device_labs: 1 or 4 fields");
     GH_HIP_CHECK(hipGetLastError());
     return GH_OK;
 }
@@ -1203,7 +1244,9 @@ int fwt_details_t(Ctx& c, const std::vector<int>& slots, std::vector<double>& de
         const int m = (int)slots.size();
         if ((rc = c.labSlots.ensure_grow((size_t)m * 4)) || (rc = clear_bad(c))) return rc;
         GH_HIP_CHECK(hipMemcpyAsync(c.labSlots.p, slots.data(), (size_t)m * 4, hipMemcpyHostToDevice, c.stream));
-        if ((rc = device_labs<Real>(c, m, c.labSlots.as<int32_t>(), c.phi[c.cur].as<Real>(), c.labBuf))) return rc;
+        const Real* fld[1] = {c.phi[c.cur].as<Real>()};
+        DevBuf* out[1] = {&c.labBuf};
+        if ((rc = device_labs<Real>(c, m, c.labSlots.as<int32_t>(), 1, fld, out))) return rc;
         hipLaunchKernelGGL(fwt_kernel<Real>, dim3((unsigned)m), dim3(512), 0, c.stream, m, lab_start(order),
                            lab_size(order), taps<Real>(order), c.labBuf.as<Real>(), d.as<double>());
         int bad = 1;
@@ -1262,10 +1305,13 @@ int apply_new_tree(Ctx& c, Octree& nt, const std::vector<Remap>& rm, const std::
             if ((rc = c.labSlots.ensure_grow(split_parents.size() * 4)) || (rc = clear_bad(c))) return rc;
             GH_HIP_CHECK(hipMemcpyAsync(c.labSlots.p, split_parents.data(), split_parents.size() * 4,
                                         hipMemcpyHostToDevice, c.stream));
-            for (int f = 0; f < 4; ++f)
-                if ((rc = device_labs<Real>(c, (int)split_parents.size(), c.labSlots.as<int32_t>(),
-                                            fields[f]->as<Real>(), labs[f])))
-                    return rc;
+            const Real* fld[4];
+            DevBuf* out[4];
+            for (int f = 0; f < 4; ++f) {
+                fld[f] = fields[f]->as<Real>();
+                out[f] = &labs[f];
+            }
+            if ((rc = device_labs<Real>(c, (int)split_parents.size(), c.labSlots.as<int32_t>(), 4, fld, out))) return rc;
             on_device = device_labs_ok(c);
         }
         ++c.adapt_count[on_device ? 0 : 1];
@@ -1281,12 +1327,16 @@ int apply_new_tree(Ctx& c, Octree& nt, const std::vector<Remap>& rm, const std::
     if ((rc = drm.ensure_grow(rm.size() * sizeof(Remap)))) return rc;
     GH_HIP_CHECK(hipMemcpyAsync(drm.p, rm.data(), rm.size() * sizeof(Remap), hipMemcpyHostToDevice, c.stream));
     DevBuf* nf = c.spare;  // the new arrays go to the spares, which then swap with the live ones
+    RemapFields<Real> rf{};
     for (int f = 0; f < 4; ++f) {
         if ((rc = nf[f].ensure_grow((size_t)nnew * BCELLS * sizeof(Real)))) return rc;
-        hipLaunchKernelGGL(remap_kernel<Real>, dim3(nnew), dim3(512), 0, c.stream, nnew, drm.as<Remap>(),
-                           fields[f]->as<Real>(), labs[f].as<Real>(), LS, LAB, taps<Real>(order), nf[f].as<Real>());
-        GH_HIP_CHECK(hipGetLastError());
+        rf.old[f] = fields[f]->as<Real>();
+        rf.lab[f] = labs[f].as<Real>();
+        rf.out[f] = nf[f].as<Real>();
     }
+    hipLaunchKernelGGL(remap_kernel<Real>, dim3(nnew), dim3(512), 0, c.stream, nnew, drm.as<Remap>(), rf, LS, LAB,
+                       taps<Real>(order));
+    GH_HIP_CHECK(hipGetLastError());
     // rm (and the split list) stay alive in the caller until after
     // apply_topology, whose uploads synchronize the stream
     for (int f = 0; f < 4; ++f) std::swap(*fields[f], nf[f]);
