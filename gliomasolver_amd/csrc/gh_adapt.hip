@@ -1429,11 +1429,12 @@ int grid_refine(Ctx& c, const std::vector<char>& flagged, int* nrefined) {
 }
 
 // Grid::compress with Compressor (MRAGrid.inl:383-486)
-int grid_compress(Ctx& c, const std::vector<char>& flagged, int* ncollapsed) {
+int grid_compress(Ctx& c, const std::vector<char>& flagged, int* ncollapsed, std::vector<uint64_t>* parents_out) {
     const auto t_plan = std::chrono::steady_clock::now();
     const Octree& t = c.tree;
     std::vector<uint64_t> parents = t.collapse_parents(flagged);
     if (ncollapsed) *ncollapsed = (int)parents.size();
+    if (parents_out) *parents_out = parents;
     if (parents.empty()) return GH_OK;
     Octree nt = t.shell();
     std::unordered_map<uint64_t, std::array<int, 8>> kids;
@@ -1589,22 +1590,132 @@ int gh_refine(gh_ctx* ctx, double tol, int max_loops, int* nrefined) {
     return GH_OK;
 }
 
-// Science::AutomaticCompression<0,0>(grid, fwt, tol, loops) (MRAGScienceCore.h:117-177)
+// Which blocks a set of changed boxes can reach.  A block's lab -- hence its
+// FWT details -- reads the leaves within its own box widened by two of its
+// block widths on every side (periodically): the lab spans [-2, 11) cells of
+// the block's level, and a ghost in a coarser region adds at most two
+// synthesis levels of 3 and 6 of those cells (order 4; less for order 2), 12 <
+// 16 cells in all; a finer region answers with the coincident point.  A
+// compression changes the tree and the data only inside the collapsed
+// parents' boxes, so a block whose widened box misses all of them keeps its
+// lab bit for bit.  The boxes are rasterized at level R = min(finest, 7) with
+// a summed-volume table; blocks finer than R are tested conservatively.
+class ChangedBoxes {
+  public:
+    ChangedBoxes(int finest, const std::vector<uint64_t>& boxes) : R_(std::min(finest, 7)), n_(1 << R_) {
+        const size_t m = (size_t)(n_ + 1);
+        sum_.assign(m * m * m, 0);
+        for (uint64_t k : boxes) {
+            const int L = (int)(k >> 48);
+            int lo[3], hi[3];
+            const int idx[3] = {(int)((k >> 32) & 0xffff), (int)((k >> 16) & 0xffff), (int)(k & 0xffff)};
+            for (int a = 0; a < 3; ++a) {
+                if (L <= R_) {
+                    lo[a] = idx[a] << (R_ - L);
+                    hi[a] = (idx[a] + 1) << (R_ - L);
+                } else {
+                    lo[a] = idx[a] >> (L - R_);
+                    hi[a] = lo[a] + 1;
+                }
+            }
+            for (int z = lo[2]; z < hi[2]; ++z)
+                for (int y = lo[1]; y < hi[1]; ++y)
+                    for (int x = lo[0]; x < hi[0]; ++x) at(x + 1, y + 1, z + 1) = 1;
+        }
+        for (int z = 1; z <= n_; ++z)  // summed-volume table
+            for (int y = 1; y <= n_; ++y)
+                for (int x = 1; x <= n_; ++x)
+                    at(x, y, z) += at(x - 1, y, z) + at(x, y - 1, z) + at(x, y, z - 1) - at(x - 1, y - 1, z) -
+                                   at(x - 1, y, z - 1) - at(x, y - 1, z - 1) + at(x - 1, y - 1, z - 1);
+    }
+    // true if block (L, idx) widened by two of its widths meets a changed box
+    bool reaches(const BlockRec& b) const {
+        int lo[2][3], hi[2][3], nseg[3];
+        for (int a = 0; a < 3; ++a) {
+            int l, h;  // [l, h) at level R, before the periodic wrap
+            if (b.level <= R_) {
+                l = (b.index[a] - 2) << (R_ - b.level);
+                h = (b.index[a] + 3) << (R_ - b.level);
+            } else {
+                const int sh = b.level - R_;
+                l = (b.index[a] - 2) >> sh;                  // floor
+                h = (b.index[a] + 3 + (1 << sh) - 1) >> sh;  // ceil
+            }
+            const int len = h - l;
+            nseg[a] = 1;
+            if (len >= n_) {
+                lo[0][a] = 0;
+                hi[0][a] = n_;
+                continue;
+            }
+            l = ((l % n_) + n_) % n_;
+            h = l + len;
+            lo[0][a] = l;
+            hi[0][a] = std::min(h, n_);
+            if (h > n_) {
+                lo[1][a] = 0;
+                hi[1][a] = h - n_;
+                nseg[a] = 2;
+            }
+        }
+        for (int i = 0; i < nseg[0]; ++i)
+            for (int j = 0; j < nseg[1]; ++j)
+                for (int k = 0; k < nseg[2]; ++k)
+                    if (box_sum(lo[i][0], hi[i][0], lo[j][1], hi[j][1], lo[k][2], hi[k][2]) > 0) return true;
+        return false;
+    }
+
+  private:
+    int R_, n_;
+    std::vector<int32_t> sum_;
+    int32_t& at(int x, int y, int z) { return sum_[(size_t)x + (size_t)(n_ + 1) * ((size_t)y + (size_t)(n_ + 1) * z)]; }
+    int32_t get(int x, int y, int z) const {
+        return sum_[(size_t)x + (size_t)(n_ + 1) * ((size_t)y + (size_t)(n_ + 1) * z)];
+    }
+    int32_t box_sum(int x0, int x1, int y0, int y1, int z0, int z1) const {
+        return get(x1, y1, z1) - get(x0, y1, z1) - get(x1, y0, z1) - get(x1, y1, z0) + get(x0, y0, z1) +
+               get(x0, y1, z0) + get(x1, y0, z0) - get(x0, y0, z0);
+    }
+};
+
+// Science::AutomaticCompression<0,0>(grid, fwt, tol, loops) (MRAGScienceCore.h:117-177).
+// Every loop needs the details of every block not yet found nice; a block
+// whose lab no collapse of the previous loop could reach (ChangedBoxes) keeps
+// the detail computed then -- the same lab, so the same value -- and only the
+// others are evaluated again (all of them when the previous loop's labs came
+// from host tables, whose recursion is not bounded by two levels).
 int gh_compress(gh_ctx* ctx, double tol, int max_loops, int* ncollapsed) {
     Ctx* c = as_ctx(ctx);
     if (!c) return set_error(GH_EINVAL, "null context");
     GH_HIP_CHECK(hipSetDevice(c->device));
     std::unordered_set<uint64_t> nice;
+    std::unordered_map<uint64_t, double> known;  // details valid on the current tree
     int loops = 0, total = 0;
     while (true) {
-        std::vector<int> cand;
+        std::vector<int> cand, need;
+        std::vector<size_t> need_at;
+        std::vector<double> det;
         for (int s = 0; s < c->topo.nblocks(); ++s)
             if (!nice.count(Octree::key(c->tree.leaves[s]))) cand.push_back(s);
-        std::vector<double> det;
+        det.assign(cand.size(), 0.0);
+        for (size_t i = 0; i < cand.size(); ++i) {
+            auto it = known.find(Octree::key(c->tree.leaves[cand[i]]));
+            if (it != known.end()) {
+                det[i] = it->second;
+            } else {
+                need.push_back(cand[i]);
+                need_at.push_back(i);
+            }
+        }
         int rc;
+        bool host_tables = false;
         {
             ProfScope fwt(c->prof[P_COMP_FWT]);
-            if ((rc = fwt_details(*c, cand, det))) return rc;
+            std::vector<double> dn;
+            const int64_t host_before = c->adapt_count[1];
+            if ((rc = fwt_details(*c, need, dn))) return rc;
+            host_tables = c->adapt_count[1] != host_before;
+            for (size_t j = 0; j < need.size(); ++j) det[need_at[j]] = dn[j];
         }
         std::vector<char> flagged(c->topo.nblocks(), 0);
         int nflag = 0;
@@ -1618,14 +1729,24 @@ int gh_compress(gh_ctx* ctx, double tol, int max_loops, int* ncollapsed) {
         }
         if (nflag == 0) break;
         int nc = 0;
+        std::vector<uint64_t> parents;
+        const int finest = c->tree.top_level;
+        std::vector<BlockRec> cand_blocks(cand.size());
+        for (size_t i = 0; i < cand.size(); ++i) cand_blocks[i] = c->tree.leaves[cand[i]];
         {
             ProfScope comp(c->prof[P_COMP]);
-            if ((rc = grid_compress(*c, flagged, &nc))) return rc;
+            if ((rc = grid_compress(*c, flagged, &nc, &parents))) return rc;
         }
         if (nc == 0) break;
         total += nc;
         ++loops;
         if (max_loops > 0 && loops >= max_loops) break;
+        known.clear();
+        if (!host_tables) {
+            const ChangedBoxes changed(finest, parents);
+            for (size_t i = 0; i < cand.size(); ++i)
+                if (!changed.reaches(cand_blocks[i])) known[Octree::key(cand_blocks[i])] = det[i];
+        }
     }
     if (ncollapsed) *ncollapsed = total;
     return GH_OK;

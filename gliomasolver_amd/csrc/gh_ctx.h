@@ -163,7 +163,9 @@ int rd_prepare_ghosted(Ctx& c, double Dw, double Dg);
 int rd_step_ghosted(Ctx& c, const void* in, void* out, double dt, bool fused);
 int fwt_details(Ctx& c, const std::vector<int>& slots, std::vector<double>& det);
 int grid_refine(Ctx& c, const std::vector<char>& flagged, int* nrefined);
-int grid_compress(Ctx& c, const std::vector<char>& flagged, int* ncollapsed);
+// parents (may be null) receives the keys of the new parent leaves
+int grid_compress(Ctx& c, const std::vector<char>& flagged, int* ncollapsed,
+                  std::vector<uint64_t>* parents = nullptr);
 
 // likelihood helpers (gh_lik.hip): -logL of the current phi; dJ/du of a state buffer
 int lik_eval_current(Ctx& c, const double* params, int stepPET, int stepTi, int use_roi, double* out,

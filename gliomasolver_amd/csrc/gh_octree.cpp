@@ -277,29 +277,27 @@ bool Octree::has_finer_direct_neighbour(int s) const {
     return false;
 }
 
-// Every neighbour of leaf s (periodic images included) is within one level of
-// L, for s at level L + 1: a subdivided region holds leaves of level >= L + 2;
-// a coarser region lies in the leaf found walking up, which must be at level
-// >= L - 1 (the walk of neighbours()).
-bool Octree::neighbours_within_one_of(int s, int L) const {
-    const int K = leaves[s].level;
+bool Octree::outside_within_one(int L, const int P[3]) const {
+    const int n = 1 << L;
     for (int code = 0; code < 27; ++code) {
         if (code == 13) continue;
-        int P[3];
-        bool wrapped;
-        const Kind k = neighbour_region(s, code, P, wrapped);
-        if (k == LEAF) {
-            if (std::abs(K - L) > 1) return false;
-        } else if (k == FINER) {
-            return false;
-        } else {
-            for (int l = K - 1; l >= 0; --l) {
-                const int sh = K - l;
-                if (kind(l, P[0] >> sh, P[1] >> sh, P[2] >> sh) == LEAF) {
-                    if (std::abs(l - L) > 1) return false;
-                    break;
-                }
+        const int d[3] = {code % 3 - 1, (code / 3) % 3 - 1, code / 9 - 1};
+        const int Q[3] = {wrapi(P[0] + d[0], n), wrapi(P[1] + d[1], n), wrapi(P[2] + d[2], n)};
+        const Kind k = kind(L, Q[0], Q[1], Q[2]);
+        if (k == LEAF) continue;
+        if (k == COARSER) {  // the leaf covering Q must be one level up, not coarser
+            if (L == 0 || kind(L - 1, Q[0] >> 1, Q[1] >> 1, Q[2] >> 1) != LEAF) return false;
+            continue;
+        }
+        // subdivided: the children of Q that face the node must be leaves
+        for (int o = 0; o < 8; ++o) {
+            const int off[3] = {o & 1, (o >> 1) & 1, (o >> 2) & 1};
+            bool faces = true;
+            for (int a = 0; a < 3; ++a) {
+                if (d[a] < 0 && off[a] != 1) faces = false;  // child on the max side faces the node
+                if (d[a] > 0 && off[a] != 0) faces = false;
             }
+            if (faces && kind(L + 1, 2 * Q[0] + off[0], 2 * Q[1] + off[1], 2 * Q[2] + off[2]) != LEAF) return false;
         }
     }
     return true;
@@ -413,30 +411,27 @@ std::vector<int> Octree::refinement_victims(const std::vector<char>& flagged) co
 
 std::vector<uint64_t> Octree::collapse_parents(const std::vector<char>& flagged) const {
     std::vector<uint64_t> out;
-    // candidate parents: interior nodes whose 8 children are leaves
-    std::vector<uint64_t> cand;  // ascending, distinct
-    cand.reserve(leaves.size());
+    // candidate parents: interior nodes whose 8 children are leaves -- each
+    // found once, from its child of octant 0 (a leaf with even coordinates)
+    std::vector<uint64_t> cand;
     for (const BlockRec& b : leaves)
-        if (b.level > 0) cand.push_back(key(b.level - 1, b.index[0] >> 1, b.index[1] >> 1, b.index[2] >> 1));
-    std::sort(cand.begin(), cand.end());
-    cand.erase(std::unique(cand.begin(), cand.end()), cand.end());
+        if (b.level > 0 && ((b.index[0] | b.index[1] | b.index[2]) & 1) == 0)
+            cand.push_back(key(b.level - 1, b.index[0] >> 1, b.index[1] >> 1, b.index[2] >> 1));
     for (uint64_t pk : cand) {
         const int L = (int)(pk >> 48);
         if (L < min_level) continue;
         const int px = (int)((pk >> 32) & 0xffff), py = (int)((pk >> 16) & 0xffff), pz = (int)(pk & 0xffff);
         bool ok = true;
-        int kids[8];
         for (int o = 0; o < 8 && ok; ++o) {
             int slot;
             if (kind(L + 1, 2 * px + (o & 1), 2 * py + ((o >> 1) & 1), 2 * pz + ((o >> 2) & 1), &slot) != LEAF) ok = false;
             else if (!(slot < (int)flagged.size() && flagged[slot])) ok = false;
-            else kids[o] = slot;
         }
         // every neighbour of every child within one level of the parent (MRAGCompressor.h:95-107)
-        for (int o = 0; o < 8 && ok; ++o)
-            if (!neighbours_within_one_of(kids[o], L)) ok = false;
-        if (ok) out.push_back(pk);
+        const int P[3] = {px, py, pz};
+        if (ok && outside_within_one(L, P)) out.push_back(pk);
     }
+    std::sort(out.begin(), out.end());  // ascending keys
     return out;
 }
 

@@ -97,7 +97,12 @@ struct Octree {
     std::vector<Nbr> neighbours(int slot) const;
     Kind neighbour_region(int slot, int code, int P[3], bool& wrapped, int* region_slot = nullptr) const;
     bool has_finer_direct_neighbour(int slot) const;
-    bool neighbours_within_one_of(int slot, int L) const;
+    // every leaf touching node (L, P)'s box from outside (periodic images
+    // included) is at level L - 1, L or L + 1: the same as every neighbour of
+    // each of the node's eight children being within one level of L, since a
+    // child's neighbours are its siblings (level L + 1) and leaves touching
+    // the node's box, and every leaf touching the box touches a child
+    bool outside_within_one(int L, const int P[3]) const;
     void chain_targets(int slot, std::vector<int>& out) const;
     // same-level neighbour slot in each of the 27 directions (-1 if none)
     void same_level_neighbours(int slot, int out[27]) const;

@@ -207,3 +207,73 @@ def test_oracle_resample_uniform_is_identity_and_coarse_replicates(oracle):
     assert np.array_equal(out, dense)
     out2 = oracle.amr_resample(rows, vals, 32)  # every cell covers 2^3 finest cells
     assert np.array_equal(out2[::2, ::2, ::2], dense) and np.array_equal(out2[1::2, 1::2, 1::2], dense)
+
+
+def _far_from(rows_b, parents, top):
+    """Block (L, x, y, z) dilated by two of its own block widths (periodic)
+    shares no point with any parent box."""
+    L = int(rows_b[0])
+    n = 1 << top
+    s = 1 << (top - L)
+    for P in parents:
+        sp = 1 << (top - int(P[0]))
+        hit = True
+        for a in range(3):
+            if 5 * s >= n:          # the dilated interval covers the whole axis
+                continue
+            lo = ((int(rows_b[1 + a]) - 2) * s) % n
+            hi = lo + 5 * s         # [lo, hi) at level top, possibly past n (wrap)
+            plo, phi = int(P[1 + a]) * sp, (int(P[1 + a]) + 1) * sp
+            if not any(max(lo - k * n, plo) < min(hi - k * n, phi) for k in (0, 1)):
+                hit = False
+                break
+        if hit:
+            return False
+    return True
+
+
+@pytest.mark.parametrize("order", [4, 2])
+def test_lab_unchanged_two_blocks_away_from_a_collapse(oracle, order):
+    """The lab of a block -- and so its FWT details -- depends only on the
+    leaves within two of its block widths: a collapse whose parent boxes stay
+    outside that reach leaves the lab bit-identical (the argument behind the
+    compression pass reusing details between its loops: lab [-2, 11) and at
+    most two synthesis levels of 3 and 6 cells reach 12 < 16 cells)."""
+    from amr_util import apply_collapse
+    rng = np.random.default_rng(17 + order)
+    far_checked = near_changed = 0
+    for trial in range(6):
+        rows = balanced_random_grid(rng, base_level=3, max_level=5, nrefine=4)
+        top = int(rows[:, 0].max())
+        cand = collapse_parents(rows, np.ones(len(rows), np.uint8), 0)
+        if not cand:
+            continue
+        pick = [cand[i] for i in sorted(rng.choice(len(cand), size=min(2, len(cand)), replace=False))]
+        new = apply_collapse(rows, pick)
+        old_index = {tuple(r): i for i, r in enumerate(rows.tolist())}
+        phi_old = rng.uniform(0, 1, (len(rows), 512)).astype(np.float32)
+        phi_new = np.zeros((len(new), 512), np.float32)
+        for j, r in enumerate(new.tolist()):
+            if tuple(r) in old_index:
+                phi_new[j] = phi_old[old_index[tuple(r)]]
+            else:                                   # a collapsed parent: the coincident child vertices
+                L, px, py, pz = r
+                c = np.arange(512)
+                cx, cy, cz = c & 7, (c >> 3) & 7, c >> 6
+                octant = (cx >> 2) | ((cy >> 2) << 1) | ((cz >> 2) << 2)
+                src = 2 * (cx & 3) + 8 * 2 * (cy & 3) + 64 * 2 * (cz & 3)
+                for o in range(8):
+                    k = old_index[(L + 1, 2 * px + (o & 1), 2 * py + ((o >> 1) & 1), 2 * pz + ((o >> 2) & 1))]
+                    m = octant == o
+                    phi_new[j, m] = phi_old[k, src[m]]
+        for j, r in enumerate(new.tolist()):
+            if tuple(r) not in old_index:
+                continue
+            a = oracle.amr_lab(rows, order, phi_old, old_index[tuple(r)])
+            b = oracle.amr_lab(new, order, phi_new, j)
+            if _far_from(r, pick, top):
+                far_checked += 1
+                assert np.array_equal(a.view(np.uint32), b.view(np.uint32)), (trial, r, pick)
+            elif not np.array_equal(a, b):
+                near_changed += 1
+    assert far_checked > 200 and near_changed > 10, (far_checked, near_changed)
