@@ -1034,20 +1034,9 @@ int apply_topology(Ctx& c) {
         GH_HIP_CHECK(hipMemcpyAsync(c.dBlk.p, blk.data(), blk.size() * sizeof(int4), hipMemcpyHostToDevice, c.stream));
     GH_HIP_CHECK(hipMemcpyAsync(c.dNbr27.p, n27.data(), n27.size() * 4, hipMemcpyHostToDevice, c.stream));
     GH_HIP_CHECK(hipMemcpyAsync(c.dLevel.p, lev.data(), lev.size() * 4, hipMemcpyHostToDevice, c.stream));
-    bool on_device = false;
-    if (c.nDense > 0 && (rc = device_face_table(c, on_device))) {
-        (void)hipStreamSynchronize(c.stream);  // host vectors above are being uploaded
-        return rc;
-    }
-    ++c.adapt_count[on_device ? 2 : 3];
-    if (!on_device) {
-        c.nFacePk = 0;
-        HostTable ft = build_face_table(t, c.topo.nbr);
-        c.nFaceEnt = (int)ft.ent.size();
-        rc = c.real_kind == GH_REAL_F32 ? upload_table<float>(c, ft, c.faceEnt, c.faceSrc, c.faceW)
-                                        : upload_table<double>(c, ft, c.faceEnt, c.faceSrc, c.faceW);
-        if (rc) return rc;
-    }
+    c.faces_stale = true;  // built by ensure_faces when an RD step or the constants need them
+    c.nFacePk = 0;
+    c.nFaceEnt = 0;
     const size_t rs = c.real_size;
     rc = c.fgPhi.ensure_grow(std::max(1, nb) * 384 * rs);
     if (!rc) rc = c.fgTau.ensure_grow(std::max(1, nb) * 384 * rs);
@@ -1060,6 +1049,27 @@ int apply_topology(Ctx& c) {
 }
 
 namespace {
+
+// the face-ghost table of the current grid (device-built, or host-built when
+// a ghost needs deeper recursion), once per grid change
+int ensure_faces(Ctx& c) {
+    if (!c.faces_stale) return GH_OK;
+    bool on_device = false;
+    int rc;
+    if (c.nDense > 0 && (rc = device_face_table(c, on_device))) return rc;
+    ++c.adapt_count[on_device ? 2 : 3];
+    if (!on_device) {
+        c.nFacePk = 0;
+        HostTable ft = build_face_table(c.tree, c.topo.nbr);
+        c.nFaceEnt = (int)ft.ent.size();
+        rc = c.real_kind == GH_REAL_F32 ? upload_table<float>(c, ft, c.faceEnt, c.faceSrc, c.faceW)
+                                        : upload_table<double>(c, ft, c.faceEnt, c.faceSrc, c.faceW);
+        if (rc) return rc;
+        GH_HIP_CHECK(hipStreamSynchronize(c.stream));  // the table's host vectors are being uploaded
+    }
+    c.faces_stale = false;
+    return GH_OK;
+}
 
 template <typename Real>
 int face_ghosts(Ctx& c, const Real* field, Real* out) {
@@ -1081,7 +1091,7 @@ int face_ghosts(Ctx& c, const Real* field, Real* out) {
 template <typename Real>
 int prepare_ghosted_t(Ctx& c, double Dw, double Dg) {
     int rc;
-    if ((rc = face_ghosts<Real>(c, c.pw.as<Real>(), c.fgPw.as<Real>())) ||
+    if ((rc = ensure_faces(c)) || (rc = face_ghosts<Real>(c, c.pw.as<Real>(), c.fgPw.as<Real>())) ||
         (rc = face_ghosts<Real>(c, c.pg.as<Real>(), c.fgPg.as<Real>())))
         return rc;
     const int ns = c.topo.nblocks();
@@ -1096,6 +1106,8 @@ int prepare_ghosted_t(Ctx& c, double Dw, double Dg) {
 
 template <typename Real, bool FUSED>
 int step_ghosted_t(Ctx& c, const Real* in, Real* out, double dt) {
+    int rc0 = ensure_faces(c);
+    if (rc0) return rc0;
     const int ns = c.topo.nblocks();
     const int per_xcd = (ns + 7) / 8;
     if (c.nFacePk > 0) {  // ghosts summed inside the step kernel
@@ -1351,8 +1363,7 @@ int apply_new_tree(Ctx& c, Octree& nt, const std::vector<Remap>& rm, const std::
     if ((rc = c.phi[1 - c.cur].ensure_grow(n)) || (rc = c.dphidt.ensure_grow(n)) || (rc = c.statD.ensure_grow(4 * n)) ||
         (rc = c.statSig.ensure_grow(n)))
         return rc;
-    ProfScope stat(c.prof[P_STAT]);
-    if (c.prepared) return rd_prepare(c, c.Dw, c.Dg, c.rho);
+    if (c.prepared) c.constants_stale = true;  // rebuilt by the next RD call (rd_ensure_prepared)
     return GH_OK;
 }
 
@@ -1749,7 +1760,10 @@ int gh_compress(gh_ctx* ctx, double tol, int max_loops, int* ncollapsed) {
         }
     }
     if (ncollapsed) *ncollapsed = total;
-    return GH_OK;
+    // the grid a compression leaves is the one that steps next: its constants
+    // (and those of any refinement before it) are built here, once
+    ProfScope stat(c->prof[P_STAT]);
+    return rd_ensure_prepared(*c);
 }
 
 // the listed blocks as per-slot flags; ids outside [0, nblocks) are refused

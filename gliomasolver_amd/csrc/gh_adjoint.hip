@@ -148,6 +148,7 @@ extern "C" int gh_rd_adjoint(gh_ctx* ctx, int64_t nsteps, double T, int ckpt_eve
     if (nsteps < 1 || ckpt_every < 1 || ckpt_every > 4096 || !(T > 0) || !(ic_sigma > 0) || !(c->Dw > 0))
         return set_error(GH_EINVAL, "gh_rd_adjoint: bad argument");
     GH_HIP_CHECK(hipSetDevice(c->device));
+    if (int rc = rd_ensure_prepared(*c)) return rc;
     return c->real_kind == GH_REAL_F32
                ? adjoint_t<float>(*c, nsteps, T, ckpt_every, lik_params, stepPET, stepTi, use_roi, ic_center,
                                   ic_sigma, loss, grad)

@@ -128,6 +128,10 @@ struct Ctx {
     // lik_ready) on a uniform grid of the same size
     bool lik_valid() const { return lik_ready && lik_version == tree_version && topo.uniform_bpd > 0; }
     bool prepared = false;
+    // set by a grid change: the face-ghost tables of the RD step, and (when
+    // prepared) the per-solve constants, are rebuilt on their next use, so a
+    // grid that the same adaptation pass changes again is never prepared
+    bool faces_stale = false, constants_stale = false;
     double Dw = 0, Dg = 0, rho = 0;
     double dt_last = 0;  // dt of the running gh_rd_steps call
     int rd_bpw = 2;      // blocks per workgroup of the step kernel (1, 2 or 4)
@@ -142,6 +146,8 @@ Ctx* as_ctx(gh_ctx* c);
 
 // RD launchers (gh_rd.hip)
 int rd_prepare(Ctx& c, double Dw, double Dg, double rho);
+// the constants of the current grid when a grid change left them stale
+int rd_ensure_prepared(Ctx& c);
 int rd_rhs(Ctx& c);
 int rd_update(Ctx& c, double dt);
 int rd_steps(Ctx& c, int64_t nsteps, double dt);

@@ -393,7 +393,7 @@ int gh_rd_prepare(gh_ctx* ctx, double Dw, double Dg, double rho) {
     if (!c) return set_error(GH_EINVAL, "null context");
     if (c->topo.nblocks() == 0) return set_error(GH_ESTATE, "gh_rd_prepare: no grid");
     // statics are invalidated by every change of p_w, p_g or the grid
-    if (c->prepared && c->Dw == Dw && c->Dg == Dg && c->rho == rho) return GH_OK;
+    if (c->prepared && !c->constants_stale && c->Dw == Dw && c->Dg == Dg && c->rho == rho) return GH_OK;
     GH_HIP_CHECK(hipSetDevice(c->device));
     return rd_prepare(*c, Dw, Dg, rho);
 }
@@ -403,6 +403,8 @@ int gh_rd_rhs(gh_ctx* ctx) {
     if (!c) return set_error(GH_EINVAL, "null context");
     if (!c->prepared) return set_error(GH_ESTATE, "gh_rd_rhs: call gh_rd_prepare first");
     GH_HIP_CHECK(hipSetDevice(c->device));
+    int rc = rd_ensure_prepared(*c);
+    if (rc) return rc;
     return rd_rhs(*c);
 }
 
@@ -420,8 +422,10 @@ int gh_rd_steps(gh_ctx* ctx, int64_t nsteps, double dt, double* elapsed_ms) {
     if (!c->prepared) return set_error(GH_ESTATE, "gh_rd_steps: call gh_rd_prepare first");
     if (nsteps < 0) return set_error(GH_EINVAL, "gh_rd_steps: nsteps < 0");
     GH_HIP_CHECK(hipSetDevice(c->device));
+    int rc = rd_ensure_prepared(*c);
+    if (rc) return rc;
     if (elapsed_ms) GH_HIP_CHECK(hipEventRecord(c->ev0, c->stream));
-    int rc = rd_steps(*c, nsteps, dt);
+    rc = rd_steps(*c, nsteps, dt);
     if (rc) return rc;
     if (elapsed_ms) {
         GH_HIP_CHECK(hipEventRecord(c->ev1, c->stream));
@@ -446,6 +450,9 @@ int gh_rd_set_tile(gh_ctx* ctx, int blocks_per_workgroup) {
 int gh_rd_schedule_info(gh_ctx* ctx, int* tile, int* clamped_pairs) {
     Ctx* c = as_ctx(ctx);
     if (!c || !tile || !clamped_pairs) return set_error(GH_EINVAL, "gh_rd_schedule_info: NULL argument");
+    GH_HIP_CHECK(hipSetDevice(c->device));
+    int rc = rd_ensure_prepared(*c);
+    if (rc) return rc;
     *tile = c->rd_bpw;
     *clamped_pairs = (c->prepared && c->fast_ok && c->topo.uniform_bpd > 0) ? 1 : 0;
     return GH_OK;
@@ -455,6 +462,9 @@ int gh_rd_zero_flux_blocks(gh_ctx* ctx, int* nblocks) {
     Ctx* c = as_ctx(ctx);
     if (!c || !nblocks) return set_error(GH_EINVAL, "gh_rd_zero_flux_blocks: NULL argument");
     if (!c->prepared) return set_error(GH_ESTATE, "gh_rd_zero_flux_blocks: call gh_rd_prepare first");
+    GH_HIP_CHECK(hipSetDevice(c->device));
+    int rc = rd_ensure_prepared(*c);
+    if (rc) return rc;
     *nblocks = c->nZeroFlux;
     return GH_OK;
 }
@@ -465,6 +475,8 @@ int gh_rd_autotune(gh_ctx* ctx, double dt, int trial_steps, int* best) {
     if (!c->prepared) return set_error(GH_ESTATE, "gh_rd_autotune: call gh_rd_prepare first");
     if (trial_steps < 1 || trial_steps > 100000) return set_error(GH_EINVAL, "gh_rd_autotune: bad trial_steps");
     GH_HIP_CHECK(hipSetDevice(c->device));
+    int rc = rd_ensure_prepared(*c);
+    if (rc) return rc;
     return rd_autotune(*c, dt, trial_steps, best);
 }
 

@@ -147,6 +147,7 @@ int prepare_t(Ctx& c, double Dw, double Dg, double rho) {
     c.Dg = Dg;
     c.rho = rho;
     c.prepared = true;
+    c.constants_stale = false;
     return GH_OK;
 }
 
@@ -218,6 +219,11 @@ int step_t(Ctx& c, Real* out) {
 
 }  // namespace
 
+int rd_ensure_prepared(Ctx& c) {
+    if (!c.prepared || !c.constants_stale) return GH_OK;
+    return rd_prepare(c, c.Dw, c.Dg, c.rho);
+}
+
 int rd_prepare(Ctx& c, double Dw, double Dg, double rho) {
     if (c.topo.uniform_bpd <= 0) {  // blocks on several levels: ghosts across level jumps
         int rc = rd_prepare_ghosted(c, Dw, Dg);
@@ -226,6 +232,7 @@ int rd_prepare(Ctx& c, double Dw, double Dg, double rho) {
         c.Dg = Dg;
         c.rho = rho;
         c.prepared = true;
+        c.constants_stale = false;
         return GH_OK;
     }
     return c.real_kind == GH_REAL_F32 ? prepare_t<float>(c, Dw, Dg, rho) : prepare_t<double>(c, Dw, Dg, rho);

@@ -99,8 +99,8 @@ def test_gpu_fwt_details_bit_exact(oracle, real, order, grid):
 
 @pytest.mark.parametrize("order", [4, 2])
 def test_gpu_fwt_details_on_deep_grid(oracle, order):
-    """Labs and face ghosts of a level-8 grid evaluated on the device (no
-    host-built table), bit-exact to the oracle."""
+    """Labs of a level-8 grid evaluated on the device (no host-built table),
+    bit-exact to the oracle."""
     rows = deep_grid()
     fields = random_fields(rows, np.float32, 3)
     say("fwt deep", order, len(rows), "blocks, levels", rows[:, 0].min(), "-", rows[:, 0].max())
@@ -111,7 +111,6 @@ def test_gpu_fwt_details_on_deep_grid(oracle, order):
     ref = np.array([oracle.amr_fwt(order, oracle.amr_lab(rows, order, fields[FIELD_PHI], b)) for b in range(len(rows))])
     assert np.array_equal(det, ref), "max |diff| %g" % np.abs(det - ref).max()
     assert counts["labs_device"] >= 1 and counts["labs_host"] == 0, counts
-    assert counts["faces_device"] >= 1 and counts["faces_host"] == 0, counts
 
 
 @pytest.mark.parametrize("real", ["f32", "f64"])
@@ -132,7 +131,7 @@ def test_gpu_multilevel_rd_steps_on_deep_grid(oracle, real):
     ref = oracle.amr_rd_steps(rows, 4, fields[FIELD_PHI], fields[FIELD_PW], fields[FIELD_PG], t(Dw), t(Dg),
                               t(rho), dt, k)
     assert bits_equal(out, ref), "max |diff| %g" % np.abs(out.astype(float) - ref).max()
-    assert counts["faces_host"] == 0, counts
+    assert counts["faces_device"] >= 1 and counts["faces_host"] == 0, counts
 
 
 @pytest.mark.parametrize("real", ["f32", "f64"])
