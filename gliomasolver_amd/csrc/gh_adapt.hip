@@ -385,6 +385,51 @@ __global__ __launch_bounds__(LAB_THREADS) void lab_kernel(int nlist, const int32
     const int4 b = blk[s];
     const int lab3 = LAB * LAB * LAB;
     int my_bad = 0;
+    // Coarse-point cache.  A lab ghost inside a coarser leaf is the Hs
+    // synthesis of level-(L-1) points; all of them lie in the PC^3 patch of
+    // coarse points from 4*idx - PL on (PC = 9, PL = 2 for order 4; 6 and 1 for
+    // order 2).  Most such points are a single leaf cell (or the coincident
+    // vertex of a finer region); their values are resolved once per lab into
+    // LDS, and a ghost's term then reads them there: the same cells, weights
+    // and order of accumulation as the recursion, which the points that need
+    // a further synthesis (flag 0) still take.
+    const int L = b.x;
+    const int PC = order == 4 ? 9 : 6, PL = order == 4 ? 2 : 1;
+    __shared__ Real s_cv[NF][729];
+    __shared__ unsigned char s_cf[729];
+    bool jump = false;
+    for (int k = 0; k < 27; ++k) jump |= (k != 13 && nbr27[27 * s + k] < 0);
+    const bool use_cache = jump && L >= 1;  // uniform per workgroup
+    if (use_cache) {
+        const int ox = 4 * b.y - PL, oy = 4 * b.z - PL, oz = 4 * b.w - PL;
+        for (int q = threadIdx.x; q < PC * PC * PC; q += blockDim.x) {
+            int Lc = L - 1, qx = ox + q % PC, qy = oy + (q / PC) % PC, qz = oz + q / (PC * PC);
+            int32_t src = -1;
+            for (;;) {  // ghost_visit's walk without synthesis
+                const int mask = (BS << Lc) - 1;
+                qx &= mask;
+                qy &= mask;
+                qz &= mask;
+                if (Lc > m.top) break;
+                const int bx = qx >> 3, by = qy >> 3, bz = qz >> 3;
+                const int32_t v = Lc < m.nlev ? m.map[m.off[Lc] + bx + (by << Lc) + (bz << (2 * Lc))]
+                                              : hashed_node(m, Lc, bx, by, bz);
+                if (v >= 0) {
+                    src = v * BCELLS + (qx & 7) + 8 * (qy & 7) + 64 * (qz & 7);
+                    break;
+                }
+                if (v != -2) break;
+                ++Lc;
+                qx *= 2;
+                qy *= 2;
+                qz *= 2;
+            }
+            s_cf[q] = src >= 0 ? 1 : 0;
+#pragma unroll
+            for (int k = 0; k < NF; ++k) s_cv[k][q] = src >= 0 ? lf.f[k][src] : (Real)0;
+        }
+        __syncthreads();
+    }
     for (int i = threadIdx.x; i < lab3; i += blockDim.x) {
         const int p = perm[i];  // lab cells in lab_order(): waves see uniform work
         const int ix = p % LAB + LS, iy = (p / LAB) % LAB + LS, iz = p / (LAB * LAB) + LS;
@@ -407,7 +452,50 @@ __global__ __launch_bounds__(LAB_THREADS) void lab_kernel(int nlist, const int32
                     v[k] = v[k] + prod;
                 }
             };
-            ghost_visit<2>(m, order, b.x, 8 * b.y + ix, 8 * b.z + iy, 8 * b.w + iz, 1.0, visit, my_bad);
+            // the level-L region of this point: inside a coarser leaf?
+            const int mask = (BS << L) - 1;
+            const int qx = (8 * b.y + ix) & mask, qy = (8 * b.z + iy) & mask, qz = (8 * b.w + iz) & mask;
+            bool coarser = false;
+            if (use_cache) {
+                const int bx = qx >> 3, by = qy >> 3, bz = qz >> 3;
+                const int32_t r = L < m.nlev ? m.map[m.off[L] + bx + (by << L) + (bz << (2 * L))]
+                                             : hashed_node(m, L, bx, by, bz);
+                coarser = r == -1;
+            }
+            if (!coarser) {
+                ghost_visit<2>(m, order, L, qx, qy, qz, 1.0, visit, my_bad);
+            } else {  // ghost_visit<2>'s synthesis step, coarse points from the cache
+                const int nc = BS << (L - 1);  // coarse lattice size
+                const int nz = synth_n(order, qz), ny = synth_n(order, qy), nx = synth_n(order, qx);
+#pragma unroll 1
+                for (int kz = 0; kz < nz; ++kz) {
+                    const int sz = synth_src(order, qz, kz);
+                    const int pz = (((sz - (4 * b.w - PL)) % nc) + nc) % nc;
+#pragma unroll 1
+                    for (int ky = 0; ky < ny; ++ky) {
+                        const int sy = synth_src(order, qy, ky);
+                        const int py = (((sy - (4 * b.z - PL)) % nc) + nc) % nc;
+#pragma unroll 1
+                        for (int kx = 0; kx < nx; ++kx) {
+                            const int sx = synth_src(order, qx, kx);
+                            const int px = (((sx - (4 * b.y - PL)) % nc) + nc) % nc;
+                            const double w = 1.0 * (synth_w(order, qz, kz) * synth_w(order, qy, ky) *
+                                                    synth_w(order, qx, kx));
+                            const int pq = px + PC * (py + PC * pz);
+                            if (px < PC && py < PC && pz < PC && s_cf[pq]) {
+                                const Real wr = (Real)w;
+#pragma unroll
+                                for (int k = 0; k < NF; ++k) {
+                                    const Real prod = s_cv[k][pq] * wr;
+                                    v[k] = v[k] + prod;
+                                }
+                            } else {
+                                ghost_visit<1>(m, order, L - 1, sx, sy, sz, w, visit, my_bad);
+                            }
+                        }
+                    }
+                }
+            }
         }
 #pragma unroll
         for (int k = 0; k < NF; ++k) lf.lab[k][(size_t)j * lab3 + p] = v[k];
