@@ -527,15 +527,22 @@ __global__ __launch_bounds__(512) void fwt_kernel(int nlist, int LS, int LAB, WT
         s[a] = bw[a] ? tp.gs0 : 0;  // Ha support [0,1)
         e[a] = bw[a] ? tp.ge0 : 1;
     }
+    // A term whose Ga tap is 0 (order 4 has two) adds a signed zero -- the lab
+    // holds finite values -- which changes no partial sum except the sign of
+    // a zero, and so not |result|: those terms are skipped (343 -> 125 for
+    // code 7), the others summed in the reference's order.
     Real result = (Real)0;
     for (int iz = s[2]; iz < e[2]; ++iz) {
         const Real wZ = bw[2] ? tp.ga[iz - tp.gs0] : (Real)1;
+        if (wZ == (Real)0) continue;
         for (int iy = s[1]; iy < e[1]; ++iy) {
             const Real wY = bw[1] ? tp.ga[iy - tp.gs0] : (Real)1;
+            if (wY == (Real)0) continue;
             const Real wZwY = wZ * wY;
             const int base = (o[2] - iz - LS) * LAB * LAB + (o[1] - iy - LS) * LAB - LS;
             for (int ix = s[0]; ix < e[0]; ++ix) {
                 const Real wX = bw[0] ? tp.ga[ix - tp.gs0] : (Real)1;
+                if (wX == (Real)0) continue;
                 const Real w = wZwY * wX;
                 const Real prod = s_lab[base + o[0] - ix] * w;
                 result = result + prod;
