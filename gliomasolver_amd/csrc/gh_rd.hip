@@ -122,6 +122,12 @@ int plan_step2f(Ctx& c) {
         while (x < 8 && acc >= total * x / 8) c.split2f[x++] = i + 1;
     }
     for (; x <= 8; ++x) c.split2f[x] = ni;
+    // within each XCD's range the blocks with diffusion go first (in slot
+    // order, so neighbouring blocks still run close in time) and the short
+    // groups of blocks without diffusion last, where they fill the tail
+    for (int r = 0; r < 8; ++r)
+        std::stable_partition(items.begin() + c.split2f[r], items.begin() + c.split2f[r + 1],
+                              [](const int2& v) { return v.y == 0; });
     if ((rc = c.items2f.ensure_grow((size_t)ni * sizeof(int2)))) return rc;
     GH_HIP_CHECK(hipMemcpyAsync(c.items2f.p, items.data(), (size_t)ni * sizeof(int2), hipMemcpyHostToDevice,
                                 c.stream));
