@@ -81,6 +81,9 @@ int adjoint_t(Ctx& c, int64_t N, double T, int K, const double* lik, int stepPET
     const double ih2 = 1. / (h * h);
     const UniformAdjParams prm{c.rho, ih2, dt, 1.0 / (double)N, 1.0 / c.Dw, c.topo.uniform_bpd,
                                 bpd_magic(c.topo.uniform_bpd)};
+    // blocks without diffusion take the reverse step's short path under the
+    // two-step schedule (tile 0), whose plan holds their flags
+    const unsigned char* zflux = (c.nItems2f > 0 && c.rd_bpw == 0) ? c.zeroFlux.as<unsigned char>() : nullptr;
     for (int64_t sgi = nck - 1; sgi >= 0; --sgi) {
         const int64_t k0 = sgi * K, k1 = std::min<int64_t>(N, k0 + K);
         GH_HIP_CHECK(hipMemcpyAsync(seg, ck + (size_t)sgi * nb, nb, hipMemcpyDeviceToDevice, c.stream));
@@ -102,11 +105,12 @@ int adjoint_t(Ctx& c, int64_t N, double T, int K, const double* lik, int stepPET
                 hipLaunchKernelGGL((adj_step_kernel<Real, UniformAdjParams, true>), dim3(grid), dim3(512), 0,
                                    c.stream, ns, per_xcd, uk, mask_out, c.adjLam[lc].as<double>(),
                                    c.adjLam[1 - lc].as<double>(), c.statF.as<Stat4<Real>>(), (const Real*)nullptr,
-                                   prm, c.adjPart.as<double>());
+                                   prm, c.adjPart.as<double>(), zflux);
             else
                 hipLaunchKernelGGL((adj_step_kernel<Real, UniformAdjParams>), dim3(grid), dim3(512), 0, c.stream,
                                    ns, per_xcd, uk, mask_out, c.adjLam[lc].as<double>(), c.adjLam[1 - lc].as<double>(),
-                                   c.statD.as<Stat4<Real>>(), c.statSig.as<Real>(), prm, c.adjPart.as<double>());
+                                   c.statD.as<Stat4<Real>>(), c.statSig.as<Real>(), prm, c.adjPart.as<double>(),
+                                   zflux);
             GH_HIP_CHECK(hipGetLastError());
             lc = 1 - lc;
         }

@@ -162,17 +162,55 @@ __global__ __launch_bounds__(256) void adj_mask_kernel(size_t n, double* __restr
 // partial[ADJ_PART s + 2 w + q] accumulates wave w's sums A (q = 0) and B
 // (q = 1) of block s, step after step.  CLAMPED: st holds the clamped-state records and
 // sig is not read (steps k >= 1 when every threshold is <= 0).
+// zflux (may be null) marks blocks without diffusion (zero_flux_kernel): every
+// face coefficient their cells use is +0, so each of adj_cell's face terms is
+// a signed zero whatever the neighbour's state, flag or weight (finite
+// values), the sums come out +0, and the cell's update is adj_cell with zero
+// neighbour data -- the same bits with no halo and no LDS.  (Grouping them
+// several to a workgroup with the two-step kernel's cost-balanced split, as
+// the forward does, made the reverse step slower: its short path is not
+// cheap enough to batch.)
 template <typename Real, typename Params, bool CLAMPED = false>
 __global__ __launch_bounds__(512) void adj_step_kernel(int nslots, int per_xcd,
                                                        const Real* __restrict__ uk, int mask_out,
                                                        const double* __restrict__ lam1, double* __restrict__ lam0,
                                                        const Stat4<Real>* __restrict__ st,
                                                        const Real* __restrict__ sig, Params prm,
-                                                       double* __restrict__ partial) {
+                                                       double* __restrict__ partial,
+                                                       const unsigned char* __restrict__ zflux) {
     const int s = (int)(blockIdx.x % 8) * per_xcd + (int)(blockIdx.x / 8);
     if (s >= nslots) return;
     double rho, ih2, dt, invN, invDw;
     prm.get(s, rho, ih2, dt, invN, invDw);
+    if (zflux && zflux[s]) {  // uniform per workgroup
+        const int t = threadIdx.x;
+        const size_t g = (size_t)s * BCELLS + t;
+        const Real uc = uk[g];
+        const Stat4<Real> sv = st[g];
+        const bool domc = adj_in_domain<CLAMPED>(uc, sv, sig, g);
+        const double l1c = lam1[g];
+        const double u0 = (double)uc;
+        const double D[6] = {0., 0., 0., 0., 0., 0.};
+        double lk, rdiff;
+        adj_cell(
+            D,
+            [&](int, int& oo, int& of, double& un, double& wn) {
+                oo = 0;
+                of = 0;
+                un = 0.;
+                wn = 0.;
+            },
+            u0, l1c, domc, rho, ih2, dt, lk, rdiff);
+        lam0[g] = (mask_out && !(uc < (Real)1)) ? 0.0 : lk;
+        const double wc = domc ? l1c : 0.0;
+        const double A = wave_sum_dpp(wc * rdiff), B = wave_sum_dpp(wc * (u0 * (1. - u0)));
+        if ((t & 63) == 0) {
+            double* pw = partial + ADJ_PART * (size_t)s + 2 * (t >> 6);
+            pw[0] += A;
+            pw[1] += B;
+        }
+        return;
+    }
     __shared__ Real s_u[1728];
     __shared__ unsigned char s_o[1728];
     __shared__ double s_w[1000];

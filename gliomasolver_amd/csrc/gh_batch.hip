@@ -625,6 +625,8 @@ int adjoint_t(Batch& b, int K, const double* lik, int stepPET, int stepTi, int u
         return rc;
     const BatchStepParams<Real> fprm{b.slotPos.as<int32_t>(), b.range.as<SlotRange<Real>>()};
     const BatchAdjParams<Real> aprm{b.slotPos.as<int32_t>(), b.range.as<SlotRange<Real>>()};
+    // blocks without diffusion take the reverse step's short path (flags exist when fast_ok)
+    const unsigned char* zflux = b.fast_ok ? b.zeroFlux.as<unsigned char>() : nullptr;
     auto step = [&](const void* in, void* out, int nslots) {  // one Euler step (tape recompute)
         const int per_xcd = ((nslots + 1) / 2 + 7) / 8;
         hipLaunchKernelGGL((rd_step_kernel<Real, true, 2, BatchStepParams<Real>>), dim3(8 * per_xcd), dim3(512), 0,
@@ -714,14 +716,16 @@ int adjoint_t(Batch& b, int K, const double* lik, int stepPET, int stepTi, int u
             const int mask_out = k >= 1;
             if (b.fast_ok && k >= 1)  // clamped states: the clamped-state records
                 hipLaunchKernelGGL((adj_step_kernel<Real, BatchAdjParams<Real>, true>), dim3(8 * per_xcd), dim3(512),
-                                   0, b.stream, nslots, per_xcd, uk, mask_out, b.lam[(k + 1) & 1].as<double>(),
+                                   0, b.stream, nslots, per_xcd, uk, mask_out,
+                                   b.lam[(k + 1) & 1].as<double>(),
                                    b.lam[k & 1].as<double>(), b.statF.as<Stat4<Real>>(), (const Real*)nullptr, aprm,
-                                   b.apart.as<double>());
+                                   b.apart.as<double>(), zflux);
             else
                 hipLaunchKernelGGL((adj_step_kernel<Real, BatchAdjParams<Real>>), dim3(8 * per_xcd), dim3(512), 0,
-                                   b.stream, nslots, per_xcd, uk, mask_out, b.lam[(k + 1) & 1].as<double>(),
+                                   b.stream, nslots, per_xcd, uk, mask_out,
+                                   b.lam[(k + 1) & 1].as<double>(),
                                    b.lam[k & 1].as<double>(), b.statD.as<Stat4<Real>>(), b.statSig.as<Real>(), aprm,
-                                   b.apart.as<double>());
+                                   b.apart.as<double>(), zflux);
         }
         GH_HIP_CHECK(hipGetLastError());
     }

@@ -129,6 +129,35 @@ def test_gpu_adjoint_phantom64_fp32_vs_tlm(oracle):
         assert err < 5e-3
 
 
+@pytest.mark.parametrize("real", ["f32", "f64"])
+def test_gpu_adjoint_tissue_free_shortcut_is_bit_identical(real):
+    """Under the two-step schedule the reverse step advances blocks without
+    diffusion (no tissue: every face coefficient +0) by the cell's own update,
+    with no halo; the loss and all six gradients equal, bit for bit, the
+    adjoint under a single-step schedule, which takes every block through the
+    full transposed stencil."""
+    from gliomasolver_amd import workloads as wl
+    case = wl.make_case(64, real, 4, Dw_phys=0.013, rho=0.05, tend=60.0)
+    N = case.nsteps
+    ctx = wl.load_context(case)
+    ctx.checkpoint_save(0)
+    ctx.rd_steps(N, case.dt)
+    u = ctx.get_field(FIELD_PHI).astype(np.float64)
+    rng = np.random.default_rng(3)
+    t1 = (u >= 0.5).astype(np.float64)
+    fl = (u >= 0.2).astype(np.float64)
+    ctx.lik_set_data(t1, fl, np.where(t1 > 0, u + rng.normal(0, 0.05, u.shape), 0.0))
+    assert ctx.rd_zero_flux_blocks() > 100
+    out = {}
+    for tile in (0, 2):
+        ctx.rd_set_tile(tile)
+        ctx.checkpoint_load(0)
+        out[tile] = ctx.rd_adjoint(N, case.tend, case.seed_voxel, 1.0, 16)
+    ctx.close()
+    say("adjoint", real, "tile 0 vs 2: loss", out[0][0][0], out[2][0][0])
+    assert np.array_equal(out[0][0], out[2][0]) and np.array_equal(out[0][1], out[2][1])
+
+
 def test_gpu_adjoint_uniform128_fp32_agrees_with_fp64():
     """The gradient configuration at full size: 128^3 phantom, T = 400 d, FP32
     state with FP64 adjoint and gradient sums, checkpoints every 64 steps,
