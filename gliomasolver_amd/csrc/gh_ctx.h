@@ -168,10 +168,12 @@ int apply_topology(Ctx& c);
 int rd_prepare_ghosted(Ctx& c, double Dw, double Dg);
 int rd_step_ghosted(Ctx& c, const void* in, void* out, double dt, bool fused);
 int fwt_details(Ctx& c, const std::vector<int>& slots, std::vector<double>& det);
-int grid_refine(Ctx& c, const std::vector<char>& flagged, int* nrefined);
+// old_slot (may be null) receives, per new slot, the old slot of the same
+// leaf or -1 for a new one (left untouched when nothing changes)
+int grid_refine(Ctx& c, const std::vector<char>& flagged, int* nrefined, std::vector<int>* old_slot = nullptr);
 // parents (may be null) receives the keys of the new parent leaves
 int grid_compress(Ctx& c, const std::vector<char>& flagged, int* ncollapsed,
-                  std::vector<uint64_t>* parents = nullptr);
+                  std::vector<uint64_t>* parents = nullptr, std::vector<int>* old_slot = nullptr);
 
 // likelihood helpers (gh_lik.hip): -logL of the current phi; dJ/du of a state buffer
 int lik_eval_current(Ctx& c, const double* params, int stepPET, int stepTi, int use_roi, double* out,

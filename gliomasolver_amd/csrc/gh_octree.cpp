@@ -191,6 +191,24 @@ std::vector<Octree::Nbr> Octree::neighbours(int s) const {
 
 void Octree::same_level_neighbours(int s, int out[27]) const {
     const BlockRec& b = leaves[s];
+    const int L = b.level;
+    if (L < dense_levels) {  // straight from the level's dense map
+        const int m = (1 << L) - 1;
+        const int32_t* map = dense[L].data();
+        size_t ox[3], oy[3], oz[3];
+        for (int d = 0; d < 3; ++d) {
+            ox[d] = (size_t)((b.index[0] + d - 1) & m);
+            oy[d] = (size_t)((b.index[1] + d - 1) & m) << L;
+            oz[d] = (size_t)((b.index[2] + d - 1) & m) << (2 * L);
+        }
+        for (int dz = 0; dz < 3; ++dz)
+            for (int dy = 0; dy < 3; ++dy)
+                for (int dx = 0; dx < 3; ++dx) {
+                    const int32_t v = map[ox[dx] + oy[dy] + oz[dz]];
+                    out[dx + 3 * dy + 9 * dz] = v >= 0 ? v : -1;
+                }
+        return;
+    }
     for (int code = 0; code < 27; ++code) {
         const int d[3] = {code % 3 - 1, (code / 3) % 3 - 1, code / 9 - 1};
         int slot = -1;
@@ -341,7 +359,7 @@ void Octree::chain_targets(int v, std::vector<int>& out) const {
         const Kind k = neighbour_region(v, code, P, wrapped, &slot);
         if (k == LEAF) continue;
         if (k == FINER) {
-            touching_deep(*this, L, P, d, L + 3, out);
+            if (L + 3 <= top_level) touching_deep(*this, L, P, d, L + 3, out);  // no leaf is finer than top_level
             continue;
         }
         for (int l = L - 1; l >= 0; --l) {

@@ -153,6 +153,28 @@ def test_gpu_multilevel_rd_steps_bit_exact(oracle, real, order, grid):
     assert bits_equal(out, ref), "max |diff| %g" % np.abs(out.astype(float) - ref).max()
 
 
+@pytest.mark.parametrize("real", ["f32", "f64"])
+@pytest.mark.parametrize("order", [4, 2])
+def test_gpu_multilevel_rd_steps_on_large_grid(oracle, real, order):
+    """A level-4 grid with scattered level-5 refinements (4,166 blocks): large
+    enough that the face ghosts are summed by their own kernel before each
+    step instead of inside the step kernel; bit-exact to the oracle."""
+    rows = balanced_random_grid(np.random.default_rng(5), base_level=4, max_level=6, nrefine=10)
+    assert len(rows) > 1450
+    t = np.float32 if real == "f32" else np.float64
+    fields = random_fields(rows, t, 6)
+    say("multilevel rd large", real, order, len(rows), "blocks")
+    ctx = make_ctx(real, rows, order, fields, max_level=6)
+    Dw, Dg, rho, dt, k = 5e-5, 5e-6, 0.08, 0.02, 3
+    ctx.rd_prepare(Dw, Dg, rho)
+    ctx.rd_steps(k, dt)
+    out = ctx.get_field_blocks(FIELD_PHI).reshape(len(rows), 512)
+    ctx.close()
+    ref = oracle.amr_rd_steps(rows, order, fields[FIELD_PHI], fields[FIELD_PW], fields[FIELD_PG], t(Dw), t(Dg),
+                              t(rho), dt, k)
+    assert bits_equal(out, ref), "max |diff| %g" % np.abs(out.astype(float) - ref).max()
+
+
 # --- oracle-driven simulation of AutomaticRefinement / AutomaticCompression --
 
 def sim_details(oracle, rows, order, phi, subset):
