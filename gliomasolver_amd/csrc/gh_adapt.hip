@@ -465,20 +465,20 @@ __global__ __launch_bounds__(LAB_THREADS) void lab_kernel(int nlist, const int32
             if (!coarser) {
                 ghost_visit<2>(m, order, L, qx, qy, qz, 1.0, visit, my_bad);
             } else {  // ghost_visit<2>'s synthesis step, coarse points from the cache
-                const int nc = BS << (L - 1);  // coarse lattice size
+                const int nc = BS << (L - 1);  // coarse lattice size (a power of two: & wraps)
                 const int nz = synth_n(order, qz), ny = synth_n(order, qy), nx = synth_n(order, qx);
 #pragma unroll 1
                 for (int kz = 0; kz < nz; ++kz) {
                     const int sz = synth_src(order, qz, kz);
-                    const int pz = (((sz - (4 * b.w - PL)) % nc) + nc) % nc;
+                    const int pz = (sz - (4 * b.w - PL)) & (nc - 1);
 #pragma unroll 1
                     for (int ky = 0; ky < ny; ++ky) {
                         const int sy = synth_src(order, qy, ky);
-                        const int py = (((sy - (4 * b.z - PL)) % nc) + nc) % nc;
+                        const int py = (sy - (4 * b.z - PL)) & (nc - 1);
 #pragma unroll 1
                         for (int kx = 0; kx < nx; ++kx) {
                             const int sx = synth_src(order, qx, kx);
-                            const int px = (((sx - (4 * b.y - PL)) % nc) + nc) % nc;
+                            const int px = (sx - (4 * b.y - PL)) & (nc - 1);
                             const double w = 1.0 * (synth_w(order, qz, kz) * synth_w(order, qy, ky) *
                                                     synth_w(order, qx, kx));
                             const int pq = px + PC * (py + PC * pz);
@@ -505,59 +505,107 @@ __global__ __launch_bounds__(LAB_THREADS) void lab_kernel(int nlist, const int32
 
 // BlockFWT::_fwt / _filter (MRAGBlockFWT.h:219-298): 64 coarse points x 8
 // tensor filters per block; the block's detail is the largest |value| over
-// the 7 wavelet codes (FWTReport::_getDetailMaxMag, :109-120).
-template <typename Real>
-__global__ __launch_bounds__(512) void fwt_kernel(int nlist, int LS, int LAB, WTaps<Real> tp,
-                                                  const Real* __restrict__ lab, double* __restrict__ detail) {
-    const int j = blockIdx.x;
-    if (j >= nlist) return;
-    __shared__ Real s_lab[13 * 13 * 13];
-    __shared__ double s_max[8];
-    const int lab3 = LAB * LAB * LAB;
-    for (int p = threadIdx.x; p < lab3; p += blockDim.x) s_lab[p] = lab[(size_t)j * lab3 + p];
-    __syncthreads();
-    // one wavelet code per wave (wave-uniform trip counts: 1 to 343 terms),
-    // one coarse point per lane
-    const int code = threadIdx.x >> 6, pt = threadIdx.x & 63;
-    const int dx = pt & 3, dy = (pt >> 2) & 3, dz = pt >> 4;
-    const int bw[3] = {code & 1, (code >> 1) & 1, (code >> 2) & 1};
-    const int o[3] = {2 * dx, 2 * dy, 2 * dz};
-    int s[3], e[3];
-    for (int a = 0; a < 3; ++a) {
-        s[a] = bw[a] ? tp.gs0 : 0;  // Ha support [0,1)
-        e[a] = bw[a] ? tp.ge0 : 1;
-    }
-    // A term whose Ga tap is 0 (order 4 has two) adds a signed zero -- the lab
-    // holds finite values -- which changes no partial sum except the sign of
-    // a zero, and so not |result|: those terms are skipped (343 -> 125 for
-    // code 7), the others summed in the reference's order.
-    Real result = (Real)0;
-    for (int iz = s[2]; iz < e[2]; ++iz) {
-        const Real wZ = bw[2] ? tp.ga[iz - tp.gs0] : (Real)1;
-        if (wZ == (Real)0) continue;
-        for (int iy = s[1]; iy < e[1]; ++iy) {
-            const Real wY = bw[1] ? tp.ga[iy - tp.gs0] : (Real)1;
-            if (wY == (Real)0) continue;
+// the 7 wavelet codes (FWTReport::_getDetailMaxMag, :109-120; the code-0
+// scaling values do not enter it and are not formed).
+//
+// The analysis taps Ga are compile-time constants.  A term whose Ga tap is 0
+// (order 4 has two) adds a signed zero -- the lab holds finite values --
+// which changes no partial sum except the sign of a zero, and so not
+// |result|: those terms are left out (343 -> 125 for code 7), the others are
+// summed in the reference's order (z, y, x taps ascending).  The tap
+// products wZ*wY*wX are exact in either precision (powers of two times 1, 9
+// or 81, 729), so folding them at compile time gives the runtime values.
+template <int ORDER>
+struct FwtGeom;
+template <>
+struct FwtGeom<4> {  // Ga support [-4, 3): 1/16, 0, -9/16, 1, -9/16, 0, 1/16
+    static constexpr int LS = -2, LAB = 13, NT = 5;
+    static constexpr int tap(int k) { return k == 0 ? -4 : k == 1 ? -2 : k == 2 ? -1 : k == 3 ? 0 : 2; }
+    static constexpr double w(int k) { return k == 0 ? 1 / 16. : k == 1 ? -9 / 16. : k == 2 ? 1. : k == 3 ? -9 / 16. : 1 / 16.; }
+};
+template <>
+struct FwtGeom<2> {  // Ga support [-2, 1): -1/2, 1, -1/2
+    static constexpr int LS = -1, LAB = 10, NT = 3;
+    static constexpr int tap(int k) { return k - 2; }
+    static constexpr double w(int k) { return k == 1 ? 1. : -0.5; }
+};
+
+// one wavelet code (BX, BY, BZ: the axes filtered with Ga, the others with
+// Ha = {1}) at the coarse point whose lab position p points at
+template <typename Real, int ORDER, int BX, int BY, int BZ>
+__device__ __forceinline__ Real fwt_code(const Real* __restrict__ p) {
+    using G = FwtGeom<ORDER>;
+    constexpr int LAB = G::LAB;
+    constexpr int nz = BZ ? G::NT : 1, ny = BY ? G::NT : 1, nx = BX ? G::NT : 1;
+    Real r = (Real)0;
+#pragma unroll
+    for (int kz = 0; kz < nz; ++kz) {
+        const Real wZ = BZ ? (Real)G::w(kz) : (Real)1;
+        const int iz = BZ ? G::tap(kz) : 0;
+#pragma unroll
+        for (int ky = 0; ky < ny; ++ky) {
+            const Real wY = BY ? (Real)G::w(ky) : (Real)1;
+            const int iy = BY ? G::tap(ky) : 0;
             const Real wZwY = wZ * wY;
-            const int base = (o[2] - iz - LS) * LAB * LAB + (o[1] - iy - LS) * LAB - LS;
-            for (int ix = s[0]; ix < e[0]; ++ix) {
-                const Real wX = bw[0] ? tp.ga[ix - tp.gs0] : (Real)1;
-                if (wX == (Real)0) continue;
+#pragma unroll
+            for (int kx = 0; kx < nx; ++kx) {
+                const Real wX = BX ? (Real)G::w(kx) : (Real)1;
+                const int ix = BX ? G::tap(kx) : 0;
                 const Real w = wZwY * wX;
-                const Real prod = s_lab[base + o[0] - ix] * w;
-                result = result + prod;
+                const Real prod = p[-(iz * LAB * LAB + iy * LAB + ix)] * w;
+                r = r + prod;
             }
         }
     }
-    double v = code == 0 ? 0.0 : (double)fabs(result);
-    for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
-    if ((threadIdx.x & 63) == 0) s_max[threadIdx.x >> 6] = v;
+    return r;
+}
+
+// Two blocks per 256-lane workgroup, two waves per block, one coarse point
+// per lane: one wave forms code 7 (125 terms at order 4), the other codes 1-6
+// (90), so that the waves of a workgroup finish together and a CU stays full
+// (one wave per code left a CU running mostly the code-7 waves alone).
+template <typename Real, int ORDER>
+__global__ __launch_bounds__(256) void fwt_kernel(int nlist, const Real* __restrict__ lab,
+                                                  double* __restrict__ detail) {
+    using G = FwtGeom<ORDER>;
+    constexpr int LAB = G::LAB, LS = G::LS, LAB3 = LAB * LAB * LAB;
+    __shared__ Real s_lab[2 * LAB3];
+    __shared__ double s_max[4];
+    const int j0 = 2 * blockIdx.x;
+    const int nb = min(2, nlist - j0);
+    if (nb <= 0) return;
+    for (int p = threadIdx.x; p < nb * LAB3; p += blockDim.x) s_lab[p] = lab[(size_t)j0 * LAB3 + p];
     __syncthreads();
-    if (threadIdx.x == 0) {
-        double m = 0.0;
-        for (int w = 0; w < 8; ++w) m = fmax(m, s_max[w]);
-        detail[j] = m;
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int b = wv >> 1;
+    const int dx = lane & 3, dy = (lane >> 2) & 3, dz = lane >> 4;
+    const Real* p = s_lab + b * LAB3 + (2 * dz - LS) * LAB * LAB + (2 * dy - LS) * LAB + (2 * dx - LS);
+    double v = 0.0;
+    if (b < nb) {
+        if ((wv & 1) == 0) {
+            v = (double)fabs(fwt_code<Real, ORDER, 1, 1, 1>(p));
+        } else {
+            v = fmax(v, (double)fabs(fwt_code<Real, ORDER, 1, 0, 0>(p)));
+            v = fmax(v, (double)fabs(fwt_code<Real, ORDER, 0, 1, 0>(p)));
+            v = fmax(v, (double)fabs(fwt_code<Real, ORDER, 1, 1, 0>(p)));
+            v = fmax(v, (double)fabs(fwt_code<Real, ORDER, 0, 0, 1>(p)));
+            v = fmax(v, (double)fabs(fwt_code<Real, ORDER, 1, 0, 1>(p)));
+            v = fmax(v, (double)fabs(fwt_code<Real, ORDER, 0, 1, 1>(p)));
+        }
     }
+    for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
+    if (lane == 0) s_max[wv] = v;
+    __syncthreads();
+    if (threadIdx.x < nb) detail[j0 + threadIdx.x] = fmax(s_max[2 * threadIdx.x], s_max[2 * threadIdx.x + 1]);
+}
+
+template <typename Real>
+void launch_fwt(int order, int nlist, const Real* lab, double* detail, hipStream_t st) {
+    const dim3 grid((unsigned)((nlist + 1) / 2));
+    if (order == 4)
+        hipLaunchKernelGGL((fwt_kernel<Real, 4>), grid, dim3(256), 0, st, nlist, lab, detail);
+    else
+        hipLaunchKernelGGL((fwt_kernel<Real, 2>), grid, dim3(256), 0, st, nlist, lab, detail);
 }
 
 // new leaf from the old grid: copy, split (Hs synthesis from the parent's
@@ -1363,8 +1411,7 @@ int fwt_details_t(Ctx& c, const std::vector<int>& slots, std::vector<double>& de
         const Real* fld[1] = {c.phi[c.cur].as<Real>()};
         DevBuf* out[1] = {&c.labBuf};
         if ((rc = device_labs<Real>(c, m, c.labSlots.as<int32_t>(), 1, fld, out))) return rc;
-        hipLaunchKernelGGL(fwt_kernel<Real>, dim3((unsigned)m), dim3(512), 0, c.stream, m, lab_start(order),
-                           lab_size(order), taps<Real>(order), c.labBuf.as<Real>(), d.as<double>());
+        launch_fwt<Real>(order, m, c.labBuf.as<Real>(), d.as<double>(), c.stream);
         int bad = 1;
         hipError_t e = hipGetLastError();
         if (e == hipSuccess) e = hipMemcpyAsync(det.data(), d.p, (size_t)m * 8, hipMemcpyDeviceToHost, c.stream);
@@ -1393,8 +1440,7 @@ int fwt_details_t(Ctx& c, const std::vector<int>& slots, std::vector<double>& de
     if ((rc = build_labs<Real>(c, n, c.labAllSlots, c.phi[c.cur].as<Real>(), c.nLabAllEnt, c.labAllEnt, c.labAllSrc,
                                c.labAllW, c.labBuf)))
         return rc;
-    hipLaunchKernelGGL(fwt_kernel<Real>, dim3((unsigned)n), dim3(512), 0, c.stream, n, lab_start(order),
-                       lab_size(order), taps<Real>(order), c.labBuf.as<Real>(), d.as<double>());
+    launch_fwt<Real>(order, n, c.labBuf.as<Real>(), d.as<double>(), c.stream);
     std::vector<double> all_det(n);
     hipError_t e = hipGetLastError();
     if (e == hipSuccess) e = hipMemcpyAsync(all_det.data(), d.p, all_det.size() * 8, hipMemcpyDeviceToHost, c.stream);
