@@ -48,32 +48,6 @@ struct ProfScope {
 };
 enum ProfSlot { P_REF_FWT = 0, P_REF = 1, P_COMP_FWT = 2, P_COMP = 3, P_PLAN = 4, P_MOVE = 5, P_TOPO = 6, P_STAT = 7 };
 
-// interpolating wavelet taps in Real (MRAGWavelets_StaticData.cpp:13-21, 31-39)
-template <typename Real>
-struct WTaps {
-    int gs0, ge0;   // Ga support [gs0, ge0)
-    Real ga[7];     // Ga[i - gs0]
-    int hs0, he0;   // Hs support
-    Real hs[7];
-};
-template <typename Real>
-WTaps<Real> taps(int order) {
-    WTaps<Real> t{};
-    if (order == 2) {
-        t.gs0 = -2; t.ge0 = 1;
-        t.ga[0] = (Real)-0.5; t.ga[1] = (Real)1.; t.ga[2] = (Real)-0.5;
-        t.hs0 = -1; t.he0 = 2;
-        t.hs[0] = (Real)0.5; t.hs[1] = (Real)1.; t.hs[2] = (Real)0.5;
-    } else {
-        t.gs0 = -4; t.ge0 = 3;
-        const double g[7] = {1 / 16., 0, -9 / 16., 1, -9 / 16., 0, 1 / 16.};
-        const double h[7] = {-1 / 16., 0, 9 / 16., 1, 9 / 16., 0, -1 / 16.};
-        for (int i = 0; i < 7; ++i) { t.ga[i] = (Real)g[i]; t.hs[i] = (Real)h[i]; }
-        t.hs0 = -3; t.he0 = 4;
-    }
-    return t;
-}
-
 // lab geometry: positions [LS, LS+LAB) per axis (MRAGrid.inl:761-789 with the
 // RD stencil [-1,2)): order 4 -> [-2, 11), order 2 -> [-1, 9)
 inline int lab_start(int order) { return order == 4 ? -2 : -1; }
@@ -361,6 +335,64 @@ __global__ __launch_bounds__(256) void face_packet_kernel(int n, const int32_t* 
     out[(size_t)faces[k] * 64 + p] = acc;
 }
 
+// BlockFWT::_fwt / _filter (MRAGBlockFWT.h:219-298): 64 coarse points x 8
+// tensor filters per block; the block's detail is the largest |value| over
+// the 7 wavelet codes (FWTReport::_getDetailMaxMag, :109-120; the code-0
+// scaling values do not enter it and are not formed).
+//
+// The analysis taps Ga (MRAGWavelets_StaticData.cpp:14, 33) are compile-time
+// constants.  A term whose Ga tap is 0 (order 4 has two) adds a signed zero
+// -- the lab holds finite values -- which changes no partial sum except the
+// sign of a zero, and so not |result|: those terms are left out (343 -> 125
+// for code 7), the others are summed in the reference's order (z, y, x taps
+// ascending).  The tap
+// products wZ*wY*wX are exact in either precision (powers of two times 1, 9
+// or 81, 729), so folding them at compile time gives the runtime values.
+template <int ORDER>
+struct FwtGeom;
+template <>
+struct FwtGeom<4> {  // Ga support [-4, 3): 1/16, 0, -9/16, 1, -9/16, 0, 1/16
+    static constexpr int LS = -2, LAB = 13, NT = 5;
+    static constexpr int tap(int k) { return k == 0 ? -4 : k == 1 ? -2 : k == 2 ? -1 : k == 3 ? 0 : 2; }
+    static constexpr double w(int k) { return k == 0 ? 1 / 16. : k == 1 ? -9 / 16. : k == 2 ? 1. : k == 3 ? -9 / 16. : 1 / 16.; }
+};
+template <>
+struct FwtGeom<2> {  // Ga support [-2, 1): -1/2, 1, -1/2
+    static constexpr int LS = -1, LAB = 10, NT = 3;
+    static constexpr int tap(int k) { return k - 2; }
+    static constexpr double w(int k) { return k == 1 ? 1. : -0.5; }
+};
+
+// one wavelet code (BX, BY, BZ: the axes filtered with Ga, the others with
+// Ha = {1}) at the coarse point whose lab position p points at
+template <typename Real, int ORDER, int BX, int BY, int BZ>
+__device__ __forceinline__ Real fwt_code(const Real* __restrict__ p) {
+    using G = FwtGeom<ORDER>;
+    constexpr int LAB = G::LAB;
+    constexpr int nz = BZ ? G::NT : 1, ny = BY ? G::NT : 1, nx = BX ? G::NT : 1;
+    Real r = (Real)0;
+#pragma unroll
+    for (int kz = 0; kz < nz; ++kz) {
+        const Real wZ = BZ ? (Real)G::w(kz) : (Real)1;
+        const int iz = BZ ? G::tap(kz) : 0;
+#pragma unroll
+        for (int ky = 0; ky < ny; ++ky) {
+            const Real wY = BY ? (Real)G::w(ky) : (Real)1;
+            const int iy = BY ? G::tap(ky) : 0;
+            const Real wZwY = wZ * wY;
+#pragma unroll
+            for (int kx = 0; kx < nx; ++kx) {
+                const Real wX = BX ? (Real)G::w(kx) : (Real)1;
+                const int ix = BX ? G::tap(kx) : 0;
+                const Real w = wZwY * wX;
+                const Real prod = p[-(iz * LAB * LAB + iy * LAB + ix)] * w;
+                r = r + prod;
+            }
+        }
+    }
+    return r;
+}
+
 // Full labs of the listed slots (slots == nullptr: every slot j), ghosts
 // evaluated on the device; *bad is set when a ghost needs more than two
 // synthesis levels (the caller then uses host tables).
@@ -373,14 +405,20 @@ struct LabFields {
     const Real* __restrict__ f[4];
     Real* __restrict__ lab[4];
 };
-template <typename Real, int NF>
+// FWT = 2 or 4 (NF = 1): the lab stays in LDS and the kernel ends with the
+// block's FWT detail (fwt_code, the wavelet order FWT) in detail[j] instead
+// of writing the lab out.
+template <typename Real, int NF, int FWT = 0>
 __global__ __launch_bounds__(LAB_THREADS) void lab_kernel(int nlist, const int32_t* __restrict__ slots, int LS, int LAB,
                                                   int order, const int32_t* __restrict__ nbr27,
                                                   const int4* __restrict__ blk, LevelMaps m,
                                                   const int16_t* __restrict__ perm, LabFields<Real> lf,
-                                                  int* __restrict__ bad) {
+                                                  double* __restrict__ detail, int* __restrict__ bad) {
+    static_assert(FWT == 0 || NF == 1, "details of one field");
     const int j = blockIdx.x;
     if (j >= nlist) return;
+    constexpr int FLAB = FWT == 4 ? 13 : 10;
+    __shared__ Real s_lab[FWT ? FLAB * FLAB * FLAB : 1];
     const int s = slots ? slots[j] : j;
     const int4 b = blk[s];
     const int lab3 = LAB * LAB * LAB;
@@ -431,7 +469,9 @@ __global__ __launch_bounds__(LAB_THREADS) void lab_kernel(int nlist, const int32
         __syncthreads();
     }
     for (int i = threadIdx.x; i < lab3; i += blockDim.x) {
-        const int p = perm[i];  // lab cells in lab_order(): waves see uniform work
+        // a lab without level jumps is copies only: cells in storage order
+        // (coalesced); else in lab_order(), so that waves see uniform work
+        const int p = jump ? perm[i] : i;
         const int ix = p % LAB + LS, iy = (p / LAB) % LAB + LS, iz = p / (LAB * LAB) + LS;
         const int code = dir_code(ix, iy, iz);
         const int src = code == 13 ? s : nbr27[27 * s + code];
@@ -497,67 +537,40 @@ __global__ __launch_bounds__(LAB_THREADS) void lab_kernel(int nlist, const int32
                 }
             }
         }
+        if constexpr (FWT != 0) {
+            s_lab[p] = v[0];
+        } else {
 #pragma unroll
-        for (int k = 0; k < NF; ++k) lf.lab[k][(size_t)j * lab3 + p] = v[k];
-    }
-    if (my_bad) atomicOr(bad, 1);
-}
-
-// BlockFWT::_fwt / _filter (MRAGBlockFWT.h:219-298): 64 coarse points x 8
-// tensor filters per block; the block's detail is the largest |value| over
-// the 7 wavelet codes (FWTReport::_getDetailMaxMag, :109-120; the code-0
-// scaling values do not enter it and are not formed).
-//
-// The analysis taps Ga are compile-time constants.  A term whose Ga tap is 0
-// (order 4 has two) adds a signed zero -- the lab holds finite values --
-// which changes no partial sum except the sign of a zero, and so not
-// |result|: those terms are left out (343 -> 125 for code 7), the others are
-// summed in the reference's order (z, y, x taps ascending).  The tap
-// products wZ*wY*wX are exact in either precision (powers of two times 1, 9
-// or 81, 729), so folding them at compile time gives the runtime values.
-template <int ORDER>
-struct FwtGeom;
-template <>
-struct FwtGeom<4> {  // Ga support [-4, 3): 1/16, 0, -9/16, 1, -9/16, 0, 1/16
-    static constexpr int LS = -2, LAB = 13, NT = 5;
-    static constexpr int tap(int k) { return k == 0 ? -4 : k == 1 ? -2 : k == 2 ? -1 : k == 3 ? 0 : 2; }
-    static constexpr double w(int k) { return k == 0 ? 1 / 16. : k == 1 ? -9 / 16. : k == 2 ? 1. : k == 3 ? -9 / 16. : 1 / 16.; }
-};
-template <>
-struct FwtGeom<2> {  // Ga support [-2, 1): -1/2, 1, -1/2
-    static constexpr int LS = -1, LAB = 10, NT = 3;
-    static constexpr int tap(int k) { return k - 2; }
-    static constexpr double w(int k) { return k == 1 ? 1. : -0.5; }
-};
-
-// one wavelet code (BX, BY, BZ: the axes filtered with Ga, the others with
-// Ha = {1}) at the coarse point whose lab position p points at
-template <typename Real, int ORDER, int BX, int BY, int BZ>
-__device__ __forceinline__ Real fwt_code(const Real* __restrict__ p) {
-    using G = FwtGeom<ORDER>;
-    constexpr int LAB = G::LAB;
-    constexpr int nz = BZ ? G::NT : 1, ny = BY ? G::NT : 1, nx = BX ? G::NT : 1;
-    Real r = (Real)0;
-#pragma unroll
-    for (int kz = 0; kz < nz; ++kz) {
-        const Real wZ = BZ ? (Real)G::w(kz) : (Real)1;
-        const int iz = BZ ? G::tap(kz) : 0;
-#pragma unroll
-        for (int ky = 0; ky < ny; ++ky) {
-            const Real wY = BY ? (Real)G::w(ky) : (Real)1;
-            const int iy = BY ? G::tap(ky) : 0;
-            const Real wZwY = wZ * wY;
-#pragma unroll
-            for (int kx = 0; kx < nx; ++kx) {
-                const Real wX = BX ? (Real)G::w(kx) : (Real)1;
-                const int ix = BX ? G::tap(kx) : 0;
-                const Real w = wZwY * wX;
-                const Real prod = p[-(iz * LAB * LAB + iy * LAB + ix)] * w;
-                r = r + prod;
-            }
+            for (int k = 0; k < NF; ++k) lf.lab[k][(size_t)j * lab3 + p] = v[k];
         }
     }
-    return r;
+    if (my_bad) atomicOr(bad, 1);
+    if constexpr (FWT != 0) {
+        // one coarse point per lane: wave 0 forms code 7, wave 1 codes 3 and
+        // 5, wave 2 codes 6, 1, 2 and 4 (125, 50 and 40 terms at order 4)
+        using G = FwtGeom<FWT>;
+        __shared__ double s_max[LAB_THREADS / 64];
+        __syncthreads();
+        const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+        const int dx = lane & 3, dy = (lane >> 2) & 3, dz = lane >> 4;
+        const Real* q = s_lab + (2 * dz - G::LS) * FLAB * FLAB + (2 * dy - G::LS) * FLAB + (2 * dx - G::LS);
+        double dv = 0.0;
+        if (wv == 0) {
+            dv = (double)fabs(fwt_code<Real, FWT, 1, 1, 1>(q));
+        } else if (wv == 1) {
+            dv = fmax(dv, (double)fabs(fwt_code<Real, FWT, 1, 1, 0>(q)));
+            dv = fmax(dv, (double)fabs(fwt_code<Real, FWT, 1, 0, 1>(q)));
+        } else if (wv == 2) {
+            dv = fmax(dv, (double)fabs(fwt_code<Real, FWT, 0, 1, 1>(q)));
+            dv = fmax(dv, (double)fabs(fwt_code<Real, FWT, 1, 0, 0>(q)));
+            dv = fmax(dv, (double)fabs(fwt_code<Real, FWT, 0, 1, 0>(q)));
+            dv = fmax(dv, (double)fabs(fwt_code<Real, FWT, 0, 0, 1>(q)));
+        }
+        for (int off = 32; off > 0; off >>= 1) dv = fmax(dv, __shfl_xor(dv, off, 64));
+        if (lane == 0) s_max[wv] = dv;
+        __syncthreads();
+        if (threadIdx.x == 0) detail[j] = fmax(s_max[0], fmax(s_max[1], s_max[2]));
+    }
 }
 
 // Two blocks per 256-lane workgroup, two waves per block, one coarse point
@@ -619,52 +632,56 @@ struct RemapFields {
     const Real* __restrict__ lab[4];
     Real* __restrict__ out[4];
 };
+// Split children: each destination cell d (0 <= d < 16 per axis in the
+// parent's doubled lattice) sums the parent-lab sources s whose Hs tap 2s - d
+// is nonzero, in increasing s -- the reference scatters every source of
+// [S0, E0) (MRAGBlockSplitter.h:49-96) in increasing order, and the terms
+// of its zero taps (order 4: 2s - d = +-2) add a signed zero to a sum that
+// starts at +0 and so is never -0: they change nothing and are left out.
+// The sources are synth_src / synth_w's (the same Hs taps): even d one
+// (weight 1), odd d two (order 2) or four (order 4).  The lanes of a
+// workgroup take the cells by parity class, one class per wave, so a wave's
+// trip counts are uniform (1, 4, 16 or 64 terms at order 4).
 template <typename Real>
 __global__ __launch_bounds__(512) void remap_kernel(int nnew, const Remap* __restrict__ rm, RemapFields<Real> rf,
-                                                    int LS, int LAB, WTaps<Real> tp) {
+                                                    int LS, int LAB, int order) {
     const int ns = blockIdx.x;
     if (ns >= nnew) return;
     const Remap r = rm[ns];
     const int t = threadIdx.x;
-    const int x = t & 7, y = (t >> 3) & 7, z = t >> 6;
     Real v[4];
+    int cell = t;
     if (r.kind == 0) {
 #pragma unroll
         for (int f = 0; f < 4; ++f) v[f] = rf.old[f][(size_t)r.src * BCELLS + t];
     } else if (r.kind == 2) {
+        const int x = t & 7, y = (t >> 3) & 7, z = t >> 6;
         const int o = (x >> 2) | ((y >> 2) << 1) | ((z >> 2) << 2);
         const int c = 2 * (x & 3) + 8 * (2 * (y & 3)) + 64 * (2 * (z & 3));
 #pragma unroll
         for (int f = 0; f < 4; ++f) v[f] = (Real)0 + rf.old[f][(size_t)r.kids[o] * BCELLS + c];
     } else {
-        // destination in the parent's doubled lattice
+        const int w = t >> 6, l = t & 63;  // parity class (wave), cell of the class (lane)
+        const int x = 2 * (l & 3) + (w & 1), y = 2 * ((l >> 2) & 3) + ((w >> 1) & 1), z = 2 * (l >> 4) + (w >> 2);
+        cell = x + 8 * y + 64 * z;
         const int d[3] = {8 * (r.octant & 1) + x, 8 * ((r.octant >> 1) & 1) + y, 8 * ((r.octant >> 2) & 1) + z};
-        // source range [S0, E0) (MRAGBlockSplitter.h:49-59); of it, source s
-        // contributes to destination d (0 <= d < 16) iff
-        // 2 s + 1 - he0 <= d < 2 s + 1 - hs0, i.e. s in [lo(d), hi(d)] below:
-        // the loops visit exactly the contributing sources, in increasing order
-        const int S0 = (int)ceil(tp.hs0 * 0.5), E0 = 1 + (int)floor((tp.he0 - 1) * 0.5 + 8);
-        int lo[3], hi[3];
-        for (int a = 0; a < 3; ++a) {
-            lo[a] = max(S0, ((d[a] - 1 + tp.hs0) >> 1) + 1);  // arithmetic shift: floor division by 2
-            hi[a] = min(E0 - 1, (d[a] - 1 + tp.he0) >> 1);
-        }
         const size_t L0 = (size_t)r.src * LAB * LAB * LAB;
 #pragma unroll
         for (int f = 0; f < 4; ++f) v[f] = (Real)0;
-        for (int sz = lo[2]; sz <= hi[2]; ++sz) {
-            const int kz = 2 * sz - d[2];
-            const double wZ = (double)tp.hs[kz - tp.hs0];
-            for (int sy = lo[1]; sy <= hi[1]; ++sy) {
-                const int ky = 2 * sy - d[1];
-                const double wZwY = wZ * (double)tp.hs[ky - tp.hs0];
-                for (int sx = lo[0]; sx <= hi[0]; ++sx) {
-                    const int kx = 2 * sx - d[0];
-                    const Real w = (Real)(wZwY * (double)tp.hs[kx - tp.hs0]);
+        const int nz = synth_n(order, d[2]), ny = synth_n(order, d[1]), nx = synth_n(order, d[0]);
+        for (int kz = 0; kz < nz; ++kz) {
+            const int sz = synth_src(order, d[2], kz);
+            const double wZ = synth_w(order, d[2], kz);
+            for (int ky = 0; ky < ny; ++ky) {
+                const int sy = synth_src(order, d[1], ky);
+                const double wZwY = wZ * synth_w(order, d[1], ky);
+                for (int kx = 0; kx < nx; ++kx) {
+                    const int sx = synth_src(order, d[0], kx);
+                    const Real wt = (Real)(wZwY * synth_w(order, d[0], kx));
                     const size_t q = L0 + (sz - LS) * LAB * LAB + (sy - LS) * LAB + (sx - LS);
 #pragma unroll
                     for (int f = 0; f < 4; ++f) {
-                        const Real prod = rf.lab[f][q] * w;
+                        const Real prod = rf.lab[f][q] * wt;
                         v[f] = v[f] + prod;
                     }
                 }
@@ -672,7 +689,7 @@ __global__ __launch_bounds__(512) void remap_kernel(int nnew, const Remap* __res
         }
     }
 #pragma unroll
-    for (int f = 0; f < 4; ++f) rf.out[f][(size_t)ns * BCELLS + t] = v[f];
+    for (int f = 0; f < 4; ++f) rf.out[f][(size_t)ns * BCELLS + cell] = v[f];
 }
 
 // per-cell constants on a multi-level grid: as rd_statics_kernel, with the
@@ -1363,13 +1380,37 @@ int device_labs(Ctx& c, int nlist, const int32_t* dslots, int nf, const Real* co
     if (nf == 1)
         hipLaunchKernelGGL((lab_kernel<Real, 1>), dim3((unsigned)nlist), dim3(LAB_THREADS), 0, c.stream, nlist, dslots,
                            LS, LAB, order, c.dNbr27.as<int32_t>(), c.dBlk.as<int4>(), m, c.labPerm.as<int16_t>(), lf,
-                           c.dBad.as<int>());
+                           (double*)nullptr, c.dBad.as<int>());
     else if (nf == 4)
         hipLaunchKernelGGL((lab_kernel<Real, 4>), dim3((unsigned)nlist), dim3(LAB_THREADS), 0, c.stream, nlist, dslots,
                            LS, LAB, order, c.dNbr27.as<int32_t>(), c.dBlk.as<int4>(), m, c.labPerm.as<int16_t>(), lf,
-                           c.dBad.as<int>());
+                           (double*)nullptr, c.dBad.as<int>());
     else
         return set_error(GH_EINVAL, "device_labs: 1 or 4 fields");
+    GH_HIP_CHECK(hipGetLastError());
+    return GH_OK;
+}
+
+// FWT details of nlist slots (dslots on the device) from labs built in LDS
+// and never stored; clears and then may set c.dBad as device_labs
+template <typename Real>
+int device_details(Ctx& c, int nlist, const int32_t* dslots, const Real* field, double* detail) {
+    const int order = c.tree.order;
+    const int LS = lab_start(order), LAB = lab_size(order);
+    int rc;
+    if (nlist == 0) return GH_OK;
+    if ((rc = lab_order(c, order))) return rc;
+    const LevelMaps m = level_maps(c);
+    LabFields<Real> lf{};
+    lf.f[0] = field;
+    if (order == 4)
+        hipLaunchKernelGGL((lab_kernel<Real, 1, 4>), dim3((unsigned)nlist), dim3(LAB_THREADS), 0, c.stream, nlist,
+                           dslots, LS, LAB, order, c.dNbr27.as<int32_t>(), c.dBlk.as<int4>(), m,
+                           c.labPerm.as<int16_t>(), lf, detail, c.dBad.as<int>());
+    else
+        hipLaunchKernelGGL((lab_kernel<Real, 1, 2>), dim3((unsigned)nlist), dim3(LAB_THREADS), 0, c.stream, nlist,
+                           dslots, LS, LAB, order, c.dNbr27.as<int32_t>(), c.dBlk.as<int4>(), m,
+                           c.labPerm.as<int16_t>(), lf, detail, c.dBad.as<int>());
     GH_HIP_CHECK(hipGetLastError());
     return GH_OK;
 }
@@ -1408,10 +1449,8 @@ int fwt_details_t(Ctx& c, const std::vector<int>& slots, std::vector<double>& de
         const int m = (int)slots.size();
         if ((rc = c.labSlots.ensure_grow((size_t)m * 4)) || (rc = clear_bad(c))) return rc;
         GH_HIP_CHECK(hipMemcpyAsync(c.labSlots.p, slots.data(), (size_t)m * 4, hipMemcpyHostToDevice, c.stream));
-        const Real* fld[1] = {c.phi[c.cur].as<Real>()};
-        DevBuf* out[1] = {&c.labBuf};
-        if ((rc = device_labs<Real>(c, m, c.labSlots.as<int32_t>(), 1, fld, out))) return rc;
-        launch_fwt<Real>(order, m, c.labBuf.as<Real>(), d.as<double>(), c.stream);
+        if ((rc = device_details<Real>(c, m, c.labSlots.as<int32_t>(), c.phi[c.cur].as<Real>(), d.as<double>())))
+            return rc;
         int bad = 1;
         hipError_t e = hipGetLastError();
         if (e == hipSuccess) e = hipMemcpyAsync(det.data(), d.p, (size_t)m * 8, hipMemcpyDeviceToHost, c.stream);
@@ -1497,7 +1536,7 @@ int apply_new_tree(Ctx& c, Octree& nt, const std::vector<Remap>& rm, const std::
         rf.out[f] = nf[f].as<Real>();
     }
     hipLaunchKernelGGL(remap_kernel<Real>, dim3(nnew), dim3(512), 0, c.stream, nnew, drm.as<Remap>(), rf, LS, LAB,
-                       taps<Real>(order));
+                       order);
     GH_HIP_CHECK(hipGetLastError());
     // rm (and the split list) stay alive in the caller until after
     // apply_topology, whose uploads synchronize the stream
