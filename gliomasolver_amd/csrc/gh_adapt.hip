@@ -94,20 +94,23 @@ __global__ __launch_bounds__(256) void ghost_entry_kernel(int n, const GhostEntr
     out[e.dst] = acc;
 }
 
-// Per-level block maps (Octree::dense on the device): map[off[l] + x +
-// (y << l) + (z << 2l)] = slot of a leaf at (l, x, y, z), -2 for a node with
+// Per-level block maps (Octree::dense on the device): map[dense_level_offset(l)
+// + x + (y << l) + (z << 2l)] = slot of a leaf at (l, x, y, z), -2 for a node with
 // children, -1 for a point inside a coarser leaf, for levels < nlev; levels
 // nlev..top in the open-addressing table hkey/hval (Octree::hash_keys, 2^hbits
 // slots, linear probing, absent = -1); levels > top hold no node.
 struct LevelMaps {
     const int32_t* __restrict__ map;
-    int off[Octree::MAXL + 1];
     int nlev;
     int top;
     const uint64_t* __restrict__ hkey;
     const int32_t* __restrict__ hval;
     int hbits;
 };
+
+// the maps of levels 0, 1, ... lie one after another: level l starts at
+// 8^0 + ... + 8^(l-1) = (8^l - 1) / 7 (l <= 7: no overflow)
+__host__ __device__ inline int dense_level_offset(int l) { return (int)(((1u << (3 * l)) - 1u) / 7u); }
 
 // value of block position (L, x, y, z) in the hashed levels (Octree::node_hash;
 // the table is at most half full, so the probe reaches a free slot)
@@ -127,7 +130,6 @@ __device__ inline int32_t hashed_node(const LevelMaps& m, int L, int x, int y, i
 inline LevelMaps level_maps(const Ctx& c) {
     LevelMaps m{};
     m.map = c.dDense.as<int32_t>();
-    for (int l = 0; l <= Octree::MAXL; ++l) m.off[l] = c.denseOff[l];
     m.nlev = c.nDense;
     m.top = c.topLevel;
     m.hkey = c.dHashKey.as<uint64_t>();
@@ -161,7 +163,7 @@ __device__ inline void ghost_visit(const LevelMaps& m, int order, int L, int qx,
         qz &= mask;
         if (L > m.top) break;  // inside a coarser leaf
         const int bx = (int)(qx >> 3), by = (int)(qy >> 3), bz = (int)(qz >> 3);
-        const int32_t v = L < m.nlev ? m.map[m.off[L] + bx + (by << L) + (bz << (2 * L))] : hashed_node(m, L, bx, by, bz);
+        const int32_t v = L < m.nlev ? m.map[dense_level_offset(L) + bx + (by << L) + (bz << (2 * L))] : hashed_node(m, L, bx, by, bz);
         if (v >= 0) {
             visit((int32_t)(v * BCELLS + (qx & 7) + 8 * (qy & 7) + 64 * (qz & 7)), w);
             return;
@@ -405,23 +407,25 @@ struct LabFields {
     const Real* __restrict__ f[4];
     Real* __restrict__ lab[4];
 };
-// FWT = 2 or 4 (NF = 1): the lab stays in LDS and the kernel ends with the
-// block's FWT detail (fwt_code, the wavelet order FWT) in detail[j] instead
-// of writing the lab out.
-template <typename Real, int NF, int FWT = 0>
-__global__ __launch_bounds__(LAB_THREADS) void lab_kernel(int nlist, const int32_t* __restrict__ slots, int LS, int LAB,
-                                                  int order, const int32_t* __restrict__ nbr27,
+// ORDER (2 or 4) fixes the lab geometry at compile time (the cell loops
+// divide by the lab size).  FWT (NF = 1): the lab stays in LDS and the kernel
+// ends with the block's FWT detail (fwt_code) in detail[j] instead of writing
+// the lab out.
+template <typename Real, int NF, int ORDER, bool FWT>
+__global__ __launch_bounds__(LAB_THREADS) void lab_kernel(int nlist, const int32_t* __restrict__ slots,
+                                                  const int32_t* __restrict__ nbr27,
                                                   const int4* __restrict__ blk, LevelMaps m,
                                                   const int16_t* __restrict__ perm, LabFields<Real> lf,
                                                   double* __restrict__ detail, int* __restrict__ bad) {
-    static_assert(FWT == 0 || NF == 1, "details of one field");
+    static_assert(!FWT || NF == 1, "details of one field");
+    static_assert(ORDER == 2 || ORDER == 4, "wavelet order");
+    constexpr int order = ORDER, LS = ORDER == 4 ? -2 : -1, LAB = ORDER == 4 ? 13 : 10;
     const int j = blockIdx.x;
     if (j >= nlist) return;
-    constexpr int FLAB = FWT == 4 ? 13 : 10;
-    __shared__ Real s_lab[FWT ? FLAB * FLAB * FLAB : 1];
+    __shared__ Real s_lab[FWT ? LAB * LAB * LAB : 1];
     const int s = slots ? slots[j] : j;
     const int4 b = blk[s];
-    const int lab3 = LAB * LAB * LAB;
+    constexpr int lab3 = LAB * LAB * LAB;
     int my_bad = 0;
     // Coarse-point cache.  A lab ghost inside a coarser leaf is the Hs
     // synthesis of level-(L-1) points; all of them lie in the PC^3 patch of
@@ -432,12 +436,24 @@ __global__ __launch_bounds__(LAB_THREADS) void lab_kernel(int nlist, const int32
     // and order of accumulation as the recursion, which the points that need
     // a further synthesis (flag 0) still take.
     const int L = b.x;
-    const int PC = order == 4 ? 9 : 6, PL = order == 4 ? 2 : 1;
+    constexpr int PC = ORDER == 4 ? 9 : 6, PL = ORDER == 4 ? 2 : 1;
     __shared__ Real s_cv[NF][729];
     __shared__ unsigned char s_cf[729];
     bool jump = false;
     for (int k = 0; k < 27; ++k) jump |= (k != 13 && nbr27[27 * s + k] < 0);
     const bool use_cache = jump && L >= 1;  // uniform per workgroup
+    // what each of the 27 level-L regions around the block is: a leaf (its
+    // slot), subdivided (-2) or inside a coarser leaf (-1) -- looked up once
+    // here instead of for every ghost point
+    __shared__ int32_t s_reg[27];
+    if (jump && threadIdx.x < 27) {
+        const int k = threadIdx.x, msk = (1 << L) - 1;
+        const int bx = (b.y + k % 3 - 1) & msk, by = (b.z + (k / 3) % 3 - 1) & msk, bz = (b.w + k / 9 - 1) & msk;
+        const int32_t v = L < m.nlev ? m.map[dense_level_offset(L) + bx + (by << L) + (bz << (2 * L))]
+                                     : hashed_node(m, L, bx, by, bz);
+        s_reg[k] = v >= 0 ? v : (v == -2 ? -2 : -1);
+    }
+    if (jump && !use_cache) __syncthreads();
     if (use_cache) {
         const int ox = 4 * b.y - PL, oy = 4 * b.z - PL, oz = 4 * b.w - PL;
         for (int q = threadIdx.x; q < PC * PC * PC; q += blockDim.x) {
@@ -450,7 +466,7 @@ __global__ __launch_bounds__(LAB_THREADS) void lab_kernel(int nlist, const int32
                 qz &= mask;
                 if (Lc > m.top) break;
                 const int bx = qx >> 3, by = qy >> 3, bz = qz >> 3;
-                const int32_t v = Lc < m.nlev ? m.map[m.off[Lc] + bx + (by << Lc) + (bz << (2 * Lc))]
+                const int32_t v = Lc < m.nlev ? m.map[dense_level_offset(Lc) + bx + (by << Lc) + (bz << (2 * Lc))]
                                               : hashed_node(m, Lc, bx, by, bz);
                 if (v >= 0) {
                     src = v * BCELLS + (qx & 7) + 8 * (qy & 7) + 64 * (qz & 7);
@@ -495,14 +511,10 @@ __global__ __launch_bounds__(LAB_THREADS) void lab_kernel(int nlist, const int32
             // the level-L region of this point: inside a coarser leaf?
             const int mask = (BS << L) - 1;
             const int qx = (8 * b.y + ix) & mask, qy = (8 * b.z + iy) & mask, qz = (8 * b.w + iz) & mask;
-            bool coarser = false;
-            if (use_cache) {
-                const int bx = qx >> 3, by = qy >> 3, bz = qz >> 3;
-                const int32_t r = L < m.nlev ? m.map[m.off[L] + bx + (by << L) + (bz << (2 * L))]
-                                             : hashed_node(m, L, bx, by, bz);
-                coarser = r == -1;
-            }
-            if (!coarser) {
+            const int32_t reg = s_reg[code];  // src < 0: the block has a level jump
+            if (reg == -2) {  // subdivided: ghost_visit<2>'s first step, the coincident fine vertex
+                ghost_visit<2>(m, order, L + 1, 2 * qx, 2 * qy, 2 * qz, 1.0, visit, my_bad);
+            } else if (!(use_cache && reg == -1)) {
                 ghost_visit<2>(m, order, L, qx, qy, qz, 1.0, visit, my_bad);
             } else {  // ghost_visit<2>'s synthesis step, coarse points from the cache
                 const int nc = BS << (L - 1);  // coarse lattice size (a power of two: & wraps)
@@ -537,7 +549,7 @@ __global__ __launch_bounds__(LAB_THREADS) void lab_kernel(int nlist, const int32
                 }
             }
         }
-        if constexpr (FWT != 0) {
+        if constexpr (FWT) {
             s_lab[p] = v[0];
         } else {
 #pragma unroll
@@ -545,10 +557,11 @@ __global__ __launch_bounds__(LAB_THREADS) void lab_kernel(int nlist, const int32
         }
     }
     if (my_bad) atomicOr(bad, 1);
-    if constexpr (FWT != 0) {
+    if constexpr (FWT) {
         // one coarse point per lane: wave 0 forms code 7, wave 1 codes 3 and
         // 5, wave 2 codes 6, 1, 2 and 4 (125, 50 and 40 terms at order 4)
-        using G = FwtGeom<FWT>;
+        using G = FwtGeom<ORDER>;
+        constexpr int FLAB = LAB;
         __shared__ double s_max[LAB_THREADS / 64];
         __syncthreads();
         const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -556,15 +569,15 @@ __global__ __launch_bounds__(LAB_THREADS) void lab_kernel(int nlist, const int32
         const Real* q = s_lab + (2 * dz - G::LS) * FLAB * FLAB + (2 * dy - G::LS) * FLAB + (2 * dx - G::LS);
         double dv = 0.0;
         if (wv == 0) {
-            dv = (double)fabs(fwt_code<Real, FWT, 1, 1, 1>(q));
+            dv = (double)fabs(fwt_code<Real, ORDER, 1, 1, 1>(q));
         } else if (wv == 1) {
-            dv = fmax(dv, (double)fabs(fwt_code<Real, FWT, 1, 1, 0>(q)));
-            dv = fmax(dv, (double)fabs(fwt_code<Real, FWT, 1, 0, 1>(q)));
+            dv = fmax(dv, (double)fabs(fwt_code<Real, ORDER, 1, 1, 0>(q)));
+            dv = fmax(dv, (double)fabs(fwt_code<Real, ORDER, 1, 0, 1>(q)));
         } else if (wv == 2) {
-            dv = fmax(dv, (double)fabs(fwt_code<Real, FWT, 0, 1, 1>(q)));
-            dv = fmax(dv, (double)fabs(fwt_code<Real, FWT, 1, 0, 0>(q)));
-            dv = fmax(dv, (double)fabs(fwt_code<Real, FWT, 0, 1, 0>(q)));
-            dv = fmax(dv, (double)fabs(fwt_code<Real, FWT, 0, 0, 1>(q)));
+            dv = fmax(dv, (double)fabs(fwt_code<Real, ORDER, 0, 1, 1>(q)));
+            dv = fmax(dv, (double)fabs(fwt_code<Real, ORDER, 1, 0, 0>(q)));
+            dv = fmax(dv, (double)fabs(fwt_code<Real, ORDER, 0, 1, 0>(q)));
+            dv = fmax(dv, (double)fabs(fwt_code<Real, ORDER, 0, 0, 1>(q)));
         }
         for (int off = 32; off > 0; off >>= 1) dv = fmax(dv, __shfl_xor(dv, off, 64));
         if (lane == 0) s_max[wv] = dv;
@@ -1170,7 +1183,7 @@ int apply_topology(Ctx& c) {
     c.topLevel = t.top_level;
     c.hashBits = t.hash_bits;
     for (int l = 0; l < c.nDense; ++l) {
-        c.denseOff[l] = (int)dense.size();
+        if ((int)dense.size() != dense_level_offset(l)) return set_error(GH_EINVAL, "level maps: layout");
         dense.insert(dense.end(), t.dense[l].begin(), t.dense[l].end());
     }
     if (c.hashBits > 0) {  // the tree outlives the asynchronous copies (synchronized below)
@@ -1360,13 +1373,15 @@ int lab_order(Ctx& c, int order) {
     return GH_OK;
 }
 
-// Labs of nlist slots (dslots on the device, or nullptr: slots 0..nlist-1)
-// with the ghosts evaluated on the device; clears and then may set c.dBad
-// (a ghost deeper than the kernel's recursion: use host tables instead).
+// Labs of the four fields (phi, p_w, p_g, p_csf) of nlist slots (dslots on
+// the device, or nullptr: slots 0..nlist-1) with the ghosts evaluated on the
+// device; may set c.dBad (a ghost deeper than the kernel's recursion: use
+// host tables instead).
 template <typename Real>
 int device_labs(Ctx& c, int nlist, const int32_t* dslots, int nf, const Real* const* fields, DevBuf* const* outs) {
     const int order = c.tree.order;
-    const int LS = lab_start(order), LAB = lab_size(order), lab3 = LAB * LAB * LAB;
+    const int LAB = lab_size(order), lab3 = LAB * LAB * LAB;
+    if (nf != 4) return set_error(GH_EINVAL, "device_labs: 4 fields");
     int rc;
     LabFields<Real> lf{};
     for (int k = 0; k < nf; ++k) {
@@ -1377,16 +1392,14 @@ int device_labs(Ctx& c, int nlist, const int32_t* dslots, int nf, const Real* co
     if (nlist == 0) return GH_OK;
     if ((rc = lab_order(c, order))) return rc;
     const LevelMaps m = level_maps(c);
-    if (nf == 1)
-        hipLaunchKernelGGL((lab_kernel<Real, 1>), dim3((unsigned)nlist), dim3(LAB_THREADS), 0, c.stream, nlist, dslots,
-                           LS, LAB, order, c.dNbr27.as<int32_t>(), c.dBlk.as<int4>(), m, c.labPerm.as<int16_t>(), lf,
-                           (double*)nullptr, c.dBad.as<int>());
-    else if (nf == 4)
-        hipLaunchKernelGGL((lab_kernel<Real, 4>), dim3((unsigned)nlist), dim3(LAB_THREADS), 0, c.stream, nlist, dslots,
-                           LS, LAB, order, c.dNbr27.as<int32_t>(), c.dBlk.as<int4>(), m, c.labPerm.as<int16_t>(), lf,
+    if (order == 4)
+        hipLaunchKernelGGL((lab_kernel<Real, 4, 4, false>), dim3((unsigned)nlist), dim3(LAB_THREADS), 0, c.stream, nlist,
+                           dslots, c.dNbr27.as<int32_t>(), c.dBlk.as<int4>(), m, c.labPerm.as<int16_t>(), lf,
                            (double*)nullptr, c.dBad.as<int>());
     else
-        return set_error(GH_EINVAL, "device_labs: 1 or 4 fields");
+        hipLaunchKernelGGL((lab_kernel<Real, 4, 2, false>), dim3((unsigned)nlist), dim3(LAB_THREADS), 0, c.stream, nlist,
+                           dslots, c.dNbr27.as<int32_t>(), c.dBlk.as<int4>(), m, c.labPerm.as<int16_t>(), lf,
+                           (double*)nullptr, c.dBad.as<int>());
     GH_HIP_CHECK(hipGetLastError());
     return GH_OK;
 }
@@ -1396,7 +1409,6 @@ int device_labs(Ctx& c, int nlist, const int32_t* dslots, int nf, const Real* co
 template <typename Real>
 int device_details(Ctx& c, int nlist, const int32_t* dslots, const Real* field, double* detail) {
     const int order = c.tree.order;
-    const int LS = lab_start(order), LAB = lab_size(order);
     int rc;
     if (nlist == 0) return GH_OK;
     if ((rc = lab_order(c, order))) return rc;
@@ -1404,13 +1416,13 @@ int device_details(Ctx& c, int nlist, const int32_t* dslots, const Real* field, 
     LabFields<Real> lf{};
     lf.f[0] = field;
     if (order == 4)
-        hipLaunchKernelGGL((lab_kernel<Real, 1, 4>), dim3((unsigned)nlist), dim3(LAB_THREADS), 0, c.stream, nlist,
-                           dslots, LS, LAB, order, c.dNbr27.as<int32_t>(), c.dBlk.as<int4>(), m,
-                           c.labPerm.as<int16_t>(), lf, detail, c.dBad.as<int>());
+        hipLaunchKernelGGL((lab_kernel<Real, 1, 4, true>), dim3((unsigned)nlist), dim3(LAB_THREADS), 0, c.stream,
+                           nlist, dslots, c.dNbr27.as<int32_t>(), c.dBlk.as<int4>(), m, c.labPerm.as<int16_t>(), lf,
+                           detail, c.dBad.as<int>());
     else
-        hipLaunchKernelGGL((lab_kernel<Real, 1, 2>), dim3((unsigned)nlist), dim3(LAB_THREADS), 0, c.stream, nlist,
-                           dslots, LS, LAB, order, c.dNbr27.as<int32_t>(), c.dBlk.as<int4>(), m,
-                           c.labPerm.as<int16_t>(), lf, detail, c.dBad.as<int>());
+        hipLaunchKernelGGL((lab_kernel<Real, 1, 2, true>), dim3((unsigned)nlist), dim3(LAB_THREADS), 0, c.stream,
+                           nlist, dslots, c.dNbr27.as<int32_t>(), c.dBlk.as<int4>(), m, c.labPerm.as<int16_t>(), lf,
+                           detail, c.dBad.as<int>());
     GH_HIP_CHECK(hipGetLastError());
     return GH_OK;
 }

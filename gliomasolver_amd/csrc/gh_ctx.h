@@ -85,7 +85,6 @@ struct Ctx {
     // per-level block maps for device-side ghost evaluation (Octree::dense,
     // concatenated; nDense = 0: not available), block (level, x, y, z) per slot
     DevBuf dDense, dBlk, dBad;
-    int denseOff[Octree::MAXL + 1] = {0};
     int nDense = 0;
     int topLevel = -1;           // finest leaf level
     DevBuf dHashKey, dHashVal;   // nodes of levels nDense..topLevel (Octree::hash_keys / hash_vals)
