@@ -398,7 +398,7 @@ __device__ __forceinline__ Real fwt_code(const Real* __restrict__ p) {
 // Full labs of the listed slots (slots == nullptr: every slot j), ghosts
 // evaluated on the device; *bad is set when a ghost needs more than two
 // synthesis levels (the caller then uses host tables).
-constexpr int LAB_THREADS = 512;  // lanes per lab (256: 12 % slower epochs, 1024: no gain)
+constexpr int LAB_THREADS = 512;  // lanes per lab (at least 256: the FWT epilogue takes four waves)
 // up to four fields whose labs are built together: the ghost recursion (map
 // lookups, terms, weights) runs once and each field accumulates its own sum
 // with the single-field arithmetic, so every lab has the bits it has alone
@@ -559,7 +559,8 @@ __global__ __launch_bounds__(LAB_THREADS) void lab_kernel(int nlist, const int32
     if (my_bad) atomicOr(bad, 1);
     if constexpr (FWT) {
         // one coarse point per lane: wave 0 forms code 7, wave 1 codes 3 and
-        // 5, wave 2 codes 6, 1, 2 and 4 (125, 50 and 40 terms at order 4)
+        // 5, wave 2 codes 6 and 1, wave 3 codes 2 and 4 (125, 50, 30 and 10
+        // terms at order 4)
         using G = FwtGeom<ORDER>;
         constexpr int FLAB = LAB;
         __shared__ double s_max[LAB_THREADS / 64];
@@ -576,13 +577,14 @@ __global__ __launch_bounds__(LAB_THREADS) void lab_kernel(int nlist, const int32
         } else if (wv == 2) {
             dv = fmax(dv, (double)fabs(fwt_code<Real, ORDER, 0, 1, 1>(q)));
             dv = fmax(dv, (double)fabs(fwt_code<Real, ORDER, 1, 0, 0>(q)));
+        } else if (wv == 3) {
             dv = fmax(dv, (double)fabs(fwt_code<Real, ORDER, 0, 1, 0>(q)));
             dv = fmax(dv, (double)fabs(fwt_code<Real, ORDER, 0, 0, 1>(q)));
         }
         for (int off = 32; off > 0; off >>= 1) dv = fmax(dv, __shfl_xor(dv, off, 64));
         if (lane == 0) s_max[wv] = dv;
         __syncthreads();
-        if (threadIdx.x == 0) detail[j] = fmax(s_max[0], fmax(s_max[1], s_max[2]));
+        if (threadIdx.x == 0) detail[j] = fmax(fmax(s_max[0], s_max[1]), fmax(s_max[2], s_max[3]));
     }
 }
 

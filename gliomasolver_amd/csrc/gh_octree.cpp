@@ -53,14 +53,30 @@ void Octree::init_uniform(int level) {
     rebuild_index();
 }
 
-// canonical slot order: (level, z, y, x) ascending
+// canonical slot order: (level, z, y, x) ascending, by an LSD radix sort of
+// the 31-bit keys level << 27 | z << 18 | y << 9 | x (levels < 16,
+// coordinates < 512) in three passes of 11 bits
 void Octree::rebuild_index() {
-    auto order_key = [](const BlockRec& b) {
-        return ((uint64_t)b.level << 48) | ((uint64_t)(uint32_t)b.index[2] << 32) |
-               ((uint64_t)(uint32_t)b.index[1] << 16) | (uint64_t)(uint32_t)b.index[0];
-    };
-    std::sort(leaves.begin(), leaves.end(),
-              [&](const BlockRec& a, const BlockRec& b) { return order_key(a) < order_key(b); });
+    {
+        const size_t nl = leaves.size();
+        std::vector<uint32_t> a(nl), b(nl);
+        for (size_t s = 0; s < nl; ++s) {
+            const BlockRec& r = leaves[s];
+            a[s] = (uint32_t)r.level << 27 | (uint32_t)r.index[2] << 18 | (uint32_t)r.index[1] << 9 |
+                   (uint32_t)r.index[0];
+        }
+        std::vector<uint32_t> cnt(2049);
+        for (int pass = 0; pass < 3; ++pass) {
+            const int sh = 11 * pass;
+            std::fill(cnt.begin(), cnt.end(), 0u);
+            for (uint32_t k : a) ++cnt[((k >> sh) & 2047u) + 1];
+            for (int i = 0; i < 2048; ++i) cnt[i + 1] += cnt[i];
+            for (uint32_t k : a) b[cnt[(k >> sh) & 2047u]++] = k;
+            a.swap(b);
+        }
+        for (size_t s = 0; s < nl; ++s)
+            leaves[s] = BlockRec{(int)(a[s] >> 27), {(int)(a[s] & 511u), (int)((a[s] >> 9) & 511u), (int)((a[s] >> 18) & 511u)}};
+    }
     for (int l = 0; l < MAXL; ++l) dense[l].clear();
     dense_levels = 0;
     top_level = -1;
@@ -68,7 +84,7 @@ void Octree::rebuild_index() {
     hash_keys.clear();
     hash_vals.clear();
     if (leaves.empty()) return;
-    const int top = levels_max();
+    const int top = leaves.back().level;  // the finest level sorts last
     const int nd = std::min(top, DENSE_MAX);
     for (int l = 0; l <= nd; ++l) dense[l].assign((size_t)1 << (3 * l), -1);
     size_t deep = 0;  // nodes below the dense levels
