@@ -398,7 +398,12 @@ __device__ __forceinline__ Real fwt_code(const Real* __restrict__ p) {
 // Full labs of the listed slots (slots == nullptr: every slot j), ghosts
 // evaluated on the device; *bad is set when a ghost needs more than two
 // synthesis levels (the caller then uses host tables).
-constexpr int LAB_THREADS = 512;  // lanes per lab (at least 256: the FWT epilogue takes four waves)
+// Lanes per lab: 512 while the labs are fewer than the CUs can hold several
+// of, 256 beyond that (more labs in flight per CU hide the dependent map and
+// field loads better; at least 256: the FWT epilogue takes four waves).
+// Details of 1,114 / 7,225 / 13,455 blocks: 0.12 / 0.41 / 0.54 ms with 512,
+// 0.15 / 0.38 / 0.47 ms with 256 lanes.
+constexpr int kLabThreadsFewMax = 2048;  // largest list built with 512 lanes per lab
 // up to four fields whose labs are built together: the ghost recursion (map
 // lookups, terms, weights) runs once and each field accumulates its own sum
 // with the single-field arithmetic, so every lab has the bits it has alone
@@ -411,7 +416,7 @@ struct LabFields {
 // divide by the lab size).  FWT (NF = 1): the lab stays in LDS and the kernel
 // ends with the block's FWT detail (fwt_code) in detail[j] instead of writing
 // the lab out.
-template <typename Real, int NF, int ORDER, bool FWT>
+template <typename Real, int NF, int ORDER, bool FWT, int LAB_THREADS>
 __global__ __launch_bounds__(LAB_THREADS) void lab_kernel(int nlist, const int32_t* __restrict__ slots,
                                                   const int32_t* __restrict__ nbr27,
                                                   const int4* __restrict__ blk, LevelMaps m,
@@ -456,7 +461,7 @@ __global__ __launch_bounds__(LAB_THREADS) void lab_kernel(int nlist, const int32
     if (jump && !use_cache) __syncthreads();
     if (use_cache) {
         const int ox = 4 * b.y - PL, oy = 4 * b.z - PL, oz = 4 * b.w - PL;
-        for (int q = threadIdx.x; q < PC * PC * PC; q += blockDim.x) {
+        for (int q = threadIdx.x; q < PC * PC * PC; q += LAB_THREADS) {
             int Lc = L - 1, qx = ox + q % PC, qy = oy + (q / PC) % PC, qz = oz + q / (PC * PC);
             int32_t src = -1;
             for (;;) {  // ghost_visit's walk without synthesis
@@ -484,7 +489,7 @@ __global__ __launch_bounds__(LAB_THREADS) void lab_kernel(int nlist, const int32
         }
         __syncthreads();
     }
-    for (int i = threadIdx.x; i < lab3; i += blockDim.x) {
+    for (int i = threadIdx.x; i < lab3; i += LAB_THREADS) {
         // a lab without level jumps is copies only: cells in storage order
         // (coalesced); else in lab_order(), so that waves see uniform work
         const int p = jump ? perm[i] : i;
@@ -1375,6 +1380,25 @@ int lab_order(Ctx& c, int order) {
     return GH_OK;
 }
 
+// lab_kernel for the wavelet order and the list size
+template <typename Real, int NF, bool FWT>
+void launch_labs(Ctx& c, int order, int nlist, const int32_t* dslots, const LevelMaps& m, const LabFields<Real>& lf,
+                 double* detail) {
+    const bool few = nlist <= kLabThreadsFewMax;
+    auto go = [&](auto kern, int threads) {
+        hipLaunchKernelGGL(kern, dim3((unsigned)nlist), dim3(threads), 0, c.stream, nlist, dslots,
+                           c.dNbr27.as<int32_t>(), c.dBlk.as<int4>(), m, c.labPerm.as<int16_t>(), lf, detail,
+                           c.dBad.as<int>());
+    };
+    if (order == 4) {
+        if (few) go(lab_kernel<Real, NF, 4, FWT, 512>, 512);
+        else go(lab_kernel<Real, NF, 4, FWT, 256>, 256);
+    } else {
+        if (few) go(lab_kernel<Real, NF, 2, FWT, 512>, 512);
+        else go(lab_kernel<Real, NF, 2, FWT, 256>, 256);
+    }
+}
+
 // Labs of the four fields (phi, p_w, p_g, p_csf) of nlist slots (dslots on
 // the device, or nullptr: slots 0..nlist-1) with the ghosts evaluated on the
 // device; may set c.dBad (a ghost deeper than the kernel's recursion: use
@@ -1394,14 +1418,7 @@ int device_labs(Ctx& c, int nlist, const int32_t* dslots, int nf, const Real* co
     if (nlist == 0) return GH_OK;
     if ((rc = lab_order(c, order))) return rc;
     const LevelMaps m = level_maps(c);
-    if (order == 4)
-        hipLaunchKernelGGL((lab_kernel<Real, 4, 4, false>), dim3((unsigned)nlist), dim3(LAB_THREADS), 0, c.stream, nlist,
-                           dslots, c.dNbr27.as<int32_t>(), c.dBlk.as<int4>(), m, c.labPerm.as<int16_t>(), lf,
-                           (double*)nullptr, c.dBad.as<int>());
-    else
-        hipLaunchKernelGGL((lab_kernel<Real, 4, 2, false>), dim3((unsigned)nlist), dim3(LAB_THREADS), 0, c.stream, nlist,
-                           dslots, c.dNbr27.as<int32_t>(), c.dBlk.as<int4>(), m, c.labPerm.as<int16_t>(), lf,
-                           (double*)nullptr, c.dBad.as<int>());
+    launch_labs<Real, 4, false>(c, order, nlist, dslots, m, lf, nullptr);
     GH_HIP_CHECK(hipGetLastError());
     return GH_OK;
 }
@@ -1417,14 +1434,7 @@ int device_details(Ctx& c, int nlist, const int32_t* dslots, const Real* field, 
     const LevelMaps m = level_maps(c);
     LabFields<Real> lf{};
     lf.f[0] = field;
-    if (order == 4)
-        hipLaunchKernelGGL((lab_kernel<Real, 1, 4, true>), dim3((unsigned)nlist), dim3(LAB_THREADS), 0, c.stream,
-                           nlist, dslots, c.dNbr27.as<int32_t>(), c.dBlk.as<int4>(), m, c.labPerm.as<int16_t>(), lf,
-                           detail, c.dBad.as<int>());
-    else
-        hipLaunchKernelGGL((lab_kernel<Real, 1, 2, true>), dim3((unsigned)nlist), dim3(LAB_THREADS), 0, c.stream,
-                           nlist, dslots, c.dNbr27.as<int32_t>(), c.dBlk.as<int4>(), m, c.labPerm.as<int16_t>(), lf,
-                           detail, c.dBad.as<int>());
+    launch_labs<Real, 1, true>(c, order, nlist, dslots, m, lf, detail);
     GH_HIP_CHECK(hipGetLastError());
     return GH_OK;
 }

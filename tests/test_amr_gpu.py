@@ -175,6 +175,45 @@ def test_gpu_multilevel_rd_steps_on_large_grid(oracle, real, order):
     assert bits_equal(out, ref), "max |diff| %g" % np.abs(out.astype(float) - ref).max()
 
 
+@pytest.mark.parametrize("real", ["f32", "f64"])
+def test_gpu_details_and_splits_on_large_grid(oracle, real):
+    """Lists of more than 2,048 labs (built with 256 lanes per lab instead of
+    512): the FWT details of all 4,166 blocks, then a refinement of over 3,000
+    blocks whose copies keep their bits and whose split children (a sample of
+    48, all four fields) equal the oracle's split of the parent's lab."""
+    rows = balanced_random_grid(np.random.default_rng(5), base_level=4, max_level=6, nrefine=10)
+    t = np.float32 if real == "f32" else np.float64
+    fields = random_fields(rows, t, 8)
+    say("large grid details + splits", real, len(rows), "blocks")
+    ctx = make_ctx(real, rows, 4, fields, max_level=6)
+    det = ctx.fwt_details()
+    ref = oracle.amr_details(rows, 4, fields[FIELD_PHI], list(range(len(rows))))
+    assert np.array_equal(det, ref), "max |diff| %g" % np.abs(det - ref).max()
+    ids = [b for b in range(len(rows)) if rows[b, 0] == 4 and rows[b, 1] % 4 != 0]
+    assert len(ids) > 2048
+    nref = ctx.refine_blocks(ids)
+    assert nref >= len(ids)
+    new = rows_of(ctx)
+    out = {f: ctx.get_field_blocks(f).reshape(len(new), 512) for f in FIELDS}
+    ctx.close()
+    old_index = {tuple(r): i for i, r in enumerate(rows.tolist())}
+    kids = []
+    for s, (L, x, y, z) in enumerate(new.tolist()):
+        if (L, x, y, z) in old_index:
+            for f in FIELDS:
+                assert bits_equal(out[f][s], fields[f][old_index[(L, x, y, z)]])
+        else:
+            kids.append(s)
+    assert len(kids) == 8 * nref
+    for s in np.random.default_rng(9).choice(kids, 48, replace=False).tolist():
+        L, x, y, z = new[s].tolist()
+        parent = old_index[(L - 1, x >> 1, y >> 1, z >> 1)]
+        o = (x & 1) | ((y & 1) << 1) | ((z & 1) << 2)
+        for f in FIELDS:
+            want = oracle.amr_split(4, oracle.amr_lab(rows, 4, fields[f], parent), o).reshape(512)
+            assert bits_equal(out[f][s], want), (s, f)
+
+
 # --- oracle-driven simulation of AutomaticRefinement / AutomaticCompression --
 
 def sim_details(oracle, rows, order, phi, subset):
