@@ -523,23 +523,35 @@ __global__ __launch_bounds__(LAB_THREADS) void lab_kernel(int nlist, const int32
                 ghost_visit<2>(m, order, L, qx, qy, qz, 1.0, visit, my_bad);
             } else {  // ghost_visit<2>'s synthesis step, coarse points from the cache
                 const int nc = BS << (L - 1);  // coarse lattice size (a power of two: & wraps)
+                constexpr int NT = ORDER == 4 ? 4 : 2;  // taps of an odd coordinate
                 const int nz = synth_n(order, qz), ny = synth_n(order, qy), nx = synth_n(order, qx);
+                // the x taps once per ghost (entries k >= nx unused)
+                int sxk[NT], pxk[NT];
+                double wxk[NT];
+#pragma unroll
+                for (int k = 0; k < NT; ++k) {
+                    sxk[k] = synth_src(order, qx, k);
+                    pxk[k] = (sxk[k] - (4 * b.y - PL)) & (nc - 1);
+                    wxk[k] = synth_w(order, qx, k);
+                }
 #pragma unroll 1
                 for (int kz = 0; kz < nz; ++kz) {
                     const int sz = synth_src(order, qz, kz);
                     const int pz = (sz - (4 * b.w - PL)) & (nc - 1);
+                    const double wz = synth_w(order, qz, kz);
 #pragma unroll 1
                     for (int ky = 0; ky < ny; ++ky) {
                         const int sy = synth_src(order, qy, ky);
                         const int py = (sy - (4 * b.z - PL)) & (nc - 1);
-#pragma unroll 1
-                        for (int kx = 0; kx < nx; ++kx) {
-                            const int sx = synth_src(order, qx, kx);
-                            const int px = (sx - (4 * b.y - PL)) & (nc - 1);
-                            const double w = 1.0 * (synth_w(order, qz, kz) * synth_w(order, qy, ky) *
-                                                    synth_w(order, qx, kx));
-                            const int pq = px + PC * (py + PC * pz);
-                            if (px < PC && py < PC && pz < PC && s_cf[pq]) {
+                        const double wzy = wz * synth_w(order, qy, ky);
+                        const bool row_in = py < PC && pz < PC;
+                        const int row = PC * (py + PC * pz);
+#pragma unroll
+                        for (int kx = 0; kx < NT; ++kx) {
+                            if (kx >= nx) break;
+                            const double w = 1.0 * (wzy * wxk[kx]);
+                            const int pq = pxk[kx] + row;
+                            if (row_in && pxk[kx] < PC && s_cf[pq]) {
                                 const Real wr = (Real)w;
 #pragma unroll
                                 for (int k = 0; k < NF; ++k) {
@@ -547,7 +559,7 @@ __global__ __launch_bounds__(LAB_THREADS) void lab_kernel(int nlist, const int32
                                     v[k] = v[k] + prod;
                                 }
                             } else {
-                                ghost_visit<1>(m, order, L - 1, sx, sy, sz, w, visit, my_bad);
+                                ghost_visit<1>(m, order, L - 1, sxk[kx], sy, sz, w, visit, my_bad);
                             }
                         }
                     }
