@@ -264,6 +264,51 @@ __global__ __launch_bounds__(1024) void scan_chunk_apply(int n, int32_t* __restr
     if (g < n) cnt[g] = (int32_t)(offs[blockIdx.x] + e);  // the caller rejects totals above INT32_MAX
 }
 
+// Same-level neighbours of every slot from the level maps, as
+// Octree::same_level_neighbours: n27[27 s + k] = slot of the leaf at (L,
+// index + d_k) (periodic) or -1; nbr6[6 s + f] = n27 entry of face f's code
+// (FXM, FXP, FYM, FYP, FZM, FZP: 12, 14, 10, 16, 4, 22); level[s] = L.
+__global__ __launch_bounds__(256) void topology_kernel(int n, const int4* __restrict__ blk, LevelMaps m,
+                                                       int32_t* __restrict__ n27, int32_t* __restrict__ nbr6,
+                                                       int32_t* __restrict__ level) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;  // n = 27 x blocks
+    const int s = i / 27, k = i - 27 * s;
+    const int4 b = blk[s];
+    const int L = b.x, msk = (1 << L) - 1;
+    const int bx = (b.y + k % 3 - 1) & msk, by = (b.z + (k / 3) % 3 - 1) & msk, bz = (b.w + k / 9 - 1) & msk;
+    const int32_t v = L < m.nlev ? m.map[dense_level_offset(L) + bx + (by << L) + (bz << (2 * L))]
+                                 : hashed_node(m, L, bx, by, bz);
+    const int32_t r = v >= 0 ? v : -1;
+    n27[i] = r;
+    const int f = k == 12 ? 0 : k == 14 ? 1 : k == 10 ? 2 : k == 16 ? 3 : k == 4 ? 4 : k == 22 ? 5 : -1;
+    if (f >= 0) nbr6[6 * s + f] = r;
+    if (k == 13) level[s] = L;
+}
+
+// jump faces (no same-level neighbour): flag[i] = 1 for i < n, flag[n] = 0,
+// then scanned in place into list positions
+__global__ __launch_bounds__(256) void face_flag_kernel(int n, const int32_t* __restrict__ nbr6,
+                                                        int32_t* __restrict__ flag) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i <= n) flag[i] = i < n && nbr6[i] < 0 ? 1 : 0;
+}
+
+// the jump faces in slot-face order: faces[pos[i]] = i and pk_of[i] = pos[i]
+// for a jump face i = 6 * slot + f, pk_of[i] = -1 for the others
+__global__ __launch_bounds__(256) void face_list_kernel(int n, const int32_t* __restrict__ nbr6,
+                                                        const int32_t* __restrict__ pos,
+                                                        int32_t* __restrict__ faces, int32_t* __restrict__ pk_of) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    if (nbr6[i] < 0) {
+        faces[pos[i]] = i;
+        pk_of[i] = pos[i];
+    } else {
+        pk_of[i] = -1;
+    }
+}
+
 // Face-ghost table on the device, pass 1: term count of ghost g = 64 * k + p
 // of jump face k (faces[k] = 6 * slot + f), p = v * 8 + u.
 // pkt (may be null): per face, 64 x the largest count of its 64 ghosts, which
@@ -1076,42 +1121,51 @@ HostTable build_lab_table(const Octree& t, const std::vector<int>& slots, const 
 }
 
 // The face-ghost table of the RD step built on the device (same entries and
-// term order as build_face_table): a counting pass, a host prefix sum, a
-// filling pass.  *done = false when a ghost is deeper than the device
-// recursion (the caller then builds it on the host).
+// term order as build_face_table): the jump faces listed, a counting pass,
+// a prefix sum, a filling pass.  *done = false when a ghost is deeper than
+// the device recursion (the caller then builds it on the host).
 int device_face_table(Ctx& c, bool& done) {
     done = false;
     c.nFacePk = 0;
-    // host side of an asynchronous upload: a context member, so it outlives
-    // this call until apply_topology synchronizes
-    std::vector<int32_t>& faces = c.hFaces;
-    faces.clear();
-    const int nb = c.topo.nblocks();
-    std::vector<int32_t>& pk_of = c.hFacePkOf;
-    pk_of.assign(6 * (size_t)nb, -1);
-    for (int s = 0; s < nb; ++s)
-        for (int f = 0; f < 6; ++f)
-            if (c.topo.nbr[6 * (size_t)s + f] < 0) {
-                pk_of[6 * (size_t)s + f] = (int32_t)faces.size();
-                faces.push_back(6 * s + f);
-            }
-    const int nf = (int)faces.size();
-    const int n = nf * 64;
     c.nFaceEnt = 0;
+    const int nb = c.topo.nblocks();
+    const int n6 = 6 * nb;
+    int rc;
+    DevBuf& dfaces = c.faceList;  // kept across calls: no allocation per adaptation
+    if ((rc = c.faceScan.ensure_grow(((size_t)n6 + 1) * 4)) || (rc = dfaces.ensure_grow(std::max(1, n6) * 4)) ||
+        (rc = c.facePkOf.ensure_grow(std::max(1, n6) * 4)))
+        return rc;
+    // the jump faces in slot-face order, listed on the device from the
+    // neighbour table: flags, their exclusive prefix sum, the list
+    int32_t* pos = c.faceScan.as<int32_t>();
+    {
+        const int np6 = n6 + 1;
+        const int nch = (np6 + 1023) / 1024;
+        if ((rc = c.scanBuf.ensure_grow(((size_t)2 * nch + 1) * sizeof(long long)))) return rc;
+        long long* sums = c.scanBuf.as<long long>();
+        hipLaunchKernelGGL(face_flag_kernel, dim3((unsigned)((np6 + 255) / 256)), dim3(256), 0, c.stream, n6,
+                           c.nbr.as<int32_t>(), pos);
+        hipLaunchKernelGGL(scan_chunk_sums, dim3(nch), dim3(1024), 0, c.stream, np6, pos, sums);
+        hipLaunchKernelGGL(scan_chunk_offsets, dim3(1), dim3(1024), 0, c.stream, nch, sums, sums + nch,
+                           sums + 2 * nch);
+        hipLaunchKernelGGL(scan_chunk_apply, dim3(nch), dim3(1024), 0, c.stream, np6, pos, sums + nch);
+        if (n6 > 0)
+            hipLaunchKernelGGL(face_list_kernel, dim3((unsigned)((n6 + 255) / 256)), dim3(256), 0, c.stream, n6,
+                               c.nbr.as<int32_t>(), pos, dfaces.as<int32_t>(), c.facePkOf.as<int32_t>());
+        GH_HIP_CHECK(hipGetLastError());
+    }
+    int nf = 0;
+    GH_HIP_CHECK(hipMemcpyAsync(&nf, pos + n6, sizeof(int32_t), hipMemcpyDeviceToHost, c.stream));
+    GH_HIP_CHECK(hipStreamSynchronize(c.stream));
+    const int n = nf * 64;
     if (n == 0) {
         done = true;
         return GH_OK;
     }
     const LevelMaps m = level_maps(c);
     const int order = c.tree.order;
-    DevBuf& dfaces = c.faceList;  // kept across calls: no allocation per adaptation
     DevBuf& dcount = c.faceCount;
-    int rc;
-    if ((rc = dfaces.ensure_grow(faces.size() * 4)) || (rc = dcount.ensure_grow((size_t)n * 4)) ||
-        (rc = c.facePkBase.ensure_grow(((size_t)nf + 1) * 4)) || (rc = c.facePkOf.ensure_grow(pk_of.size() * 4)))
-        return rc;
-    GH_HIP_CHECK(hipMemcpyAsync(dfaces.p, faces.data(), faces.size() * 4, hipMemcpyHostToDevice, c.stream));
-    GH_HIP_CHECK(hipMemcpyAsync(c.facePkOf.p, pk_of.data(), pk_of.size() * 4, hipMemcpyHostToDevice, c.stream));
+    if ((rc = dcount.ensure_grow((size_t)n * 4)) || (rc = c.facePkBase.ensure_grow(((size_t)nf + 1) * 4))) return rc;
     GH_HIP_CHECK(hipMemsetAsync(c.dBad.p, 0, sizeof(int), c.stream));
     GH_HIP_CHECK(hipMemsetAsync(c.facePkBase.as<int32_t>() + nf, 0, sizeof(int32_t), c.stream));
     const unsigned grid = (unsigned)((n + 255) / 256);
@@ -1152,14 +1206,15 @@ int device_face_table(Ctx& c, bool& done) {
                            c.faceW.as<double>());
     GH_HIP_CHECK(hipGetLastError());
     c.nFacePk = nf;
-    done = true;  // the upload above reads c.hFaces until apply_topology synchronizes
+    done = true;
     return GH_OK;
 }
 
 }  // namespace
 
-// topology derived from the octree: slots, same-level neighbours, levels,
-// face-ghost table; uploads everything and sizes the per-slot ghost buffers
+// topology derived from the octree: slots, level maps, block coordinates
+// uploaded; same-level neighbours and levels built on the device from the
+// maps (topology_kernel); the per-slot ghost buffers sized
 int apply_topology(Ctx& c) {
     Octree& t = c.tree;
     ++c.tree_version;  // invalidates the cached lab tables
@@ -1167,35 +1222,12 @@ int apply_topology(Ctx& c) {
     Topology topo;
     topo.blocks = t.leaves;
     const int nb = (int)t.leaves.size();
-    topo.nbr.resize(6 * (size_t)nb);
-    std::vector<int32_t> n27(27 * (size_t)nb), lev(nb);
-    static const int face_code[6] = {12, 14, 10, 16, 4, 22};
-    auto fill = [&](int s0, int s1) {  // disjoint output ranges per slot
-        for (int s = s0; s < s1; ++s) {
-            int o[27];
-            t.same_level_neighbours(s, o);
-            for (int k = 0; k < 27; ++k) n27[27 * (size_t)s + k] = o[k];
-            for (int f = 0; f < 6; ++f) topo.nbr[6 * (size_t)s + f] = o[face_code[f]];
-            lev[s] = t.leaves[s].level;
-        }
-    };
-    // large grids: the 27 dense-map reads per block on a few host threads
-    // (13,910 blocks: 0.61 ms on one thread, 0.25 ms on four, 0.34 ms on eight)
-    const int nth = nb >= 8192 ? (int)std::min(4u, std::max(1u, std::thread::hardware_concurrency())) : 1;
-    if (nth <= 1) {
-        fill(0, nb);
-    } else {
-        std::vector<std::thread> th;
-        for (int k = 0; k < nth; ++k)
-            th.emplace_back(fill, (int)((int64_t)nb * k / nth), (int)((int64_t)nb * (k + 1) / nth));
-        for (auto& x : th) x.join();
-    }
     topo.uniform_bpd = t.is_uniform() ? (1 << t.levels_min()) : 0;
-    topo.nbr27 = n27;
     c.topo = std::move(topo);
     int rc;
-    if ((rc = c.upload_topology())) return rc;
-    if ((rc = c.dNbr27.ensure_grow(n27.size() * 4)) || (rc = c.dLevel.ensure_grow(lev.size() * 4))) return rc;
+    if ((rc = c.nbr.ensure_grow(std::max(1, 6 * nb) * 4)) || (rc = c.dNbr27.ensure_grow(std::max(1, 27 * nb) * 4)) ||
+        (rc = c.dLevel.ensure_grow(std::max(1, nb) * 4)))
+        return rc;
     // level maps and block coordinates for the device-side lab ghosts
     std::vector<int32_t> dense;
     c.nDense = t.dense_levels;
@@ -1225,8 +1257,12 @@ int apply_topology(Ctx& c) {
         GH_HIP_CHECK(hipMemcpyAsync(c.dDense.p, dense.data(), dense.size() * 4, hipMemcpyHostToDevice, c.stream));
     if (nb > 0)
         GH_HIP_CHECK(hipMemcpyAsync(c.dBlk.p, blk.data(), blk.size() * sizeof(int4), hipMemcpyHostToDevice, c.stream));
-    GH_HIP_CHECK(hipMemcpyAsync(c.dNbr27.p, n27.data(), n27.size() * 4, hipMemcpyHostToDevice, c.stream));
-    GH_HIP_CHECK(hipMemcpyAsync(c.dLevel.p, lev.data(), lev.size() * 4, hipMemcpyHostToDevice, c.stream));
+    if (nb > 0) {
+        hipLaunchKernelGGL(topology_kernel, dim3((unsigned)((27 * nb + 255) / 256)), dim3(256), 0, c.stream, 27 * nb,
+                           c.dBlk.as<int4>(), level_maps(c), c.dNbr27.as<int32_t>(), c.nbr.as<int32_t>(),
+                           c.dLevel.as<int32_t>());
+        GH_HIP_CHECK(hipGetLastError());
+    }
     c.faces_stale = true;  // built by ensure_faces when an RD step or the constants need them
     c.nFacePk = 0;
     c.nFaceEnt = 0;
@@ -1236,9 +1272,25 @@ int apply_topology(Ctx& c) {
     if (!rc) rc = c.fgDm.ensure_grow(std::max(1, nb) * 192 * rs);
     if (!rc) rc = c.fgPw.ensure_grow(std::max(1, nb) * 384 * rs);
     if (!rc) rc = c.fgPg.ensure_grow(std::max(1, nb) * 384 * rs);
-    // the uploads above read host vectors of this call and of the face-table build
+    // the uploads above read host vectors of this call
     GH_HIP_CHECK(hipStreamSynchronize(c.stream));
     return rc;
+}
+
+// the host copies of the neighbour tables (c.topo.nbr, c.topo.nbr27), for a
+// host-built ghost table; the device's own are built by apply_topology
+void host_neighbours(Ctx& c) {
+    const int nb = c.topo.nblocks();
+    if ((int)c.topo.nbr27.size() == 27 * nb && (int)c.topo.nbr.size() == 6 * nb) return;
+    static const int face_code[6] = {12, 14, 10, 16, 4, 22};
+    c.topo.nbr.assign(6 * (size_t)nb, -1);
+    c.topo.nbr27.assign(27 * (size_t)nb, -1);
+    for (int s = 0; s < nb; ++s) {
+        int o[27];
+        c.tree.same_level_neighbours(s, o);
+        for (int k = 0; k < 27; ++k) c.topo.nbr27[27 * (size_t)s + k] = o[k];
+        for (int f = 0; f < 6; ++f) c.topo.nbr[6 * (size_t)s + f] = o[face_code[f]];
+    }
 }
 
 namespace {
@@ -1253,6 +1305,7 @@ int ensure_faces(Ctx& c) {
     ++c.adapt_count[on_device ? 2 : 3];
     if (!on_device) {
         c.nFacePk = 0;
+        host_neighbours(c);
         HostTable ft = build_face_table(c.tree, c.topo.nbr);
         c.nFaceEnt = (int)ft.ent.size();
         rc = c.real_kind == GH_REAL_F32 ? upload_table<float>(c, ft, c.faceEnt, c.faceSrc, c.faceW)
@@ -1504,6 +1557,7 @@ int fwt_details_t(Ctx& c, const std::vector<int>& slots, std::vector<double>& de
     if (c.labAllVersion != c.tree_version) {
         all.resize(n);
         for (int i = 0; i < n; ++i) all[i] = i;
+        host_neighbours(c);
         HostTable tab = build_lab_table(c.tree, all, c.topo.nbr27, order);
         if ((rc = upload_table<Real>(c, tab, c.labAllEnt, c.labAllSrc, c.labAllW)) ||
             (rc = c.labAllSlots.ensure_grow(std::max<size_t>(1, all.size()) * 4)))
@@ -1553,6 +1607,7 @@ int apply_new_tree(Ctx& c, Octree& nt, const std::vector<Remap>& rm, const std::
         }
         ++c.adapt_count[on_device ? 0 : 1];
         if (!on_device) {
+            host_neighbours(c);
             HostTable tab = build_lab_table(c.tree, split_parents, c.topo.nbr27, order);
             if ((rc = upload_table<Real>(c, tab, c.labEnt, c.labSrc, c.labW))) return rc;
             for (int f = 0; f < 4; ++f)

@@ -61,15 +61,14 @@ struct Ctx {
     DevBuf dLevel, dNbr27;
     DevBuf faceEnt, faceSrc, faceW;  // GhostEntry / int32 / Real
     DevBuf faceList, faceCount;      // device face-table build: jump faces, per-ghost term counts
-    std::vector<int32_t> hFaces;     // the face list's host side
-    DevBuf scanBuf;                  // face-table prefix sum: chunk sums, offsets, total
+    DevBuf faceScan;                 // per slot face + 1: jump flag, then its exclusive prefix sum
+    DevBuf scanBuf;                  // face-table prefix sums: chunk sums, offsets, total
     int nFaceEnt = 0;
     // device-built face table in packets: per jump face, its 64 ghosts' term
     // lists interleaved (term k of ghost p at facePkBase[face] + 64 k + p for
     // k below its own term count in faceCount), so a wave's loads are contiguous
     DevBuf facePkBase;               // int32 per face + 1: packet starts (exclusive scan)
     DevBuf facePkOf;                 // int32 per slot face: its packet (index in the face list), -1 if none
-    std::vector<int32_t> hFacePkOf;  // its host side (uploaded asynchronously, like hFaces)
     int nFacePk = 0;                 // faces in packets (0: the GhostEntry table is used)
     DevBuf fgPhi, fgTau, fgDm;       // per slot: 6*64 ghost phi, 6*64 ghost tau, 3*64 minus-face D
     DevBuf fgPw, fgPg;               // per slot: 6*64 ghost p_w, p_g (statics)
@@ -137,7 +136,6 @@ struct Ctx {
 
     size_t ncells() const { return (size_t)topo.nblocks() * BCELLS; }
     int alloc_fields();
-    int upload_topology();
     void release_all();  // every device buffer of the context
 };
 

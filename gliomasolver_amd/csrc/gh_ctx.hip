@@ -71,16 +71,6 @@ void Ctx::release_all() {
     labPermOrder = 0;
 }
 
-int Ctx::upload_topology() {
-    int rc = nbr.ensure_grow(topo.nbr.size() * sizeof(int32_t));
-    if (rc) return rc;
-    // topo.nbr lives until the next topology change; the caller synchronizes
-    // before that (apply_topology's face-table build)
-    GH_HIP_CHECK(hipMemcpyAsync(nbr.p, topo.nbr.data(), topo.nbr.size() * sizeof(int32_t), hipMemcpyHostToDevice,
-                                stream));
-    return GH_OK;
-}
-
 // dense x-fastest n^3 <-> block-major [slot][z][y][x] for a uniform grid in
 // slot order bx + bpd*(by + bpd*bz)
 template <typename T>

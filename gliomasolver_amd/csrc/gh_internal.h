@@ -36,6 +36,8 @@ struct BlockRec {
 // Leaf-block topology in device slot order.
 struct Topology {
     std::vector<BlockRec> blocks;
+    // host copies of the device's neighbour tables, filled only when a
+    // host-built ghost table needs them (empty otherwise)
     std::vector<int32_t> nbr;    // 6 per block: same-level face neighbour slot, -1 across a level jump
     std::vector<int32_t> nbr27;  // 27 per block: same-level neighbour slot per direction code, -1 if none
     int uniform_bpd = 0;       // >0 when every block sits on one level (bpd^3 blocks)

@@ -153,6 +153,34 @@ def test_gpu_multilevel_rd_steps_bit_exact(oracle, real, order, grid):
     assert bits_equal(out, ref), "max |diff| %g" % np.abs(out.astype(float) - ref).max()
 
 
+def test_gpu_host_tables_on_a_three_level_jump(oracle):
+    """A tiling with a level-4 block touching level-1 blocks (no 2:1 balance):
+    some ghosts need three synthesis levels, more than the device recursion
+    takes, so the labs and the face ghosts come from host-built tables (with
+    the host copies of the neighbour tables); details and RD steps bit-exact
+    to the oracle."""
+    rows = canonical([[1, x, y, z] for z in range(2) for y in range(2) for x in range(2)])
+    for corner in ((1, 1, 1, 1), (2, 2, 2, 2), (3, 4, 4, 4)):  # refine the low corner three times
+        rows = apply_refine(rows, [k for k in range(len(rows)) if tuple(rows[k]) == corner])
+    fields = random_fields(rows, np.float32, 12)
+    say("three-level jump", len(rows), "blocks")
+    ctx = make_ctx("f32", rows, 4, fields, max_level=4)
+    det = ctx.fwt_details()
+    ref = np.array([oracle.amr_fwt(4, oracle.amr_lab(rows, 4, fields[FIELD_PHI], b)) for b in range(len(rows))])
+    assert np.array_equal(det, ref), "max |diff| %g" % np.abs(det - ref).max()
+    Dw, Dg, rho, dt, k = 1.5e-4, 1.5e-5, 0.08, 0.02, 3
+    ctx.rd_prepare(Dw, Dg, rho)
+    ctx.rd_steps(k, dt)
+    out = ctx.get_field_blocks(FIELD_PHI).reshape(len(rows), 512)
+    counts = ctx.adapt_counters()
+    ctx.close()
+    t = np.float32
+    want = oracle.amr_rd_steps(rows, 4, fields[FIELD_PHI], fields[FIELD_PW], fields[FIELD_PG], t(Dw), t(Dg), t(rho),
+                               dt, k)
+    assert bits_equal(out, want), "max |diff| %g" % np.abs(out.astype(float) - want).max()
+    assert counts["labs_host"] >= 1 and counts["faces_host"] >= 1, counts
+
+
 @pytest.mark.parametrize("real", ["f32", "f64"])
 @pytest.mark.parametrize("order", [4, 2])
 def test_gpu_multilevel_rd_steps_on_large_grid(oracle, real, order):
