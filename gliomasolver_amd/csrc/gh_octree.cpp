@@ -367,6 +367,37 @@ static void touching_deep(const Octree& t, int L, const int P[3], const int d[3]
 // always qualifies; inside a finer region only leaves of level >= L + 3 do.
 void Octree::chain_targets(int v, std::vector<int>& out) const {
     const int L = leaves[v].level;
+    if (L < dense_levels) {  // the same walk on the dense maps of levels L and L - 1
+        const int m = (1 << L) - 1;
+        const int32_t* map = dense[L].data();
+        const BlockRec& b = leaves[v];
+        for (int code = 0; code < 27; ++code) {
+            if (code == 13) continue;
+            const int d[3] = {code % 3 - 1, (code / 3) % 3 - 1, code / 9 - 1};
+            const int P[3] = {(b.index[0] + d[0]) & m, (b.index[1] + d[1]) & m, (b.index[2] + d[2]) & m};
+            const int32_t r = map[(size_t)P[0] + ((size_t)P[1] << L) + ((size_t)P[2] << (2 * L))];
+            if (r >= 0) continue;  // a same-level leaf
+            if (r == -2) {         // subdivided
+                if (L + 3 <= top_level) touching_deep(*this, L, P, d, L + 3, out);
+                continue;
+            }
+            const int32_t r1 = dense[L - 1][(size_t)(P[0] >> 1) + ((size_t)(P[1] >> 1) << (L - 1)) +
+                                            ((size_t)(P[2] >> 1) << (2 * (L - 1)))];
+            if (r1 >= 0) {
+                out.push_back(r1);
+                continue;
+            }
+            int slot;
+            for (int l = L - 2; l >= 0; --l) {
+                const int sh = L - l;
+                if (kind(l, P[0] >> sh, P[1] >> sh, P[2] >> sh, &slot) == LEAF) {
+                    out.push_back(slot);
+                    break;
+                }
+            }
+        }
+        return;
+    }
     for (int code = 0; code < 27; ++code) {
         if (code == 13) continue;
         const int d[3] = {code % 3 - 1, (code / 3) % 3 - 1, code / 9 - 1};
@@ -397,9 +428,11 @@ std::vector<int> Octree::refinement_victims(const std::vector<char>& flagged) co
     // subdivided region of the leaf's level next to it without the wrap
     // (has_finer_direct_neighbour), so the leaves to flag are the same-level
     // leaves next to each interior node, found from the interior nodes.
-    for (uint64_t k : interior) {
-        const int L = (int)(k >> 48), n = 1 << L;
-        const int I[3] = {(int)((k >> 32) & 0xffff), (int)((k >> 16) & 0xffff), (int)(k & 0xffff)};
+    // Levels whose dense map is small are scanned for their subdivided nodes,
+    // the others visited from the interior set.
+    constexpr int kScanMax = 5;  // 8^5 = 32,768 map entries
+    auto extend = [&](int L, const int I[3]) {
+        const int n = 1 << L;
         for (int code = 0; code < 27; ++code) {
             if (code == 13) continue;
             const int d[3] = {code % 3 - 1, (code / 3) % 3 - 1, code / 9 - 1};
@@ -408,6 +441,28 @@ std::vector<int> Octree::refinement_victims(const std::vector<char>& flagged) co
             int slot;
             if (kind(L, P[0], P[1], P[2], &slot) == LEAF) flag[slot] = 1;
         }
+    };
+    const int nscan = std::min(dense_levels, kScanMax + 1);
+    for (int L = 0; L < nscan; ++L) {
+        const int n = 1 << L;
+        const int32_t* map = dense[L].data();
+        for (int z = 0; z < n; ++z)
+            for (int y = 0; y < n; ++y)
+                for (int x = 0; x < n; ++x) {
+                    if (map[(size_t)x + ((size_t)y << L) + ((size_t)z << (2 * L))] != -2) continue;
+                    for (int zz = std::max(z - 1, 0); zz <= std::min(z + 1, n - 1); ++zz)
+                        for (int yy = std::max(y - 1, 0); yy <= std::min(y + 1, n - 1); ++yy)
+                            for (int xx = std::max(x - 1, 0); xx <= std::min(x + 1, n - 1); ++xx) {
+                                const int32_t v = map[(size_t)xx + ((size_t)yy << L) + ((size_t)zz << (2 * L))];
+                                if (v >= 0) flag[v] = 1;  // the node itself is -2, never a leaf
+                            }
+                }
+    }
+    for (uint64_t k : interior) {
+        const int L = (int)(k >> 48);
+        if (L < nscan) continue;
+        const int I[3] = {(int)((k >> 32) & 0xffff), (int)((k >> 16) & 0xffff), (int)(k & 0xffff)};
+        extend(L, I);
     }
     // nothing at the maximum level is refined (:54-63)
     for (int s = 0; s < nl; ++s)

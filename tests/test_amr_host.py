@@ -55,6 +55,29 @@ def test_compression_plan_matches_restatement(seed, min_level):
     assert got == collapse_parents(rows, flags, min_level)
 
 
+def test_plans_on_a_grid_deeper_than_the_scanned_maps():
+    """Levels 2-8 around one point (2:1-balanced): the plans read the small
+    levels' maps directly and the deep levels (dense 6-7, hashed 8) through
+    the general lookup; both plans equal the restatement."""
+    rows = canonical([[1, x, y, z] for z in range(2) for y in range(2) for x in range(2)])
+    p = (0.3, 0.45, 0.6)
+    for L in range(1, 8):
+        rows = apply_refine(rows, [k for k in range(len(rows)) if rows[k, 0] == L and
+                                   all(int(p[a] * (1 << L)) == rows[k, 1 + a] for a in range(3))])
+        while True:
+            nb = neighbours_by_ring(rows)
+            bad = sorted({j for s in range(len(rows)) for j, _ in nb[s] if rows[s, 0] - rows[j, 0] > 1})
+            if not bad:
+                break
+            rows = apply_refine(rows, bad)
+    assert rows[:, 0].max() == 8
+    rng = np.random.default_rng(3)
+    flags = (rng.uniform(0, 1, len(rows)) < 0.2).astype(np.uint8)
+    assert gh.plan_refine(rows, flags, 9) == refine_victims(rows, flags, 9)
+    flags = (rng.uniform(0, 1, len(rows)) < 0.8).astype(np.uint8)
+    assert gh.plan_compress(rows, flags, 0) == collapse_parents(rows, flags, 0)
+
+
 def test_space_extension_refines_coarse_neighbours_of_fine_blocks():
     """With no flags at all, a leaf next to a finer leaf is still refined
     (MRAGRefiner_SpaceExtension.h:35-51) -- except across the periodic wrap."""
