@@ -28,7 +28,6 @@
 #include <exception>
 #include <memory>
 #include <thread>
-#include <unordered_map>
 #include <vector>
 
 #include "gh_ctx.h"
@@ -1673,6 +1672,7 @@ int grid_refine(Ctx& c, const std::vector<char>& flagged, int* nrefined, std::ve
     if (nrefined) *nrefined = (int)victims.size();
     if (victims.empty()) return GH_OK;
     Octree nt = t.shell();
+    nt.leaves.reserve(t.leaves.size() + 7 * victims.size());
     std::vector<char> is_victim(t.leaves.size(), 0);
     for (int v : victims) is_victim[v] = 1;
     std::vector<int> victim_index(t.leaves.size(), -1);  // old slot -> compact parent index
@@ -1733,9 +1733,10 @@ int grid_compress(Ctx& c, const std::vector<char>& flagged, int* ncollapsed, std
     if (parents_out) *parents_out = parents;
     if (parents.empty()) return GH_OK;
     Octree nt = t.shell();
-    std::unordered_map<uint64_t, std::array<int, 8>> kids;
+    std::vector<std::array<int, 8>> kids(parents.size());  // per parent, in the (ascending) order of the keys
     std::vector<char> gone(t.leaves.size(), 0);
-    for (uint64_t pk : parents) {
+    for (size_t i = 0; i < parents.size(); ++i) {
+        const uint64_t pk = parents[i];
         const int L = (int)(pk >> 48);
         const int px = (int)((pk >> 32) & 0xffff), py = (int)((pk >> 16) & 0xffff), pz = (int)(pk & 0xffff);
         std::array<int, 8> k{};
@@ -1745,9 +1746,10 @@ int grid_compress(Ctx& c, const std::vector<char>& flagged, int* ncollapsed, std
             k[o] = slot;
             gone[slot] = 1;
         }
-        kids[pk] = k;
+        kids[i] = k;
         nt.interior.erase(pk);
     }
+    nt.leaves.reserve(t.leaves.size() - 7 * parents.size());
     for (int s = 0; s < (int)t.leaves.size(); ++s)
         if (!gone[s]) nt.leaves.push_back(t.leaves[s]);
     for (uint64_t pk : parents)
@@ -1765,7 +1767,9 @@ int grid_compress(Ctx& c, const std::vector<char>& flagged, int* ncollapsed, std
             r.src = old_slot;
         } else {
             r.kind = 2;
-            const auto& kk = kids.at(k);
+            const auto it = std::lower_bound(parents.begin(), parents.end(), k);
+            if (it == parents.end() || *it != k) return set_error(GH_EINVAL, "grid_compress: inconsistent plan");
+            const auto& kk = kids[it - parents.begin()];
             for (int o = 0; o < 8; ++o) r.kids[o] = kk[o];
         }
         rm[s] = r;

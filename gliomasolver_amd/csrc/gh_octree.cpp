@@ -311,8 +311,58 @@ bool Octree::has_finer_direct_neighbour(int s) const {
     return false;
 }
 
+// For each of the 27 direction codes: the children (octant bits x | y << 1 |
+// z << 2) of the neighbour region in that direction that touch the node's
+// box -- on the max side of an axis where the region lies below the node, on
+// the min side where it lies above.  Faces have 4, edges 2, corners 1.
+struct FacingChildren {
+    int n[27];
+    int oct[27][8];
+    FacingChildren() {
+        for (int code = 0; code < 27; ++code) {
+            const int d[3] = {code % 3 - 1, (code / 3) % 3 - 1, code / 9 - 1};
+            n[code] = 0;
+            for (int o = 0; o < 8; ++o) {
+                const int off[3] = {o & 1, (o >> 1) & 1, (o >> 2) & 1};
+                bool faces = true;
+                for (int a = 0; a < 3; ++a) {
+                    if (d[a] < 0 && off[a] != 1) faces = false;
+                    if (d[a] > 0 && off[a] != 0) faces = false;
+                }
+                if (faces) oct[code][n[code]++] = o;
+            }
+        }
+    }
+};
+static const FacingChildren kFacing;
+
 bool Octree::outside_within_one(int L, const int P[3]) const {
     const int n = 1 << L;
+    if (L + 1 < dense_levels) {  // the same tests on the dense maps of levels L - 1, L and L + 1
+        const int m = n - 1;
+        const int32_t* map = dense[L].data();
+        const int32_t* fine = dense[L + 1].data();
+        for (int code = 0; code < 27; ++code) {
+            if (code == 13) continue;
+            const int d[3] = {code % 3 - 1, (code / 3) % 3 - 1, code / 9 - 1};
+            const int Q[3] = {(P[0] + d[0]) & m, (P[1] + d[1]) & m, (P[2] + d[2]) & m};
+            const int32_t v = map[(size_t)Q[0] + ((size_t)Q[1] << L) + ((size_t)Q[2] << (2 * L))];
+            if (v >= 0) continue;  // a leaf
+            if (v == -1) {         // inside a coarser leaf: it must be one level up
+                if (L == 0 || dense[L - 1][(size_t)(Q[0] >> 1) + ((size_t)(Q[1] >> 1) << (L - 1)) +
+                                           ((size_t)(Q[2] >> 1) << (2 * (L - 1)))] < 0)
+                    return false;
+                continue;
+            }
+            for (int i = 0; i < kFacing.n[code]; ++i) {  // subdivided: the children facing the node must be leaves
+                const int o = kFacing.oct[code][i];
+                if (fine[(size_t)(2 * Q[0] + (o & 1)) + ((size_t)(2 * Q[1] + ((o >> 1) & 1)) << (L + 1)) +
+                         ((size_t)(2 * Q[2] + (o >> 2)) << (2 * (L + 1)))] < 0)
+                    return false;
+            }
+        }
+        return true;
+    }
     for (int code = 0; code < 27; ++code) {
         if (code == 13) continue;
         const int d[3] = {code % 3 - 1, (code / 3) % 3 - 1, code / 9 - 1};
@@ -511,10 +561,21 @@ std::vector<uint64_t> Octree::collapse_parents(const std::vector<char>& flagged)
         if (L < min_level) continue;
         const int px = (int)((pk >> 32) & 0xffff), py = (int)((pk >> 16) & 0xffff), pz = (int)(pk & 0xffff);
         bool ok = true;
-        for (int o = 0; o < 8 && ok; ++o) {
-            int slot;
-            if (kind(L + 1, 2 * px + (o & 1), 2 * py + ((o >> 1) & 1), 2 * pz + ((o >> 2) & 1), &slot) != LEAF) ok = false;
-            else if (!(slot < (int)flagged.size() && flagged[slot])) ok = false;
+        if (L + 1 < dense_levels) {  // the children straight from their level's map
+            const int32_t* map = dense[L + 1].data();
+            for (int o = 0; o < 8 && ok; ++o) {
+                const int32_t slot = map[(size_t)(2 * px + (o & 1)) + ((size_t)(2 * py + ((o >> 1) & 1)) << (L + 1)) +
+                                         ((size_t)(2 * pz + ((o >> 2) & 1)) << (2 * (L + 1)))];
+                ok = slot >= 0 && slot < (int)flagged.size() && flagged[slot];
+            }
+        } else {
+            for (int o = 0; o < 8 && ok; ++o) {
+                int slot;
+                if (kind(L + 1, 2 * px + (o & 1), 2 * py + ((o >> 1) & 1), 2 * pz + ((o >> 2) & 1), &slot) != LEAF)
+                    ok = false;
+                else if (!(slot < (int)flagged.size() && flagged[slot]))
+                    ok = false;
+            }
         }
         // every neighbour of every child within one level of the parent (MRAGCompressor.h:95-107)
         const int P[3] = {px, py, pz};
