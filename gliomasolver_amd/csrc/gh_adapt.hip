@@ -25,6 +25,7 @@
 #include <array>
 #include <chrono>
 #include <cmath>
+#include <cstring>
 #include <exception>
 #include <memory>
 #include <thread>
@@ -1227,14 +1228,19 @@ int apply_topology(Ctx& c) {
     if ((rc = c.nbr.ensure_grow(std::max(1, 6 * nb) * 4)) || (rc = c.dNbr27.ensure_grow(std::max(1, 27 * nb) * 4)) ||
         (rc = c.dLevel.ensure_grow(std::max(1, nb) * 4)))
         return rc;
-    // level maps and block coordinates for the device-side lab ghosts
-    std::vector<int32_t> dense;
+    // level maps and block coordinates for the device-side lab ghosts, staged
+    // in pinned memory (read by the uploads until the synchronization below)
     c.nDense = t.dense_levels;
     c.topLevel = t.top_level;
     c.hashBits = t.hash_bits;
+    const size_t ndense = c.nDense > 0 ? (size_t)dense_level_offset(c.nDense) : 0;
+    if ((rc = c.pinDense.ensure_grow(std::max<size_t>(1, ndense) * 4)) ||
+        (rc = c.pinBlk.ensure_grow(std::max(1, nb) * sizeof(int4))))
+        return rc;
+    int32_t* dense = c.pinDense.as<int32_t>();
     for (int l = 0; l < c.nDense; ++l) {
-        if ((int)dense.size() != dense_level_offset(l)) return set_error(GH_EINVAL, "level maps: layout");
-        dense.insert(dense.end(), t.dense[l].begin(), t.dense[l].end());
+        if (t.dense[l].size() != ((size_t)1 << (3 * l))) return set_error(GH_EINVAL, "level maps: layout");
+        std::memcpy(dense + dense_level_offset(l), t.dense[l].data(), t.dense[l].size() * 4);
     }
     if (c.hashBits > 0) {  // the tree outlives the asynchronous copies (synchronized below)
         int rc2;
@@ -1246,16 +1252,15 @@ int apply_topology(Ctx& c) {
         GH_HIP_CHECK(hipMemcpyAsync(c.dHashVal.p, t.hash_vals.data(), t.hash_vals.size() * 4, hipMemcpyHostToDevice,
                                     c.stream));
     }
-    std::vector<int4> blk(nb);
+    int4* blk = c.pinBlk.as<int4>();
     for (int s = 0; s < nb; ++s)
         blk[s] = make_int4(t.leaves[s].level, t.leaves[s].index[0], t.leaves[s].index[1], t.leaves[s].index[2]);
-    if ((rc = c.dDense.ensure_grow(std::max<size_t>(1, dense.size()) * 4)) ||
-        (rc = c.dBlk.ensure_grow(std::max<size_t>(1, blk.size()) * sizeof(int4))) || (rc = c.dBad.ensure_grow(sizeof(int))))
+    if ((rc = c.dDense.ensure_grow(std::max<size_t>(1, ndense) * 4)) ||
+        (rc = c.dBlk.ensure_grow(std::max(1, nb) * sizeof(int4))) || (rc = c.dBad.ensure_grow(sizeof(int))))
         return rc;
-    if (!dense.empty())
-        GH_HIP_CHECK(hipMemcpyAsync(c.dDense.p, dense.data(), dense.size() * 4, hipMemcpyHostToDevice, c.stream));
-    if (nb > 0)
-        GH_HIP_CHECK(hipMemcpyAsync(c.dBlk.p, blk.data(), blk.size() * sizeof(int4), hipMemcpyHostToDevice, c.stream));
+    if (ndense > 0)
+        GH_HIP_CHECK(hipMemcpyAsync(c.dDense.p, dense, ndense * 4, hipMemcpyHostToDevice, c.stream));
+    if (nb > 0) GH_HIP_CHECK(hipMemcpyAsync(c.dBlk.p, blk, (size_t)nb * sizeof(int4), hipMemcpyHostToDevice, c.stream));
     if (nb > 0) {
         hipLaunchKernelGGL(topology_kernel, dim3((unsigned)((27 * nb + 255) / 256)), dim3(256), 0, c.stream, 27 * nb,
                            c.dBlk.as<int4>(), level_maps(c), c.dNbr27.as<int32_t>(), c.nbr.as<int32_t>(),
@@ -1535,16 +1540,24 @@ int fwt_details_t(Ctx& c, const std::vector<int>& slots, std::vector<double>& de
         // labs and details of the listed slots only; the recursion check is
         // read back together with the details (one synchronization)
         const int m = (int)slots.size();
-        if ((rc = c.labSlots.ensure_grow((size_t)m * 4)) || (rc = clear_bad(c))) return rc;
-        GH_HIP_CHECK(hipMemcpyAsync(c.labSlots.p, slots.data(), (size_t)m * 4, hipMemcpyHostToDevice, c.stream));
+        if ((rc = c.labSlots.ensure_grow((size_t)m * 4)) || (rc = clear_bad(c)) ||
+            (rc = c.pinSlots.ensure_grow((size_t)m * 4)) || (rc = c.pinDet.ensure_grow((size_t)m * 8 + 8)))
+            return rc;
+        std::memcpy(c.pinSlots.p, slots.data(), (size_t)m * 4);
+        GH_HIP_CHECK(hipMemcpyAsync(c.labSlots.p, c.pinSlots.p, (size_t)m * 4, hipMemcpyHostToDevice, c.stream));
         if ((rc = device_details<Real>(c, m, c.labSlots.as<int32_t>(), c.phi[c.cur].as<Real>(), d.as<double>())))
             return rc;
-        int bad = 1;
+        // details, then the recursion flag after them, into the pinned buffer
+        double* pd = c.pinDet.as<double>();
+        int* pbad = reinterpret_cast<int*>(pd + m);
+        *pbad = 1;
         hipError_t e = hipGetLastError();
-        if (e == hipSuccess) e = hipMemcpyAsync(det.data(), d.p, (size_t)m * 8, hipMemcpyDeviceToHost, c.stream);
-        if (e == hipSuccess) e = hipMemcpyAsync(&bad, c.dBad.p, sizeof(int), hipMemcpyDeviceToHost, c.stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(pd, d.p, (size_t)m * 8, hipMemcpyDeviceToHost, c.stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(pbad, c.dBad.p, sizeof(int), hipMemcpyDeviceToHost, c.stream);
         if (e == hipSuccess) e = hipStreamSynchronize(c.stream);
         if (e != hipSuccess) return set_error(GH_EHIP, std::string("fwt: ") + hipGetErrorString(e));
+        std::memcpy(det.data(), pd, (size_t)m * 8);
+        const int bad = *pbad;
         if (bad == 0) {
             ++c.adapt_count[0];
             return GH_OK;
@@ -1582,7 +1595,9 @@ int fwt_details_t(Ctx& c, const std::vector<int>& slots, std::vector<double>& de
 // old parent's lab) or a collapse of 8 old leaves; field arrays are rebuilt
 // on the device and the per-solve constants recomputed
 template <typename Real>
-int apply_new_tree(Ctx& c, Octree& nt, const std::vector<Remap>& rm, const std::vector<int>& split_parents) {
+// rm: one Remap per new leaf, in c.pinRemap (pinned; read by the upload
+// until apply_topology synchronizes)
+int apply_new_tree(Ctx& c, Octree& nt, const Remap* rm, const std::vector<int>& split_parents) {
     const int order = c.tree.order;
     const int LS = lab_start(order), LAB = lab_size(order);
     int rc;
@@ -1592,9 +1607,12 @@ int apply_new_tree(Ctx& c, Octree& nt, const std::vector<Remap>& rm, const std::
     if (!split_parents.empty()) {
         bool on_device = false;
         if (c.nDense > 0) {
-            if ((rc = c.labSlots.ensure_grow(split_parents.size() * 4)) || (rc = clear_bad(c))) return rc;
-            GH_HIP_CHECK(hipMemcpyAsync(c.labSlots.p, split_parents.data(), split_parents.size() * 4,
-                                        hipMemcpyHostToDevice, c.stream));
+            if ((rc = c.labSlots.ensure_grow(split_parents.size() * 4)) || (rc = clear_bad(c)) ||
+                (rc = c.pinSlots.ensure_grow(split_parents.size() * 4)))
+                return rc;
+            std::memcpy(c.pinSlots.p, split_parents.data(), split_parents.size() * 4);
+            GH_HIP_CHECK(hipMemcpyAsync(c.labSlots.p, c.pinSlots.p, split_parents.size() * 4, hipMemcpyHostToDevice,
+                                        c.stream));
             const Real* fld[4];
             DevBuf* out[4];
             for (int f = 0; f < 4; ++f) {
@@ -1615,8 +1633,8 @@ int apply_new_tree(Ctx& c, Octree& nt, const std::vector<Remap>& rm, const std::
     }
     const int nnew = (int)nt.leaves.size();
     DevBuf& drm = c.remapBuf;
-    if ((rc = drm.ensure_grow(rm.size() * sizeof(Remap)))) return rc;
-    GH_HIP_CHECK(hipMemcpyAsync(drm.p, rm.data(), rm.size() * sizeof(Remap), hipMemcpyHostToDevice, c.stream));
+    if ((rc = drm.ensure_grow((size_t)nnew * sizeof(Remap)))) return rc;
+    GH_HIP_CHECK(hipMemcpyAsync(drm.p, rm, (size_t)nnew * sizeof(Remap), hipMemcpyHostToDevice, c.stream));
     DevBuf* nf = c.spare;  // the new arrays go to the spares, which then swap with the live ones
     RemapFields<Real> rf{};
     for (int f = 0; f < 4; ++f) {
@@ -1694,14 +1712,17 @@ int grid_refine(Ctx& c, const std::vector<char>& flagged, int* nrefined, std::ve
         }
     }
     nt.rebuild_index();
-    std::vector<Remap> rm(nt.leaves.size());
-    for (int s = 0; s < (int)nt.leaves.size(); ++s) {
+    const size_t nrm = nt.leaves.size();
+    int rc;
+    if ((rc = c.pinRemap.ensure_grow(nrm * sizeof(Remap)))) return rc;
+    Remap* rm = c.pinRemap.as<Remap>();
+    for (int s = 0; s < (int)nrm; ++s) {
         const BlockRec& nb = nt.leaves[s];
         Remap r{};
-        int old_slot = -1;
-        if (t.kind(nb.level, nb.index[0], nb.index[1], nb.index[2], &old_slot) == Octree::LEAF) {
+        int os = -1;
+        if (t.kind(nb.level, nb.index[0], nb.index[1], nb.index[2], &os) == Octree::LEAF) {
             r.kind = 0;
-            r.src = old_slot;
+            r.src = os;
         } else {  // a child of a split leaf: the parent is the old leaf one level up
             const BlockRec& ch = nt.leaves[s];
             int ps = -1;
@@ -1715,8 +1736,8 @@ int grid_refine(Ctx& c, const std::vector<char>& flagged, int* nrefined, std::ve
         rm[s] = r;
     }
     if (old_slot) {
-        old_slot->resize(rm.size());
-        for (size_t s = 0; s < rm.size(); ++s) (*old_slot)[s] = rm[s].kind == 0 ? rm[s].src : -1;
+        old_slot->resize(nrm);
+        for (size_t s = 0; s < nrm; ++s) (*old_slot)[s] = rm[s].kind == 0 ? rm[s].src : -1;
     }
     c.prof[P_PLAN] += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_plan).count();
     return c.real_kind == GH_REAL_F32 ? apply_new_tree<float>(c, nt, rm, victims)
@@ -1756,15 +1777,18 @@ int grid_compress(Ctx& c, const std::vector<char>& flagged, int* ncollapsed, std
         nt.leaves.push_back(BlockRec{(int)(pk >> 48), {(int)((pk >> 32) & 0xffff), (int)((pk >> 16) & 0xffff),
                                                        (int)(pk & 0xffff)}});
     nt.rebuild_index();
-    std::vector<Remap> rm(nt.leaves.size());
-    for (int s = 0; s < (int)nt.leaves.size(); ++s) {
+    const size_t nrm = nt.leaves.size();
+    int rc;
+    if ((rc = c.pinRemap.ensure_grow(nrm * sizeof(Remap)))) return rc;
+    Remap* rm = c.pinRemap.as<Remap>();
+    for (int s = 0; s < (int)nrm; ++s) {
         const BlockRec& nb = nt.leaves[s];
         const uint64_t k = Octree::key(nb);
         Remap r{};
-        int old_slot = -1;
-        if (t.kind(nb.level, nb.index[0], nb.index[1], nb.index[2], &old_slot) == Octree::LEAF) {
+        int os = -1;
+        if (t.kind(nb.level, nb.index[0], nb.index[1], nb.index[2], &os) == Octree::LEAF) {
             r.kind = 0;
-            r.src = old_slot;
+            r.src = os;
         } else {
             r.kind = 2;
             const auto it = std::lower_bound(parents.begin(), parents.end(), k);
@@ -1775,8 +1799,8 @@ int grid_compress(Ctx& c, const std::vector<char>& flagged, int* ncollapsed, std
         rm[s] = r;
     }
     if (old_slot) {
-        old_slot->resize(rm.size());
-        for (size_t s = 0; s < rm.size(); ++s) (*old_slot)[s] = rm[s].kind == 0 ? rm[s].src : -1;
+        old_slot->resize(nrm);
+        for (size_t s = 0; s < nrm; ++s) (*old_slot)[s] = rm[s].kind == 0 ? rm[s].src : -1;
     }
     c.prof[P_PLAN] += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_plan).count();
     return c.real_kind == GH_REAL_F32 ? apply_new_tree<float>(c, nt, rm, {})

@@ -1,5 +1,6 @@
 // gh_ctx.hip -- context, grid topology, field transfer and schedule entry
 // points of the C ABI (include/gliomahip.h).
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <new>
@@ -39,6 +40,20 @@ void DevBuf::release() {
     bytes = 0;
 }
 
+int PinnedBuf::ensure_grow(size_t nbytes) {
+    if (nbytes <= bytes && p) return GH_OK;
+    release();
+    const size_t n = nbytes + nbytes / 4;
+    GH_HIP_CHECK(hipHostMalloc(&p, std::max<size_t>(n, 256), hipHostMallocDefault));
+    bytes = std::max<size_t>(n, 256);
+    return GH_OK;
+}
+void PinnedBuf::release() {
+    if (p) (void)hipHostFree(p);
+    p = nullptr;
+    bytes = 0;
+}
+
 Ctx* as_ctx(gh_ctx* c) { return reinterpret_cast<Ctx*>(c); }
 
 int Ctx::alloc_fields() {
@@ -68,6 +83,10 @@ void Ctx::release_all() {
                       &obsPet,   &obsRoi,  &likPart,    &adjCk,     &adjSeg,    &adjLam[0],   &adjLam[1], &adjPart};
     for (DevBuf* b : bufs) b->release();
     for (DevBuf& b : ckpt) b.release();
+    faceScan.release();
+    dHashKey.release();
+    dHashVal.release();
+    for (PinnedBuf* b : {&pinRemap, &pinDense, &pinBlk, &pinSlots, &pinDet}) b->release();
     labPermOrder = 0;
 }
 
