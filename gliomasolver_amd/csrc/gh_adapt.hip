@@ -1212,6 +1212,18 @@ int device_face_table(Ctx& c, bool& done) {
 
 }  // namespace
 
+// the staging buffers and the tree's hash table may be rewritten once the
+// uploads recorded by stage_mark have completed
+int stage_wait(Ctx& c) {
+    if (c.evStage) GH_HIP_CHECK(hipEventSynchronize(c.evStage));
+    return GH_OK;
+}
+int stage_mark(Ctx& c) {
+    if (!c.evStage) GH_HIP_CHECK(hipEventCreateWithFlags(&c.evStage, hipEventDisableTiming));
+    GH_HIP_CHECK(hipEventRecord(c.evStage, c.stream));
+    return GH_OK;
+}
+
 // topology derived from the octree: slots, level maps, block coordinates
 // uploaded; same-level neighbours and levels built on the device from the
 // maps (topology_kernel); the per-slot ghost buffers sized
@@ -1229,7 +1241,8 @@ int apply_topology(Ctx& c) {
         (rc = c.dLevel.ensure_grow(std::max(1, nb) * 4)))
         return rc;
     // level maps and block coordinates for the device-side lab ghosts, staged
-    // in pinned memory (read by the uploads until the synchronization below)
+    // in pinned memory (read by the uploads until stage_mark's event passes)
+    if ((rc = stage_wait(c))) return rc;
     c.nDense = t.dense_levels;
     c.topLevel = t.top_level;
     c.hashBits = t.hash_bits;
@@ -1242,7 +1255,7 @@ int apply_topology(Ctx& c) {
         if (t.dense[l].size() != ((size_t)1 << (3 * l))) return set_error(GH_EINVAL, "level maps: layout");
         std::memcpy(dense + dense_level_offset(l), t.dense[l].data(), t.dense[l].size() * 4);
     }
-    if (c.hashBits > 0) {  // the tree outlives the asynchronous copies (synchronized below)
+    if (c.hashBits > 0) {  // the tree outlives the asynchronous copies (apply_new_tree waits for them)
         int rc2;
         if ((rc2 = c.dHashKey.ensure_grow(t.hash_keys.size() * 8)) ||
             (rc2 = c.dHashVal.ensure_grow(t.hash_vals.size() * 4)))
@@ -1276,9 +1289,10 @@ int apply_topology(Ctx& c) {
     if (!rc) rc = c.fgDm.ensure_grow(std::max(1, nb) * 192 * rs);
     if (!rc) rc = c.fgPw.ensure_grow(std::max(1, nb) * 384 * rs);
     if (!rc) rc = c.fgPg.ensure_grow(std::max(1, nb) * 384 * rs);
-    // the uploads above read host vectors of this call
-    GH_HIP_CHECK(hipStreamSynchronize(c.stream));
-    return rc;
+    if (rc) return rc;
+    // no synchronization: the next reader of the device tables is on the same
+    // stream, and the staging buffers are rewritten only after stage_wait
+    return stage_mark(c);
 }
 
 // the host copies of the neighbour tables (c.topo.nbr, c.topo.nbr27), for a
@@ -1540,7 +1554,7 @@ int fwt_details_t(Ctx& c, const std::vector<int>& slots, std::vector<double>& de
         // labs and details of the listed slots only; the recursion check is
         // read back together with the details (one synchronization)
         const int m = (int)slots.size();
-        if ((rc = c.labSlots.ensure_grow((size_t)m * 4)) || (rc = clear_bad(c)) ||
+        if ((rc = stage_wait(c)) || (rc = c.labSlots.ensure_grow((size_t)m * 4)) || (rc = clear_bad(c)) ||
             (rc = c.pinSlots.ensure_grow((size_t)m * 4)) || (rc = c.pinDet.ensure_grow((size_t)m * 8 + 8)))
             return rc;
         std::memcpy(c.pinSlots.p, slots.data(), (size_t)m * 4);
@@ -1646,13 +1660,14 @@ int apply_new_tree(Ctx& c, Octree& nt, const Remap* rm, const std::vector<int>& 
     hipLaunchKernelGGL(remap_kernel<Real>, dim3(nnew), dim3(512), 0, c.stream, nnew, drm.as<Remap>(), rf, LS, LAB,
                        order);
     GH_HIP_CHECK(hipGetLastError());
-    // rm (and the split list) stay alive in the caller until after
-    // apply_topology, whose uploads synchronize the stream
+    // rm (in c.pinRemap) and the split list (copied to c.pinSlots) are read by
+    // the uploads until the event apply_topology records has passed
     for (int f = 0; f < 4; ++f) std::swap(*fields[f], nf[f]);
     c.prof[P_MOVE] += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_move).count();
     {
         ProfScope topo(c.prof[P_TOPO]);
-        c.tree = std::move(nt);  // nt is the caller's scratch tree
+        if ((rc = stage_wait(c))) return rc;  // the old tree's hash table may still be uploading
+        c.tree = std::move(nt);                 // nt is the caller's scratch tree
         if ((rc = apply_topology(c))) return rc;
     }
     // the other state buffers follow the new size
@@ -1714,7 +1729,7 @@ int grid_refine(Ctx& c, const std::vector<char>& flagged, int* nrefined, std::ve
     nt.rebuild_index();
     const size_t nrm = nt.leaves.size();
     int rc;
-    if ((rc = c.pinRemap.ensure_grow(nrm * sizeof(Remap)))) return rc;
+    if ((rc = stage_wait(c)) || (rc = c.pinRemap.ensure_grow(nrm * sizeof(Remap)))) return rc;
     Remap* rm = c.pinRemap.as<Remap>();
     for (int s = 0; s < (int)nrm; ++s) {
         const BlockRec& nb = nt.leaves[s];
@@ -1779,7 +1794,7 @@ int grid_compress(Ctx& c, const std::vector<char>& flagged, int* ncollapsed, std
     nt.rebuild_index();
     const size_t nrm = nt.leaves.size();
     int rc;
-    if ((rc = c.pinRemap.ensure_grow(nrm * sizeof(Remap)))) return rc;
+    if ((rc = stage_wait(c)) || (rc = c.pinRemap.ensure_grow(nrm * sizeof(Remap)))) return rc;
     Remap* rm = c.pinRemap.as<Remap>();
     for (int s = 0; s < (int)nrm; ++s) {
         const BlockRec& nb = nt.leaves[s];
@@ -1854,6 +1869,7 @@ int gh_grid_init_leaves(gh_ctx* ctx, int nleaves, const int* level, const int* i
         if (t.interior.count(Octree::key(b)))
             return set_error(GH_EINVAL, "gh_grid_init_leaves: a leaf contains another leaf");
     if (vol != 1.0) return set_error(GH_EINVAL, "gh_grid_init_leaves: leaves do not tile the domain");
+    GH_HIP_CHECK(hipStreamSynchronize(c->stream));  // the old tree's tables may still be uploading
     c->tree = t;
     int rc = apply_topology(*c);
     if (rc) return rc;

@@ -71,6 +71,10 @@ struct Ctx {
     size_t real_size = 4;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    // recorded after the last uploads from the pinned staging buffers (and from
+    // the tree's hash table); waited for before any of them is rewritten, so a
+    // tree change need not wait for its remap and uploads to finish
+    hipEvent_t evStage = nullptr;
     Topology topo;
     Octree tree;  // host octree the topology is derived from
 

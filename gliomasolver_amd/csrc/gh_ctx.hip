@@ -210,7 +210,9 @@ static int init_uniform(Ctx* c, int bpd) {
         return set_error(GH_EINVAL, "bpd must be a power of two in [1,64]");
     int level = 0;
     while ((1 << level) < bpd) ++level;
-    // slot order (level, z, y, x) = bx + bpd*(by + bpd*bz) on one level
+    // slot order (level, z, y, x) = bx + bpd*(by + bpd*bz) on one level; the
+    // old tree's tables may still be uploading
+    GH_HIP_CHECK(hipStreamSynchronize(c->stream));
     c->tree.init_uniform(level);
     if (c->tree.max_level < level) c->tree.max_level = level;
     int rc = apply_topology(*c);
@@ -285,6 +287,7 @@ int gh_ctx_destroy(gh_ctx* ctx) {
     c->release_all();
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
+    if (c->evStage) (void)hipEventDestroy(c->evStage);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return GH_OK;
