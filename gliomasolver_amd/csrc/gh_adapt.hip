@@ -1542,9 +1542,18 @@ int clear_bad(Ctx& c) {
 // on the host); a lab does not depend on which other labs are built, so both
 // give the details of labs built for the list alone.
 template <typename Real>
-int fwt_details_t(Ctx& c, const std::vector<int>& slots, std::vector<double>& det) {
+int fwt_details_t(Ctx& c, const std::vector<int>& slots, std::vector<double>& det,
+                  const std::function<void()>& meanwhile) {
     det.assign(slots.size(), 0.0);
-    if (slots.empty()) return GH_OK;
+    bool ran = false;
+    auto run_meanwhile = [&] {
+        if (!ran && meanwhile) meanwhile();
+        ran = true;
+    };
+    if (slots.empty()) {
+        run_meanwhile();
+        return GH_OK;
+    }
     const int order = c.tree.order;
     const int n = c.topo.nblocks();
     int rc;
@@ -1568,6 +1577,7 @@ int fwt_details_t(Ctx& c, const std::vector<int>& slots, std::vector<double>& de
         hipError_t e = hipGetLastError();
         if (e == hipSuccess) e = hipMemcpyAsync(pd, d.p, (size_t)m * 8, hipMemcpyDeviceToHost, c.stream);
         if (e == hipSuccess) e = hipMemcpyAsync(pbad, c.dBad.p, sizeof(int), hipMemcpyDeviceToHost, c.stream);
+        if (e == hipSuccess) run_meanwhile();
         if (e == hipSuccess) e = hipStreamSynchronize(c.stream);
         if (e != hipSuccess) return set_error(GH_EHIP, std::string("fwt: ") + hipGetErrorString(e));
         std::memcpy(det.data(), pd, (size_t)m * 8);
@@ -1578,6 +1588,7 @@ int fwt_details_t(Ctx& c, const std::vector<int>& slots, std::vector<double>& de
         }
     }
     ++c.adapt_count[1];
+    run_meanwhile();
     // host-built lab table of every slot (cached per tree version)
     std::vector<int> all;  // source of an async upload: lives until the sync below
     if (c.labAllVersion != c.tree_version) {
@@ -1693,15 +1704,18 @@ int rd_step_ghosted(Ctx& c, const void* in, void* out, double dt, bool fused) {
                  : step_ghosted_t<double, false>(c, (const double*)in, (double*)out, dt);
 }
 
-int fwt_details(Ctx& c, const std::vector<int>& slots, std::vector<double>& det) {
-    return c.real_kind == GH_REAL_F32 ? fwt_details_t<float>(c, slots, det) : fwt_details_t<double>(c, slots, det);
+int fwt_details(Ctx& c, const std::vector<int>& slots, std::vector<double>& det,
+                const std::function<void()>& meanwhile) {
+    return c.real_kind == GH_REAL_F32 ? fwt_details_t<float>(c, slots, det, meanwhile)
+                                      : fwt_details_t<double>(c, slots, det, meanwhile);
 }
 
 // Grid::refine with Refiner_SpaceExtension (MRAGrid.inl:247-355)
-int grid_refine(Ctx& c, const std::vector<char>& flagged, int* nrefined, std::vector<int>* old_slot) {
+int grid_refine(Ctx& c, const std::vector<char>& flagged, int* nrefined, std::vector<int>* old_slot,
+                const std::vector<char>* ext) {
     const auto t_plan = std::chrono::steady_clock::now();
     const Octree& t = c.tree;
-    std::vector<int> victims = t.refinement_victims(flagged);
+    std::vector<int> victims = t.refinement_victims(flagged, ext);
     if (nrefined) *nrefined = (int)victims.size();
     if (victims.empty()) return GH_OK;
     Octree nt = t.shell();
@@ -1761,10 +1775,10 @@ int grid_refine(Ctx& c, const std::vector<char>& flagged, int* nrefined, std::ve
 
 // Grid::compress with Compressor (MRAGrid.inl:383-486)
 int grid_compress(Ctx& c, const std::vector<char>& flagged, int* ncollapsed, std::vector<uint64_t>* parents_out,
-                  std::vector<int>* old_slot) {
+                  std::vector<int>* old_slot, const std::vector<Octree::CollapseCand>* cands) {
     const auto t_plan = std::chrono::steady_clock::now();
     const Octree& t = c.tree;
-    std::vector<uint64_t> parents = t.collapse_parents(flagged);
+    std::vector<uint64_t> parents = t.collapse_parents(flagged, cands);
     if (ncollapsed) *ncollapsed = (int)parents.size();
     if (parents_out) *parents_out = parents;
     if (parents.empty()) return GH_OK;
@@ -1908,10 +1922,11 @@ int gh_refine(gh_ctx* ctx, double tol, int max_loops, int* nrefined) {
         }
         if (cand.empty()) break;
         std::vector<double> det;
+        std::vector<char> ext;  // the space extension, found while the device evaluates the details
         int rc;
         {
             ProfScope fwt(c->prof[P_REF_FWT]);
-            if ((rc = fwt_details(*c, cand, det))) return rc;
+            if ((rc = fwt_details(*c, cand, det, [&] { ext = c->tree.space_extension(); }))) return rc;
         }
         std::vector<char> flagged(c->topo.nblocks(), 0);
         int nflag = 0;
@@ -1928,7 +1943,7 @@ int gh_refine(gh_ctx* ctx, double tol, int max_loops, int* nrefined) {
         old_slot.clear();
         {
             ProfScope ref(c->prof[P_REF]);
-            if ((rc = grid_refine(*c, flagged, &nr, &old_slot))) return rc;
+            if ((rc = grid_refine(*c, flagged, &nr, &old_slot, &ext))) return rc;
         }
         if (!old_slot.empty()) {
             carried.assign(old_slot.size(), 0);
@@ -2065,11 +2080,12 @@ int gh_compress(gh_ctx* ctx, double tol, int max_loops, int* ncollapsed) {
         }
         int rc;
         bool host_tables = false;
+        std::vector<Octree::CollapseCand> pcand;  // found while the device evaluates the details
         {
             ProfScope fwt(c->prof[P_COMP_FWT]);
             std::vector<double> dn;
             const int64_t host_before = c->adapt_count[1];
-            if ((rc = fwt_details(*c, need, dn))) return rc;
+            if ((rc = fwt_details(*c, need, dn, [&] { pcand = c->tree.collapse_candidates(); }))) return rc;
             host_tables = c->adapt_count[1] != host_before;
             for (size_t j = 0; j < need.size(); ++j) det[need_at[j]] = dn[j];
         }
@@ -2090,7 +2106,7 @@ int gh_compress(gh_ctx* ctx, double tol, int max_loops, int* ncollapsed) {
         old_slot.clear();
         {
             ProfScope comp(c->prof[P_COMP]);
-            if ((rc = grid_compress(*c, flagged, &nc, &parents, &old_slot))) return rc;
+            if ((rc = grid_compress(*c, flagged, &nc, &parents, &old_slot, &pcand))) return rc;
         }
         if (nc == 0) break;
         total += nc;

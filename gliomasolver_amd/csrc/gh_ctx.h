@@ -2,6 +2,8 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <functional>
+
 #include "gh_internal.h"
 #include "gh_octree.h"
 
@@ -187,13 +189,20 @@ int rd_autotune(Ctx& c, double dt, int trial_steps, int* best);
 int apply_topology(Ctx& c);
 int rd_prepare_ghosted(Ctx& c, double Dw, double Dg);
 int rd_step_ghosted(Ctx& c, const void* in, void* out, double dt, bool fused);
-int fwt_details(Ctx& c, const std::vector<int>& slots, std::vector<double>& det);
+// meanwhile (may be empty) runs on the host while the device evaluates the
+// details (host work that does not need them: the tree-only parts of a plan)
+int fwt_details(Ctx& c, const std::vector<int>& slots, std::vector<double>& det,
+                const std::function<void()>& meanwhile = {});
 // old_slot (may be null) receives, per new slot, the old slot of the same
-// leaf or -1 for a new one (left untouched when nothing changes)
-int grid_refine(Ctx& c, const std::vector<char>& flagged, int* nrefined, std::vector<int>* old_slot = nullptr);
-// parents (may be null) receives the keys of the new parent leaves
+// leaf or -1 for a new one (left untouched when nothing changes); ext (may be
+// null) is the current tree's space_extension()
+int grid_refine(Ctx& c, const std::vector<char>& flagged, int* nrefined, std::vector<int>* old_slot = nullptr,
+                const std::vector<char>* ext = nullptr);
+// parents (may be null) receives the keys of the new parent leaves; cands
+// (may be null) is the current tree's collapse_candidates()
 int grid_compress(Ctx& c, const std::vector<char>& flagged, int* ncollapsed,
-                  std::vector<uint64_t>* parents = nullptr, std::vector<int>* old_slot = nullptr);
+                  std::vector<uint64_t>* parents = nullptr, std::vector<int>* old_slot = nullptr,
+                  const std::vector<Octree::CollapseCand>* cands = nullptr);
 
 // likelihood helpers (gh_lik.hip): -logL of the current phi; dJ/du of a state buffer
 int lik_eval_current(Ctx& c, const double* params, int stepPET, int stepTi, int use_roi, double* out,

@@ -469,17 +469,15 @@ void Octree::chain_targets(int v, std::vector<int>& out) const {
     }
 }
 
-std::vector<int> Octree::refinement_victims(const std::vector<char>& flagged) const {
-    const int nl = (int)leaves.size();
-    std::vector<char> flag(flagged.begin(), flagged.end());
-    flag.resize(nl, 0);
-    // space extension: a leaf with a finer non-periodic-image neighbour is refined
+// space extension: a leaf with a finer non-periodic-image neighbour is refined
     // (MRAGRefiner_SpaceExtension.h:35-51).  Such a neighbour lies in a
     // subdivided region of the leaf's level next to it without the wrap
     // (has_finer_direct_neighbour), so the leaves to flag are the same-level
     // leaves next to each interior node, found from the interior nodes.
-    // Levels whose dense map is small are scanned for their subdivided nodes,
-    // the others visited from the interior set.
+// Levels whose dense map is small are scanned for their subdivided nodes,
+// the others visited from the interior set.
+std::vector<char> Octree::space_extension() const {
+    std::vector<char> flag(leaves.size(), 0);
     constexpr int kScanMax = 5;  // 8^5 = 32,768 map entries
     auto extend = [&](int L, const int I[3]) {
         const int n = 1 << L;
@@ -514,6 +512,14 @@ std::vector<int> Octree::refinement_victims(const std::vector<char>& flagged) co
         const int I[3] = {(int)((k >> 32) & 0xffff), (int)((k >> 16) & 0xffff), (int)(k & 0xffff)};
         extend(L, I);
     }
+    return flag;
+}
+
+std::vector<int> Octree::refinement_victims(const std::vector<char>& flagged, const std::vector<char>* ext) const {
+    const int nl = (int)leaves.size();
+    std::vector<char> flag = ext ? *ext : space_extension();
+    flag.resize(nl, 0);
+    for (int s = 0; s < nl && s < (int)flagged.size(); ++s) flag[s] |= flagged[s];
     // nothing at the maximum level is refined (:54-63)
     for (int s = 0; s < nl; ++s)
         if (leaves[s].level >= max_level) flag[s] = 0;
@@ -548,38 +554,48 @@ std::vector<int> Octree::refinement_victims(const std::vector<char>& flagged) co
     return out;
 }
 
-std::vector<uint64_t> Octree::collapse_parents(const std::vector<char>& flagged) const {
-    std::vector<uint64_t> out;
-    // candidate parents: interior nodes whose 8 children are leaves -- each
-    // found once, from its child of octant 0 (a leaf with even coordinates)
-    std::vector<uint64_t> cand;
-    for (const BlockRec& b : leaves)
-        if (b.level > 0 && ((b.index[0] | b.index[1] | b.index[2]) & 1) == 0)
-            cand.push_back(key(b.level - 1, b.index[0] >> 1, b.index[1] >> 1, b.index[2] >> 1));
-    for (uint64_t pk : cand) {
-        const int L = (int)(pk >> 48);
+std::vector<Octree::CollapseCand> Octree::collapse_candidates() const {
+    std::vector<CollapseCand> out;
+    // interior nodes whose 8 children are leaves -- each found once, from its
+    // child of octant 0 (a leaf with even coordinates)
+    for (const BlockRec& b : leaves) {
+        if (b.level == 0 || ((b.index[0] | b.index[1] | b.index[2]) & 1) != 0) continue;
+        const int L = b.level - 1;
         if (L < min_level) continue;
-        const int px = (int)((pk >> 32) & 0xffff), py = (int)((pk >> 16) & 0xffff), pz = (int)(pk & 0xffff);
+        const int px = b.index[0] >> 1, py = b.index[1] >> 1, pz = b.index[2] >> 1;
+        CollapseCand c{key(L, px, py, pz), {}};
         bool ok = true;
         if (L + 1 < dense_levels) {  // the children straight from their level's map
             const int32_t* map = dense[L + 1].data();
             for (int o = 0; o < 8 && ok; ++o) {
-                const int32_t slot = map[(size_t)(2 * px + (o & 1)) + ((size_t)(2 * py + ((o >> 1) & 1)) << (L + 1)) +
-                                         ((size_t)(2 * pz + ((o >> 2) & 1)) << (2 * (L + 1)))];
-                ok = slot >= 0 && slot < (int)flagged.size() && flagged[slot];
+                c.kids[o] = map[(size_t)(2 * px + (o & 1)) + ((size_t)(2 * py + ((o >> 1) & 1)) << (L + 1)) +
+                                ((size_t)(2 * pz + ((o >> 2) & 1)) << (2 * (L + 1)))];
+                ok = c.kids[o] >= 0;
             }
         } else {
-            for (int o = 0; o < 8 && ok; ++o) {
-                int slot;
-                if (kind(L + 1, 2 * px + (o & 1), 2 * py + ((o >> 1) & 1), 2 * pz + ((o >> 2) & 1), &slot) != LEAF)
-                    ok = false;
-                else if (!(slot < (int)flagged.size() && flagged[slot]))
-                    ok = false;
-            }
+            for (int o = 0; o < 8 && ok; ++o)
+                ok = kind(L + 1, 2 * px + (o & 1), 2 * py + ((o >> 1) & 1), 2 * pz + ((o >> 2) & 1), &c.kids[o]) ==
+                     LEAF;
         }
         // every neighbour of every child within one level of the parent (MRAGCompressor.h:95-107)
         const int P[3] = {px, py, pz};
-        if (ok && outside_within_one(L, P)) out.push_back(pk);
+        if (ok && outside_within_one(L, P)) out.push_back(c);
+    }
+    return out;
+}
+
+std::vector<uint64_t> Octree::collapse_parents(const std::vector<char>& flagged,
+                                               const std::vector<CollapseCand>* cands) const {
+    std::vector<CollapseCand> own;
+    if (!cands) {
+        own = collapse_candidates();
+        cands = &own;
+    }
+    std::vector<uint64_t> out;
+    for (const CollapseCand& c : *cands) {
+        bool ok = true;
+        for (int o = 0; o < 8 && ok; ++o) ok = c.kids[o] < (int)flagged.size() && flagged[c.kids[o]];
+        if (ok) out.push_back(c.key);
     }
     std::sort(out.begin(), out.end());  // ascending keys
     return out;

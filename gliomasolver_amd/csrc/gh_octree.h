@@ -113,11 +113,23 @@ struct Octree {
     void value_terms(int L, const int64_t q[3], double w, std::vector<GhostTerm>& out) const;
 
     // --- plans (return the new leaf list; data movement is done by the caller)
-    // Science::AutomaticRefinement's refine(set): space extension + 2:1 chain reaction
-    std::vector<int> refinement_victims(const std::vector<char>& flagged) const;
+    // Science::AutomaticRefinement's refine(set): space extension + 2:1 chain
+    // reaction.  The space extension depends on the tree only: ext (may be
+    // null) is space_extension() of this tree, computed beforehand.
+    std::vector<char> space_extension() const;
+    std::vector<int> refinement_victims(const std::vector<char>& flagged, const std::vector<char>* ext = nullptr) const;
     // Compressor::createPlan: parents whose 8 children are flagged leaves and whose
-    // children's neighbours are all within one level of the parent
-    std::vector<uint64_t> collapse_parents(const std::vector<char>& flagged) const;
+    // children's neighbours are all within one level of the parent.  The
+    // candidates (parents of 8 leaves at or above min_level that pass the
+    // level test) depend on the tree only: cands (may be null) is
+    // collapse_candidates() of this tree, computed beforehand.
+    struct CollapseCand {
+        uint64_t key;  // the parent
+        int kids[8];   // slots of its children by octant
+    };
+    std::vector<CollapseCand> collapse_candidates() const;
+    std::vector<uint64_t> collapse_parents(const std::vector<char>& flagged,
+                                           const std::vector<CollapseCand>* cands = nullptr) const;
 };
 
 // how a leaf of the new grid is produced from the old one
