@@ -15,12 +15,13 @@
 // Ghost values: a ghost lattice point at the block's level is a same-level
 // copy when a same-level leaf covers it, else a linear combination of leaf
 // cells: injection from finer regions (Ha = {1}), Hs synthesis from coarser
-// ones, recursively (Octree::value_terms on the host, ghost_value on the
+// ones, recursively (Octree::value_terms on the host, ghost_visit on the
 // device).  Terms are accumulated in Real, product by product, in recursion
 // order (synthesis taps z-major, then y, then x); the weights are dyadic,
-// exact in float and double.  The labs of the FWT and of the split are
-// evaluated on the device from per-level block maps; the RD face ghosts of a
-// multi-level grid use a host-built table (applied every step).
+// exact in float and double.  The labs of the FWT and of the split, the
+// neighbour tables and the RD face-ghost table of a multi-level grid are built
+// on the device from per-level block maps (host-built tables only for ghosts
+// deeper than the device recursion); the face ghosts are gathered every step.
 #include <algorithm>
 #include <array>
 #include <chrono>
