@@ -739,11 +739,25 @@ __global__ __launch_bounds__(512) void remap_kernel(int nnew, const Remap* __res
 #pragma unroll
         for (int f = 0; f < 4; ++f) v[f] = (Real)0 + rf.old[f][(size_t)r.kids[o] * BCELLS + c];
     } else {
+        // the sources of this octant's cells: a W^3 window of the parent's lab
+        // (W = 7 from 4 * octant - 1 at order 4, 5 from 4 * octant at order
+        // 2), staged in LDS for the four fields
+        __shared__ Real s_src[4 * 343];
+        const int W = order == 4 ? 7 : 5;
+        const int ws[3] = {4 * (r.octant & 1) - (order == 4 ? 1 : 0), 4 * ((r.octant >> 1) & 1) - (order == 4 ? 1 : 0),
+                           4 * ((r.octant >> 2) & 1) - (order == 4 ? 1 : 0)};
+        const size_t L0 = (size_t)r.src * LAB * LAB * LAB;
+        const int W3 = W * W * W;
+        for (int i = t; i < 4 * W3; i += 512) {
+            const int f = i / W3, e = i - f * W3;
+            const int ex = e % W, ey = (e / W) % W, ez = e / (W * W);
+            s_src[i] = rf.lab[f][L0 + (size_t)((ws[2] + ez - LS) * LAB * LAB + (ws[1] + ey - LS) * LAB + (ws[0] + ex - LS))];
+        }
+        __syncthreads();
         const int w = t >> 6, l = t & 63;  // parity class (wave), cell of the class (lane)
         const int x = 2 * (l & 3) + (w & 1), y = 2 * ((l >> 2) & 3) + ((w >> 1) & 1), z = 2 * (l >> 4) + (w >> 2);
         cell = x + 8 * y + 64 * z;
         const int d[3] = {8 * (r.octant & 1) + x, 8 * ((r.octant >> 1) & 1) + y, 8 * ((r.octant >> 2) & 1) + z};
-        const size_t L0 = (size_t)r.src * LAB * LAB * LAB;
 #pragma unroll
         for (int f = 0; f < 4; ++f) v[f] = (Real)0;
         const int nz = synth_n(order, d[2]), ny = synth_n(order, d[1]), nx = synth_n(order, d[0]);
@@ -756,10 +770,10 @@ __global__ __launch_bounds__(512) void remap_kernel(int nnew, const Remap* __res
                 for (int kx = 0; kx < nx; ++kx) {
                     const int sx = synth_src(order, d[0], kx);
                     const Real wt = (Real)(wZwY * synth_w(order, d[0], kx));
-                    const size_t q = L0 + (sz - LS) * LAB * LAB + (sy - LS) * LAB + (sx - LS);
+                    const int q = (sz - ws[2]) * W * W + (sy - ws[1]) * W + (sx - ws[0]);
 #pragma unroll
                     for (int f = 0; f < 4; ++f) {
-                        const Real prod = rf.lab[f][q] * wt;
+                        const Real prod = s_src[f * W3 + q] * wt;
                         v[f] = v[f] + prod;
                     }
                 }
