@@ -853,34 +853,50 @@ struct FacePackets {
 // jump, from the interpolated face ghosts -- precomputed in fgPhi, or (PK)
 // summed here from the face's packet by the wave that owns the face, term by
 // term in face_packet_kernel's order, so both give the same bits
-template <typename Real, bool FUSED, bool PK = false>
-__global__ __launch_bounds__(512) void rd_step_ghosted_kernel(int nslots, int per_xcd, const int32_t* __restrict__ nbr,
-                                                              const int32_t* __restrict__ level,
-                                                              const Real* __restrict__ phi, Real* __restrict__ out,
-                                                              const Stat4<Real>* __restrict__ st,
-                                                              const Real* __restrict__ sig,
-                                                              const Real* __restrict__ fgPhi,
-                                                              const Real* __restrict__ fgTau,
-                                                              const Real* __restrict__ fgDm, Real rho, double dt,
-                                                              FacePackets<Real> fp) {
+template <typename Real, bool FUSED, bool PK, int NT>
+__global__ __launch_bounds__(NT) void rd_step_ghosted_kernel(int nslots, int per_xcd, const int32_t* __restrict__ nbr,
+                                                             const int32_t* __restrict__ level,
+                                                             const Real* __restrict__ phi, Real* __restrict__ out,
+                                                             const Stat4<Real>* __restrict__ st,
+                                                             const Real* __restrict__ sig,
+                                                             const Real* __restrict__ fgPhi,
+                                                             const Real* __restrict__ fgTau,
+                                                             const Real* __restrict__ fgDm, Real rho, double dt,
+                                                             FacePackets<Real> fp) {
+    static_assert(NT == 512 || NT == 256, "one or two cells per lane");
+    constexpr int CPT = 512 / NT;  // cells per lane: i = t + c * NT
     const int s = (int)(blockIdx.x % 8) * per_xcd + (int)(blockIdx.x / 8);
     if (s >= nslots) return;
     __shared__ Real s_phi[1000];
     __shared__ unsigned char s_out[1000];
     __shared__ Real s_Dx[576], s_Dy[576], s_Dz[576];
     const int t = threadIdx.x;
-    const int x = t & 7, y = (t >> 3) & 7, z = t >> 6;
-    const size_t g = (size_t)s * BCELLS + t;
-    const Real pc = phi[g];
-    const Stat4<Real> sv = st[g];
-    const Real sc = sig[g];
-    s_phi[P(x, y, z)] = pc;
-    s_out[P(x, y, z)] = pc < sv.tau;
-    s_Dx[z * 72 + y * 9 + x + 1] = sv.dx;
-    s_Dy[z * 72 + (y + 1) * 8 + x] = sv.dy;
-    s_Dz[(z + 1) * 64 + y * 8 + x] = sv.dz;
-    if (t < 384) {
-        const int f = t >> 6, u = t & 7, v = (t >> 3) & 7;
+    Real pc[CPT], sc[CPT];
+    Stat4<Real> sv[CPT];
+#pragma unroll
+    for (int c = 0; c < CPT; ++c) {
+        const int i = t + c * NT;
+        const size_t g = (size_t)s * BCELLS + i;
+        pc[c] = phi[g];
+        sv[c] = st[g];
+        sc[c] = sig[g];
+    }
+#pragma unroll
+    for (int c = 0; c < CPT; ++c) {
+        const int i = t + c * NT;
+        const int x = i & 7, y = (i >> 3) & 7, z = i >> 6;
+        s_phi[P(x, y, z)] = pc[c];
+        s_out[P(x, y, z)] = pc[c] < sv[c].tau;
+        s_Dx[z * 72 + y * 9 + x + 1] = sv[c].dx;
+        s_Dy[z * 72 + (y + 1) * 8 + x] = sv[c].dy;
+        s_Dz[(z + 1) * 64 + y * 8 + x] = sv[c].dz;
+    }
+    // the 6 x 64 face halo points, one wave per face
+#pragma unroll
+    for (int c = 0; c < (384 + NT - 1) / NT; ++c) {
+        const int fi = t + c * NT;
+        if (fi >= 384) break;
+        const int f = fi >> 6, u = fi & 7, v = (fi >> 3) & 7;
         int src, dst;
         switch (f) {
             case FXM: src = 7 + 8 * u + 64 * v; dst = P(-1, u, v); break;
@@ -901,7 +917,7 @@ __global__ __launch_bounds__(512) void rd_step_ghosted_kernel(int nslots, int pe
         } else {
             if (PK) {
                 const int k = fp.pkOf[6 * s + f];
-                const int p = t & 63;
+                const int p = fi & 63;
                 const int b0 = fp.base[k] + p, len = fp.count[64 * k + p];
                 Real acc = (Real)0;
 #pragma unroll 4
@@ -911,9 +927,9 @@ __global__ __launch_bounds__(512) void rd_step_ghosted_kernel(int nslots, int pe
                 }
                 ph = acc;
             } else {
-                ph = fgPhi[(size_t)s * 384 + t];
+                ph = fgPhi[(size_t)s * 384 + fi];
             }
-            tau = fgTau[(size_t)s * 384 + t];
+            tau = fgTau[(size_t)s * 384 + fi];
             if (f == FXM || f == FYM || f == FZM) dm = fgDm[(size_t)s * 192 + (f >> 1) * 64 + v * 8 + u];
         }
         s_phi[dst] = ph;
@@ -925,41 +941,47 @@ __global__ __launch_bounds__(512) void rd_step_ghosted_kernel(int nslots, int pe
     __syncthreads();
     const double h = ldexp(1.0, -level[s]) / 8;  // info.h[0] (MRAGrid.inl:200-206)
     const double ih2 = 1. / (h * h);
-    Real d0 = s_Dx[z * 72 + y * 9 + x], d1 = sv.dx, d2 = s_Dy[z * 72 + y * 8 + x], d3 = sv.dy;
-    Real d4 = s_Dz[z * 64 + y * 8 + x], d5 = sv.dz;
-    if (s_out[P(x - 1, y, z)]) d1 = (Real)((double)d1 * 2.0);
-    if (s_out[P(x + 1, y, z)]) d0 = (Real)((double)d0 * 2.0);
-    if (s_out[P(x, y - 1, z)]) d3 = (Real)((double)d3 * 2.0);
-    if (s_out[P(x, y + 1, z)]) d2 = (Real)((double)d2 * 2.0);
-    if (s_out[P(x, y, z - 1)]) d5 = (Real)((double)d5 * 2.0);
-    if (s_out[P(x, y, z + 1)]) d4 = (Real)((double)d4 * 2.0);
-    const Real p0 = s_phi[P(x - 1, y, z)], p1 = s_phi[P(x + 1, y, z)], p2 = s_phi[P(x, y - 1, z)];
-    const Real p3 = s_phi[P(x, y + 1, z)], p4 = s_phi[P(x, y, z - 1)], p5 = s_phi[P(x, y, z + 1)];
-    Real sin_ = d0 * p0;
-    sin_ = sin_ + d1 * p1;
-    sin_ = sin_ + d2 * p2;
-    sin_ = sin_ + d3 * p3;
-    sin_ = sin_ + d4 * p4;
-    sin_ = sin_ + d5 * p5;
-    const double fin = ih2 * (double)sin_;
-    Real sdf = d0 + d1;
-    sdf = sdf + d2;
-    sdf = sdf + d3;
-    sdf = sdf + d4;
-    sdf = sdf + d5;
-    const Real sdp = sdf * pc;
-    const double fout = -((double)sdp * ih2);
-    const Real rp = rho * pc;
-    const double react = (double)rp * (1. - (double)pc);
-    Real dphidt = (Real)((fout + fin) + react);
-    if (!(pc > sc)) dphidt = (Real)0.;
-    if (FUSED) {
-        Real p = (Real)((double)pc + dt * (double)dphidt);
-        p = ((Real)0. < p) ? p : (Real)0.;
-        p = (p < (Real)1.) ? p : (Real)1.;
-        out[g] = p;
-    } else {
-        out[g] = dphidt;
+#pragma unroll
+    for (int c = 0; c < CPT; ++c) {
+        const int i = t + c * NT;
+        const int x = i & 7, y = (i >> 3) & 7, z = i >> 6;
+        Real d0 = s_Dx[z * 72 + y * 9 + x], d1 = sv[c].dx, d2 = s_Dy[z * 72 + y * 8 + x], d3 = sv[c].dy;
+        Real d4 = s_Dz[z * 64 + y * 8 + x], d5 = sv[c].dz;
+        if (s_out[P(x - 1, y, z)]) d1 = (Real)((double)d1 * 2.0);
+        if (s_out[P(x + 1, y, z)]) d0 = (Real)((double)d0 * 2.0);
+        if (s_out[P(x, y - 1, z)]) d3 = (Real)((double)d3 * 2.0);
+        if (s_out[P(x, y + 1, z)]) d2 = (Real)((double)d2 * 2.0);
+        if (s_out[P(x, y, z - 1)]) d5 = (Real)((double)d5 * 2.0);
+        if (s_out[P(x, y, z + 1)]) d4 = (Real)((double)d4 * 2.0);
+        const Real p0 = s_phi[P(x - 1, y, z)], p1 = s_phi[P(x + 1, y, z)], p2 = s_phi[P(x, y - 1, z)];
+        const Real p3 = s_phi[P(x, y + 1, z)], p4 = s_phi[P(x, y, z - 1)], p5 = s_phi[P(x, y, z + 1)];
+        Real sin_ = d0 * p0;
+        sin_ = sin_ + d1 * p1;
+        sin_ = sin_ + d2 * p2;
+        sin_ = sin_ + d3 * p3;
+        sin_ = sin_ + d4 * p4;
+        sin_ = sin_ + d5 * p5;
+        const double fin = ih2 * (double)sin_;
+        Real sdf = d0 + d1;
+        sdf = sdf + d2;
+        sdf = sdf + d3;
+        sdf = sdf + d4;
+        sdf = sdf + d5;
+        const Real sdp = sdf * pc[c];
+        const double fout = -((double)sdp * ih2);
+        const Real rp = rho * pc[c];
+        const double react = (double)rp * (1. - (double)pc[c]);
+        Real dphidt = (Real)((fout + fin) + react);
+        if (!(pc[c] > sc[c])) dphidt = (Real)0.;
+        const size_t g = (size_t)s * BCELLS + i;
+        if (FUSED) {
+            Real p = (Real)((double)pc[c] + dt * (double)dphidt);
+            p = ((Real)0. < p) ? p : (Real)0.;
+            p = (p < (Real)1.) ? p : (Real)1.;
+            out[g] = p;
+        } else {
+            out[g] = dphidt;
+        }
     }
 }
 
@@ -1385,6 +1407,26 @@ int prepare_ghosted_t(Ctx& c, double Dw, double Dg) {
 
 constexpr int kFusedGhostMaxBlocks = 1450;  // largest grid whose step kernel gathers its own ghosts
 
+// The step kernel with 512 lanes per block when it gathers the face ghosts
+// itself (one wave per face: with 256 lanes two waves would gather two faces
+// each, in turn: 1,114 blocks 15.9 -> 23.0 us per step), else with 256 lanes
+// of two cells each (more blocks in flight per CU: 7,225 / 13,455 blocks 45.3
+// / 91.6 -> 44.6 / 86.1 us per step with the separate gather).
+template <typename Real, bool FUSED, bool PK>
+void launch_step_ghosted(Ctx& c, int ns, int per_xcd, const Real* in, Real* out, double dt, const Real* fg,
+                         const FacePackets<Real>& fp) {
+    if (!PK)
+        hipLaunchKernelGGL((rd_step_ghosted_kernel<Real, FUSED, PK, 256>), dim3(8 * per_xcd), dim3(256), 0, c.stream,
+                           ns, per_xcd, c.nbr.as<int32_t>(), c.dLevel.as<int32_t>(), in, out,
+                           c.statD.as<Stat4<Real>>(), c.statSig.as<Real>(), fg, c.fgTau.as<Real>(),
+                           c.fgDm.as<Real>(), (Real)c.rho, dt, fp);
+    else
+        hipLaunchKernelGGL((rd_step_ghosted_kernel<Real, FUSED, PK, 512>), dim3(8 * per_xcd), dim3(512), 0, c.stream,
+                           ns, per_xcd, c.nbr.as<int32_t>(), c.dLevel.as<int32_t>(), in, out,
+                           c.statD.as<Stat4<Real>>(), c.statSig.as<Real>(), fg, c.fgTau.as<Real>(),
+                           c.fgDm.as<Real>(), (Real)c.rho, dt, fp);
+}
+
 template <typename Real, bool FUSED>
 int step_ghosted_t(Ctx& c, const Real* in, Real* out, double dt) {
     int rc0 = ensure_faces(c);
@@ -1400,19 +1442,13 @@ int step_ghosted_t(Ctx& c, const Real* in, Real* out, double dt) {
     if (c.nFacePk > 0 && ns <= kFusedGhostMaxBlocks) {
         const FacePackets<Real> fp{c.facePkOf.as<int32_t>(), c.facePkBase.as<int32_t>(), c.faceCount.as<int32_t>(),
                                    c.faceSrc.as<int32_t>(), c.faceW.as<Real>()};
-        hipLaunchKernelGGL((rd_step_ghosted_kernel<Real, FUSED, true>), dim3(8 * per_xcd), dim3(512), 0, c.stream, ns,
-                           per_xcd, c.nbr.as<int32_t>(), c.dLevel.as<int32_t>(), in, out, c.statD.as<Stat4<Real>>(),
-                           c.statSig.as<Real>(), (const Real*)nullptr, c.fgTau.as<Real>(), c.fgDm.as<Real>(),
-                           (Real)c.rho, dt, fp);
+        launch_step_ghosted<Real, FUSED, true>(c, ns, per_xcd, in, out, dt, nullptr, fp);
         GH_HIP_CHECK(hipGetLastError());
         return GH_OK;
     }
     int rc = face_ghosts<Real>(c, in, c.fgPhi.as<Real>());
     if (rc) return rc;
-    hipLaunchKernelGGL((rd_step_ghosted_kernel<Real, FUSED>), dim3(8 * per_xcd), dim3(512), 0, c.stream, ns, per_xcd,
-                       c.nbr.as<int32_t>(), c.dLevel.as<int32_t>(), in, out, c.statD.as<Stat4<Real>>(),
-                       c.statSig.as<Real>(), c.fgPhi.as<Real>(), c.fgTau.as<Real>(), c.fgDm.as<Real>(), (Real)c.rho,
-                       dt, FacePackets<Real>{});
+    launch_step_ghosted<Real, FUSED, false>(c, ns, per_xcd, in, out, dt, c.fgPhi.as<Real>(), FacePackets<Real>{});
     GH_HIP_CHECK(hipGetLastError());
     return GH_OK;
 }
