@@ -1614,7 +1614,9 @@ int fwt_details_t(Ctx& c, const std::vector<int>& slots, std::vector<double>& de
         // labs and details of the listed slots only; the recursion check is
         // read back together with the details (one synchronization)
         const int m = (int)slots.size();
-        if ((rc = stage_wait(c)) || (rc = c.labSlots.ensure_grow((size_t)m * 4)) || (rc = clear_bad(c)) ||
+        // pinSlots and pinDet are used here only, each call synchronizing
+        // before it returns
+        if ((rc = c.labSlots.ensure_grow((size_t)m * 4)) || (rc = clear_bad(c)) ||
             (rc = c.pinSlots.ensure_grow((size_t)m * 4)) || (rc = c.pinDet.ensure_grow((size_t)m * 8 + 8)))
             return rc;
         std::memcpy(c.pinSlots.p, slots.data(), (size_t)m * 4);
@@ -1684,10 +1686,10 @@ int apply_new_tree(Ctx& c, Octree& nt, const Remap* rm, const std::vector<int>& 
         bool on_device = false;
         if (c.nDense > 0) {
             if ((rc = c.labSlots.ensure_grow(split_parents.size() * 4)) || (rc = clear_bad(c)) ||
-                (rc = c.pinSlots.ensure_grow(split_parents.size() * 4)))
+                (rc = c.pinSplit.ensure_grow(split_parents.size() * 4)))
                 return rc;
-            std::memcpy(c.pinSlots.p, split_parents.data(), split_parents.size() * 4);
-            GH_HIP_CHECK(hipMemcpyAsync(c.labSlots.p, c.pinSlots.p, split_parents.size() * 4, hipMemcpyHostToDevice,
+            std::memcpy(c.pinSplit.p, split_parents.data(), split_parents.size() * 4);
+            GH_HIP_CHECK(hipMemcpyAsync(c.labSlots.p, c.pinSplit.p, split_parents.size() * 4, hipMemcpyHostToDevice,
                                         c.stream));
             const Real* fld[4];
             DevBuf* out[4];
@@ -1722,7 +1724,7 @@ int apply_new_tree(Ctx& c, Octree& nt, const Remap* rm, const std::vector<int>& 
     hipLaunchKernelGGL(remap_kernel<Real>, dim3(nnew), dim3(512), 0, c.stream, nnew, drm.as<Remap>(), rf, LS, LAB,
                        order);
     GH_HIP_CHECK(hipGetLastError());
-    // rm (in c.pinRemap) and the split list (copied to c.pinSlots) are read by
+    // rm (in c.pinRemap) and the split list (copied to c.pinSplit) are read by
     // the uploads until the event apply_topology records has passed
     for (int f = 0; f < 4; ++f) std::swap(*fields[f], nf[f]);
     c.prof[P_MOVE] += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_move).count();

@@ -116,7 +116,7 @@ struct Ctx {
     // adaptation scratch kept across epochs: spare field arrays swapped with
     // the live ones by apply_new_tree, split labs, remap list, FWT details
     DevBuf spare[4], labSplit[4], remapBuf, detBuf;
-    PinnedBuf pinRemap, pinDense, pinBlk, pinSlots, pinDet;  // staging of the adaptation's host<->device tables
+    PinnedBuf pinRemap, pinDense, pinBlk, pinSplit, pinSlots, pinDet;  // staging of the adaptation's host<->device tables
     // adaptation profile (gh_adapt_profile), host wall time in ms per phase
     double prof[8] = {0};
     // how ghosts were built (gh_adapt_counters): lab batches on the device,

@@ -86,7 +86,7 @@ void Ctx::release_all() {
     faceScan.release();
     dHashKey.release();
     dHashVal.release();
-    for (PinnedBuf* b : {&pinRemap, &pinDense, &pinBlk, &pinSlots, &pinDet}) b->release();
+    for (PinnedBuf* b : {&pinRemap, &pinDense, &pinBlk, &pinSplit, &pinSlots, &pinDet}) b->release();
     labPermOrder = 0;
 }
 
