@@ -1777,14 +1777,8 @@ int grid_refine(Ctx& c, const std::vector<char>& flagged, int* nrefined, std::ve
     for (int v : victims) is_victim[v] = 1;
     std::vector<int> victim_index(t.leaves.size(), -1);  // old slot -> compact parent index
     for (int j = 0; j < (int)victims.size(); ++j) victim_index[victims[j]] = j;
-    for (int s = 0; s < (int)t.leaves.size(); ++s) {
-        const BlockRec& b = t.leaves[s];
-        if (!is_victim[s]) {
-            nt.leaves.push_back(b);
-            continue;
-        }
-        nt.interior.insert(Octree::key(b));
-    }
+    for (int s = 0; s < (int)t.leaves.size(); ++s)
+        if (!is_victim[s]) nt.leaves.push_back(t.leaves[s]);
     for (int j = 0; j < (int)victims.size(); ++j) {
         const BlockRec& b = t.leaves[victims[j]];
         for (int o = 0; o < 8; ++o) {
@@ -1850,7 +1844,6 @@ int grid_compress(Ctx& c, const std::vector<char>& flagged, int* ncollapsed, std
             gone[slot] = 1;
         }
         kids[i] = k;
-        nt.interior.erase(pk);
     }
     nt.leaves.reserve(t.leaves.size() - 7 * parents.size());
     for (int s = 0; s < (int)t.leaves.size(); ++s)
@@ -1912,9 +1905,7 @@ int gh_grid_init_leaves(gh_ctx* ctx, int nleaves, const int* level, const int* i
     Ctx* c = as_ctx(ctx);
     if (!c || nleaves <= 0 || !level || !index) return set_error(GH_EINVAL, "gh_grid_init_leaves: bad argument");
     GH_HIP_CHECK(hipSetDevice(c->device));
-    Octree t = c->tree;
-    t.leaves.clear();
-    t.interior.clear();
+    Octree t = c->tree.shell();
     double vol = 0;
     for (int i = 0; i < nleaves; ++i) {
         const int L = level[i];
@@ -1924,17 +1915,12 @@ int gh_grid_init_leaves(gh_ctx* ctx, int nleaves, const int* level, const int* i
                 return set_error(GH_EINVAL, "gh_grid_init_leaves: index out of range");
         t.leaves.push_back(BlockRec{L, {index[3 * i], index[3 * i + 1], index[3 * i + 2]}});
         vol += std::ldexp(1.0, -3 * L);
-        for (int l = L - 1; l >= 0; --l)
-            t.interior.insert(Octree::key(l, index[3 * i] >> (L - l), index[3 * i + 1] >> (L - l),
-                                          index[3 * i + 2] >> (L - l)));
     }
     t.rebuild_index();
     for (int s = 1; s < nleaves; ++s)  // canonical order puts equal leaves next to each other
         if (Octree::key(t.leaves[s]) == Octree::key(t.leaves[s - 1]))
             return set_error(GH_EINVAL, "gh_grid_init_leaves: duplicate leaf");
-    for (auto& b : t.leaves)
-        if (t.interior.count(Octree::key(b)))
-            return set_error(GH_EINVAL, "gh_grid_init_leaves: a leaf contains another leaf");
+    if (t.nested) return set_error(GH_EINVAL, "gh_grid_init_leaves: a leaf contains another leaf");
     if (vol != 1.0) return set_error(GH_EINVAL, "gh_grid_init_leaves: leaves do not tile the domain");
     GH_HIP_CHECK(hipStreamSynchronize(c->stream));  // the old tree's tables may still be uploading
     c->tree = t;

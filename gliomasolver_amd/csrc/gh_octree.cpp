@@ -39,17 +39,10 @@ Octree::Kind Octree::kind(int L, int x, int y, int z, int* slot) const {
 
 void Octree::init_uniform(int level) {
     leaves.clear();
-    interior.clear();
     const int n = 1 << level;
     for (int z = 0; z < n; ++z)
         for (int y = 0; y < n; ++y)
             for (int x = 0; x < n; ++x) leaves.push_back(BlockRec{level, {x, y, z}});
-    for (int l = 0; l < level; ++l) {
-        const int m = 1 << l;
-        for (int z = 0; z < m; ++z)
-            for (int y = 0; y < m; ++y)
-                for (int x = 0; x < m; ++x) interior.insert(key(l, x, y, z));
-    }
     rebuild_index();
 }
 
@@ -83,48 +76,70 @@ void Octree::rebuild_index() {
     hash_bits = 0;
     hash_keys.clear();
     hash_vals.clear();
+    interior_deep.clear();
+    nested = false;
     if (leaves.empty()) return;
     const int top = leaves.back().level;  // the finest level sorts last
     const int nd = std::min(top, DENSE_MAX);
     for (int l = 0; l <= nd; ++l) dense[l].assign((size_t)1 << (3 * l), -1);
-    size_t deep = 0;  // nodes below the dense levels
-    for (uint64_t k : interior) {
-        const int l = (int)(k >> 48);
-        if (l > top) continue;
-        if (l > nd) {
-            ++deep;
-            continue;
-        }
-        const size_t x = (k >> 32) & 0xffff, y = (k >> 16) & 0xffff, z = k & 0xffff;
-        dense[l][x + (y << l) + (z << (2 * l))] = -2;
-    }
+    size_t deep_leaves = 0;  // leaves below the dense levels
     for (size_t s = 0; s < leaves.size(); ++s) {
         const BlockRec& b = leaves[s];
         if (b.level > nd) {
-            ++deep;
+            ++deep_leaves;
             continue;
         }
-        dense[b.level][(size_t)b.index[0] + ((size_t)b.index[1] << b.level) + ((size_t)b.index[2] << (2 * b.level))] =
-            (int32_t)s;
+        int32_t& e =
+            dense[b.level][(size_t)b.index[0] + ((size_t)b.index[1] << b.level) + ((size_t)b.index[2] << (2 * b.level))];
+        if (e != -1) nested = true;  // the same leaf twice
+        e = (int32_t)s;
     }
+    // the interior nodes are the strict ancestors of the leaves: marked -2 by a
+    // walk up from each leaf that stops at a node marked before (its own
+    // ancestors are marked already); reaching a leaf means nesting.  Nodes of
+    // levels >= SCAN_MAX + 1 (and all of the hashed levels) are listed.
+    std::vector<uint64_t>& deep = interior_deep;
+    for (const BlockRec& b : leaves) {
+        for (int l = b.level - 1; l >= 0; --l) {
+            const int sh = b.level - l;
+            const int x = b.index[0] >> sh, y = b.index[1] >> sh, z = b.index[2] >> sh;
+            if (l > nd) {
+                deep.push_back(key(l, x, y, z));
+                continue;
+            }
+            int32_t& e = dense[l][(size_t)x + ((size_t)y << l) + ((size_t)z << (2 * l))];
+            if (e == -2) break;
+            if (e >= 0) nested = true;  // a leaf containing another leaf
+            e = -2;
+            if (l > SCAN_MAX) deep.push_back(key(l, x, y, z));
+        }
+    }
+    std::sort(deep.begin(), deep.end());
+    deep.erase(std::unique(deep.begin(), deep.end()), deep.end());
     dense_levels = nd + 1;
     top_level = top;
-    if (deep == 0) return;
+    size_t deep_nodes = 0;  // interior nodes below the dense levels
+    for (uint64_t k : deep) deep_nodes += (int)(k >> 48) > nd;
+    if (deep_leaves + deep_nodes == 0) return;
     hash_bits = 4;
-    while (((size_t)1 << hash_bits) < 2 * deep) ++hash_bits;
+    while (((size_t)1 << hash_bits) < 2 * (deep_leaves + deep_nodes)) ++hash_bits;
     const size_t cap = (size_t)1 << hash_bits;
     hash_keys.assign(cap, kEmptyKey);
     hash_vals.assign(cap, -1);
     auto put = [&](uint64_t k, int32_t v) {
         size_t h = node_hash(k, hash_bits);
-        while (hash_keys[h] != kEmptyKey) h = (h + 1) & (cap - 1);
+        while (hash_keys[h] != kEmptyKey) {
+            if (hash_keys[h] == k) {  // a leaf that is also an interior node, or twice
+                nested = true;
+                return;
+            }
+            h = (h + 1) & (cap - 1);
+        }
         hash_keys[h] = k;
         hash_vals[h] = v;
     };
-    for (uint64_t k : interior) {
-        const int l = (int)(k >> 48);
-        if (l > nd && l <= top) put(k, -2);
-    }
+    for (uint64_t k : deep)
+        if ((int)(k >> 48) > nd) put(k, -2);
     for (size_t s = 0; s < leaves.size(); ++s)
         if (leaves[s].level > nd) put(key(leaves[s]), (int32_t)s);
 }
@@ -475,10 +490,9 @@ void Octree::chain_targets(int v, std::vector<int>& out) const {
     // (has_finer_direct_neighbour), so the leaves to flag are the same-level
     // leaves next to each interior node, found from the interior nodes.
 // Levels whose dense map is small are scanned for their subdivided nodes,
-// the others visited from the interior set.
+// the others visited from the listed interior nodes of the deeper levels.
 std::vector<char> Octree::space_extension() const {
     std::vector<char> flag(leaves.size(), 0);
-    constexpr int kScanMax = 5;  // 8^5 = 32,768 map entries
     auto extend = [&](int L, const int I[3]) {
         const int n = 1 << L;
         for (int code = 0; code < 27; ++code) {
@@ -490,7 +504,7 @@ std::vector<char> Octree::space_extension() const {
             if (kind(L, P[0], P[1], P[2], &slot) == LEAF) flag[slot] = 1;
         }
     };
-    const int nscan = std::min(dense_levels, kScanMax + 1);
+    const int nscan = std::min(dense_levels, SCAN_MAX + 1);
     for (int L = 0; L < nscan; ++L) {
         const int n = 1 << L;
         const int32_t* map = dense[L].data();
@@ -506,7 +520,7 @@ std::vector<char> Octree::space_extension() const {
                             }
                 }
     }
-    for (uint64_t k : interior) {
+    for (uint64_t k : interior_deep) {  // the levels not scanned (interior_deep holds levels >= SCAN_MAX + 1)
         const int L = (int)(k >> 48);
         if (L < nscan) continue;
         const int I[3] = {(int)((k >> 32) & 0xffff), (int)((k >> 16) & 0xffff), (int)(k & 0xffff)};
@@ -605,7 +619,6 @@ std::vector<uint64_t> Octree::collapse_parents(const std::vector<char>& flagged,
 static int tree_from_rows(int nleaves, const int* rows, Octree& t) {
     if (nleaves <= 0 || !rows) return set_error(GH_EINVAL, "plan: no leaves");
     t.leaves.clear();
-    t.interior.clear();
     for (int i = 0; i < nleaves; ++i) {
         const int L = rows[4 * i];
         if (L < 0 || L > 9) return set_error(GH_EINVAL, "plan: level out of range");
@@ -613,11 +626,10 @@ static int tree_from_rows(int nleaves, const int* rows, Octree& t) {
         for (int a = 0; a < 3; ++a)
             if (b.index[a] < 0 || b.index[a] >= (1 << L)) return set_error(GH_EINVAL, "plan: index out of range");
         t.leaves.push_back(b);
-        for (int l = L - 1; l >= 0; --l)
-            t.interior.insert(Octree::key(l, b.index[0] >> (L - l), b.index[1] >> (L - l), b.index[2] >> (L - l)));
     }
     std::vector<BlockRec> given = t.leaves;
     t.rebuild_index();
+    if (t.nested) return set_error(GH_EINVAL, "plan: a leaf lies inside another leaf");
     for (int i = 0; i < nleaves; ++i)
         if (Octree::key(given[i]) != Octree::key(t.leaves[i]))
             return set_error(GH_EINVAL, "plan: leaves must be in canonical (level, z, y, x) order");

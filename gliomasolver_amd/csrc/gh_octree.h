@@ -12,7 +12,6 @@
 #include <map>
 #include <set>
 #include <unordered_map>
-#include <unordered_set>
 #include <vector>
 
 #include "gh_internal.h"
@@ -37,8 +36,13 @@ struct Octree {
     int order = 4;         // interpolating wavelet order: 2 or 4 (MRAGWavelets_Interp{2nd,4th}Order)
     int max_level = 4;     // Refiner_SpaceExtension maxLevel
     int min_level = 0;     // compression floor (0 = MRAG: no floor)
-    std::vector<BlockRec> leaves;           // slot order
-    std::unordered_set<uint64_t> interior;  // keys of nodes that have children
+    std::vector<BlockRec> leaves;  // slot order; the tree is these leaves (interior nodes = their ancestors)
+    // built by rebuild_index: keys of the interior nodes of levels >= SCAN_MAX + 1
+    // (ascending), which the plans visit instead of scanning those levels' maps,
+    // and whether the leaves nest (a leaf inside another, or a duplicate)
+    static constexpr int SCAN_MAX = 5;  // levels whose maps (8^5 = 32,768 entries) the plans scan
+    std::vector<uint64_t> interior_deep;
+    bool nested = false;
     // dense per-level form of the two maps for levels < dense_levels =
     // min(finest leaf level, DENSE_MAX) + 1 (built by rebuild_index): slot of a
     // leaf, -2 for a node with children, -1 for a point inside a coarser leaf
@@ -69,20 +73,18 @@ struct Octree {
     // subdivided node, or nothing (a coarser leaf covers it)
     Kind kind(int L, int x, int y, int z, int* slot = nullptr) const;
 
-    // a copy of the configuration and the node sets without the index
-    // (dense maps, hashed levels): for building a new tree that rebuild_index
-    // then indexes
+    // a copy of the configuration without leaves or index: for building a
+    // new tree that rebuild_index then indexes
     Octree shell() const {
         Octree o;
         o.order = order;
         o.max_level = max_level;
         o.min_level = min_level;
-        o.interior = interior;
         return o;
     }
 
     void init_uniform(int level);
-    void rebuild_index();  // canonical slot order (level, z, y, x) + maps; call after any change of `interior`
+    void rebuild_index();  // canonical slot order (level, z, y, x) + maps; call after any change of `leaves`
     int levels_min() const;
     int levels_max() const;
     bool is_uniform() const { return levels_min() == levels_max(); }

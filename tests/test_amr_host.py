@@ -78,6 +78,27 @@ def test_plans_on_a_grid_deeper_than_the_scanned_maps():
     assert gh.plan_compress(rows, flags, 0) == collapse_parents(rows, flags, 0)
 
 
+def test_plans_reject_nested_and_repeated_leaves():
+    """A leaf inside another leaf, or a leaf given twice, is not a tree: the
+    plans refuse it, on the dense levels and on the hashed ones (levels > 7)."""
+    from gliomasolver_amd._capi import GHError
+    rows = canonical([[1, x, y, z] for z in range(2) for y in range(2) for x in range(2)])
+    zero = lambda r: np.zeros(len(r), np.uint8)
+    cases = [
+        canonical(np.vstack([rows, [[2, 0, 0, 0]]])),          # level-2 leaf inside a level-1 leaf
+        canonical(np.vstack([rows, [[8, 1, 1, 1]]])),          # hashed leaf inside a dense leaf
+        canonical(np.vstack([rows, [[1, 0, 0, 0]]])),          # repeated dense leaf
+    ]
+    deep = canonical(np.vstack([rows[1:], [[8, 0, 0, 0], [8, 0, 0, 0]]]))  # repeated hashed leaf
+    cases.append(deep)
+    deep2 = canonical(np.vstack([rows[1:], [[8, 0, 0, 0], [9, 0, 0, 1]]]))  # hashed leaf inside a hashed leaf
+    cases.append(deep2)
+    for r in cases:
+        with pytest.raises(GHError, match="inside another leaf"):
+            gh.plan_refine(r, zero(r), 9)
+    assert gh.plan_refine(rows, zero(rows), 3) == []
+
+
 def test_space_extension_refines_coarse_neighbours_of_fine_blocks():
     """With no flags at all, a leaf next to a finer leaf is still refined
     (MRAGRefiner_SpaceExtension.h:35-51) -- except across the periodic wrap."""
