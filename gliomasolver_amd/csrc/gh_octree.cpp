@@ -46,30 +46,68 @@ void Octree::init_uniform(int level) {
     rebuild_index();
 }
 
-// canonical slot order: (level, z, y, x) ascending, by an LSD radix sort of
-// the 31-bit keys level << 27 | z << 18 | y << 9 | x (levels < 16,
-// coordinates < 512) in three passes of 11 bits
-void Octree::rebuild_index() {
-    {
-        const size_t nl = leaves.size();
-        std::vector<uint32_t> a(nl), b(nl);
-        for (size_t s = 0; s < nl; ++s) {
-            const BlockRec& r = leaves[s];
-            a[s] = (uint32_t)r.level << 27 | (uint32_t)r.index[2] << 18 | (uint32_t)r.index[1] << 9 |
-                   (uint32_t)r.index[0];
-        }
-        std::vector<uint32_t> cnt(2049);
-        for (int pass = 0; pass < 3; ++pass) {
-            const int sh = 11 * pass;
-            std::fill(cnt.begin(), cnt.end(), 0u);
-            for (uint32_t k : a) ++cnt[((k >> sh) & 2047u) + 1];
-            for (int i = 0; i < 2048; ++i) cnt[i + 1] += cnt[i];
-            for (uint32_t k : a) b[cnt[(k >> sh) & 2047u]++] = k;
-            a.swap(b);
-        }
-        for (size_t s = 0; s < nl; ++s)
-            leaves[s] = BlockRec{(int)(a[s] >> 27), {(int)(a[s] & 511u), (int)((a[s] >> 9) & 511u), (int)((a[s] >> 18) & 511u)}};
+// (level, z, y, x) ascending by an LSD radix sort of the 31-bit keys
+// level << 27 | z << 18 | y << 9 | x (levels < 16, coordinates < 512) in three
+// passes of 11 bits
+static void radix_sort_leaves(std::vector<BlockRec>& v) {
+    const size_t nl = v.size();
+    std::vector<uint32_t> a(nl), b(nl);
+    for (size_t s = 0; s < nl; ++s) {
+        const BlockRec& r = v[s];
+        a[s] = (uint32_t)r.level << 27 | (uint32_t)r.index[2] << 18 | (uint32_t)r.index[1] << 9 | (uint32_t)r.index[0];
     }
+    std::vector<uint32_t> cnt(2049);
+    for (int pass = 0; pass < 3; ++pass) {
+        const int sh = 11 * pass;
+        std::fill(cnt.begin(), cnt.end(), 0u);
+        for (uint32_t k : a) ++cnt[((k >> sh) & 2047u) + 1];
+        for (int i = 0; i < 2048; ++i) cnt[i + 1] += cnt[i];
+        for (uint32_t k : a) b[cnt[(k >> sh) & 2047u]++] = k;
+        a.swap(b);
+    }
+    for (size_t s = 0; s < nl; ++s)
+        v[s] = BlockRec{(int)(a[s] >> 27), {(int)(a[s] & 511u), (int)((a[s] >> 9) & 511u), (int)((a[s] >> 18) & 511u)}};
+}
+
+// canonical slot order: (level, z, y, x) ascending.  The leaves of levels up
+// to 7 are marked in one bit set per level (8^l bits, x fastest), read back in
+// bit order; deeper leaves are radix sorted after them.  A leaf given twice
+// leaves the whole list to the radix sort, so that it stays in the list.
+static void sort_leaves(std::vector<BlockRec>& leaves, int top) {
+    const int nb = std::min(top, 7);
+    std::vector<uint64_t> occ[8];
+    for (int l = 0; l <= nb; ++l) occ[l].assign((((size_t)1 << (3 * l)) + 63) / 64, 0);
+    std::vector<BlockRec> deep;
+    uint64_t twice = 0;
+    for (const BlockRec& b : leaves) {
+        if (b.level > nb) {
+            deep.push_back(b);
+            continue;
+        }
+        const size_t i = (size_t)b.index[0] + ((size_t)b.index[1] << b.level) + ((size_t)b.index[2] << (2 * b.level));
+        uint64_t& w = occ[b.level][i >> 6];
+        const uint64_t bit = 1ull << (i & 63);
+        twice |= w & bit;
+        w |= bit;
+    }
+    if (twice) {
+        radix_sort_leaves(leaves);
+        return;
+    }
+    radix_sort_leaves(deep);
+    size_t s = 0;
+    for (int l = 0; l <= nb; ++l) {
+        const size_t m = ((size_t)1 << l) - 1;
+        for (size_t wi = 0; wi < occ[l].size(); ++wi)
+            for (uint64_t w = occ[l][wi]; w; w &= w - 1) {
+                const size_t i = (wi << 6) + (size_t)__builtin_ctzll(w);
+                leaves[s++] = BlockRec{l, {(int)(i & m), (int)((i >> l) & m), (int)(i >> (2 * l))}};
+            }
+    }
+    for (const BlockRec& b : deep) leaves[s++] = b;
+}
+
+void Octree::rebuild_index() {
     for (int l = 0; l < MAXL; ++l) dense[l].clear();
     dense_levels = 0;
     top_level = -1;
@@ -79,7 +117,9 @@ void Octree::rebuild_index() {
     interior_deep.clear();
     nested = false;
     if (leaves.empty()) return;
-    const int top = leaves.back().level;  // the finest level sorts last
+    int top = 0;
+    for (const BlockRec& b : leaves) top = std::max(top, b.level);
+    sort_leaves(leaves, top);
     const int nd = std::min(top, DENSE_MAX);
     for (int l = 0; l <= nd; ++l) dense[l].assign((size_t)1 << (3 * l), -1);
     size_t deep_leaves = 0;  // leaves below the dense levels
@@ -568,8 +608,42 @@ std::vector<int> Octree::refinement_victims(const std::vector<char>& flagged, co
     return out;
 }
 
+// The neighbourhood test of outside_within_one for a parent P at level L,
+// restated on the cells of level l = L + 1: the 56 cells around P's children
+// (the 4x4x4 box from 2P - 1, periodic, less the children themselves) are the
+// children of P's 26 neighbours that face P, and the test fails exactly when
+// one of them is subdivided or lies in a leaf coarser than L - 1.  With
+// cov(c) = level of the leaf covering c, -1 when c is subdivided, a cell c of
+// level l is bad when cov(c) < l - 2; P's own children are leaves, never bad,
+// so P passes iff its whole box holds no bad cell.  The bad cells of each
+// level 2..6 are kept as rows of x bits, 16 row tests per parent.
 std::vector<Octree::CollapseCand> Octree::collapse_candidates() const {
     std::vector<CollapseCand> out;
+    const int lrow = std::min(dense_levels - 1, 6);
+    std::vector<uint64_t> bad_rows[7];
+    if (lrow >= 2) {
+        std::vector<int8_t> prev(1, dense[0][0] >= 0 ? 0 : -1), cov;
+        for (int l = 1; l <= lrow; ++l) {
+            const int n = 1 << l;
+            const int32_t* map = dense[l].data();
+            cov.resize((size_t)n * n * n);
+            if (l >= 2) bad_rows[l].assign((size_t)n * n, 0);
+            for (int z = 0; z < n; ++z)
+                for (int y = 0; y < n; ++y) {
+                    const size_t row = (size_t)y + ((size_t)z << l);
+                    const int8_t* up = prev.data() + ((size_t)(y >> 1) << (l - 1)) + ((size_t)(z >> 1) << (2 * (l - 1)));
+                    uint64_t bits = 0;
+                    for (int x = 0; x < n; ++x) {
+                        const int32_t v = map[(row << l) + x];
+                        const int8_t cv = v >= 0 ? (int8_t)l : v == -2 ? (int8_t)-1 : up[x >> 1];
+                        cov[(row << l) + x] = cv;
+                        bits |= (uint64_t)(cv < l - 2) << x;
+                    }
+                    if (l >= 2) bad_rows[l][row] = bits;
+                }
+            prev.swap(cov);
+        }
+    }
     // interior nodes whose 8 children are leaves -- each found once, from its
     // child of octant 0 (a leaf with even coordinates)
     for (const BlockRec& b : leaves) {
@@ -591,9 +665,21 @@ std::vector<Octree::CollapseCand> Octree::collapse_candidates() const {
                 ok = kind(L + 1, 2 * px + (o & 1), 2 * py + ((o >> 1) & 1), 2 * pz + ((o >> 2) & 1), &c.kids[o]) ==
                      LEAF;
         }
+        if (!ok) continue;
         // every neighbour of every child within one level of the parent (MRAGCompressor.h:95-107)
-        const int P[3] = {px, py, pz};
-        if (ok && outside_within_one(L, P)) out.push_back(c);
+        if (L >= 1 && L + 1 <= lrow) {
+            const int l = L + 1, m = (1 << l) - 1;
+            uint64_t win = 0;
+            for (int k = 0; k < 4; ++k) win |= 1ull << ((2 * px - 1 + k) & m);
+            const uint64_t* rows = bad_rows[l].data();
+            for (int kz = 0; kz < 4 && ok; ++kz)
+                for (int ky = 0; ky < 4 && ok; ++ky)
+                    ok = (rows[(size_t)((2 * py - 1 + ky) & m) + ((size_t)((2 * pz - 1 + kz) & m) << l)] & win) == 0;
+        } else {
+            const int P[3] = {px, py, pz};
+            ok = outside_within_one(L, P);
+        }
+        if (ok) out.push_back(c);
     }
     return out;
 }

@@ -55,6 +55,35 @@ def test_compression_plan_matches_restatement(seed, min_level):
     assert got == collapse_parents(rows, flags, min_level)
 
 
+@pytest.mark.parametrize("seed", range(4))
+def test_compression_neighbourhood_rule_on_every_parent(seed):
+    """All leaves flagged, so the plan is every parent of 8 leaves whose
+    children's neighbours lie within one level of it: the neighbourhood rule
+    alone, on grids of levels 2-5 (parents at levels 1-4, periodic wrap)."""
+    rng = np.random.default_rng(300 + seed)
+    rows = canonical([[2, x, y, z] for z in range(4) for y in range(4) for x in range(4)])
+    pts = rng.uniform(0, 1, (3, 3))
+    for L in range(2, 5):  # refine around three points down to level 5, keeping 2:1 balance
+        rows = apply_refine(rows, [k for k in range(len(rows)) if rows[k, 0] == L and
+                                   any(all(int(p[a] * (1 << L)) == rows[k, 1 + a] for a in range(3)) for p in pts)])
+        while True:
+            nb = neighbours_by_ring(rows)
+            bad = sorted({j for s in range(len(rows)) for j, _ in nb[s] if rows[s, 0] - rows[j, 0] > 1})
+            if not bad:
+                break
+            rows = apply_refine(rows, bad)
+    assert rows[:, 0].max() == 5
+    flags = np.ones(len(rows), np.uint8)
+    got = gh.plan_compress(rows, flags, 0)
+    assert got == collapse_parents(rows, flags, 0)
+    assert got  # some parents pass, and (below) some fail the rule
+    allp = {(L - 1, x >> 1, y >> 1, z >> 1) for L, x, y, z in rows.tolist() if L > 0}
+    key = {tuple(r) for r in rows.tolist()}
+    full = [p for p in allp if all((p[0] + 1, 2 * p[1] + (o & 1), 2 * p[2] + ((o >> 1) & 1), 2 * p[3] + (o >> 2))
+                                   in key for o in range(8))]
+    assert len(full) > len(got)
+
+
 def test_plans_on_a_grid_deeper_than_the_scanned_maps():
     """Levels 2-8 around one point (2:1-balanced): the plans read the small
     levels' maps directly and the deep levels (dense 6-7, hashed 8) through
