@@ -1765,12 +1765,16 @@ int fwt_details(Ctx& c, const std::vector<int>& slots, std::vector<double>& det,
 
 // Grid::refine with Refiner_SpaceExtension (MRAGrid.inl:247-355)
 int grid_refine(Ctx& c, const std::vector<char>& flagged, int* nrefined, std::vector<int>* old_slot,
-                const std::vector<char>* ext) {
+                const std::vector<char>* ext, std::vector<uint64_t>* split_keys) {
     const auto t_plan = std::chrono::steady_clock::now();
     const Octree& t = c.tree;
     std::vector<int> victims = t.refinement_victims(flagged, ext);
     if (nrefined) *nrefined = (int)victims.size();
     if (victims.empty()) return GH_OK;
+    if (split_keys) {
+        split_keys->resize(victims.size());
+        for (size_t j = 0; j < victims.size(); ++j) (*split_keys)[j] = Octree::key(t.leaves[victims[j]]);
+    }
     Octree nt = t.shell();
     nt.leaves.reserve(t.leaves.size() + 7 * victims.size());
     std::vector<char> is_victim(t.leaves.size(), 0);
@@ -1942,62 +1946,6 @@ int gh_fwt_details(gh_ctx* ctx, double* detail) {
     return GH_OK;
 }
 
-// Science::AutomaticRefinement<0,0>(grid, fwt, tol, maxLevel, loops) (MRAGScienceCore.h:42-108)
-int gh_refine(gh_ctx* ctx, double tol, int max_loops, int* nrefined) {
-    Ctx* c = as_ctx(ctx);
-    if (!c) return set_error(GH_EINVAL, "null context");
-    GH_HIP_CHECK(hipSetDevice(c->device));
-    // blocks found nice (detail < tol) stay nice while they exist: a flag per
-    // slot, carried to the surviving leaves' new slots after each refinement
-    std::vector<char> nice(c->topo.nblocks(), 0), carried;
-    std::vector<int> old_slot;
-    int loops = 0, total = 0;
-    while (true) {
-        std::vector<int> cand;
-        for (int s = 0; s < c->topo.nblocks(); ++s) {
-            const BlockRec& b = c->tree.leaves[s];
-            if (c->tree.max_level >= 0 && c->tree.max_level <= b.level) continue;
-            if (!nice[s]) cand.push_back(s);
-        }
-        if (cand.empty()) break;
-        std::vector<double> det;
-        std::vector<char> ext;  // the space extension, found while the device evaluates the details
-        int rc;
-        {
-            ProfScope fwt(c->prof[P_REF_FWT]);
-            if ((rc = fwt_details(*c, cand, det, [&] { ext = c->tree.space_extension(); }))) return rc;
-        }
-        std::vector<char> flagged(c->topo.nblocks(), 0);
-        int nflag = 0;
-        for (size_t i = 0; i < cand.size(); ++i) {
-            if (det[i] >= tol) {
-                flagged[cand[i]] = 1;
-                ++nflag;
-            } else {
-                nice[cand[i]] = 1;
-            }
-        }
-        if (nflag == 0) break;
-        int nr = 0;
-        old_slot.clear();
-        {
-            ProfScope ref(c->prof[P_REF]);
-            if ((rc = grid_refine(*c, flagged, &nr, &old_slot, &ext))) return rc;
-        }
-        if (!old_slot.empty()) {
-            carried.assign(old_slot.size(), 0);
-            for (size_t s = 0; s < old_slot.size(); ++s)
-                if (old_slot[s] >= 0) carried[s] = nice[old_slot[s]];
-            nice.swap(carried);
-        }
-        total += nr;
-        ++loops;
-        if (max_loops > 0 && loops >= max_loops) break;
-    }
-    if (nrefined) *nrefined = total;
-    return GH_OK;
-}
-
 // Which blocks a set of changed boxes can reach.  A block's lab -- hence its
 // FWT details -- reads the leaves within its own box widened by two of its
 // block widths on every side (periodically): the lab spans [-2, 11) cells of
@@ -2005,8 +1953,8 @@ int gh_refine(gh_ctx* ctx, double tol, int max_loops, int* nrefined) {
 // synthesis levels of 3 and 6 of those cells (order 4; less for order 2), 12 <
 // 16 cells in all; a finer region answers with the coincident point.  A
 // compression changes the tree and the data only inside the collapsed
-// parents' boxes, so a block whose widened box misses all of them keeps its
-// lab bit for bit.  The boxes are rasterized at level R = min(finest, 7) with
+// parents' boxes, a refinement only inside the split blocks' boxes, so a
+// block whose widened box misses all of them keeps its lab bit for bit.  The boxes are rasterized at level R = min(finest, 7) with
 // a summed-volume table; blocks finer than R are tested conservatively.
 class ChangedBoxes {
   public:
@@ -2086,6 +2034,98 @@ class ChangedBoxes {
     }
 };
 
+// Science::AutomaticRefinement<0,0>(grid, fwt, tol, maxLevel, loops) (MRAGScienceCore.h:42-108)
+int gh_refine(gh_ctx* ctx, double tol, int max_loops, int* nrefined) {
+    Ctx* c = as_ctx(ctx);
+    if (!c) return set_error(GH_EINVAL, "null context");
+    GH_HIP_CHECK(hipSetDevice(c->device));
+    // blocks found nice (detail < tol) stay nice while they exist: a flag per
+    // slot, carried to the surviving leaves' new slots after each refinement
+    std::vector<char> nice(c->topo.nblocks(), 0), carried;
+    std::vector<int> old_slot;
+    // the details of the last evaluation, per slot of the current tree, where
+    // still valid (for the compression that may follow: see Ctx::carry_call)
+    c->carry_known.clear();
+    c->carry_det.clear();
+    std::vector<char> known;
+    std::vector<double> kdet;
+    int loops = 0, total = 0;
+    while (true) {
+        std::vector<int> cand;
+        for (int s = 0; s < c->topo.nblocks(); ++s) {
+            const BlockRec& b = c->tree.leaves[s];
+            if (c->tree.max_level >= 0 && c->tree.max_level <= b.level) continue;
+            if (!nice[s]) cand.push_back(s);
+        }
+        if (cand.empty()) break;
+        std::vector<double> det;
+        std::vector<char> ext;  // the space extension, found while the device evaluates the details
+        int rc;
+        bool host_tables = false;
+        {
+            ProfScope fwt(c->prof[P_REF_FWT]);
+            const int64_t host_before = c->adapt_count[1];
+            if ((rc = fwt_details(*c, cand, det, [&] { ext = c->tree.space_extension(); }))) return rc;
+            host_tables = c->adapt_count[1] != host_before;
+        }
+        known.assign(c->topo.nblocks(), 0);
+        kdet.assign(c->topo.nblocks(), 0.0);
+        if (!host_tables)  // (host-built labs are not bounded by the two-block reach)
+            for (size_t i = 0; i < cand.size(); ++i) {
+                known[cand[i]] = 1;
+                kdet[cand[i]] = det[i];
+            }
+        std::vector<char> flagged(c->topo.nblocks(), 0);
+        int nflag = 0;
+        for (size_t i = 0; i < cand.size(); ++i) {
+            if (det[i] >= tol) {
+                flagged[cand[i]] = 1;
+                ++nflag;
+            } else {
+                nice[cand[i]] = 1;
+            }
+        }
+        if (nflag == 0) break;
+        int nr = 0;
+        old_slot.clear();
+        std::vector<uint64_t> split;
+        const int finest = c->tree.top_level;
+        {
+            ProfScope ref(c->prof[P_REF]);
+            if ((rc = grid_refine(*c, flagged, &nr, &old_slot, &ext, &split))) return rc;
+        }
+        if (!old_slot.empty()) {
+            carried.assign(old_slot.size(), 0);
+            for (size_t s = 0; s < old_slot.size(); ++s)
+                if (old_slot[s] >= 0) carried[s] = nice[old_slot[s]];
+            nice.swap(carried);
+            // details the split boxes cannot reach stay valid on the new tree
+            const ChangedBoxes changed(finest, split);
+            std::vector<char> known2(old_slot.size(), 0);
+            std::vector<double> kdet2(old_slot.size(), 0.0);
+            for (size_t s = 0; s < old_slot.size(); ++s) {
+                const int o = old_slot[s];
+                if (o >= 0 && known[o] && !changed.reaches(c->tree.leaves[s])) {
+                    known2[s] = 1;
+                    kdet2[s] = kdet[o];
+                }
+            }
+            known.swap(known2);
+            kdet.swap(kdet2);
+        }
+        total += nr;
+        ++loops;
+        if (max_loops > 0 && loops >= max_loops) break;
+    }
+    if (nrefined) *nrefined = total;
+    if (known.size() == (size_t)c->topo.nblocks()) {
+        c->carry_known.swap(known);
+        c->carry_det.swap(kdet);
+        c->carry_call = c->calls;
+    }
+    return GH_OK;
+}
+
 // Science::AutomaticCompression<0,0>(grid, fwt, tol, loops) (MRAGScienceCore.h:117-177).
 // Every loop needs the details of every block not yet found nice; a block
 // whose lab no collapse of the previous loop could reach (ChangedBoxes) keeps
@@ -2100,6 +2140,14 @@ int gh_compress(gh_ctx* ctx, double tol, int max_loops, int* ncollapsed) {
     // both carried to the surviving leaves' new slots after each compression
     std::vector<char> nice(c->topo.nblocks(), 0), known(c->topo.nblocks(), 0), nice2, known2;
     std::vector<double> kdet(c->topo.nblocks(), 0.0), kdet2;
+    // a refinement called just before (no other call on this context in
+    // between) leaves the details still valid on this tree and data
+    if (c->carry_call + 1 == c->calls && c->carry_known.size() == known.size()) {
+        known.swap(c->carry_known);
+        kdet.swap(c->carry_det);
+    }
+    c->carry_known.clear();
+    c->carry_det.clear();
     std::vector<int> old_slot;
     int loops = 0, total = 0;
     while (true) {

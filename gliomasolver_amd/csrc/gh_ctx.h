@@ -122,6 +122,14 @@ struct Ctx {
     // how ghosts were built (gh_adapt_counters): lab batches on the device,
     // lab batches from host tables, face tables on the device, from the host
     int64_t adapt_count[4] = {0};
+    // FWT details (a flag and the value per slot) that gh_refine leaves valid
+    // on the tree and data it ends with, for a gh_compress called right after
+    // it: `calls` counts entry-point calls on this context (as_ctx), and the
+    // details are used only when the compression is the next call after the
+    // refinement's (carry_call), so that nothing can have changed them
+    uint64_t calls = 0, carry_call = 0;
+    std::vector<char> carry_known;
+    std::vector<double> carry_det;
 
     DevBuf phi[2];      // ping-pong tumor density
     int cur = 0;        // phi[cur] holds the current state
@@ -195,9 +203,10 @@ int fwt_details(Ctx& c, const std::vector<int>& slots, std::vector<double>& det,
                 const std::function<void()>& meanwhile = {});
 // old_slot (may be null) receives, per new slot, the old slot of the same
 // leaf or -1 for a new one (left untouched when nothing changes); ext (may be
-// null) is the current tree's space_extension()
+// null) is the current tree's space_extension(); split_keys (may be null)
+// receives the keys of the split leaves
 int grid_refine(Ctx& c, const std::vector<char>& flagged, int* nrefined, std::vector<int>* old_slot = nullptr,
-                const std::vector<char>* ext = nullptr);
+                const std::vector<char>* ext = nullptr, std::vector<uint64_t>* split_keys = nullptr);
 // parents (may be null) receives the keys of the new parent leaves; cands
 // (may be null) is the current tree's collapse_candidates()
 int grid_compress(Ctx& c, const std::vector<char>& flagged, int* ncollapsed,

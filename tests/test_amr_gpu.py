@@ -552,6 +552,41 @@ def test_gpu_adaptive_levels_3_to_5_full_size_properties():
     assert refined > 0 and collapsed > 0
 
 
+def test_gpu_compression_reuses_refinement_details_bit_exactly():
+    """A compression called right after a refinement starts from the
+    refinement's details where no split block can reach the lab; with any
+    other call in between it evaluates every block.  Twin contexts on the
+    256^3 levels 3-5 configuration, 40 epochs: same grids, same bits."""
+    n = 256
+    gm, wm, csf, sv = gh.phantom(n, 3)
+    Dw, Dg, dt, _ = gh.schedule("f32", 0.013, 17.6, 500.0, n, 0.9)
+    ctxs = []
+    for _ in range(2):
+        ctx = gh.Context(0, "f32")
+        ctx.grid_configure(4, 5, 3)
+        ctx.grid_init_uniform(n // 8)
+        ctx.set_anatomy(gm, wm, csf)
+        ctx.set_field(FIELD_PHI, gh.gaussian_ic("f32", ctx.get_field(FIELD_PW), ctx.get_field(FIELD_PG), sv, 1.0, 0.1))
+        ctx.rd_prepare(Dw, Dg, 0.025)
+        ctx.compress(1e-6, -1)
+        ctxs.append(ctx)
+    del gm, wm, csf
+    a, b = ctxs
+    for epoch in range(40):
+        na = (a.rd_steps(10, dt), a.refine(1e-4, 1), a.compress(1e-6, -1))[1:]
+        b.rd_steps(10, dt)
+        nr = b.refine(1e-4, 1)
+        b.nblocks()  # any call in between: the compression evaluates every block
+        nb = (nr, b.compress(1e-6, -1))
+        assert na == nb, epoch
+        if (epoch + 1) % 10 == 0:
+            say("epoch", epoch + 1, "blocks", a.nblocks(), "refined/collapsed", na)
+            assert np.array_equal(rows_of(a), rows_of(b)), epoch
+            assert bits_equal(a.get_field_blocks(FIELD_PHI), b.get_field_blocks(FIELD_PHI)), epoch
+    a.close()
+    b.close()
+
+
 @pytest.mark.parametrize("real", ["f32", "f64"])
 def test_gpu_refine_and_compress_caller_plans_equal_automatic(real):
     """Grid::refine(set) / Grid::compress(set, int&) with the sets that
