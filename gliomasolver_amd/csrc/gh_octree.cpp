@@ -544,21 +544,35 @@ std::vector<char> Octree::space_extension() const {
             if (kind(L, P[0], P[1], P[2], &slot) == LEAF) flag[slot] = 1;
         }
     };
+    // a scanned level as rows of x bits: the subdivided nodes, widened by one
+    // along x, then along y and z (no wrap), meet the rows of leaves
     const int nscan = std::min(dense_levels, SCAN_MAX + 1);
-    for (int L = 0; L < nscan; ++L) {
+    std::vector<uint64_t> sub, leaf;
+    for (int L = 1; L < nscan && L < top_level; ++L) {  // (1 node on level 0; none subdivided on the finest)
         const int n = 1 << L;
+        const uint64_t full = n == 64 ? ~0ull : (1ull << n) - 1;
         const int32_t* map = dense[L].data();
+        sub.assign((size_t)n * n, 0);
+        leaf.assign((size_t)n * n, 0);
+        for (size_t r = 0; r < (size_t)n * n; ++r) {
+            uint64_t sb = 0, lb = 0;
+            for (int x = 0; x < n; ++x) {
+                const int32_t v = map[(r << L) + x];
+                sb |= (uint64_t)(v == -2) << x;
+                lb |= (uint64_t)(v >= 0) << x;
+            }
+            sub[r] = (sb | sb << 1 | sb >> 1) & full;
+            leaf[r] = lb;
+        }
         for (int z = 0; z < n; ++z)
-            for (int y = 0; y < n; ++y)
-                for (int x = 0; x < n; ++x) {
-                    if (map[(size_t)x + ((size_t)y << L) + ((size_t)z << (2 * L))] != -2) continue;
-                    for (int zz = std::max(z - 1, 0); zz <= std::min(z + 1, n - 1); ++zz)
-                        for (int yy = std::max(y - 1, 0); yy <= std::min(y + 1, n - 1); ++yy)
-                            for (int xx = std::max(x - 1, 0); xx <= std::min(x + 1, n - 1); ++xx) {
-                                const int32_t v = map[(size_t)xx + ((size_t)yy << L) + ((size_t)zz << (2 * L))];
-                                if (v >= 0) flag[v] = 1;  // the node itself is -2, never a leaf
-                            }
-                }
+            for (int y = 0; y < n; ++y) {
+                uint64_t near = 0;
+                for (int zz = std::max(z - 1, 0); zz <= std::min(z + 1, n - 1); ++zz)
+                    for (int yy = std::max(y - 1, 0); yy <= std::min(y + 1, n - 1); ++yy)
+                        near |= sub[(size_t)yy + ((size_t)zz << L)];
+                const size_t r = (size_t)y + ((size_t)z << L);
+                for (uint64_t m = near & leaf[r]; m; m &= m - 1) flag[map[(r << L) + __builtin_ctzll(m)]] = 1;
+            }
     }
     for (uint64_t k : interior_deep) {  // the levels not scanned (interior_deep holds levels >= SCAN_MAX + 1)
         const int L = (int)(k >> 48);
@@ -588,10 +602,37 @@ std::vector<int> Octree::refinement_victims(const std::vector<char>& flagged, co
             victim[s] = 1;
             front.push_back(s);
         }
+    // a leaf of level L (1 <= L <= 6, L + 3 > top_level: no leaf 3 levels
+    // finer) has targets only where a cell of level L next to it lies inside
+    // a coarser leaf (dense map -1): rows of x bits of such cells answer that
+    // for most leaves without the walk
+    std::vector<uint64_t> coarse[7];
+    auto no_targets = [&](int v) {
+        const BlockRec& b = leaves[v];
+        const int L = b.level;
+        if (L < 1 || L > 6 || L >= dense_levels || L + 3 <= top_level) return false;
+        const int n = 1 << L, m = n - 1;
+        std::vector<uint64_t>& rows = coarse[L];
+        if (rows.empty()) {
+            rows.assign((size_t)n * n, 0);
+            const int32_t* map = dense[L].data();
+            for (size_t r = 0; r < (size_t)n * n; ++r) {
+                uint64_t bits = 0;
+                for (int x = 0; x < n; ++x) bits |= (uint64_t)(map[(r << L) + x] == -1) << x;
+                rows[r] = bits;
+            }
+        }
+        const uint64_t win = 1ull << ((b.index[0] - 1) & m) | 1ull << b.index[0] | 1ull << ((b.index[0] + 1) & m);
+        for (int dz = -1; dz <= 1; ++dz)
+            for (int dy = -1; dy <= 1; ++dy)
+                if (rows[(size_t)((b.index[1] + dy) & m) + ((size_t)((b.index[2] + dz) & m) << L)] & win) return false;
+        return true;
+    };
     std::vector<int> tg;
     while (!front.empty()) {
         next.clear();
         for (int v : front) {
+            if (no_targets(v)) continue;
             tg.clear();
             chain_targets(v, tg);
             for (int n : tg)

@@ -128,6 +128,29 @@ def test_plans_reject_nested_and_repeated_leaves():
     assert gh.plan_refine(rows, zero(rows), 3) == []
 
 
+@pytest.mark.parametrize("axis", [0, 1, 2])
+@pytest.mark.parametrize("side", ["low", "high"])
+def test_chain_reaction_across_the_periodic_wrap(axis, side):
+    """Half of a level-2 grid refined to level 3 along one axis; one level-3
+    block on the periodic boundary is flagged.  Its only coarser neighbour
+    lies across the wrap, where the space extension does not look, so only
+    the 2:1 chain reaction (MRAGRefiner.h:65-152) refines it."""
+    rows = canonical([[2, x, y, z] for z in range(4) for y in range(4) for x in range(4)])
+    half = {0, 1} if side == "low" else {2, 3}
+    rows = apply_refine(rows, [k for k in range(len(rows)) if rows[k, 1 + axis] in half])
+    idx = [3, 3, 3] if side == "low" else [4, 4, 4]
+    idx[axis] = 0 if side == "low" else 7
+    v = next(k for k in range(len(rows)) if rows[k, 0] == 3 and list(rows[k, 1:]) == idx)
+    flags = np.zeros(len(rows), np.uint8)
+    flags[v] = 1
+    got = gh.plan_refine(rows, flags, 5)
+    assert got == refine_victims(rows, flags, 5)
+    across = [i // 2 for i in idx]
+    across[axis] = 3 if side == "low" else 0
+    w = next(k for k in range(len(rows)) if rows[k, 0] == 2 and list(rows[k, 1:]) == across)
+    assert v in got and w in got
+
+
 def test_space_extension_refines_coarse_neighbours_of_fine_blocks():
     """With no flags at all, a leaf next to a finer leaf is still refined
     (MRAGRefiner_SpaceExtension.h:35-51) -- except across the periodic wrap."""
