@@ -1796,25 +1796,33 @@ int grid_refine(Ctx& c, const std::vector<char>& flagged, int* nrefined, std::ve
     int rc;
     if ((rc = stage_wait(c)) || (rc = c.pinRemap.ensure_grow(nrm * sizeof(Remap)))) return rc;
     Remap* rm = c.pinRemap.as<Remap>();
+    // the leaves not split keep their (canonical) order: new slots and old
+    // slots not split pair up in one pass; the rest are children
+    int j = 0;
+    const int nold = (int)t.leaves.size();
+    auto same = [](const BlockRec& a, const BlockRec& b) {
+        return a.level == b.level && a.index[0] == b.index[0] && a.index[1] == b.index[1] && a.index[2] == b.index[2];
+    };
     for (int s = 0; s < (int)nrm; ++s) {
         const BlockRec& nb = nt.leaves[s];
+        while (j < nold && is_victim[j]) ++j;
         Remap r{};
-        int os = -1;
-        if (t.kind(nb.level, nb.index[0], nb.index[1], nb.index[2], &os) == Octree::LEAF) {
+        if (j < nold && same(nb, t.leaves[j])) {
             r.kind = 0;
-            r.src = os;
+            r.src = j++;
         } else {  // a child of a split leaf: the parent is the old leaf one level up
-            const BlockRec& ch = nt.leaves[s];
             int ps = -1;
-            if (ch.level == 0 || t.kind(ch.level - 1, ch.index[0] >> 1, ch.index[1] >> 1, ch.index[2] >> 1, &ps) !=
+            if (nb.level == 0 || t.kind(nb.level - 1, nb.index[0] >> 1, nb.index[1] >> 1, nb.index[2] >> 1, &ps) !=
                                      Octree::LEAF || victim_index[ps] < 0)
                 return set_error(GH_EINVAL, "grid_refine: inconsistent plan");
             r.kind = 1;
             r.src = victim_index[ps];
-            r.octant = (ch.index[0] & 1) | (ch.index[1] & 1) << 1 | (ch.index[2] & 1) << 2;
+            r.octant = (nb.index[0] & 1) | (nb.index[1] & 1) << 1 | (nb.index[2] & 1) << 2;
         }
         rm[s] = r;
     }
+    while (j < nold && is_victim[j]) ++j;
+    if (j != nold) return set_error(GH_EINVAL, "grid_refine: inconsistent plan");
     if (old_slot) {
         old_slot->resize(nrm);
         for (size_t s = 0; s < nrm; ++s) (*old_slot)[s] = rm[s].kind == 0 ? rm[s].src : -1;
@@ -1829,26 +1837,16 @@ int grid_compress(Ctx& c, const std::vector<char>& flagged, int* ncollapsed, std
                   std::vector<int>* old_slot, const std::vector<Octree::CollapseCand>* cands) {
     const auto t_plan = std::chrono::steady_clock::now();
     const Octree& t = c.tree;
-    std::vector<uint64_t> parents = t.collapse_parents(flagged, cands);
+    const std::vector<Octree::CollapseCand> plan = t.collapse_plan(flagged, cands);  // ascending keys
+    std::vector<uint64_t> parents(plan.size());
+    for (size_t i = 0; i < plan.size(); ++i) parents[i] = plan[i].key;
     if (ncollapsed) *ncollapsed = (int)parents.size();
     if (parents_out) *parents_out = parents;
     if (parents.empty()) return GH_OK;
     Octree nt = t.shell();
-    std::vector<std::array<int, 8>> kids(parents.size());  // per parent, in the (ascending) order of the keys
     std::vector<char> gone(t.leaves.size(), 0);
-    for (size_t i = 0; i < parents.size(); ++i) {
-        const uint64_t pk = parents[i];
-        const int L = (int)(pk >> 48);
-        const int px = (int)((pk >> 32) & 0xffff), py = (int)((pk >> 16) & 0xffff), pz = (int)(pk & 0xffff);
-        std::array<int, 8> k{};
-        for (int o = 0; o < 8; ++o) {
-            int slot = -1;
-            t.kind(L + 1, 2 * px + (o & 1), 2 * py + ((o >> 1) & 1), 2 * pz + ((o >> 2) & 1), &slot);
-            k[o] = slot;
-            gone[slot] = 1;
-        }
-        kids[i] = k;
-    }
+    for (const Octree::CollapseCand& p : plan)
+        for (int o = 0; o < 8; ++o) gone[p.kids[o]] = 1;
     nt.leaves.reserve(t.leaves.size() - 7 * parents.size());
     for (int s = 0; s < (int)t.leaves.size(); ++s)
         if (!gone[s]) nt.leaves.push_back(t.leaves[s]);
@@ -1860,23 +1858,32 @@ int grid_compress(Ctx& c, const std::vector<char>& flagged, int* ncollapsed, std
     int rc;
     if ((rc = stage_wait(c)) || (rc = c.pinRemap.ensure_grow(nrm * sizeof(Remap)))) return rc;
     Remap* rm = c.pinRemap.as<Remap>();
+    // the surviving leaves keep their (canonical) order: new slots and old
+    // slots not collapsed pair up in one pass; the rest are the parents
+    int j = 0;
+    const int nold = (int)t.leaves.size();
+    auto same = [](const BlockRec& a, const BlockRec& b) {
+        return a.level == b.level && a.index[0] == b.index[0] && a.index[1] == b.index[1] && a.index[2] == b.index[2];
+    };
     for (int s = 0; s < (int)nrm; ++s) {
         const BlockRec& nb = nt.leaves[s];
-        const uint64_t k = Octree::key(nb);
+        while (j < nold && gone[j]) ++j;
         Remap r{};
-        int os = -1;
-        if (t.kind(nb.level, nb.index[0], nb.index[1], nb.index[2], &os) == Octree::LEAF) {
+        if (j < nold && same(nb, t.leaves[j])) {
             r.kind = 0;
-            r.src = os;
+            r.src = j++;
         } else {
             r.kind = 2;
+            const uint64_t k = Octree::key(nb);
             const auto it = std::lower_bound(parents.begin(), parents.end(), k);
             if (it == parents.end() || *it != k) return set_error(GH_EINVAL, "grid_compress: inconsistent plan");
-            const auto& kk = kids[it - parents.begin()];
-            for (int o = 0; o < 8; ++o) r.kids[o] = kk[o];
+            const Octree::CollapseCand& p = plan[it - parents.begin()];
+            for (int o = 0; o < 8; ++o) r.kids[o] = p.kids[o];
         }
         rm[s] = r;
     }
+    while (j < nold && gone[j]) ++j;
+    if (j != nold) return set_error(GH_EINVAL, "grid_compress: inconsistent plan");
     if (old_slot) {
         old_slot->resize(nrm);
         for (size_t s = 0; s < nrm; ++s) (*old_slot)[s] = rm[s].kind == 0 ? rm[s].src : -1;

@@ -70,12 +70,12 @@ static void radix_sort_leaves(std::vector<BlockRec>& v) {
 }
 
 // canonical slot order: (level, z, y, x) ascending.  The leaves of levels up
-// to 7 are marked in one bit set per level (8^l bits, x fastest), read back in
-// bit order; deeper leaves are radix sorted after them.  A leaf given twice
-// leaves the whole list to the radix sort, so that it stays in the list.
-static void sort_leaves(std::vector<BlockRec>& leaves, int top) {
+// to 7 are marked in one bit set per level (occ[l]: 8^l bits, x fastest), read
+// back in bit order; deeper leaves are radix sorted after them.  A leaf given
+// twice leaves the whole list to the radix sort, so that it stays in the list,
+// and the result is false.
+static bool sort_leaves(std::vector<BlockRec>& leaves, int top, std::vector<uint64_t> (&occ)[8]) {
     const int nb = std::min(top, 7);
-    std::vector<uint64_t> occ[8];
     for (int l = 0; l <= nb; ++l) occ[l].assign((((size_t)1 << (3 * l)) + 63) / 64, 0);
     std::vector<BlockRec> deep;
     uint64_t twice = 0;
@@ -92,7 +92,7 @@ static void sort_leaves(std::vector<BlockRec>& leaves, int top) {
     }
     if (twice) {
         radix_sort_leaves(leaves);
-        return;
+        return false;
     }
     radix_sort_leaves(deep);
     size_t s = 0;
@@ -105,6 +105,17 @@ static void sort_leaves(std::vector<BlockRec>& leaves, int top) {
             }
     }
     for (const BlockRec& b : deep) leaves[s++] = b;
+    return true;
+}
+
+// bit k of the result = bit 2k of v
+static inline uint64_t even_bits(uint64_t v) {
+    v &= 0x5555555555555555ull;
+    v = (v | (v >> 1)) & 0x3333333333333333ull;
+    v = (v | (v >> 2)) & 0x0f0f0f0f0f0f0f0full;
+    v = (v | (v >> 4)) & 0x00ff00ff00ff00ffull;
+    v = (v | (v >> 8)) & 0x0000ffff0000ffffull;
+    return (v | (v >> 16)) & 0x00000000ffffffffull;
 }
 
 void Octree::rebuild_index() {
@@ -119,7 +130,8 @@ void Octree::rebuild_index() {
     if (leaves.empty()) return;
     int top = 0;
     for (const BlockRec& b : leaves) top = std::max(top, b.level);
-    sort_leaves(leaves, top);
+    std::vector<uint64_t> occ[8];
+    const bool marked = sort_leaves(leaves, top, occ);
     const int nd = std::min(top, DENSE_MAX);
     for (int l = 0; l <= nd; ++l) dense[l].assign((size_t)1 << (3 * l), -1);
     size_t deep_leaves = 0;  // leaves below the dense levels
@@ -134,24 +146,55 @@ void Octree::rebuild_index() {
         if (e != -1) nested = true;  // the same leaf twice
         e = (int32_t)s;
     }
-    // the interior nodes are the strict ancestors of the leaves: marked -2 by a
-    // walk up from each leaf that stops at a node marked before (its own
-    // ancestors are marked already); reaching a leaf means nesting.  Nodes of
-    // levels >= SCAN_MAX + 1 (and all of the hashed levels) are listed.
+    // the interior nodes are the strict ancestors of the leaves, marked -2; an
+    // interior node that is also a leaf means nesting.  Nodes of levels >=
+    // SCAN_MAX + 1 (and all of the hashed levels) are listed.
     std::vector<uint64_t>& deep = interior_deep;
-    for (const BlockRec& b : leaves) {
-        for (int l = b.level - 1; l >= 0; --l) {
-            const int sh = b.level - l;
-            const int x = b.index[0] >> sh, y = b.index[1] >> sh, z = b.index[2] >> sh;
-            if (l > nd) {
-                deep.push_back(key(l, x, y, z));
-                continue;
+    if (marked && top <= 6) {
+        // level by level from the finest up, on rows of x bits (n <= 64): a
+        // node is interior iff one of its 8 children is a leaf or interior
+        // (no level here is listed: all are <= SCAN_MAX)
+        auto occ_row = [&](int l, size_t r) -> uint64_t {
+            if (l == 6) return occ[l][r];
+            const size_t b = r << l;
+            return (occ[l][b >> 6] >> (b & 63)) & ((1ull << (1 << l)) - 1);
+        };
+        std::vector<uint64_t> below((size_t)1 << (2 * top)), cur;  // rows of level l + 1: leaves and interior nodes
+        for (size_t r = 0; r < below.size(); ++r) below[r] = occ_row(top, r);
+        for (int l = top - 1; l >= 0; --l) {
+            const int n = 1 << l;
+            const size_t dz = (size_t)1 << (l + 1);
+            cur.assign((size_t)n * n, 0);
+            for (int z = 0; z < n; ++z)
+                for (int y = 0; y < n; ++y) {
+                    const size_t r0 = (size_t)(2 * y) + (size_t)(2 * z) * dz;
+                    uint64_t v = below[r0] | below[r0 + 1] | below[r0 + dz] | below[r0 + dz + 1];
+                    v = even_bits(v | v >> 1);
+                    const size_t r = (size_t)y + ((size_t)z << l);
+                    for (uint64_t m = v; m; m &= m - 1) {
+                        int32_t& e = dense[l][(r << l) + (size_t)__builtin_ctzll(m)];
+                        if (e >= 0) nested = true;  // a leaf containing another leaf
+                        e = -2;
+                    }
+                    cur[r] = v | occ_row(l, r);
+                }
+            below.swap(cur);
+        }
+    } else {  // a walk up from each leaf that stops at a node marked before
+        for (const BlockRec& b : leaves) {
+            for (int l = b.level - 1; l >= 0; --l) {
+                const int sh = b.level - l;
+                const int x = b.index[0] >> sh, y = b.index[1] >> sh, z = b.index[2] >> sh;
+                if (l > nd) {
+                    deep.push_back(key(l, x, y, z));
+                    continue;
+                }
+                int32_t& e = dense[l][(size_t)x + ((size_t)y << l) + ((size_t)z << (2 * l))];
+                if (e == -2) break;
+                if (e >= 0) nested = true;  // a leaf containing another leaf
+                e = -2;
+                if (l > SCAN_MAX) deep.push_back(key(l, x, y, z));
             }
-            int32_t& e = dense[l][(size_t)x + ((size_t)y << l) + ((size_t)z << (2 * l))];
-            if (e == -2) break;
-            if (e >= 0) nested = true;  // a leaf containing another leaf
-            e = -2;
-            if (l > SCAN_MAX) deep.push_back(key(l, x, y, z));
         }
     }
     std::sort(deep.begin(), deep.end());
@@ -725,20 +768,28 @@ std::vector<Octree::CollapseCand> Octree::collapse_candidates() const {
     return out;
 }
 
-std::vector<uint64_t> Octree::collapse_parents(const std::vector<char>& flagged,
-                                               const std::vector<CollapseCand>* cands) const {
+std::vector<Octree::CollapseCand> Octree::collapse_plan(const std::vector<char>& flagged,
+                                                        const std::vector<CollapseCand>* cands) const {
     std::vector<CollapseCand> own;
     if (!cands) {
         own = collapse_candidates();
         cands = &own;
     }
-    std::vector<uint64_t> out;
+    std::vector<CollapseCand> out;
     for (const CollapseCand& c : *cands) {
         bool ok = true;
         for (int o = 0; o < 8 && ok; ++o) ok = c.kids[o] < (int)flagged.size() && flagged[c.kids[o]];
-        if (ok) out.push_back(c.key);
+        if (ok) out.push_back(c);
     }
-    std::sort(out.begin(), out.end());  // ascending keys
+    std::sort(out.begin(), out.end(), [](const CollapseCand& a, const CollapseCand& b) { return a.key < b.key; });
+    return out;
+}
+
+std::vector<uint64_t> Octree::collapse_parents(const std::vector<char>& flagged,
+                                               const std::vector<CollapseCand>* cands) const {
+    std::vector<CollapseCand> plan = collapse_plan(flagged, cands);
+    std::vector<uint64_t> out(plan.size());
+    for (size_t i = 0; i < plan.size(); ++i) out[i] = plan[i].key;
     return out;
 }
 

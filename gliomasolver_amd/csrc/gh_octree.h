@@ -130,6 +130,9 @@ struct Octree {
         int kids[8];   // slots of its children by octant
     };
     std::vector<CollapseCand> collapse_candidates() const;
+    // the chosen parents with their children, ascending keys
+    std::vector<CollapseCand> collapse_plan(const std::vector<char>& flagged,
+                                            const std::vector<CollapseCand>* cands = nullptr) const;
     std::vector<uint64_t> collapse_parents(const std::vector<char>& flagged,
                                            const std::vector<CollapseCand>* cands = nullptr) const;
 };
