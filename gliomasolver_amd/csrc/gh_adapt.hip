@@ -1961,26 +1961,24 @@ int gh_fwt_details(gh_ctx* ctx, double* detail) {
 // 16 cells in all; a finer region answers with the coincident point.  A
 // compression changes the tree and the data only inside the collapsed
 // parents' boxes, a refinement only inside the split blocks' boxes, so a
-// block whose widened box misses all of them keeps its lab bit for bit.  The boxes are rasterized at level R = min(finest, 7) with
-// a summed-volume table; blocks finer than R are tested conservatively.
+// block whose widened box misses all of them keeps its lab bit for bit.
+// The boxes are rasterized at level R = min(finest, 7).  Up to R = 6 as rows
+// of x bits: the changed cells of level R are OR-reduced to every coarser
+// level L and widened there by two cells each way (periodic), so that a block
+// of level L is one bit test.  At R = 7 with a summed-volume table, blocks
+// finer than R tested conservatively.
 class ChangedBoxes {
   public:
     ChangedBoxes(int finest, const std::vector<uint64_t>& boxes) : R_(std::min(finest, 7)), n_(1 << R_) {
+        if (R_ <= 6) {
+            build_rows(boxes);
+            return;
+        }
         const size_t m = (size_t)(n_ + 1);
         sum_.assign(m * m * m, 0);
         for (uint64_t k : boxes) {
-            const int L = (int)(k >> 48);
             int lo[3], hi[3];
-            const int idx[3] = {(int)((k >> 32) & 0xffff), (int)((k >> 16) & 0xffff), (int)(k & 0xffff)};
-            for (int a = 0; a < 3; ++a) {
-                if (L <= R_) {
-                    lo[a] = idx[a] << (R_ - L);
-                    hi[a] = (idx[a] + 1) << (R_ - L);
-                } else {
-                    lo[a] = idx[a] >> (L - R_);
-                    hi[a] = lo[a] + 1;
-                }
-            }
+            raster(k, lo, hi);
             for (int z = lo[2]; z < hi[2]; ++z)
                 for (int y = lo[1]; y < hi[1]; ++y)
                     for (int x = lo[0]; x < hi[0]; ++x) at(x + 1, y + 1, z + 1) = 1;
@@ -1993,6 +1991,11 @@ class ChangedBoxes {
     }
     // true if block (L, idx) widened by two of its widths meets a changed box
     bool reaches(const BlockRec& b) const {
+        if (R_ <= 6) {  // rows of the block's level (none finer than R: conservatively reached)
+            const int L = b.level;
+            if (L > R_) return true;
+            return (wide_[L][(size_t)b.index[1] + ((size_t)b.index[2] << L)] >> b.index[0]) & 1;
+        }
         int lo[2][3], hi[2][3], nseg[3];
         for (int a = 0; a < 3; ++a) {
             int l, h;  // [l, h) at level R, before the periodic wrap
@@ -2031,6 +2034,80 @@ class ChangedBoxes {
   private:
     int R_, n_;
     std::vector<int32_t> sum_;
+    std::vector<uint64_t> wide_[7];  // per level L <= R: rows of x bits, the cells a changed box can reach
+    // cells [lo, hi) of level R covered by box k (the containing cell for a box finer than R)
+    void raster(uint64_t k, int lo[3], int hi[3]) const {
+        const int L = (int)(k >> 48);
+        const int idx[3] = {(int)((k >> 32) & 0xffff), (int)((k >> 16) & 0xffff), (int)(k & 0xffff)};
+        for (int a = 0; a < 3; ++a) {
+            if (L <= R_) {
+                lo[a] = idx[a] << (R_ - L);
+                hi[a] = (idx[a] + 1) << (R_ - L);
+            } else {
+                lo[a] = idx[a] >> (L - R_);
+                hi[a] = lo[a] + 1;
+            }
+        }
+    }
+    static uint64_t even_bits(uint64_t v) {  // bit k of the result = bit 2k of v
+        v &= 0x5555555555555555ull;
+        v = (v | (v >> 1)) & 0x3333333333333333ull;
+        v = (v | (v >> 2)) & 0x0f0f0f0f0f0f0f0full;
+        v = (v | (v >> 4)) & 0x00ff00ff00ff00ffull;
+        v = (v | (v >> 8)) & 0x0000ffff0000ffffull;
+        return (v | (v >> 16)) & 0x00000000ffffffffull;
+    }
+    void build_rows(const std::vector<uint64_t>& boxes) {
+        std::vector<uint64_t> cells((size_t)n_ * n_, 0);  // the changed cells of level R
+        for (uint64_t k : boxes) {
+            int lo[3], hi[3];
+            raster(k, lo, hi);
+            const uint64_t span = (hi[0] - lo[0] == 64 ? ~0ull : ((1ull << (hi[0] - lo[0])) - 1)) << lo[0];
+            for (int z = lo[2]; z < hi[2]; ++z)
+                for (int y = lo[1]; y < hi[1]; ++y) cells[(size_t)y + ((size_t)z << R_)] |= span;
+        }
+        std::vector<uint64_t> wx, wy;
+        for (int L = R_; L >= 0; --L) {
+            const int n = 1 << L;
+            const uint64_t full = n == 64 ? ~0ull : (1ull << n) - 1;
+            if (L < R_) {  // OR-reduce the level L + 1 rows
+                std::vector<uint64_t> up((size_t)n * n);
+                const size_t dz = (size_t)1 << (L + 1);
+                for (int z = 0; z < n; ++z)
+                    for (int y = 0; y < n; ++y) {
+                        const size_t r0 = (size_t)(2 * y) + (size_t)(2 * z) * dz;
+                        const uint64_t v = cells[r0] | cells[r0 + 1] | cells[r0 + dz] | cells[r0 + dz + 1];
+                        up[(size_t)y + ((size_t)z << L)] = even_bits(v | v >> 1);
+                    }
+                cells.swap(up);
+            }
+            // widen by two cells each way, periodically: x by rotations, then y, then z
+            auto rot = [&](uint64_t v, int k) {  // rotate the n-bit row v left by k (mod n)
+                k = ((k % n) + n) % n;
+                return k == 0 ? v : ((v << k) | (v >> (n - k))) & full;
+            };
+            wx.assign((size_t)n * n, 0);
+            for (size_t r = 0; r < (size_t)n * n; ++r) {
+                const uint64_t v = cells[r];
+                wx[r] = v | rot(v, 1) | rot(v, 2) | rot(v, -1) | rot(v, -2);
+            }
+            wy.assign((size_t)n * n, 0);
+            for (int z = 0; z < n; ++z)
+                for (int y = 0; y < n; ++y) {
+                    uint64_t v = 0;
+                    for (int d = -2; d <= 2; ++d) v |= wx[(size_t)((y + d) & (n - 1)) + ((size_t)z << L)];
+                    wy[(size_t)y + ((size_t)z << L)] = v;
+                }
+            std::vector<uint64_t>& w = wide_[L];
+            w.assign((size_t)n * n, 0);
+            for (int z = 0; z < n; ++z)
+                for (int y = 0; y < n; ++y) {
+                    uint64_t v = 0;
+                    for (int d = -2; d <= 2; ++d) v |= wy[(size_t)y + ((size_t)((z + d) & (n - 1)) << L)];
+                    w[(size_t)y + ((size_t)z << L)] = v;
+                }
+        }
+    }
     int32_t& at(int x, int y, int z) { return sum_[(size_t)x + (size_t)(n_ + 1) * ((size_t)y + (size_t)(n_ + 1) * z)]; }
     int32_t get(int x, int y, int z) const {
         return sum_[(size_t)x + (size_t)(n_ + 1) * ((size_t)y + (size_t)(n_ + 1) * z)];
