@@ -66,6 +66,8 @@ int gh_grid_blocks(gh_ctx* ctx, int* level, int* index);
  *   Refiner_SpaceExtension; *nrefined = refined blocks.
  * gh_compress: Science::AutomaticCompression<0,0>(grid, fwt, tol, max_loops)
  *   (:117-177); *ncollapsed = collapsed parents.  max_loops <= 0: until no change.
+ *   As the next call after gh_refine on the context it reuses the details the
+ *   refinement left valid (same result, fewer evaluations).
  * Both move phi, p_w, p_g, p_csf (split: Hs synthesis, MRAGBlockSplitter.h:24-99;
  * collapse: Ha injection, MRAGBlockCollapser.h:24-85) and recompute the
  * per-solve constants if gh_rd_prepare was called.
