@@ -127,6 +127,7 @@ void Octree::rebuild_index() {
     hash_vals.clear();
     interior_deep.clear();
     nested = false;
+    row_levels = 0;
     if (leaves.empty()) return;
     int top = 0;
     for (const BlockRec& b : leaves) top = std::max(top, b.level);
@@ -160,7 +161,12 @@ void Octree::rebuild_index() {
             return (occ[l][b >> 6] >> (b & 63)) & ((1ull << (1 << l)) - 1);
         };
         std::vector<uint64_t> below((size_t)1 << (2 * top)), cur;  // rows of level l + 1: leaves and interior nodes
-        for (size_t r = 0; r < below.size(); ++r) below[r] = occ_row(top, r);
+        for (int l = 0; l <= top; ++l) {
+            leaf_rows[l].resize((size_t)1 << (2 * l));
+            for (size_t r = 0; r < leaf_rows[l].size(); ++r) leaf_rows[l][r] = occ_row(l, r);
+            node_rows[l].assign((size_t)1 << (2 * l), 0);
+        }
+        below = leaf_rows[top];
         for (int l = top - 1; l >= 0; --l) {
             const int n = 1 << l;
             const size_t dz = (size_t)1 << (l + 1);
@@ -176,10 +182,12 @@ void Octree::rebuild_index() {
                         if (e >= 0) nested = true;  // a leaf containing another leaf
                         e = -2;
                     }
-                    cur[r] = v | occ_row(l, r);
+                    node_rows[l][r] = v;
+                    cur[r] = v | leaf_rows[l][r];
                 }
             below.swap(cur);
         }
+        row_levels = top + 1;
     } else {  // a walk up from each leaf that stops at a node marked before
         for (const BlockRec& b : leaves) {
             for (int l = b.level - 1; l >= 0; --l) {
@@ -701,10 +709,97 @@ std::vector<int> Octree::refinement_victims(const std::vector<char>& flagged, co
 // level l is bad when cov(c) < l - 2; P's own children are leaves, never bad,
 // so P passes iff its whole box holds no bad cell.  The bad cells of each
 // level 2..6 are kept as rows of x bits, 16 row tests per parent.
+// bit k of the result = bits 4k .. 4k + 3 all equal to bit k of v (v < 2^16)
+static inline uint64_t spread4(uint64_t v) {
+    auto part1by1 = [](uint64_t x) {  // bit k -> bit 2k (x < 2^32)
+        x &= 0x00000000ffffffffull;
+        x = (x | (x << 16)) & 0x0000ffff0000ffffull;
+        x = (x | (x << 8)) & 0x00ff00ff00ff00ffull;
+        x = (x | (x << 4)) & 0x0f0f0f0f0f0f0f0full;
+        x = (x | (x << 2)) & 0x3333333333333333ull;
+        return (x | (x << 1)) & 0x5555555555555555ull;
+    };
+    uint64_t t = part1by1(v);
+    t |= t << 1;
+    t = part1by1(t);
+    return t | t << 1;
+}
+
 std::vector<Octree::CollapseCand> Octree::collapse_candidates() const {
     std::vector<CollapseCand> out;
     const int lrow = std::min(dense_levels - 1, 6);
     std::vector<uint64_t> bad_rows[7];
+    if (row_levels == lrow + 1 && lrow >= 2) {
+        // from the index's rows: a cell of level l is bad when it is interior
+        // or its ancestor two levels up is neither a leaf nor interior (it
+        // lies in a leaf coarser than l - 2)
+        for (int l = 2; l <= lrow; ++l) {
+            const int n = 1 << l;
+            const uint64_t full = n == 64 ? ~0ull : (1ull << n) - 1;
+            bad_rows[l].resize((size_t)n * n);
+            for (int z = 0; z < n; ++z)
+                for (int y = 0; y < n; ++y) {
+                    const size_t up = (size_t)(y >> 2) + ((size_t)(z >> 2) << (l - 2));
+                    const uint64_t present = leaf_rows[l - 2][up] | node_rows[l - 2][up];
+                    const size_t r = (size_t)y + ((size_t)z << l);
+                    bad_rows[l][r] = node_rows[l][r] | (~spread4(present) & full);
+                }
+            // widened to the box [x - 1, x + 2] x [y - 1, y + 2] x [z - 1, z + 2]
+            // (periodic): a parent P of level l - 1 passes iff the bit of
+            // (2P) is clear
+            auto rot = [&](uint64_t v, int k) {  // rotate the n-bit row left by k (mod n)
+                k = ((k % n) + n) % n;
+                return k == 0 ? v : ((v << k) | (v >> (n - k))) & full;
+            };
+            std::vector<uint64_t> wx((size_t)n * n), wy((size_t)n * n);
+            for (size_t r = 0; r < wx.size(); ++r) {
+                const uint64_t v = bad_rows[l][r];
+                wx[r] = v | rot(v, 1) | rot(v, -1) | rot(v, -2);
+            }
+            for (int z = 0; z < n; ++z)
+                for (int y = 0; y < n; ++y) {
+                    uint64_t v = 0;
+                    for (int d = -1; d <= 2; ++d) v |= wx[(size_t)((y + d) & (n - 1)) + ((size_t)z << l)];
+                    wy[(size_t)y + ((size_t)z << l)] = v;
+                }
+            for (int z = 0; z < n; ++z)
+                for (int y = 0; y < n; ++y) {
+                    uint64_t v = 0;
+                    for (int d = -1; d <= 2; ++d) v |= wy[(size_t)y + ((size_t)((z + d) & (n - 1)) << l)];
+                    bad_rows[l][(size_t)y + ((size_t)z << l)] = v;
+                }
+        }
+        // parents of levels 1 .. lrow - 1 whose 8 children are leaves, from
+        // the leaf rows of the level below; the root as before
+        if (min_level <= 0) {
+            CollapseCand c{key(0, 0, 0, 0), {}};
+            bool ok = true;
+            for (int o = 0; o < 8 && ok; ++o) ok = (c.kids[o] = dense[1][o]) >= 0;
+            const int P[3] = {0, 0, 0};
+            if (ok && outside_within_one(0, P)) out.push_back(c);
+        }
+        for (int L = std::max(1, min_level); L < lrow; ++L) {
+            const int n = 1 << L, l = L + 1;
+            const size_t dz = (size_t)1 << l;
+            const uint64_t* rows = leaf_rows[l].data();
+            const uint64_t* bad = bad_rows[l].data();
+            const int32_t* map = dense[l].data();
+            for (int pz = 0; pz < n; ++pz)
+                for (int py = 0; py < n; ++py) {
+                    const size_t r0 = (size_t)(2 * py) + (size_t)(2 * pz) * dz;
+                    uint64_t t = rows[r0] & rows[r0 + 1] & rows[r0 + dz] & rows[r0 + dz + 1];
+                    for (uint64_t c8 = even_bits(t & (t >> 1) & ~bad[r0]); c8; c8 &= c8 - 1) {
+                        const int px = __builtin_ctzll(c8);
+                        CollapseCand c{key(L, px, py, pz), {}};
+                        for (int o = 0; o < 8; ++o)
+                            c.kids[o] = map[(size_t)(2 * px + (o & 1)) + ((size_t)(2 * py + ((o >> 1) & 1)) << l) +
+                                            ((size_t)(2 * pz + ((o >> 2) & 1)) << (2 * l))];
+                        out.push_back(c);
+                    }
+                }
+        }
+        return out;
+    }
     if (lrow >= 2) {
         std::vector<int8_t> prev(1, dense[0][0] >= 0 ? 0 : -1), cov;
         for (int l = 1; l <= lrow; ++l) {

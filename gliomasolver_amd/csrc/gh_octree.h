@@ -43,6 +43,11 @@ struct Octree {
     static constexpr int SCAN_MAX = 5;  // levels whose maps (8^5 = 32,768 entries) the plans scan
     std::vector<uint64_t> interior_deep;
     bool nested = false;
+    // for trees up to level 6 (row_levels = finest level + 1, else 0): per
+    // level, rows of x bits (row y + z * 2^l) of its leaves and of its
+    // interior nodes, which the compression candidates are read from
+    int row_levels = 0;
+    std::vector<uint64_t> leaf_rows[7], node_rows[7];
     // dense per-level form of the two maps for levels < dense_levels =
     // min(finest leaf level, DENSE_MAX) + 1 (built by rebuild_index): slot of a
     // leaf, -2 for a node with children, -1 for a point inside a coarser leaf
