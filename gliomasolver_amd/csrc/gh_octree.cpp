@@ -235,16 +235,9 @@ void Octree::rebuild_index() {
         if (leaves[s].level > nd) put(key(leaves[s]), (int32_t)s);
 }
 
-int Octree::levels_min() const {
-    int m = MAXL;
-    for (auto& b : leaves) m = std::min(m, b.level);
-    return m;
-}
-int Octree::levels_max() const {
-    int m = 0;
-    for (auto& b : leaves) m = std::max(m, b.level);
-    return m;
-}
+// the canonical order puts the coarsest leaf first and the finest last
+int Octree::levels_min() const { return leaves.empty() ? MAXL : leaves.front().level; }
+int Octree::levels_max() const { return leaves.empty() ? 0 : leaves.back().level; }
 
 // leaves of the subtree at (L, P) that touch the face/edge/corner facing
 // direction -d (the side adjacent to the querying block)

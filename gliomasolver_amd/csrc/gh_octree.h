@@ -90,7 +90,7 @@ struct Octree {
 
     void init_uniform(int level);
     void rebuild_index();  // canonical slot order (level, z, y, x) + maps; call after any change of `leaves`
-    int levels_min() const;
+    int levels_min() const;  // (leaves in canonical order: after rebuild_index)
     int levels_max() const;
     bool is_uniform() const { return levels_min() == levels_max(); }
 
